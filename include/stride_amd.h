@@ -1,0 +1,149 @@
+/*
+ * stride_amd: MI355X (gfx950) iso-acoustic forward / adjoint solver, C ABI.
+ *
+ * One solver object holds one shot's worth of device state on one GPU: the
+ * medium, boundary description, source / receiver taps, the wavefield buffers
+ * and the saved forward wavefield that the adjoint consumes. It replaces the
+ * three Devito operators of Stride's IsoAcousticDevito (acoustic_iso_state,
+ * acoustic_iso_state_save, acoustic_iso_adjoint;
+ * stride/physics/iso_acoustic/devito.py:151-162) and the buffers GridDevito
+ * keeps for them (stride/physics/common/devito.py:197-885).
+ *
+ * Conventions
+ *  - Arrays are float32, C order, last axis fastest, on the extended grid
+ *    (inner shape + 2*extra), exactly as ScalarField.extended_data
+ *    (stride/problem/data.py:284-293).
+ *  - Time series are time-major: (nt, npoint).
+ *  - Functions return 0 on success, a negative SA_E* code on failure;
+ *    sa_last_error() gives the message (thread-local).
+ *  - "_dev" entry points take device pointers and do not synchronise; the
+ *    others take host pointers, copy, run and synchronise.
+ *  - All work is issued on the solver's stream (sa_set_stream; default the
+ *    null stream).
+ */
+#ifndef STRIDE_AMD_H
+#define STRIDE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SA_OK 0
+#define SA_E_ARG -1      /* invalid argument (ValueError in the reference) */
+#define SA_E_STATE -2    /* call out of order (e.g. adjoint before a saving forward) */
+#define SA_E_HIP -3      /* HIP runtime error */
+#define SA_E_NOMEM -4    /* device allocation failed */
+#define SA_E_UNSUPPORTED -5
+
+typedef struct sa_solver sa_solver;
+
+/* Grid and time axis: Space/Time (stride/problem/domain.py:48-74, 285-330).
+ * absorbing: boundary layer thickness; the interior is middle(absorbing)
+ * (stride/physics/common/devito.py:35-47, 244). */
+typedef struct {
+    int dim;             /* 2 or 3 */
+    int shape[3];        /* extended shape */
+    double spacing[3];   /* metres */
+    int absorbing[3];
+    double dt;           /* seconds */
+    int nt;              /* time.num */
+    int device;          /* HIP device ordinal */
+} sa_grid;
+
+/* Scheme selection (IsoAcousticDevito kwargs, devito.py:40-115 and _check_conditions :1354-1464). */
+typedef struct {
+    int kernel;          /* 2: OT2, 4: OT4 */
+    int rho_mode;        /* 0: no density, 1: constant density (div grad), 2: variable density */
+    int att_power;       /* -2: no attenuation, 0, 2, -1: attenuation_power=None */
+    int boundary;        /* -1: none, 0: sponge_boundary_2, 1: complex_frequency_shift_PML_2 */
+    int diff_source;     /* source injected as is (0) or already differentiated (1): changes the dt power */
+    int undersampling;   /* forward wavefield save every f steps */
+    int save_mode;       /* 0: no save, 1: p.dt2 (vp gradient), 2: raw p (rho / wavelet gradients),
+                            3: raw p every step (exact adjoint) */
+    int tb0, tb1;        /* time_bounds */
+    int grad_vp, grad_rho, grad_src;
+} sa_scheme;
+
+const char *sa_last_error(void);
+const char *sa_version(void);
+
+/* IsoAcousticDevito.__init__ / GridDevito (devito.py:122-183): allocate the solver on grid->device. */
+int sa_create(const sa_grid *grid, sa_solver **out);
+/* deallocate everything (deallocate=True path, devito.py:673-683, 969-979) */
+int sa_destroy(sa_solver *s);
+/* stream the solver issues work on (a hipStream_t); NULL is the null stream */
+int sa_set_stream(sa_solver *s, void *stream);
+
+/* _check_problem / _stencil choices (devito.py:1266-1352, 1466-1608) */
+int sa_set_scheme(sa_solver *s, const sa_scheme *scheme);
+
+/* Medium (devito.py:476-497): vp required; rho used for rho_mode 2; alpha (already in
+ * nepers, i.e. times db_to_neper) used when att_power != -2. Host arrays, extended shape. */
+int sa_set_medium(sa_solver *s, const float *vp, const float *rho, const float *alpha);
+
+/* SpongeBoundary2 layer values per axis before the velocity factor: tables[d * len + l] for
+ * layer l counted from the outer edge (stride/physics/boundaries/boundary.py:59-171). */
+int sa_set_sponge(sa_solver *s, const float *tables, int len);
+
+/* ComplexFrequencyShiftPML2 profiles: profiles[(d * 4 + r) * stride + x], r = sigma, alpha,
+ * dsigma/dx, dalpha/dx (stride/physics/boundaries/devito.py:100-131). */
+int sa_set_pml(sa_solver *s, const float *profiles, int stride);
+
+/* Sparse points: npoint x ntap linear indices into the extended grid (-1 = no tap) and
+ * interpolation weights (src.inject / rec.interpolate, devito.py:338-364). */
+int sa_set_sources(sa_solver *s, int npoint, int ntap, const int64_t *index, const float *weight);
+int sa_set_receivers(sa_solver *s, int npoint, int ntap, const int64_t *index, const float *weight);
+
+/* State run (run_forward, devito.py:515-581): time steps 1 .. tb1-2, src (nt, nsrc) already
+ * windowed, rec (nt, nrec) out. Saves the wavefield per scheme->save_mode. */
+int sa_forward(sa_solver *s, const float *src, float *rec);
+int sa_forward_dev(sa_solver *s, const float *src_dev, float *rec_dev);
+
+/* As-written adjoint (run_adjoint, devito.py:875-920): steps tb1-1-f down to tb0+1,
+ * residual res (nt, nrec) already windowed. Gradient and preconditioner accumulators
+ * are returned on the whole extended grid (non-zero on the interior only), before the
+ * get_grad_vp rescale; any output pointer may be NULL. grad_src (nt, nsrc). */
+int sa_adjoint(sa_solver *s, const float *res, float *grad_vp, float *prec_vp, float *grad_rho,
+               float *prec_rho, float *grad_src);
+
+#define SA_WANT_VP 1
+#define SA_WANT_RHO 2
+/* Device-resident form: res_dev and grad_src_dev (nt, nsrc; may be NULL) are device pointers;
+ * the gradients requested in `want` stay in the solver until read with sa_read_gradient. */
+int sa_adjoint_dev(sa_solver *s, const float *res_dev, int want, float *grad_src_dev);
+/* which: 0 grad_vp, 1 prec_vp, 2 grad_rho, 3 prec_rho -> host array (extended grid) */
+int sa_read_gradient(sa_solver *s, int which, float *out);
+
+/* Exact adjoint of the state run for J = sum_t <res[t], rec[t]> (no reference counterpart:
+ * the reference adjoint above is not the discrete transpose). Requires save_mode 3.
+ * Outputs (extended grid): sensitivities of J to the per-point step coefficients
+ * A, B, G, m and to the buoyancy b (see stride_amd.physics.gradients); lam_src
+ * (nt, nsrc * ntap) holds the adjoint field at each source tap. */
+int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float *gB, float *gG, float *gm,
+                     float *gb, float *lam_src);
+
+/* Copy a wavefield level to the host (extended grid). which = 0: last computed level,
+ * 1: the level before it. Used for wavefield dumps (dump_forward_wavefield). */
+int sa_get_wavefield(sa_solver *s, int which, float *out);
+
+/* deallocate_wavefield (devito.py:191-196): release the saved forward wavefield. */
+int sa_free_saved(sa_solver *s);
+
+/* Kernel timing: when enabled, every launch of the wavefield-update kernel is bracketed by
+ * HIP events on the solver stream. sa_timing_read returns the summed kernel milliseconds
+ * and launch count since the last reset. */
+int sa_timing_enable(sa_solver *s, int enable);
+int sa_timing_read(sa_solver *s, double *kernel_ms, int64_t *launches);
+
+/* Device buffer helpers for callers without another allocator. */
+int sa_device_alloc(sa_solver *s, int64_t bytes, void **out);
+int sa_device_free(sa_solver *s, void *ptr);
+int sa_memcpy(sa_solver *s, void *dst, const void *src, int64_t bytes, int kind /* 1 h2d, 2 d2h, 3 d2d */);
+int sa_synchronize(sa_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

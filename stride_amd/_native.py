@@ -1,0 +1,231 @@
+"""ctypes binding of the C ABI in include/stride_amd.h.
+
+The HIP library is mandatory: when libstride_amd.so is missing or does not load,
+every solver call raises ``NativeLibraryError``. There is no CPU fallback.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ['lib', 'NativeLibraryError', 'SolverError', 'Solver', 'library_path']
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, '_lib', 'libstride_amd.so')
+
+
+class NativeLibraryError(RuntimeError):
+    pass
+
+
+class SolverError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__('stride_amd error %d: %s' % (code, message))
+        self.code = code
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [('dim', ctypes.c_int), ('shape', ctypes.c_int * 3), ('spacing', ctypes.c_double * 3),
+                ('absorbing', ctypes.c_int * 3), ('dt', ctypes.c_double), ('nt', ctypes.c_int),
+                ('device', ctypes.c_int)]
+
+
+class _Scheme(ctypes.Structure):
+    _fields_ = [('kernel', ctypes.c_int), ('rho_mode', ctypes.c_int), ('att_power', ctypes.c_int),
+                ('boundary', ctypes.c_int), ('diff_source', ctypes.c_int), ('undersampling', ctypes.c_int),
+                ('save_mode', ctypes.c_int), ('tb0', ctypes.c_int), ('tb1', ctypes.c_int),
+                ('grad_vp', ctypes.c_int), ('grad_rho', ctypes.c_int), ('grad_src', ctypes.c_int)]
+
+
+EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_stream', 'sa_set_scheme',
+           'sa_set_medium', 'sa_set_sponge', 'sa_set_pml', 'sa_set_sources', 'sa_set_receivers',
+           'sa_forward', 'sa_forward_dev', 'sa_adjoint', 'sa_adjoint_dev', 'sa_read_gradient',
+           'sa_adjoint_exact', 'sa_get_wavefield', 'sa_free_saved', 'sa_timing_enable', 'sa_timing_read',
+           'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize']
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise NativeLibraryError('%s is not built: run python -m stride_amd.build' % library_path)
+        try:
+            handle = ctypes.CDLL(library_path)
+        except OSError as exc:
+            raise NativeLibraryError('cannot load %s: %s' % (library_path, exc))
+        vp = ctypes.c_void_p
+        handle.sa_last_error.restype = ctypes.c_char_p
+        handle.sa_version.restype = ctypes.c_char_p
+        for name in EXPORTS[2:]:
+            getattr(handle, name).restype = ctypes.c_int
+        handle.sa_create.argtypes = [ctypes.POINTER(_Grid), ctypes.POINTER(vp)]
+        _lib = handle
+    return _lib
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Solver:
+    """Owner of one ``sa_solver`` (one GPU, one grid)."""
+
+    def __init__(self, dim, shape, spacing, absorbing, dt, nt, device=0):
+        self._lib = lib()
+        g = _Grid()
+        g.dim = dim
+        for i in range(3):
+            g.shape[i] = shape[i] if i < dim else 1
+            g.spacing[i] = spacing[i] if i < dim else 1.0
+            g.absorbing[i] = absorbing[i] if i < dim else 0
+        g.dt = dt
+        g.nt = nt
+        g.device = device
+        self.key = (dim, tuple(shape), tuple(spacing), tuple(absorbing), float(dt), int(nt), int(device))
+        self.shape = tuple(shape)
+        self.nt = nt
+        self._h = ctypes.c_void_p()
+        self._check(self._lib.sa_create(ctypes.byref(g), ctypes.byref(self._h)))
+        self.nsrc = self.nrec = 0
+        self.src_ntap = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._lib.sa_last_error().decode(errors='replace')
+            if rc == -1:
+                raise ValueError(msg)
+            raise SolverError(rc, msg)
+
+    def close(self):
+        if self._h:
+            self._lib.sa_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_scheme(self, kernel, rho_mode, att_power, boundary, diff_source, undersampling, save_mode,
+                   tb0, tb1, grad_vp=0, grad_rho=0, grad_src=0):
+        s = _Scheme(kernel, rho_mode, att_power, boundary, int(diff_source), undersampling, save_mode,
+                    tb0, tb1, int(grad_vp), int(grad_rho), int(grad_src))
+        self._check(self._lib.sa_set_scheme(self._h, ctypes.byref(s)))
+
+    def set_medium(self, vp, rho=None, alpha=None):
+        vp, rho, alpha = _f32(vp), _f32(rho), _f32(alpha)
+        for a in (vp, rho, alpha):
+            if a is not None and a.shape != self.shape:
+                raise ValueError('medium array of shape %s, expected %s' % (a.shape, self.shape))
+        self._check(self._lib.sa_set_medium(self._h, _p(vp), _p(rho), _p(alpha)))
+
+    def set_sponge(self, tables):
+        tables = _f32(tables)
+        self._check(self._lib.sa_set_sponge(self._h, _p(tables), ctypes.c_int(tables.shape[-1])))
+
+    def set_pml(self, profiles):
+        profiles = _f32(profiles)
+        self._check(self._lib.sa_set_pml(self._h, _p(profiles), ctypes.c_int(profiles.shape[-1])))
+
+    def set_sources(self, idx, w):
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        w = _f32(w)
+        self._check(self._lib.sa_set_sources(self._h, ctypes.c_int(idx.shape[0]), ctypes.c_int(idx.shape[1]),
+                                             _p(idx), _p(w)))
+        self.nsrc, self.src_ntap = idx.shape
+
+    def set_receivers(self, idx, w):
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        w = _f32(w)
+        self._check(self._lib.sa_set_receivers(self._h, ctypes.c_int(idx.shape[0]), ctypes.c_int(idx.shape[1]),
+                                               _p(idx), _p(w)))
+        self.nrec = idx.shape[0]
+
+    def forward(self, src):
+        src = _f32(src)
+        if src.shape != (self.nt, self.nsrc):
+            raise ValueError('source samples of shape %s, expected %s' % (src.shape, (self.nt, self.nsrc)))
+        rec = np.zeros((self.nt, self.nrec), dtype=np.float32)
+        self._check(self._lib.sa_forward(self._h, _p(src), _p(rec)))
+        return rec
+
+    def forward_dev(self, src_ptr, rec_ptr):
+        self._check(self._lib.sa_forward_dev(self._h, ctypes.c_void_p(src_ptr), ctypes.c_void_p(rec_ptr)))
+
+    def adjoint(self, res, want_vp=True, want_rho=False, want_src=False):
+        res = _f32(res)
+        if res.shape != (self.nt, self.nrec):
+            raise ValueError('adjoint source of shape %s, expected %s' % (res.shape, (self.nt, self.nrec)))
+        n = self.shape
+        gv = np.zeros(n, np.float32) if want_vp else None
+        pv = np.zeros(n, np.float32) if want_vp else None
+        gr = np.zeros(n, np.float32) if want_rho else None
+        pr = np.zeros(n, np.float32) if want_rho else None
+        gs = np.zeros((self.nt, self.nsrc), np.float32) if want_src else None
+        self._check(self._lib.sa_adjoint(self._h, _p(res), _p(gv), _p(pv), _p(gr), _p(pr), _p(gs)))
+        return dict(grad_vp=gv, prec_vp=pv, grad_rho=gr, prec_rho=pr, grad_src=gs)
+
+    def adjoint_dev(self, res_ptr, want=1, grad_src_ptr=None):
+        self._check(self._lib.sa_adjoint_dev(self._h, ctypes.c_void_p(res_ptr), ctypes.c_int(want),
+                                             ctypes.c_void_p(grad_src_ptr) if grad_src_ptr else None))
+
+    def read_gradient(self, which):
+        out = np.zeros(self.shape, np.float32)
+        self._check(self._lib.sa_read_gradient(self._h, ctypes.c_int(which), _p(out)))
+        return out
+
+    def adjoint_exact(self, res, want_m=False, want_b=False):
+        res = _f32(res)
+        n = self.shape
+        out = {k: np.zeros(n, np.float32) for k in ('gA', 'gB', 'gG')}
+        out['gm'] = np.zeros(n, np.float32) if want_m else None
+        out['gb'] = np.zeros(n, np.float32) if want_b else None
+        lam = np.zeros((self.nt, self.nsrc * self.src_ntap), np.float32)
+        self._check(self._lib.sa_adjoint_exact(self._h, _p(res), _p(out['gA']), _p(out['gB']), _p(out['gG']),
+                                               _p(out['gm']), _p(out['gb']), _p(lam)))
+        out['lam_src'] = lam.reshape(self.nt, self.nsrc, self.src_ntap)
+        return out
+
+    def wavefield(self, which=0):
+        out = np.zeros(self.shape, np.float32)
+        self._check(self._lib.sa_get_wavefield(self._h, ctypes.c_int(which), _p(out)))
+        return out
+
+    def free_saved(self):
+        self._check(self._lib.sa_free_saved(self._h))
+
+    def timing(self, enable=None):
+        if enable is not None:
+            self._check(self._lib.sa_timing_enable(self._h, ctypes.c_int(int(enable))))
+            return None
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._check(self._lib.sa_timing_read(self._h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def device_alloc(self, nbytes):
+        ptr = ctypes.c_void_p()
+        self._check(self._lib.sa_device_alloc(self._h, ctypes.c_int64(nbytes), ctypes.byref(ptr)))
+        return ptr.value
+
+    def device_free(self, ptr):
+        self._check(self._lib.sa_device_free(self._h, ctypes.c_void_p(ptr)))
+
+    def memcpy(self, dst, src, nbytes, kind):
+        self._check(self._lib.sa_memcpy(self._h, ctypes.c_void_p(dst), ctypes.c_void_p(src),
+                                        ctypes.c_int64(nbytes), ctypes.c_int(kind)))
+
+    def synchronize(self):
+        self._check(self._lib.sa_synchronize(self._h))

@@ -1,0 +1,49 @@
+"""Build the HIP library in-tree: stride_amd/_lib/libstride_amd.so (gfx950).
+
+Run ``python -m stride_amd.build``. Compiles each .hip file under csrc/ with
+hipcc for gfx950 only and links one shared library exposing the C ABI declared
+in include/stride_amd.h.
+"""
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, 'csrc')
+OUT_DIR = os.path.join(HERE, '_lib')
+LIB = os.path.join(OUT_DIR, 'libstride_amd.so')
+ARCH = 'gfx950'
+
+# packed FP32 instructions are disabled for all device code
+FLAGS = ['-O3', '--offload-arch=%s' % ARCH, '-fPIC', '-std=c++17', '-Wall', '-Wno-unused-result',
+         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+         '-I', os.path.join(HERE, '..', 'include')]
+
+
+def sources():
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith('.hip'))
+
+
+def build(verbose=False):
+    hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+    os.makedirs(OUT_DIR, exist_ok=True)
+    objs = []
+    for src in sources():
+        obj = os.path.join(OUT_DIR, os.path.basename(src)[:-4] + '.o')
+        cmd = [hipcc, '-c', src, '-o', obj] + FLAGS
+        if verbose:
+            print(' '.join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB + '.tmp'
+    cmd = [hipcc, '-shared', '-o', tmp] + objs + ['--offload-arch=%s' % ARCH, '-fPIC']
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    for obj in objs:
+        os.remove(obj)
+    return LIB
+
+
+if __name__ == '__main__':
+    print(build(verbose='-v' in sys.argv))

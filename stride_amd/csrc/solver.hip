@@ -1,0 +1,1330 @@
+// stride_amd solver: C ABI (include/stride_amd.h), host orchestration of the time
+// loops, and the general kernels covering every scheme option.
+//
+// The equations are those Stride hands to Devito for IsoAcousticDevito
+// (stride/physics/iso_acoustic/devito.py:1466-1608); the time loops follow
+// run_forward / run_adjoint (devito.py:515-581, 875-920). Every time step is
+//
+//   out = A u + B other + G (K u [+ dt^2/12 K(m K u)]) + Cbeta (lap u - lap other)
+//         - Cpml sum_d u1_d ,                                               (1)
+//
+// with per-point coefficients built once per shot by k_coefficients:
+//   a = dt (alpha vp [power 0] + vp^2 D [sponge]),  e = dt^2 vp^2 D^2 [sponge],
+//   D = 7 dt vp s(x) (sponge layer rule, boundaries/boundary.py:59-171),
+//   W = alpha [power None] else 1,  ia = 1 / (1 + a),
+//   A = W (2 - e) ia,  B = -W (1 - a) ia,  Cq = W dt^2 ia,  m = vp^2 [rho],  G = Cq m,
+//   Cbeta = Cq 2 alpha vp^3 / dt [power 2],  Cpml = Cq vp^2 [PML].
+// This is the solve() of p.dt2 = L + vp^2 (attenuation) - vp^2 (boundary term)
+// for p[t+1] (forward, other = p[t-1]) or p[t-1] (adjoint, other = p[t+1]); the
+// time-reversed form is the same because the adjoint uses the transposed centred
+// first time derivative (boundaries/devito.py:65-67, devito.py:1545).
+// K is the SO10 Laplacian (no density), or sum_d D-_d (bbar_d D+_d) with SO10
+// staggered derivatives (constant density: bbar = 1; variable density: bbar the
+// half-point mean of the buoyancy).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/stride_amd.h"
+#include "kernels.h"
+#include "fast3d.h"
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(x)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(SA_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+#define CHECK_LAUNCH() HIP_TRY(hipGetLastError())
+
+// SO10 weights (Fornberg closed forms)
+static const double kW2[6] = {-5269.0 / 1800.0, 5.0 / 3.0, -5.0 / 21.0, 5.0 / 126.0, -5.0 / 1008.0, 1.0 / 3150.0};
+static const double kW1[6] = {0.0, 5.0 / 6.0, -5.0 / 21.0, 5.0 / 84.0, -5.0 / 504.0, 1.0 / 1260.0};
+static const double kWS[5] = {1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0,
+                              35.0 / 294912.0};
+
+// ------------------------------------------------------------------------------------------
+// kernels
+
+#define DOMAIN_INDEX                                         \
+    int k = blockIdx.x * blockDim.x + threadIdx.x;           \
+    int j = blockIdx.y * blockDim.y + threadIdx.y;           \
+    int i = blockIdx.z;                                      \
+    if (k >= g.n[2] || j >= g.n[1] || i >= g.n[0]) return; \
+    long long p = sa_pidx(g, i, j, k);
+
+// per-point step coefficients (1)
+__global__ void k_coefficients(Geo g, const float *__restrict__ vp, const float *__restrict__ rho,
+                               const float *__restrict__ alpha, const float *__restrict__ tables, int tlen,
+                               int boundary, int att_power, int rho_mode, float dt, float *A, float *B, float *G,
+                               float *M, float *Cbeta, float *Cpml) {
+    DOMAIN_INDEX
+    float v = vp[p];
+    float v2 = v * v;
+    float m = rho_mode == 2 ? v2 * rho[p] : v2;
+    float a = 0.f, e = 0.f, W = 1.f;
+    if (att_power == 0) a += dt * alpha[p] * v;
+    if (boundary == 0) {
+        // sponge layer rule: axis d contributes its layer value at distance l from the
+        // outer edge when every other axis is at least l cells from its edges
+        int x[3] = {i, j, k};
+        int dist[3];
+        for (int d = 0; d < 3; ++d) dist[d] = g.act[d] ? min(x[d], g.n[d] - 1 - x[d]) : 0x7fffffff;
+        float s = 0.f;
+        for (int d = 0; d < 3; ++d) {
+            if (!g.act[d] || g.nabs[d] == 0) continue;
+            int other = 0x7fffffff;
+            for (int e2 = 0; e2 < 3; ++e2)
+                if (e2 != d) other = min(other, dist[e2]);
+            int l0 = x[d], l1 = g.n[d] - 1 - x[d];
+            if (l0 < g.nabs[d] && other >= l0) s += tables[d * tlen + l0];
+            if (l1 < g.nabs[d] && other >= l1) s += tables[d * tlen + l1];
+        }
+        float D = (7.f * (s * v)) * dt;
+        a += dt * v2 * D;
+        e = dt * dt * v2 * D * D;
+    }
+    if (att_power == -1) W = alpha[p];
+    float ia = 1.f / (1.f + a);
+    A[p] = W * (2.f - e) * ia;
+    B[p] = -W * (1.f - a) * ia;
+    float cq = W * (dt * dt) * ia;
+    G[p] = cq * m;
+    M[p] = m;
+    if (Cbeta) Cbeta[p] = cq * 2.f * alpha[p] * v2 * v / dt;
+    if (Cpml) Cpml[p] = cq * v2;
+}
+
+// out = K(u * uscale) for the Laplacian form
+__global__ void k_laplace(Geo g, const float *__restrict__ u, const float *__restrict__ uscale,
+                          float *__restrict__ out) {
+    DOMAIN_INDEX
+    if (!uscale) {
+        out[p] = sa_lap(g, u, p);
+        return;
+    }
+    float acc = 0.f;
+    for (int a = 0; a < 3; ++a) {
+        if (!g.act[a]) continue;
+        long long s = sa_stride(g, a);
+        float v = g.w2[a][0] * u[p] * uscale[p];
+        for (int r = 1; r <= 5; ++r)
+            v += g.w2[a][r] * (u[p + r * s] * uscale[p + r * s] + u[p - r * s] * uscale[p - r * s]);
+        acc += v;
+    }
+    out[p] = acc;
+}
+
+// st = bbar * D+ (u * uscale) over the domain widened by 5 cells along axis a
+__global__ void k_stagger(Geo g, int a, const float *__restrict__ u, const float *__restrict__ uscale,
+                          const float *__restrict__ b, float *__restrict__ st) {
+    int lo[3] = {0, 0, 0};
+    lo[a] = -5;
+    int ext[3] = {g.n[0], g.n[1], g.n[2]};
+    ext[a] += 10;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int i = blockIdx.z;
+    if (k >= ext[2] || j >= ext[1] || i >= ext[0]) return;
+    long long q = sa_pidx(g, i + lo[0], j + lo[1], k + lo[2]);
+    long long s = sa_stride(g, a);
+    float v;
+    if (!uscale) {
+        v = sa_dplus(g, u, q, a);
+    } else {
+        v = 0.f;
+        for (int r = 0; r < 5; ++r)
+            v += g.ws[a][r] * (u[q + (r + 1) * s] * uscale[q + (r + 1) * s] - u[q - r * s] * uscale[q - r * s]);
+    }
+    if (b) v *= 0.5f * (b[q] + b[q + s]);
+    st[q] = v;
+}
+
+__global__ void k_divergence(Geo g, const float *__restrict__ st0, const float *__restrict__ st1,
+                             const float *__restrict__ st2, float *__restrict__ out) {
+    DOMAIN_INDEX
+    float acc = 0.f;
+    if (g.act[0]) acc += sa_dminus(g, st0, p, 0);
+    if (g.act[1]) acc += sa_dminus(g, st1, p, 1);
+    if (g.act[2]) acc += sa_dminus(g, st2, p, 2);
+    out[p] = acc;
+}
+
+struct UpdateArgs {
+    const float *u, *other;
+    float *out;
+    const float *A, *B, *G, *Ku, *KmKu, *Cbeta, *Cpml;
+    const float *u1[3];
+    float c4;  // dt^2 / 12 (OT4)
+    float *save;
+    int save_mode;  // 0 none, 1 dt2, 2 raw current level
+    Box box;
+    float idt2;
+};
+
+__global__ void k_update(Geo g, UpdateArgs x) {
+    DOMAIN_INDEX
+    float u = x.u[p], o = x.other[p];
+    float q = x.Ku[p];
+    if (x.KmKu) q += x.c4 * x.KmKu[p];
+    float acc = x.G[p] * q;
+    if (x.Cbeta) acc += x.Cbeta[p] * (sa_lap(g, x.u, p) - sa_lap(g, x.other, p));
+    if (x.Cpml) {
+        float s1 = 0.f;
+        for (int a = 0; a < 3; ++a)
+            if (x.u1[a]) s1 += x.u1[a][p];
+        acc -= x.Cpml[p] * s1;
+    }
+    float out = x.A[p] * u + x.B[p] * o + acc;
+    x.out[p] = out;
+    if (x.save && sa_inbox(x.box, i, j, k)) {
+        long long b = sa_boxidx(x.box, i, j, k);
+        x.save[b] = x.save_mode == 1 ? (out - 2.f * u + o) * x.idt2 : u;
+    }
+}
+
+// CFS-PML auxiliary update on the two slabs of axis a (boundaries/devito.py:133-171)
+__global__ void k_pml(Geo g, int a, const float *__restrict__ F, float *u1, float *u2, float *u3,
+                      const float *__restrict__ sig, const float *__restrict__ alp, const float *__restrict__ dsig,
+                      const float *__restrict__ dalp, float dt, int backward) {
+    DOMAIN_INDEX
+    int x = a == 0 ? i : (a == 1 ? j : k);
+    if (!(x < g.nabs[a] || x >= g.n[a] - g.nabs[a])) return;
+    float s = sig[x], al = alp[x], ds = dsig[x], da = dalp[x];
+    float dF = sa_d1(g, F, p, a), d2F = sa_d2(g, F, p, a);
+    float v3, v2, v1;
+    if (!backward) {
+        v3 = u3[p] - dt * ((al + s) * u3[p] - s * s * (ds + da) * dF);
+        v2 = u2[p] - dt * ((al + s) * u2[p] - s * (2.f * ds * dF + da * dF + s * d2F) + v3);
+        v1 = u1[p] - dt * ((al + s) * u1[p] - (ds * dF + 2.f * s * d2F) + v2);
+    } else {
+        v3 = u3[p] - dt * ((al + s) * u3[p] + s * s * (ds + da) * dF);
+        v2 = u2[p] - dt * ((al + s) * u2[p] + s * (2.f * ds * dF + da * dF) - s * s * d2F + v3);
+        v1 = u1[p] - dt * ((al + s) * u1[p] + ds * dF - 2.f * s * d2F + v2);
+    }
+    u3[p] = v3;
+    u2[p] = v2;
+    u1[p] = v1;
+}
+
+__global__ void k_scale_copy(Geo g, const float *__restrict__ src, const float *__restrict__ scale,
+                             float *__restrict__ dst) {
+    DOMAIN_INDEX
+    dst[p] = scale ? scale[p] * src[p] : src[p];
+}
+
+// sparse injection, deterministic: one thread per destination cell sums its contributions in
+// a fixed order. dest: unpadded linear index; coef per contribution; values[point].
+__global__ void k_inject(Geo g, int ndest, const long long *__restrict__ dest, const int *__restrict__ off,
+                         const int *__restrict__ point, const float *__restrict__ coef,
+                         const float *__restrict__ values, float sign, float *field, float *save, Box box,
+                         float save_scale) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ndest) return;
+    float acc = 0.f;
+    for (int c = off[d]; c < off[d + 1]; ++c) acc += coef[c] * values[point[c]];
+    acc *= sign;
+    long long lin = dest[d];
+    long long p = sa_lin2p(g, lin);
+    field[p] += acc;
+    if (save) {
+        int k = (int)(lin % g.n[2]);
+        long long r = lin / g.n[2];
+        int j = (int)(r % g.n[1]);
+        int i = (int)(r / g.n[1]);
+        if (sa_inbox(box, i, j, k)) save[sa_boxidx(box, i, j, k)] += acc * save_scale;
+    }
+}
+
+// rec[r] = sign * sum_q w * field (receiver interpolation, wavelet gradient)
+__global__ void k_gather(Geo g, int npt, int ntap, const long long *__restrict__ idx, const float *__restrict__ w,
+                         const float *__restrict__ field, float sign, float *__restrict__ out) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= npt) return;
+    float acc = 0.f;
+    for (int q = 0; q < ntap; ++q) {
+        long long lin = idx[(long long)r * ntap + q];
+        if (lin < 0) continue;
+        acc += w[(long long)r * ntap + q] * field[sa_lin2p(g, lin)];
+    }
+    out[r] = sign * acc;
+}
+
+// per-tap values field / G (exact adjoint: lambda at each source tap)
+__global__ void k_gather_taps(Geo g, int nq, const long long *__restrict__ idx, const float *__restrict__ field,
+                              const float *__restrict__ G, float *__restrict__ out) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    long long lin = idx[q];
+    if (lin < 0) {
+        out[q] = 0.f;
+        return;
+    }
+    long long p = sa_lin2p(g, lin);
+    out[q] = field[p] / G[p];
+}
+
+// injection coefficients per contribution: 0 source (w 2 dt^k vp / h), 1 compat adjoint
+// (w dt^2 vp^2), 2 exact adjoint (w G)
+__global__ void k_tap_coef(Geo g, int nnz, const long long *__restrict__ lin, const float *__restrict__ w,
+                           const float *__restrict__ vp, const float *__restrict__ G, int kind, float scale,
+                           float dt2, float *__restrict__ out) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nnz) return;
+    long long p = sa_lin2p(g, lin[c]);
+    float v = vp[p];
+    if (kind == 0) out[c] = w[c] * scale * v;
+    else if (kind == 1) out[c] = w[c] * dt2 * v * v;
+    else out[c] = w[c] * G[p];
+}
+
+// snapshot (box layout) -> padded work field (zero outside the box)
+__global__ void k_unpack(Geo g, Box box, const float *__restrict__ snap, float *__restrict__ dst) {
+    DOMAIN_INDEX
+    dst[p] = sa_inbox(box, i, j, k) ? snap[sa_boxidx(box, i, j, k)] : 0.f;
+}
+
+// compat vp gradient on the interior (prepare_grad_vp, devito.py:985-1024)
+__global__ void k_grad_vp(Geo g, Box box, const float *__restrict__ ps, const float *__restrict__ psm,
+                          const float *__restrict__ psp, float fdt2, const float *__restrict__ pa, float *grad,
+                          float *prec) {
+    DOMAIN_INDEX
+    if (i < g.nabs[0] || i >= g.n[0] - g.nabs[0] || j < g.nabs[1] || j >= g.n[1] - g.nabs[1] || k < g.nabs[2] ||
+        k >= g.n[2] - g.nabs[2])
+        return;
+    if (!sa_inbox(box, i, j, k)) return;
+    long long b = sa_boxidx(box, i, j, k);
+    float pdt = psm ? (psp[b] - 2.f * ps[b] + psm[b]) * fdt2 : ps[b];
+    grad[p] += pdt * pa[p];
+    prec[p] += pdt * pdt;
+}
+
+// compat rho gradient (prepare_grad_rho, devito.py:1115-1150): work = padded raw snapshot
+__global__ void k_grad_rho(Geo g, const float *__restrict__ work, const float *__restrict__ b,
+                           const float *__restrict__ pa, float *grad, float *prec) {
+    DOMAIN_INDEX
+    if (i < g.nabs[0] || i >= g.n[0] - g.nabs[0] || j < g.nabs[1] || j >= g.n[1] - g.nabs[1] || k < g.nabs[2] ||
+        k >= g.n[2] - g.nabs[2])
+        return;
+    float acc = 0.f;
+    for (int a = 0; a < 3; ++a)
+        if (g.act[a]) acc += sa_dminus(g, b, p, a) * sa_dminus(g, work, p, a);
+    float f = -acc - b[p] * sa_lap(g, work, p);
+    grad[p] += f * pa[p];
+    prec[p] += f * f;
+}
+
+// exact adjoint accumulation for equation t: lambda = mu_{t+1} / G
+__global__ void k_exact_acc(Geo g, const float *__restrict__ mu, const float *__restrict__ G,
+                            const float *__restrict__ u, const float *__restrict__ um, const float *__restrict__ Ku,
+                            float *gA, float *gB, float *gG) {
+    DOMAIN_INDEX
+    float lam = mu[p] / G[p];
+    gA[p] += lam * u[p];
+    gB[p] += lam * um[p];
+    gG[p] += lam * Ku[p];
+}
+
+__global__ void k_exact_acc_m(Geo g, const float *__restrict__ Kmu, const float *__restrict__ Ku, float c4,
+                              float *gm) {
+    DOMAIN_INDEX
+    gm[p] += c4 * Kmu[p] * Ku[p];
+}
+
+__global__ void k_axpy_domain(Geo g, float alpha, const float *__restrict__ x, float *y) {
+    DOMAIN_INDEX
+    y[p] += alpha * x[p];
+}
+
+// d/db through the half-point buoyancy: -1/2 sum_a (D+ mu D+ u)(p) + (D+ mu D+ u)(p - s_a)
+__global__ void k_exact_acc_b(Geo g, const float *su0, const float *su1, const float *su2, const float *sm0,
+                              const float *sm1, const float *sm2, float *gb) {
+    DOMAIN_INDEX
+    const float *su[3] = {su0, su1, su2}, *sm[3] = {sm0, sm1, sm2};
+    float acc = 0.f;
+    for (int a = 0; a < 3; ++a) {
+        if (!g.act[a]) continue;
+        long long s = sa_stride(g, a);
+        acc += sm[a][p] * su[a][p] + sm[a][p - s] * su[a][p - s];
+    }
+    gb[p] += -0.5f * acc;
+}
+
+// ------------------------------------------------------------------------------------------
+// host side
+
+struct DevBuf {
+    float *p = nullptr;
+    size_t n = 0;
+};
+
+struct Sparse {
+    int npoint = 0, ntap = 0;
+    std::vector<long long> idx;  // host copy
+    std::vector<float> w;
+    long long *d_idx = nullptr;
+    float *d_w = nullptr;
+    // deterministic scatter: destinations sorted, CSR of contributions
+    int ndest = 0, nnz = 0;
+    long long *d_dest = nullptr, *d_lin = nullptr;
+    int *d_off = nullptr, *d_point = nullptr;
+    float *d_cw = nullptr;    // raw weights per contribution
+    float *d_coef = nullptr;  // scaled coefficients per contribution
+};
+
+struct sa_solver {
+    sa_grid grid;
+    sa_scheme sch;
+    bool have_scheme = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Geo geo;
+    int P[3];
+    long long total = 0, npts = 0;
+    float dt = 0.f;
+    double hmax = 0;
+    // medium (padded)
+    float *vp = nullptr, *rho = nullptr, *b = nullptr, *alpha = nullptr;
+    bool medium_set = false;
+    // coefficients
+    float *A = nullptr, *B = nullptr, *G = nullptr, *M = nullptr, *Cbeta = nullptr, *Cpml = nullptr;
+    bool coef_dirty = true;
+    // boundary
+    float *tables = nullptr;
+    int tlen = 0;
+    float *pml[3][4] = {};
+    float *u1[3] = {}, *u2[3] = {}, *u3[3] = {};
+    // wavefields and work
+    float *buf[2] = {};
+    float *Ku = nullptr, *L2 = nullptr, *KmKu = nullptr, *lapo = nullptr;
+    float *st[3] = {};
+    float *st2[3] = {};
+    float *work0 = nullptr, *work1 = nullptr;
+    int cur = 0;  // index of the last computed level in buf
+    // sparse
+    Sparse src, rec;
+    // saved wavefield
+    float *saved = nullptr;
+    size_t saved_bytes = 0;
+    int nsave = 0;
+    Box box;
+    int saved_mode = 0;
+    int saved_f = 0;
+    // host staging
+    float *d_series = nullptr;
+    size_t series_n = 0;
+    float *d_out = nullptr;
+    size_t out_n = 0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    double kernel_ms = 0;
+    long long launches = 0;
+    // gradient accumulators (padded): grad_vp, prec_vp, grad_rho, prec_rho
+    float *gacc[4] = {};
+    // fast 3D path
+    Fast3D fast;
+};
+
+static int dalloc(float **p, size_t n) {
+    if (*p) return SA_OK;
+    if (hipMalloc((void **)p, n * sizeof(float)) != hipSuccess) {
+        *p = nullptr;
+        return fail(SA_E_NOMEM, "hipMalloc of %zu floats failed", n);
+    }
+    if (hipMemset(*p, 0, n * sizeof(float)) != hipSuccess) return fail(SA_E_HIP, "hipMemset failed");
+    return SA_OK;
+}
+
+static void dfree(void *p) {
+    if (p) (void)hipFree(p);
+}
+
+#define TRY(x)                  \
+    do {                        \
+        int rc_ = (x);          \
+        if (rc_ != SA_OK) return rc_; \
+    } while (0)
+
+static dim3 domain_grid(const Geo &g, dim3 blk) {
+    return dim3((g.n[2] + blk.x - 1) / blk.x, (g.n[1] + blk.y - 1) / blk.y, g.n[0]);
+}
+
+static const dim3 kBlk(64, 4, 1);
+
+static int zero_field(sa_solver *s, float *p) {
+    HIP_TRY(hipMemsetAsync(p, 0, s->total * sizeof(float), s->stream));
+    return SA_OK;
+}
+
+// padded copy-in from an unpadded extended-grid host array; edge: replicate into halo
+static void pad_host(const sa_solver *s, std::vector<float> &dst, const float *src, bool edge) {
+    const Geo &g = s->geo;
+    dst.assign(s->total, 0.f);
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        int halo_lo = g.act[a] ? (a == 2 ? SA_LEAD : SA_HALO) : 0;
+        int halo_hi = g.act[a] ? (s->P[a] - halo_lo - g.n[a]) : 0;
+        lo[a] = edge ? -halo_lo : 0;
+        hi[a] = edge ? g.n[a] + halo_hi : g.n[a];
+    }
+    for (int i = lo[0]; i < hi[0]; ++i)
+        for (int j = lo[1]; j < hi[1]; ++j)
+            for (int k = lo[2]; k < hi[2]; ++k) {
+                int ii = std::min(std::max(i, 0), g.n[0] - 1);
+                int jj = std::min(std::max(j, 0), g.n[1] - 1);
+                int kk = std::min(std::max(k, 0), g.n[2] - 1);
+                long long p = g.base + (long long)i * g.s0 + (long long)j * g.s1 + k;
+                dst[p] = src[((long long)ii * g.n[1] + jj) * g.n[2] + kk];
+            }
+}
+
+static int upload_padded(sa_solver *s, float **dev, const float *host, bool edge) {
+    TRY(dalloc(dev, s->total));
+    std::vector<float> tmp;
+    pad_host(s, tmp, host, edge);
+    HIP_TRY(hipMemcpy(*dev, tmp.data(), s->total * sizeof(float), hipMemcpyHostToDevice));
+    return SA_OK;
+}
+
+static int download_unpadded(sa_solver *s, float *host, const float *dev) {
+    const Geo &g = s->geo;
+    std::vector<float> tmp(s->total);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(tmp.data(), dev, s->total * sizeof(float), hipMemcpyDeviceToHost));
+    for (int i = 0; i < g.n[0]; ++i)
+        for (int j = 0; j < g.n[1]; ++j)
+            for (int k = 0; k < g.n[2]; ++k)
+                host[((long long)i * g.n[1] + j) * g.n[2] + k] = tmp[g.base + (long long)i * g.s0 + (long long)j * g.s1 + k];
+    return SA_OK;
+}
+
+extern "C" const char *sa_last_error(void) { return g_err.c_str(); }
+extern "C" const char *sa_version(void) { return "stride_amd 0.1 gfx950"; }
+
+extern "C" int sa_create(const sa_grid *grid, sa_solver **out) {
+    if (!grid || !out) return fail(SA_E_ARG, "null argument");
+    if (grid->dim != 2 && grid->dim != 3) return fail(SA_E_ARG, "dim must be 2 or 3, got %d", grid->dim);
+    if (grid->nt < 3) return fail(SA_E_ARG, "nt must be >= 3");
+    for (int d = 0; d < grid->dim; ++d) {
+        if (grid->shape[d] < 1) return fail(SA_E_ARG, "shape[%d] = %d", d, grid->shape[d]);
+        if (grid->absorbing[d] < 0 || 2 * grid->absorbing[d] > grid->shape[d])
+            return fail(SA_E_ARG, "absorbing[%d] = %d does not fit shape %d", d, grid->absorbing[d], grid->shape[d]);
+        if (!(grid->spacing[d] > 0)) return fail(SA_E_ARG, "spacing[%d] must be positive", d);
+    }
+    if (!(grid->dt > 0)) return fail(SA_E_ARG, "dt must be positive");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (grid->device < 0 || grid->device >= ndev) return fail(SA_E_ARG, "device %d not present (%d devices)", grid->device, ndev);
+    HIP_TRY(hipSetDevice(grid->device));
+    sa_solver *s = new sa_solver();
+    s->grid = *grid;
+    s->device = grid->device;
+    Geo &g = s->geo;
+    memset(&g, 0, sizeof(g));
+    int off = 3 - grid->dim;
+    int lead[3];
+    for (int a = 0; a < 3; ++a) {
+        int d = a - off;
+        g.act[a] = d >= 0;
+        g.n[a] = d >= 0 ? grid->shape[d] : 1;
+        g.nabs[a] = d >= 0 ? grid->absorbing[d] : 0;
+        lead[a] = g.act[a] ? (a == 2 ? SA_LEAD : SA_HALO) : 0;
+        s->P[a] = lead[a] + g.n[a] + (g.act[a] ? SA_HALO : 0);
+        double h = d >= 0 ? grid->spacing[d] : 1.0;
+        for (int r = 0; r < 6; ++r) {
+            g.w2[a][r] = g.act[a] ? (float)(kW2[r] / (h * h)) : 0.f;
+            g.w1[a][r] = g.act[a] ? (float)(kW1[r] / h) : 0.f;
+        }
+        for (int r = 0; r < 5; ++r) g.ws[a][r] = g.act[a] ? (float)(kWS[r] / h) : 0.f;
+        if (d >= 0) s->hmax = std::max(s->hmax, h);
+    }
+    s->P[2] = (s->P[2] + 31) / 32 * 32;
+    g.s1 = s->P[2];
+    g.s0 = (long long)s->P[1] * s->P[2];
+    g.base = (long long)lead[0] * g.s0 + (long long)lead[1] * g.s1 + lead[2];
+    g.total = (long long)s->P[0] * g.s0;
+    s->total = g.total;
+    s->npts = (long long)g.n[0] * g.n[1] * g.n[2];
+    s->dt = (float)grid->dt;
+    int rc = dalloc(&s->buf[0], s->total);
+    if (rc == SA_OK) rc = dalloc(&s->buf[1], s->total);
+    if (rc == SA_OK) rc = dalloc(&s->Ku, s->total);
+    if (rc != SA_OK) {
+        sa_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return SA_OK;
+}
+
+static void free_sparse(Sparse &sp) {
+    dfree(sp.d_idx); dfree(sp.d_w); dfree(sp.d_dest); dfree(sp.d_lin); dfree(sp.d_off); dfree(sp.d_point);
+    dfree(sp.d_cw); dfree(sp.d_coef);
+    sp = Sparse();
+}
+
+extern "C" int sa_destroy(sa_solver *s) {
+    if (!s) return SA_OK;
+    (void)hipSetDevice(s->device);
+    (void)hipDeviceSynchronize();
+    float *fs[] = {s->vp, s->rho, s->b, s->alpha, s->A, s->B, s->G, s->M, s->Cbeta, s->Cpml, s->tables,
+                   s->buf[0], s->buf[1], s->Ku, s->L2, s->KmKu, s->lapo, s->work0, s->work1, s->saved,
+                   s->d_series, s->d_out};
+    for (float *p : fs) dfree(p);
+    for (float *p : s->gacc) dfree(p);
+    for (int a = 0; a < 3; ++a) {
+        dfree(s->u1[a]); dfree(s->u2[a]); dfree(s->u3[a]); dfree(s->st[a]); dfree(s->st2[a]);
+        for (int r = 0; r < 4; ++r) dfree(s->pml[a][r]);
+    }
+    free_sparse(s->src);
+    free_sparse(s->rec);
+    fast3d_free(s->fast);
+    for (auto e : s->ev) (void)hipEventDestroy(e);
+    delete s;
+    return SA_OK;
+}
+
+extern "C" int sa_set_stream(sa_solver *s, void *stream) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    s->stream = (hipStream_t)stream;
+    return SA_OK;
+}
+
+extern "C" int sa_set_scheme(sa_solver *s, const sa_scheme *sch) {
+    if (!s || !sch) return fail(SA_E_ARG, "null argument");
+    if (sch->kernel != 2 && sch->kernel != 4) return fail(SA_E_ARG, "Unrecognized kernel %d", sch->kernel);
+    if (sch->rho_mode < 0 || sch->rho_mode > 2) return fail(SA_E_ARG, "rho_mode %d", sch->rho_mode);
+    if (!(sch->att_power == -2 || sch->att_power == -1 || sch->att_power == 0 || sch->att_power == 2))
+        return fail(SA_E_ARG, "The \"attenuation_exponent\" can only take values (0, 2).");
+    if (sch->boundary < -1 || sch->boundary > 1) return fail(SA_E_ARG, "boundary %d", sch->boundary);
+    if (sch->undersampling < 1) return fail(SA_E_ARG, "undersampling must be >= 1");
+    if (sch->tb0 < 0 || sch->tb1 > s->grid.nt || sch->tb1 <= sch->tb0 + 2)
+        return fail(SA_E_ARG, "time bounds (%d, %d) invalid for nt %d", sch->tb0, sch->tb1, s->grid.nt);
+    if (sch->save_mode < 0 || sch->save_mode > 3) return fail(SA_E_ARG, "save_mode %d", sch->save_mode);
+    if (!s->have_scheme || memcmp(&s->sch, sch, sizeof(*sch)) != 0) s->coef_dirty = true;
+    s->sch = *sch;
+    s->have_scheme = true;
+    return SA_OK;
+}
+
+extern "C" int sa_set_medium(sa_solver *s, const float *vp, const float *rho, const float *alpha) {
+    if (!s || !vp) return fail(SA_E_ARG, "vp is required");
+    HIP_TRY(hipSetDevice(s->device));
+    TRY(upload_padded(s, &s->vp, vp, true));
+    if (rho) {
+        std::vector<float> b(s->npts);
+        for (long long q = 0; q < s->npts; ++q) {
+            if (!(rho[q] > 0)) return fail(SA_E_ARG, "rho must be positive");
+            b[q] = 1.f / rho[q];
+        }
+        TRY(upload_padded(s, &s->rho, rho, true));
+        TRY(upload_padded(s, &s->b, b.data(), true));
+    }
+    if (alpha) TRY(upload_padded(s, &s->alpha, alpha, true));
+    s->medium_set = true;
+    s->coef_dirty = true;
+    return SA_OK;
+}
+
+extern "C" int sa_set_sponge(sa_solver *s, const float *tables, int len) {
+    if (!s || !tables || len < 1) return fail(SA_E_ARG, "sponge tables required");
+    for (int d = 0; d < s->grid.dim; ++d)
+        if (s->grid.absorbing[d] > len) return fail(SA_E_ARG, "sponge table shorter than the layer");
+    HIP_TRY(hipSetDevice(s->device));
+    dfree(s->tables);
+    s->tables = nullptr;
+    // re-index tables to internal axes (2D: axis 0 unused)
+    int off = 3 - s->grid.dim;
+    std::vector<float> t(3 * (size_t)len, 0.f);
+    for (int d = 0; d < s->grid.dim; ++d)
+        for (int l = 0; l < len; ++l) t[(size_t)(d + off) * len + l] = tables[(size_t)d * len + l];
+    TRY(dalloc(&s->tables, t.size()));
+    HIP_TRY(hipMemcpy(s->tables, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    s->tlen = len;
+    s->coef_dirty = true;
+    return SA_OK;
+}
+
+extern "C" int sa_set_pml(sa_solver *s, const float *profiles, int stride) {
+    if (!s || !profiles) return fail(SA_E_ARG, "pml profiles required");
+    HIP_TRY(hipSetDevice(s->device));
+    int off = 3 - s->grid.dim;
+    for (int d = 0; d < s->grid.dim; ++d) {
+        int a = d + off;
+        if (stride < s->geo.n[a]) return fail(SA_E_ARG, "pml stride too short");
+        for (int r = 0; r < 4; ++r) {
+            TRY(dalloc(&s->pml[a][r], s->geo.n[a]));
+            HIP_TRY(hipMemcpy(s->pml[a][r], profiles + ((size_t)d * 4 + r) * stride, s->geo.n[a] * sizeof(float),
+                              hipMemcpyHostToDevice));
+        }
+    }
+    return SA_OK;
+}
+
+static int set_sparse(sa_solver *s, Sparse &sp, int npoint, int ntap, const int64_t *index, const float *weight) {
+    if (npoint < 0 || ntap < 1 || (npoint > 0 && (!index || !weight)))
+        return fail(SA_E_ARG, "sparse points need index and weight arrays");
+    HIP_TRY(hipSetDevice(s->device));
+    free_sparse(sp);
+    sp.npoint = npoint;
+    sp.ntap = ntap;
+    size_t nq = (size_t)npoint * ntap;
+    sp.idx.assign(index, index + nq);
+    sp.w.assign(weight, weight + nq);
+    for (size_t q = 0; q < nq; ++q)
+        if (sp.idx[q] >= s->npts) return fail(SA_E_ARG, "tap index %lld outside the grid", (long long)sp.idx[q]);
+    if (nq == 0) return SA_OK;
+    HIP_TRY(hipMalloc((void **)&sp.d_idx, nq * sizeof(long long)));
+    HIP_TRY(hipMalloc((void **)&sp.d_w, nq * sizeof(float)));
+    HIP_TRY(hipMemcpy(sp.d_idx, sp.idx.data(), nq * sizeof(long long), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sp.d_w, sp.w.data(), nq * sizeof(float), hipMemcpyHostToDevice));
+    // destination-sorted CSR for the deterministic scatter (stable order of contributions)
+    std::vector<size_t> ord;
+    for (size_t q = 0; q < nq; ++q)
+        if (sp.idx[q] >= 0) ord.push_back(q);
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return sp.idx[a] < sp.idx[b]; });
+    std::vector<long long> dest, lin;
+    std::vector<int> offv, point;
+    std::vector<float> cw;
+    for (size_t c = 0; c < ord.size(); ++c) {
+        size_t q = ord[c];
+        if (c == 0 || sp.idx[q] != sp.idx[ord[c - 1]]) {
+            dest.push_back(sp.idx[q]);
+            offv.push_back((int)c);
+        }
+        point.push_back((int)(q / ntap));
+        lin.push_back(sp.idx[q]);
+        cw.push_back(sp.w[q]);
+    }
+    offv.push_back((int)ord.size());
+    sp.ndest = (int)dest.size();
+    sp.nnz = (int)ord.size();
+    if (sp.nnz == 0) return SA_OK;
+    HIP_TRY(hipMalloc((void **)&sp.d_dest, dest.size() * sizeof(long long)));
+    HIP_TRY(hipMalloc((void **)&sp.d_lin, lin.size() * sizeof(long long)));
+    HIP_TRY(hipMalloc((void **)&sp.d_off, offv.size() * sizeof(int)));
+    HIP_TRY(hipMalloc((void **)&sp.d_point, point.size() * sizeof(int)));
+    HIP_TRY(hipMalloc((void **)&sp.d_cw, cw.size() * sizeof(float)));
+    HIP_TRY(hipMalloc((void **)&sp.d_coef, cw.size() * sizeof(float)));
+    HIP_TRY(hipMemcpy(sp.d_dest, dest.data(), dest.size() * sizeof(long long), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sp.d_lin, lin.data(), lin.size() * sizeof(long long), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sp.d_off, offv.data(), offv.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sp.d_point, point.data(), point.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sp.d_cw, cw.data(), cw.size() * sizeof(float), hipMemcpyHostToDevice));
+    return SA_OK;
+}
+
+extern "C" int sa_set_sources(sa_solver *s, int npoint, int ntap, const int64_t *index, const float *weight) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    return set_sparse(s, s->src, npoint, ntap, index, weight);
+}
+
+extern "C" int sa_set_receivers(sa_solver *s, int npoint, int ntap, const int64_t *index, const float *weight) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    return set_sparse(s, s->rec, npoint, ntap, index, weight);
+}
+
+static bool use_fast(const sa_solver *s) {
+    const sa_scheme &c = s->sch;
+    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 && c.att_power == -2 && c.boundary != 1 &&
+           (c.save_mode == 0 || c.save_mode == 1) && fast3d_enabled();
+}
+
+// build per-point coefficients and boundary state for the current scheme + medium
+static int prepare(sa_solver *s) {
+    if (!s->have_scheme) return fail(SA_E_STATE, "sa_set_scheme must be called first");
+    if (!s->medium_set) return fail(SA_E_STATE, "sa_set_medium must be called first");
+    const sa_scheme &c = s->sch;
+    if (c.rho_mode == 2 && !s->rho) return fail(SA_E_ARG, "variable density needs rho");
+    if (c.att_power != -2 && !s->alpha) return fail(SA_E_ARG, "attenuation needs alpha");
+    if (c.boundary == 0 && !s->tables) return fail(SA_E_ARG, "sponge boundary needs sa_set_sponge");
+    if (c.boundary == 1)
+        for (int a = 0; a < 3; ++a)
+            if (s->geo.act[a] && !s->pml[a][0]) return fail(SA_E_ARG, "PML boundary needs sa_set_pml");
+    HIP_TRY(hipSetDevice(s->device));
+    if (!s->coef_dirty) return SA_OK;
+    TRY(dalloc(&s->A, s->total));
+    TRY(dalloc(&s->B, s->total));
+    TRY(dalloc(&s->G, s->total));
+    TRY(dalloc(&s->M, s->total));
+    if (c.att_power == 2) TRY(dalloc(&s->Cbeta, s->total));
+    if (c.boundary == 1) TRY(dalloc(&s->Cpml, s->total));
+    const Geo &g = s->geo;
+    k_coefficients<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(
+        g, s->vp, s->rho, s->alpha, s->tables, s->tlen, c.boundary, c.att_power, c.rho_mode, s->dt, s->A, s->B,
+        s->G, s->M, c.att_power == 2 ? s->Cbeta : nullptr, c.boundary == 1 ? s->Cpml : nullptr);
+    CHECK_LAUNCH();
+    if (c.rho_mode != 0)
+        for (int a = 0; a < 3; ++a)
+            if (g.act[a]) TRY(dalloc(&s->st[a], s->total));
+    if (c.kernel == 4) {
+        TRY(dalloc(&s->L2, s->total));
+        TRY(dalloc(&s->KmKu, s->total));
+    }
+    if (c.boundary == 1) {
+        for (int a = 0; a < 3; ++a)
+            if (g.act[a]) {
+                TRY(dalloc(&s->u1[a], s->total));
+                TRY(dalloc(&s->u2[a], s->total));
+                TRY(dalloc(&s->u3[a], s->total));
+            }
+        if (c.kernel == 4) TRY(dalloc(&s->lapo, s->total));
+    }
+    if (use_fast(s)) TRY(fast3d_prepare(s->fast, g, s->vp, s->tables, s->tlen, c.boundary, s->dt, s->stream));
+    s->coef_dirty = false;
+    return SA_OK;
+}
+
+static int timed_begin(sa_solver *s, hipEvent_t *e0) {
+    if (!s->timing) return SA_OK;
+    if (s->ev_used + 2 > s->ev.size()) {
+        for (int q = 0; q < 256; ++q) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            s->ev.push_back(e);
+        }
+    }
+    *e0 = s->ev[s->ev_used];
+    HIP_TRY(hipEventRecord(*e0, s->stream));
+    return SA_OK;
+}
+
+static int timed_end(sa_solver *s) {
+    if (!s->timing) return SA_OK;
+    hipEvent_t e1 = s->ev[s->ev_used + 1];
+    HIP_TRY(hipEventRecord(e1, s->stream));
+    s->ev_used += 2;
+    s->launches += 1;
+    if (s->ev_used + 2 > 4096) {  // drain the pool
+        HIP_TRY(hipEventSynchronize(s->ev[s->ev_used - 1]));
+        for (size_t q = 0; q < s->ev_used; q += 2) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, s->ev[q], s->ev[q + 1]));
+            s->kernel_ms += ms;
+        }
+        s->ev_used = 0;
+    }
+    return SA_OK;
+}
+
+static int drain_timing(sa_solver *s) {
+    if (!s->timing || s->ev_used == 0) return SA_OK;
+    HIP_TRY(hipEventSynchronize(s->ev[s->ev_used - 1]));
+    for (size_t q = 0; q < s->ev_used; q += 2) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev[q], s->ev[q + 1]));
+        s->kernel_ms += ms;
+    }
+    s->ev_used = 0;
+    return SA_OK;
+}
+
+// K(src * scale) -> dst (domain)
+static int apply_K(sa_solver *s, const float *src, const float *scale, float *dst) {
+    const Geo &g = s->geo;
+    if (s->sch.rho_mode == 0) {
+        k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, src, scale, dst);
+        CHECK_LAUNCH();
+        return SA_OK;
+    }
+    const float *b = s->sch.rho_mode == 2 ? s->b : nullptr;
+    for (int a = 0; a < 3; ++a) {
+        if (!g.act[a]) continue;
+        int ext[3] = {g.n[0], g.n[1], g.n[2]};
+        ext[a] += 10;
+        dim3 grid((ext[2] + kBlk.x - 1) / kBlk.x, (ext[1] + kBlk.y - 1) / kBlk.y, ext[0]);
+        k_stagger<<<grid, kBlk, 0, s->stream>>>(g, a, src, scale, b, s->st[a]);
+        CHECK_LAUNCH();
+    }
+    k_divergence<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->st[0], s->st[1], s->st[2], dst);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
+// one time step of (1): out = step(u, other); optional fused save
+static int step_general(sa_solver *s, const float *u, const float *other, float *out, int backward, float *save,
+                        int save_mode) {
+    const Geo &g = s->geo;
+    const sa_scheme &c = s->sch;
+    hipEvent_t e0;
+    TRY(timed_begin(s, &e0));
+    TRY(apply_K(s, u, nullptr, s->Ku));
+    const float *F = u;
+    if (c.kernel == 4) {
+        TRY(apply_K(s, s->Ku, s->M, s->KmKu));
+        k_scale_copy<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->Ku, s->M, s->L2);
+        CHECK_LAUNCH();
+        F = s->L2;
+    }
+    if (c.boundary == 1) {
+        const float *Fp = c.kernel == 4 ? s->lapo : F;
+        for (int a = 0; a < 3; ++a) {
+            if (!g.act[a]) continue;
+            k_pml<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, a, Fp, s->u1[a], s->u2[a], s->u3[a], s->pml[a][0],
+                                                                s->pml[a][1], s->pml[a][2], s->pml[a][3], s->dt,
+                                                                backward);
+            CHECK_LAUNCH();
+        }
+    }
+    UpdateArgs x;
+    memset(&x, 0, sizeof(x));
+    x.u = u;
+    x.other = other;
+    x.out = out;
+    x.A = s->A;
+    x.B = s->B;
+    x.G = s->G;
+    x.Ku = s->Ku;
+    x.KmKu = c.kernel == 4 ? s->KmKu : nullptr;
+    x.c4 = s->dt * s->dt / 12.f;
+    x.Cbeta = c.att_power == 2 ? s->Cbeta : nullptr;
+    x.Cpml = c.boundary == 1 ? s->Cpml : nullptr;
+    for (int a = 0; a < 3; ++a) x.u1[a] = c.boundary == 1 ? s->u1[a] : nullptr;
+    x.save = save;
+    x.save_mode = save_mode;
+    x.box = s->box;
+    x.idt2 = 1.f / (s->dt * s->dt);
+    k_update<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, x);
+    CHECK_LAUNCH();
+    if (c.kernel == 4 && c.boundary == 1)
+        HIP_TRY(hipMemcpyAsync(s->lapo, s->L2, s->total * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+    TRY(timed_end(s));
+    return SA_OK;
+}
+
+static int coef_sparse(sa_solver *s, Sparse &sp, int kind) {
+    if (sp.nnz == 0) return SA_OK;
+    float scale = s->sch.diff_source ? (float)(2.0 * s->grid.dt * s->grid.dt / s->hmax)
+                                     : (float)(2.0 * s->grid.dt / s->hmax);
+    k_tap_coef<<<(sp.nnz + 255) / 256, 256, 0, s->stream>>>(s->geo, sp.nnz, sp.d_lin, sp.d_cw, s->vp, s->G, kind,
+                                                            scale, s->dt * s->dt, sp.d_coef);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
+static int inject(sa_solver *s, Sparse &sp, const float *values, float sign, float *field, float *save,
+                  float save_scale) {
+    if (sp.ndest == 0) return SA_OK;
+    k_inject<<<(sp.ndest + 127) / 128, 128, 0, s->stream>>>(s->geo, sp.ndest, sp.d_dest, sp.d_off, sp.d_point,
+                                                            sp.d_coef, values, sign, field, save, s->box, save_scale);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
+static int gather(sa_solver *s, Sparse &sp, const float *field, float sign, float *out) {
+    if (sp.npoint == 0) return SA_OK;
+    k_gather<<<(sp.npoint + 127) / 128, 128, 0, s->stream>>>(s->geo, sp.npoint, sp.ntap, sp.d_idx, sp.d_w, field,
+                                                             sign, out);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
+static int ensure_staging(sa_solver *s, float **p, size_t *have, size_t n) {
+    if (*have >= n && *p) return SA_OK;
+    dfree(*p);
+    *p = nullptr;
+    *have = 0;
+    if (hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(float)) != hipSuccess)
+        return fail(SA_E_NOMEM, "staging allocation failed");
+    *have = n;
+    return SA_OK;
+}
+
+static int alloc_saved(sa_solver *s) {
+    const sa_scheme &c = s->sch;
+    const Geo &g = s->geo;
+    int f = c.save_mode == 3 ? 1 : c.undersampling;
+    // vp-only gradient on the fast path needs the interior only; everything else keeps the domain
+    Box b;
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = 0;
+        b.n[a] = g.n[a];
+    }
+    if (c.save_mode == 1) {
+        for (int a = 0; a < 3; ++a) {
+            b.lo[a] = g.nabs[a];
+            b.n[a] = g.n[a] - 2 * g.nabs[a];
+        }
+    }
+    int nsave = c.tb1 / f + 1;
+    size_t per = (size_t)b.n[0] * b.n[1] * b.n[2];
+    size_t bytes = per * nsave * sizeof(float);
+    if (s->saved && s->saved_bytes >= bytes) {
+        HIP_TRY(hipMemsetAsync(s->saved, 0, bytes, s->stream));
+    } else {
+        dfree(s->saved);
+        s->saved = nullptr;
+        s->saved_bytes = 0;
+        if (hipMalloc((void **)&s->saved, bytes) != hipSuccess) {
+            s->saved = nullptr;
+            return fail(SA_E_NOMEM, "cannot allocate %.2f GB for the saved wavefield", bytes / 1e9);
+        }
+        s->saved_bytes = bytes;
+        HIP_TRY(hipMemsetAsync(s->saved, 0, bytes, s->stream));
+    }
+    s->box = b;
+    s->nsave = nsave;
+    s->saved_mode = c.save_mode;
+    s->saved_f = f;
+    return SA_OK;
+}
+
+static size_t box_points(const Box &b) { return (size_t)b.n[0] * b.n[1] * b.n[2]; }
+
+extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    TRY(prepare(s));
+    const sa_scheme &c = s->sch;
+    TRY(coef_sparse(s, s->src, 0));
+    TRY(zero_field(s, s->buf[0]));
+    TRY(zero_field(s, s->buf[1]));
+    if (c.boundary == 1)
+        for (int a = 0; a < 3; ++a)
+            if (s->geo.act[a]) {
+                TRY(zero_field(s, s->u1[a]));
+                TRY(zero_field(s, s->u2[a]));
+                TRY(zero_field(s, s->u3[a]));
+            }
+    if (s->lapo) TRY(zero_field(s, s->lapo));
+    if (s->rec.npoint > 0) HIP_TRY(hipMemsetAsync(rec, 0, sizeof(float) * (size_t)s->grid.nt * s->rec.npoint, s->stream));
+    if (c.save_mode != 0) TRY(alloc_saved(s));
+    const int f = c.save_mode == 3 ? 1 : c.undersampling;
+    const size_t per = box_points(s->box);
+    const float idt2 = 1.f / (s->dt * s->dt);
+    const bool fast = use_fast(s);
+    // levels: buf[t % 2] = p[t]; step t overwrites p[t-1] with p[t+1]
+    int tlast = c.tb1 - 2;
+    for (int t = 1; t <= tlast; ++t) {
+        float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
+        float *save = nullptr;
+        int mode = 0;
+        if (c.save_mode != 0 && t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave) {
+            save = s->saved + (size_t)(t / f) * per;
+            mode = c.save_mode == 1 ? 1 : 2;
+        }
+        if (fast) {
+            hipEvent_t e0;
+            TRY(timed_begin(s, &e0));
+            TRY(fast3d_step(s->fast, s->geo, u, o, o, save, s->box, idt2, s->stream));
+            TRY(timed_end(s));
+        } else {
+            TRY(step_general(s, u, o, o, 0, save, mode));
+        }
+        TRY(inject(s, s->src, src + (size_t)t * s->src.npoint, 1.f, o, mode == 1 ? save : nullptr, idt2));
+        TRY(gather(s, s->rec, o, 1.f, rec + (size_t)t * s->rec.npoint));
+    }
+    s->cur = (tlast + 1) % 2;
+    return SA_OK;
+}
+
+extern "C" int sa_forward(sa_solver *s, const float *src, float *rec) {
+    if (!s || !src || !rec) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    size_t ns = (size_t)s->grid.nt * s->src.npoint, nr = (size_t)s->grid.nt * s->rec.npoint;
+    TRY(ensure_staging(s, &s->d_series, &s->series_n, ns));
+    TRY(ensure_staging(s, &s->d_out, &s->out_n, nr));
+    if (ns) HIP_TRY(hipMemcpyAsync(s->d_series, src, ns * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    TRY(sa_forward_dev(s, s->d_series, s->d_out));
+    if (nr) HIP_TRY(hipMemcpyAsync(rec, s->d_out, nr * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    TRY(drain_timing(s));
+    return SA_OK;
+}
+
+static int zero_aux(sa_solver *s) {
+    TRY(zero_field(s, s->buf[0]));
+    TRY(zero_field(s, s->buf[1]));
+    if (s->sch.boundary == 1)
+        for (int a = 0; a < 3; ++a)
+            if (s->geo.act[a]) {
+                TRY(zero_field(s, s->u1[a]));
+                TRY(zero_field(s, s->u2[a]));
+                TRY(zero_field(s, s->u3[a]));
+            }
+    if (s->lapo) TRY(zero_field(s, s->lapo));
+    return SA_OK;
+}
+
+// compat adjoint with device-resident outputs (padded accumulators gv, pv, gr, pr; gsrc (nt, nsrc))
+static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, float *gr, float *pr, float *gsrc) {
+    const sa_scheme &c = s->sch;
+    const Geo &g = s->geo;
+    if (!s->saved || s->saved_mode == 0 || s->saved_mode == 3)
+        return fail(SA_E_STATE, "the adjoint needs a forward run that saved the wavefield (save_mode 1 or 2)");
+    if (gr && s->saved_mode != 2) return fail(SA_E_STATE, "the density gradient needs the raw wavefield (save_mode 2)");
+    TRY(coef_sparse(s, s->rec, 1));
+    TRY(zero_aux(s));
+    const int f = s->saved_f;
+    const size_t per = box_points(s->box);
+    const float fdt2 = (float)(1.0 / ((double)f * s->grid.dt * (double)f * s->grid.dt));
+    if (gr) {
+        TRY(dalloc(&s->work0, s->total));
+    }
+    const bool fast = use_fast(s);
+    int tstart = c.tb1 - 1 - f, tend = c.tb0 + 1;
+    // buf[t % 2] = p_a[t]; step t overwrites p_a[t+1] with p_a[t-1]
+    for (int t = tstart; t >= tend; --t) {
+        float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
+        bool guard = t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave && (gv || gr);
+        const float *ps = guard ? s->saved + (size_t)(t / f) * per : nullptr;
+        if (fast) {
+            hipEvent_t e0;
+            TRY(timed_begin(s, &e0));
+            TRY(fast3d_step_adjoint(s->fast, g, u, o, o, s->saved_mode == 1 ? ps : nullptr, s->box, gv, pv, s->stream));
+            TRY(timed_end(s));
+        } else {
+            TRY(step_general(s, u, o, o, 1, nullptr, 0));
+        }
+        TRY(inject(s, s->rec, res + (size_t)t * s->rec.npoint, -1.f, o, nullptr, 0.f));
+        if (gsrc) TRY(gather(s, s->src, u, -1.f, gsrc + (size_t)t * s->src.npoint));
+        if (!guard) continue;
+        if (gv && !(fast && s->saved_mode == 1)) {
+            const float *psm = nullptr, *psp = nullptr;
+            if (s->saved_mode == 2) {
+                psm = t / f >= 1 ? s->saved + (size_t)(t / f - 1) * per : nullptr;
+                psp = t / f + 1 < s->nsave ? s->saved + (size_t)(t / f + 1) * per : nullptr;
+                if (!psm || !psp) {
+                    // out-of-range neighbours read as zero: stage a zero snapshot
+                    TRY(dalloc(&s->work1, std::max<long long>(s->total, (long long)per)));
+                    HIP_TRY(hipMemsetAsync(s->work1, 0, per * sizeof(float), s->stream));
+                    if (!psm) psm = s->work1;
+                    if (!psp) psp = s->work1;
+                }
+            }
+            k_grad_vp<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, ps, psm, psp, fdt2, u, gv, pv);
+            CHECK_LAUNCH();
+        }
+        if (gr) {
+            k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, ps, s->work0);
+            CHECK_LAUNCH();
+            k_grad_rho<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->work0, s->b, u, gr, pr);
+            CHECK_LAUNCH();
+        }
+    }
+    s->cur = (tend - 1 + 2) % 2;
+    return SA_OK;
+}
+
+static int grad_outputs(sa_solver *s, int n, float **dev) {
+    for (int q = 0; q < n; ++q) TRY(dalloc(&dev[q], s->total));
+    for (int q = 0; q < n; ++q) TRY(zero_field(s, dev[q]));
+    return SA_OK;
+}
+
+extern "C" int sa_adjoint_dev(sa_solver *s, const float *res, int want, float *grad_src) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    HIP_TRY(hipSetDevice(s->device));
+    TRY(prepare(s));
+    bool want_v = want & SA_WANT_VP, want_r = want & SA_WANT_RHO;
+    if (want_v) TRY(grad_outputs(s, 2, s->gacc));
+    if (want_r) TRY(grad_outputs(s, 2, s->gacc + 2));
+    if (grad_src && s->src.npoint > 0)
+        HIP_TRY(hipMemsetAsync(grad_src, 0, sizeof(float) * (size_t)s->grid.nt * s->src.npoint, s->stream));
+    return adjoint_loop(s, res, want_v ? s->gacc[0] : nullptr, want_v ? s->gacc[1] : nullptr,
+                        want_r ? s->gacc[2] : nullptr, want_r ? s->gacc[3] : nullptr, grad_src);
+}
+
+extern "C" int sa_read_gradient(sa_solver *s, int which, float *out) {
+    if (!s || !out || which < 0 || which > 3) return fail(SA_E_ARG, "bad arguments");
+    if (!s->gacc[which]) return fail(SA_E_STATE, "gradient %d was not computed", which);
+    HIP_TRY(hipSetDevice(s->device));
+    return download_unpadded(s, out, s->gacc[which]);
+}
+
+extern "C" int sa_adjoint(sa_solver *s, const float *res, float *grad_vp, float *prec_vp, float *grad_rho,
+                          float *prec_rho, float *grad_src) {
+    if (!s || !res) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    TRY(prepare(s));
+    size_t nr = (size_t)s->grid.nt * s->rec.npoint, ns = (size_t)s->grid.nt * s->src.npoint;
+    TRY(ensure_staging(s, &s->d_series, &s->series_n, nr));
+    TRY(ensure_staging(s, &s->d_out, &s->out_n, ns));
+    if (nr) HIP_TRY(hipMemcpyAsync(s->d_series, res, nr * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    int want = ((grad_vp || prec_vp) ? SA_WANT_VP : 0) | ((grad_rho || prec_rho) ? SA_WANT_RHO : 0);
+    TRY(sa_adjoint_dev(s, s->d_series, want, grad_src ? s->d_out : nullptr));
+    float *outs[4] = {grad_vp, prec_vp, grad_rho, prec_rho};
+    for (int q = 0; q < 4; ++q)
+        if (outs[q]) TRY(download_unpadded(s, outs[q], s->gacc[q]));
+    if (grad_src && ns) HIP_TRY(hipMemcpyAsync(grad_src, s->d_out, ns * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return drain_timing(s);
+}
+
+extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float *gB, float *gG, float *gm,
+                                float *gb, float *lam_src) {
+    if (!s || !res) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    TRY(prepare(s));
+    const sa_scheme &c = s->sch;
+    const Geo &g = s->geo;
+    if (c.boundary == 1 || c.att_power == 2 || (c.kernel == 4 && c.rho_mode == 2))
+        return fail(SA_E_UNSUPPORTED, "exact adjoint supports sponge/no boundary, attenuation none/0/None, "
+                                      "and OT4 only without variable density");
+    if (!s->saved || s->saved_mode != 3)
+        return fail(SA_E_STATE, "the exact adjoint needs a forward run with save_mode 3");
+    const int nt = s->grid.nt;
+    size_t nr = (size_t)nt * s->rec.npoint;
+    size_t nq = (size_t)s->src.npoint * s->src.ntap;
+    TRY(ensure_staging(s, &s->d_series, &s->series_n, nr));
+    TRY(ensure_staging(s, &s->d_out, &s->out_n, (size_t)nt * nq));
+    if (nr) HIP_TRY(hipMemcpyAsync(s->d_series, res, nr * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    if (nq) HIP_TRY(hipMemsetAsync(s->d_out, 0, (size_t)nt * nq * sizeof(float), s->stream));
+    TRY(coef_sparse(s, s->rec, 2));
+    TRY(zero_aux(s));
+    float *acc[5] = {};
+    TRY(grad_outputs(s, 3, acc));
+    if (c.kernel == 4) TRY(grad_outputs(s, 1, acc + 3));
+    if (c.rho_mode == 2) {
+        TRY(grad_outputs(s, 1, acc + 4));
+        for (int a = 0; a < 3; ++a)
+            if (g.act[a]) TRY(dalloc(&s->st2[a], s->total));
+    }
+    TRY(dalloc(&s->work0, s->total));
+    TRY(dalloc(&s->work1, s->total));
+    float *KuT = nullptr, *Kmu = nullptr;
+    TRY(dalloc(&KuT, s->total));
+    if (c.kernel == 4) TRY(dalloc(&Kmu, s->total));
+    const size_t per = box_points(s->box);
+    const float c4 = s->dt * s->dt / 12.f;
+    int rc = SA_OK;
+    // buf[lv % 2] = mu_lv
+    for (int lv = nt - 1; lv >= 1 && rc == SA_OK; --lv) {
+        float *m1 = s->buf[(lv + 1) % 2], *m2 = s->buf[lv % 2];  // mu_{lv+1}, mu_{lv+2}
+        if (lv >= 2) {
+            rc = step_general(s, m1, m2, m2, 1, nullptr, 0);  // mu_lv over mu_{lv+2}
+            if (rc == SA_OK && lv - 1 >= 1 && lv - 1 <= nt - 2)
+                rc = inject(s, s->rec, s->d_series + (size_t)(lv - 1) * s->rec.npoint, 1.f, m2, nullptr, 0.f);
+        }
+        int t = lv;
+        if (rc != SA_OK || t < 1 || t > nt - 2) continue;
+        if (nq) {
+            k_gather_taps<<<((int)nq + 127) / 128, 128, 0, s->stream>>>(g, (int)nq, s->src.d_idx, m1, s->G,
+                                                                         s->d_out + (size_t)t * nq);
+        }
+        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, s->saved + (size_t)t * per, s->work0);
+        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, s->saved + (size_t)(t - 1) * per, s->work1);
+        if ((rc = (hipGetLastError() == hipSuccess ? SA_OK : fail(SA_E_HIP, "launch failed"))) != SA_OK) break;
+        if ((rc = apply_K(s, s->work0, nullptr, KuT)) != SA_OK) break;
+        if (c.kernel == 4) {
+            // gm += dt^2/12 (K mu)(K u); Ku += dt^2/12 K(m Ku)
+            if ((rc = apply_K(s, m1, nullptr, Kmu)) != SA_OK) break;
+            k_exact_acc_m<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, Kmu, KuT, c4, acc[3]);
+            if ((rc = apply_K(s, KuT, s->M, Kmu)) != SA_OK) break;
+            k_axpy_domain<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, c4, Kmu, KuT);
+        }
+        if (c.rho_mode == 2) {
+            // staggered derivatives of u_t (work0) and mu_{t+1} (m1) without buoyancy
+            for (int a = 0; a < 3; ++a) {
+                if (!g.act[a]) continue;
+                int ext[3] = {g.n[0], g.n[1], g.n[2]};
+                ext[a] += 10;
+                dim3 grid((ext[2] + kBlk.x - 1) / kBlk.x, (ext[1] + kBlk.y - 1) / kBlk.y, ext[0]);
+                k_stagger<<<grid, kBlk, 0, s->stream>>>(g, a, s->work0, nullptr, nullptr, s->st[a]);
+                k_stagger<<<grid, kBlk, 0, s->stream>>>(g, a, m1, nullptr, nullptr, s->st2[a]);
+            }
+            k_exact_acc_b<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->st[0], s->st[1], s->st[2], s->st2[0],
+                                                                        s->st2[1], s->st2[2], acc[4]);
+        }
+        k_exact_acc<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, m1, s->G, s->work0, s->work1, KuT, acc[0],
+                                                                  acc[1], acc[2]);
+        if (hipGetLastError() != hipSuccess) rc = fail(SA_E_HIP, "launch failed");
+    }
+    float *outs[5] = {gA, gB, gG, gm, gb};
+    for (int q = 0; q < 5 && rc == SA_OK; ++q)
+        if (outs[q] && acc[q]) rc = download_unpadded(s, outs[q], acc[q]);
+    if (rc == SA_OK && lam_src && nq) {
+        if (hipMemcpy(lam_src, s->d_out, (size_t)nt * nq * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(SA_E_HIP, "copy of lam_src failed");
+    }
+    for (float *p : acc) dfree(p);
+    dfree(KuT);
+    dfree(Kmu);
+    s->cur = 2 % 2;
+    if (rc == SA_OK) rc = drain_timing(s);
+    return rc;
+}
+
+extern "C" int sa_get_wavefield(sa_solver *s, int which, float *out) {
+    if (!s || !out) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    int idx = which == 0 ? s->cur : 1 - s->cur;
+    return download_unpadded(s, out, s->buf[idx]);
+}
+
+extern "C" int sa_free_saved(sa_solver *s) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    dfree(s->saved);
+    s->saved = nullptr;
+    s->saved_bytes = 0;
+    s->saved_mode = 0;
+    return SA_OK;
+}
+
+extern "C" int sa_timing_enable(sa_solver *s, int enable) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    TRY(drain_timing(s));
+    s->timing = enable != 0;
+    s->kernel_ms = 0;
+    s->launches = 0;
+    return SA_OK;
+}
+
+extern "C" int sa_timing_read(sa_solver *s, double *kernel_ms, int64_t *launches) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    TRY(drain_timing(s));
+    if (kernel_ms) *kernel_ms = s->kernel_ms;
+    if (launches) *launches = s->launches;
+    return SA_OK;
+}
+
+extern "C" int sa_device_alloc(sa_solver *s, int64_t bytes, void **out) {
+    if (!s || !out || bytes < 0) return fail(SA_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(s->device));
+    if (hipMalloc(out, std::max<int64_t>(bytes, 1)) != hipSuccess) return fail(SA_E_NOMEM, "hipMalloc failed");
+    return SA_OK;
+}
+
+extern "C" int sa_device_free(sa_solver *s, void *ptr) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    HIP_TRY(hipSetDevice(s->device));
+    dfree(ptr);
+    return SA_OK;
+}
+
+extern "C" int sa_memcpy(sa_solver *s, void *dst, const void *src, int64_t bytes, int kind) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    HIP_TRY(hipSetDevice(s->device));
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, k, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SA_OK;
+}
+
+extern "C" int sa_synchronize(sa_solver *s) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return drain_timing(s);
+}

@@ -1,0 +1,370 @@
+"""Drop-in iso-acoustic operator backed by the gfx950 HIP library.
+
+``IsoAcousticHIP`` keeps the plugin surface of Stride's ``IsoAcousticDevito``
+(stride/physics/iso_acoustic/devito.py:23-1707) on top of ``ProblemTypeBase``
+(stride/physics/problem_type.py:12-321):
+
+- ``forward(wavelets, vp, rho=None, alpha=None, problem=..., **kwargs) -> Traces``
+  = before_forward / run_forward / after_forward;
+- ``adjoint(adjoint_source, wavelets, vp, rho=None, alpha=None, problem=..., **kwargs)``
+  returns one entry per positional input (gradient or None), vp / rho gradients
+  on the inner grid with a ``.prec`` attribute;
+- ``prepare_grad_*`` / ``init_grad_*`` / ``get_grad_*`` per variable,
+  ``clear_operators``, ``deallocate_wavefield``.
+
+The same keyword options are accepted (kernel, boundary_type, interpolation_type,
+attenuation_power, save_wavefield, save_undersampling, time_bounds,
+diff_source, platform, deviceid, deallocate). ``adjoint_mode='exact'`` selects
+the exact discrete adjoint (every forward level stored) instead of the
+reference's as-written adjoint; the default, ``'compat'``, reproduces the
+reference.
+
+The forward state (solver, saved wavefield, plan) is kept on the instance
+between ``forward`` and ``adjoint`` of a shot, as the reference keeps
+``_wavefield`` and its Devito functions.
+"""
+
+import logging
+
+import numpy as np
+
+from .._native import Solver
+from ..problem import ScalarField, Traces
+from .boundaries import SPONGE, PML
+from .gradients import compat_finish, exact_gradients
+from .plan import build_plan, adjoint_source_samples, RHO_VARIABLE
+
+__all__ = ['ProblemTypeBase', 'IsoAcousticHIP', 'IsoAcousticDevito']
+
+logger = logging.getLogger('stride_amd')
+
+
+class ProblemTypeBase:
+    """before/run/after template and per-variable gradient dispatch
+    (stride/physics/problem_type.py:58-321)."""
+
+    def __init__(self, name=None, grid=None, space=None, time=None, **kwargs):
+        self.name = name or self.__class__.__name__.lower()
+        if grid is not None:
+            space = space or grid.space
+            time = time or grid.time
+        self.space = space
+        self.time = time
+
+    def forward(self, *args, **kwargs):
+        self.before_forward(*args, **kwargs)
+        fw = self.run_forward(*args, **kwargs)
+        out = self.after_forward(*args, **kwargs)
+        return out if out is not None else fw
+
+    def adjoint(self, *args, **kwargs):
+        self.before_adjoint(*args, **kwargs)
+        ad = self.run_adjoint(*args, **kwargs)
+        out = self.after_adjoint(*args, **kwargs)
+        return out if out is not None else ad
+
+    __call__ = forward
+
+    def _dispatch(self, prefix, variable):
+        methods = [getattr(self, prefix + part, None) for part in variable.name.split('_')]
+        methods = [m for m in methods if m is not None]
+        if not methods:
+            raise ValueError('Variable %s not implemented' % variable.name)
+        return methods[0]
+
+    def prepare_grad(self, *wrt, **kwargs):
+        updates = []
+        for variable in wrt:
+            if variable is None or not variable.needs_grad:
+                continue
+            update = self._dispatch('prepare_grad_', variable)(variable, wrt=wrt, **kwargs)
+            updates += list(update) if isinstance(update, tuple) else [update]
+        return updates
+
+    def init_grad(self, *wrt, **kwargs):
+        for variable in wrt:
+            if variable is None or not variable.needs_grad:
+                continue
+            self._dispatch('init_grad_', variable)(variable, wrt=wrt, **kwargs)
+
+    def get_grad(self, *wrt, **kwargs):
+        grads = []
+        for variable in wrt:
+            if variable is None or not variable.needs_grad:
+                grads.append(None)
+                continue
+            grad = self._dispatch('get_grad_', variable)(variable, wrt=wrt, **kwargs)
+            data = grad.data if hasattr(grad, 'data') else grad
+            if np.any(np.isnan(data)) or np.any(np.isinf(data)):
+                msg = 'Nan or inf detected in %s' % self.name
+                problem = kwargs.get('problem')
+                if problem is not None:
+                    msg = '(ShotID %d) ' % problem.shot.id + msg
+                logger.warning(msg)
+                return None
+            grads.append(grad)
+        return tuple(grads)
+
+
+def _needs_grad(*variables):
+    return any(v is not None and getattr(v, 'needs_grad', False) for v in variables)
+
+
+class IsoAcousticHIP(ProblemTypeBase):
+    space_order = 10
+    time_order = 2
+
+    def __init__(self, name=None, grid=None, space=None, time=None, deviceid=0, **kwargs):
+        super().__init__(name=name, grid=grid, space=space, time=time)
+        self.kernel = 'OT4'
+        self.undersampling_factor = 4
+        self.boundary_type = 'sponge_boundary_2'
+        self.interpolation_type = 'linear'
+        self.attenuation_power = 0
+        self.deviceid = deviceid
+        self._solver = None
+        self._plan = None
+        self._traces = None
+        self._grads = None
+        self._save_mode = 0
+        self._bandwidth = (0., 0., 0.)
+
+    # operator lifetime -------------------------------------------------------------------
+
+    def clear_operators(self):
+        if self._solver is not None:
+            self._solver.close()
+        self._solver = None
+
+    def deallocate_wavefield(self, platform='gpu', deallocate=False, **kwargs):
+        if self._solver is not None:
+            self._solver.free_saved()
+        if deallocate:
+            self.clear_operators()
+
+    @property
+    def solver(self):
+        return self._solver
+
+    def _get_solver(self, plan, device):
+        key = (plan.dim, plan.shape, plan.spacing, plan.absorbing, float(plan.dt), int(plan.nt), int(device))
+        if self._solver is None or self._solver.key != key:
+            self.clear_operators()
+            self._solver = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, plan.nt,
+                                  device=device)
+        return self._solver
+
+    # forward -----------------------------------------------------------------------------
+
+    def before_forward(self, wavelets, vp, rho=None, alpha=None, **kwargs):
+        problem = kwargs.get('problem')
+        shot = problem.shot
+        platform = kwargs.get('platform', 'gpu')
+        if platform not in (None, 'gpu', 'amd-hip', 'cpu'):
+            raise ValueError('Unknown platform %s' % platform)
+        device = kwargs.get('deviceid', self.deviceid)
+        mode = kwargs.get('adjoint_mode', 'compat')
+        if mode not in ('compat', 'exact'):
+            raise ValueError('adjoint_mode must be "compat" or "exact"')
+
+        save = kwargs.get('save_wavefield', False) or _needs_grad(vp, rho, alpha)
+        plan = build_plan(self.space, self.time, wavelets.data, vp.extended_data,
+                          rho=None if rho is None else rho.extended_data,
+                          alpha=None if alpha is None else alpha.extended_data,
+                          source_coordinates=shot.source_coordinates,
+                          receiver_coordinates=shot.receiver_coordinates,
+                          kernel=kwargs.get('kernel'),
+                          boundary_type=kwargs.get('boundary_type', 'sponge_boundary_2'),
+                          interpolation_type=kwargs.get('interpolation_type', 'linear'),
+                          attenuation_power=kwargs.get('attenuation_power', 0),
+                          save_undersampling=kwargs.get('save_undersampling'),
+                          time_bounds=kwargs.get('time_bounds'),
+                          diff_source=kwargs.get('diff_source', False),
+                          grad_vp=bool(vp.needs_grad), grad_rho=bool(rho is not None and rho.needs_grad),
+                          grad_alpha=bool(alpha is not None and alpha.needs_grad),
+                          grad_src=bool(getattr(wavelets, 'needs_grad', False)))
+        plan.adjoint_mode = mode
+        for msg in plan.conditions.messages():
+            logger.debug('(ShotID %d) %s', shot.id, msg)
+        if 'OT4' in plan.kernel and plan.boundary == PML:
+            logger.warning('(ShotID %d) Higdon and PML boundary conditions are unstable beyond OT2 limits',
+                           shot.id)
+        self.kernel = plan.kernel
+        self.undersampling_factor = plan.undersampling
+        self.boundary_type = plan.boundary_type
+        self.interpolation_type = plan.interpolation_type
+        self.attenuation_power = plan.attenuation_power
+        self._bandwidth = (plan.conditions.f_min, plan.conditions.f_centre, plan.conditions.f_max)
+
+        if not save:
+            save_mode = 0
+        elif mode == 'exact':
+            save_mode = 3
+        elif _needs_grad(wavelets, rho, alpha):
+            save_mode = 2
+        else:
+            save_mode = 1
+        self._save_mode = save_mode
+
+        solver = self._get_solver(plan, device)
+        solver.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
+                          plan.undersampling, save_mode, plan.tb0, plan.tb1, plan.grad_vp, plan.grad_rho,
+                          plan.grad_src)
+        solver.set_medium(plan.vp, plan.rho, plan.alpha)
+        if plan.boundary == SPONGE:
+            solver.set_sponge(plan.sponge_tables)
+        elif plan.boundary == PML:
+            solver.set_pml(plan.pml)
+        solver.set_sources(plan.src_idx, plan.src_w)
+        solver.set_receivers(plan.rec_idx, plan.rec_w)
+        self._plan = plan
+
+    def run_forward(self, wavelets, vp, rho=None, alpha=None, **kwargs):
+        self._traces = None
+        if np.linalg.norm(wavelets.data) < 1e-31:
+            logger.warning('(ShotID %d) Empty wavelets, not running forward', kwargs['problem'].shot_id)
+            return
+        self._traces = self._solver.forward(self._plan.src_data)
+
+    def after_forward(self, wavelets, vp, rho=None, alpha=None, **kwargs):
+        shot = kwargs['problem'].shot
+        plan = self._plan
+        data = self._traces.T if self._traces is not None else np.zeros((plan.nrec, plan.nt), np.float32)
+        traces = shot.observed.alike(name='modelled', data=np.ascontiguousarray(data, dtype=np.float32))
+        if kwargs.get('deallocate', False) and self._save_mode == 0:
+            self.clear_operators()
+        return traces
+
+    # adjoint -----------------------------------------------------------------------------
+
+    def before_adjoint(self, adjoint_source, wavelets, vp, rho=None, alpha=None, **kwargs):
+        if self._plan is None or self._solver is None:
+            raise RuntimeError('adjoint called before a forward run of this shot')
+        data = adjoint_source.data if hasattr(adjoint_source, 'data') else adjoint_source
+        self._res = adjoint_source_samples(self._plan, data)
+        self._res_norm = np.linalg.norm(data)
+        self.init_grad(wavelets, vp, rho, alpha, **kwargs)
+
+    def run_adjoint(self, adjoint_source, wavelets, vp, rho=None, alpha=None, **kwargs):
+        plan = self._plan
+        self._grads = None
+        if self._res_norm < 1e-31:
+            logger.warning('(ShotID %d) Empty adjoint source, not running adjoint', kwargs['problem'].shot_id)
+            return
+        if plan.adjoint_mode == 'exact':
+            acc = self._solver.adjoint_exact(self._res, want_m=plan.kernel_order == 4,
+                                             want_b=plan.rho_mode == RHO_VARIABLE)
+            self._grads = ('exact', exact_gradients(plan, acc, acc['lam_src']))
+        else:
+            out = self._solver.adjoint(self._res, want_vp=bool(vp.needs_grad),
+                                       want_rho=bool(rho is not None and rho.needs_grad),
+                                       want_src=bool(getattr(wavelets, 'needs_grad', False)))
+            self._grads = ('compat', out)
+
+    def after_adjoint(self, adjoint_source, wavelets, vp, rho=None, alpha=None, **kwargs):
+        grads = self.get_grad(wavelets, vp, rho, alpha, **kwargs)
+        self.deallocate_wavefield(deallocate=kwargs.get('deallocate', False))
+        return grads
+
+    # gradients ---------------------------------------------------------------------------
+
+    def _inner(self, field):
+        return np.ascontiguousarray(field[self.space.inner], dtype=np.float32)
+
+    def prepare_grad_vp(self, vp, **kwargs):
+        return ()
+
+    def init_grad_vp(self, vp, **kwargs):
+        pass
+
+    def get_grad_vp(self, vp, **kwargs):
+        kind, g = self._grads if self._grads is not None else (None, None)
+        shape = self.space.shape
+        if kind == 'exact':
+            grad = self._inner(g['vp'])
+            prec = None
+        elif kind == 'compat':
+            grad, prec = compat_finish(self._inner(g['grad_vp']), self._inner(g['prec_vp']),
+                                       vp.data, transform=vp.transform)
+        else:
+            grad, prec = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
+        out = ScalarField('vp_grad', space=self.space, data=grad, shape=grad.shape, extended_shape=grad.shape)
+        out.prec = None if prec is None else ScalarField('vp_prec', space=self.space, data=prec, shape=prec.shape,
+                                                         extended_shape=prec.shape)
+        return out
+
+    def prepare_grad_rho(self, rho, **kwargs):
+        return ()
+
+    def init_grad_rho(self, rho, **kwargs):
+        pass
+
+    def get_grad_rho(self, rho, **kwargs):
+        kind, g = self._grads if self._grads is not None else (None, None)
+        shape = self.space.shape
+        if kind == 'exact':
+            if 'rho' not in g:
+                raise ValueError('exact density gradient needs variable density')
+            grad, prec = self._inner(g['rho']), None
+        elif kind == 'compat':
+            grad, prec = self._inner(g['grad_rho']), self._inner(g['prec_rho'])
+        else:
+            grad, prec = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
+        out = ScalarField('rho_grad', space=self.space, data=grad, shape=grad.shape, extended_shape=grad.shape)
+        out.prec = None if prec is None else ScalarField('rho_prec', space=self.space, data=prec,
+                                                         shape=prec.shape, extended_shape=prec.shape)
+        return out
+
+    # the reference has no alpha gradient (problem_type.py:226-231 raises for it); the exact
+    # adjoint provides one
+    def _check_alpha(self):
+        if self._plan is None or self._plan.adjoint_mode != 'exact':
+            raise ValueError('Variable alpha not implemented')
+
+    def prepare_grad_alpha(self, alpha, **kwargs):
+        self._check_alpha()
+        return ()
+
+    def init_grad_alpha(self, alpha, **kwargs):
+        self._check_alpha()
+
+    def get_grad_alpha(self, alpha, **kwargs):
+        kind, g = self._grads if self._grads is not None else (None, None)
+        grad = self._inner(g['alpha']) if kind == 'exact' else np.zeros(self.space.shape, np.float32)
+        out = ScalarField('alpha_grad', space=self.space, data=grad, shape=grad.shape, extended_shape=grad.shape)
+        out.prec = None
+        return out
+
+    def prepare_grad_wavelets(self, wavelets, **kwargs):
+        return ()
+
+    def init_grad_wavelets(self, wavelets, **kwargs):
+        if self._plan is not None and self._plan.adjoint_mode == 'exact' and self._plan.diff_source:
+            raise ValueError('exact wavelet gradient with diff_source is not supported')
+
+    def get_grad_wavelets(self, wavelets, **kwargs):
+        kind, g = self._grads if self._grads is not None else (None, None)
+        plan = self._plan
+        if kind == 'exact':
+            window = np.pad(_tukey_for(plan), ((0, plan.nt - plan.tb1),), mode='constant')
+            data = (g['src_samples'] * window[:, None]).T
+        elif kind == 'compat':
+            data = g['grad_src'].T
+        else:
+            data = np.zeros(wavelets.data.shape, np.float32)
+        return Traces('wavelets_grad', data=np.ascontiguousarray(data, dtype=np.float32))
+
+    # convenience -------------------------------------------------------------------------
+
+    def wavefield(self, which=0):
+        """Last (0) or previous (1) wavefield level left in the solver, extended grid."""
+        return self._solver.wavefield(which)
+
+
+def _tukey_for(plan):
+    from ..utils import tukey
+    return tukey(plan.tb1, 0.001)
+
+
+# drop-in name: user code written for Stride's Devito operator imports this name
+IsoAcousticDevito = IsoAcousticHIP

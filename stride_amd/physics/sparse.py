@@ -1,0 +1,100 @@
+"""Off-grid point operators: interpolation taps for sources and receivers.
+
+Each sparse point becomes a fixed-size list of (grid cell, weight) taps:
+
+- ``linear``: multilinear 2^d taps from ``(x - pml_origin) / h`` (Devito's
+  ``SparseTimeFunction`` default interpolation, used by
+  stride/physics/common/devito.py:631-632).
+- ``hicks``: Kaiser-windowed sinc (Hicks 2002, Geophysics 67(1)), half-width 3,
+  b = 4.14, window extended to 3/0.99, tensor product of 7 taps per axis; sources
+  get the 5-tap smoothing blend (stride/physics/common/devito.py:843-885).
+  Support offsets are -3..+3 around ``floor`` of the grid coordinate.
+
+Taps that fall outside the extended grid are dropped (weight 0, index -1).
+Coordinates are shifted by 1e-3*h first (stride/physics/iso_acoustic/devito.py:307-309).
+"""
+
+import numpy as np
+import scipy.special
+
+__all__ = ['linear_taps', 'hicks_coefficients', 'hicks_taps', 'point_taps', 'INTERPOLATION']
+
+INTERPOLATION = ('linear', 'hicks')
+
+
+def _grid_coordinates(space, coordinates):
+    coords = np.asarray(coordinates, dtype=np.float32).astype(np.float64)
+    return (coords - np.array(space.pml_origin)) / np.array(space.spacing)
+
+
+def _flatten(space, cells, weights):
+    """cells (n, taps, dim) int64, weights (n, taps) -> linear indices, dropping
+    out-of-grid taps."""
+    shape = np.array(space.extended_shape)
+    inside = np.all((cells >= 0) & (cells < shape), axis=-1)
+    strides = np.array([int(np.prod(shape[d + 1:])) for d in range(space.dim)], dtype=np.int64)
+    index = np.where(inside, (np.clip(cells, 0, shape - 1) * strides).sum(-1), -1)
+    weights = np.where(inside, weights, 0.)
+    return index.astype(np.int64), weights.astype(np.float32)
+
+
+def linear_taps(space, coordinates):
+    g = _grid_coordinates(space, coordinates)
+    base = np.floor(g).astype(np.int64)
+    frac = (g - base).astype(np.float32)
+    dim = space.dim
+    corners = np.array(np.meshgrid(*[[0, 1]] * dim, indexing='ij')).reshape(dim, -1).T
+    cells = base[:, None, :] + corners[None, :, :]
+    w = np.ones((g.shape[0], corners.shape[0]), dtype=np.float32)
+    for d in range(dim):
+        wd = np.where(corners[None, :, d] == 1, frac[:, d:d + 1], np.float32(1) - frac[:, d:d + 1])
+        w = w * wd
+    return _flatten(space, cells, w)
+
+
+def hicks_coefficients(space, coordinates, smooth=False):
+    """(reference cells (n, dim) int32, coefficients (n, dim, 8) float64), the
+    8th coefficient always zero."""
+    g = _grid_coordinates(space, coordinates)
+    ref = np.floor(g).astype(np.int32)
+    offsets = g - ref
+    b = 4.14
+    half = 3
+    den = scipy.special.iv(0, b)
+    width = half / 0.99
+    coef = np.zeros((g.shape[0], space.dim, 2 * half + 2))
+    for k in range(-half, half + 1):
+        x = offsets - k
+        w = np.minimum((x / width) ** 2, 1.)
+        coef[:, :, half + k] = np.sinc(x) * scipy.special.iv(0, b * np.sqrt(1 - w)) / den
+    if smooth:
+        n = half - 1
+        a, bb, c = coef[:, :, n].copy(), coef[:, :, n + 1].copy(), coef[:, :, n + 2].copy()
+        coef[:, :, n - 1] += a * 0.01
+        coef[:, :, n] = a * 0.98 + bb * 0.03
+        coef[:, :, n + 1] = bb * 0.94 + (a + c) * 0.01
+        coef[:, :, n + 2] = c * 0.98 + bb * 0.03
+        coef[:, :, n + 3] += c * 0.01
+    return ref, coef
+
+
+def hicks_taps(space, coordinates, smooth=False):
+    ref, coef = hicks_coefficients(space, coordinates, smooth=smooth)
+    dim = space.dim
+    k = np.arange(-3, 4)
+    grids = np.array(np.meshgrid(*[k] * dim, indexing='ij')).reshape(dim, -1).T  # (7^d, dim)
+    cells = ref[:, None, :].astype(np.int64) + grids[None, :, :]
+    w = np.ones((ref.shape[0], grids.shape[0]))
+    for d in range(dim):
+        w = w * coef[:, d, grids[:, d] + 3]
+    return _flatten(space, cells, w)
+
+
+def point_taps(space, coordinates, interpolation_type='linear', smooth=False):
+    """Shift by 1e-3*h and return (index (n, taps) int64, weight (n, taps) float32)."""
+    coords = np.asarray(coordinates, dtype=np.float32) + 1e-3 * np.array(space.spacing).reshape(1, -1)
+    if interpolation_type == 'linear':
+        return linear_taps(space, coords)
+    if interpolation_type == 'hicks':
+        return hicks_taps(space, coords, smooth=smooth)
+    raise ValueError('Only "linear" and "hicks" interpolations are allowed.')
