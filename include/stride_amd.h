@@ -131,6 +131,11 @@ int sa_get_wavefield(sa_solver *s, int which, float *out);
 /* deallocate_wavefield (devito.py:191-196): release the saved forward wavefield. */
 int sa_free_saved(sa_solver *s);
 
+/* Tile of the specialised 3D update (3D, OT2, no density, sponge or no boundary, no
+ * attenuation): 64 x 4*rows cells per workgroup (rows in {2, 4, 8}), marching `chunk`
+ * planes of the slowest axis. Used by the autotuner (stride_amd.autotune). */
+int sa_set_tile(sa_solver *s, int rows, int chunk);
+
 /* Kernel timing: when enabled, every launch of the wavefield-update kernel is bracketed by
  * HIP events on the solver stream. sa_timing_read returns the summed kernel milliseconds
  * and launch count since the last reset. */

@@ -68,7 +68,8 @@ typedef struct {
 /* SO10 weights (Fornberg closed forms) */
 static const double W2[6] = {-5269.0 / 1800.0, 5.0 / 3.0, -5.0 / 21.0, 5.0 / 126.0, -5.0 / 1008.0, 1.0 / 3150.0};
 static const double W1[6] = {0.0, 5.0 / 6.0, -5.0 / 21.0, 5.0 / 84.0, -5.0 / 504.0, 1.0 / 1260.0};
-static const double WS[5] = {1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0, 35.0 / 294912.0};
+static const double WS[5] = {19845.0 / 16384.0, -735.0 / 8192.0, 567.0 / 40960.0, -405.0 / 229376.0,
+                              35.0 / 294912.0};
 
 typedef struct {
     const oracle_cfg *c;
@@ -766,5 +767,30 @@ int SFX(oracle_adjoint_exact)(const oracle_cfg *c, const float *vp, const float 
     free(wk.Ku); free(wk.L2); free(wk.KL2); free(wk.lapu); free(wk.lapo);
     kops_free(&ko);
     medium_free(&md);
+    return 0;
+}
+
+/* Spatial operator alone, for pinning the weights: out = K u on the domain, where K is
+ * the SO10 Laplacian (rho_mode 0) or sum_d D-_d (bbar_d D+_d u) (rho_mode 1 / 2, with rho
+ * giving the buoyancy b = 1/rho, edge-padded). u is zero outside the domain. */
+int SFX(oracle_apply_K)(const oracle_cfg *c, const float *rho, const float *u, float *out) {
+    geom G0, *g = &G0;
+    geom_init(g, c);
+    medium md;
+    memset(&md, 0, sizeof(md));
+    if (c->rho_mode == 2) {
+        md.rho = zalloc(g->total);
+        md.b = zalloc(g->total);
+        pad_in(g, md.rho, rho, 1);
+        for (long p = 0; p < g->total; ++p) md.b[p] = md.rho[p] != 0 ? (REAL)1 / md.rho[p] : 0;
+    }
+    kops ko;
+    kops_init(g, &md, c->rho_mode, &ko);
+    REAL *up = zalloc(g->total), *op = zalloc(g->total);
+    pad_in(g, up, u, 0);
+    apply_K(g, &ko, c->rho_mode, up, op);
+    unpad_out(g, out, op);
+    free(up); free(op); free(md.rho); free(md.b);
+    kops_free(&ko);
     return 0;
 }

@@ -171,3 +171,19 @@ def forward_states(plan, precision='f32'):
     _, _, final = forward(plan, save=False, precision=precision, final_state=True)
     states[plan.tb1 - 1] = final[0]
     return states
+
+
+def apply_K(shape, spacing, u, rho_mode=0, rho=None, precision='f64'):
+    """Spatial operator alone (SO10 Laplacian or staggered div(b grad))."""
+    lib = _lib(precision)
+    c = OracleCfg()
+    c.dim = len(shape)
+    for i in range(3):
+        c.n[i] = shape[i] if i < c.dim else 1
+        c.h[i] = spacing[i] if i < c.dim else 1.0
+    c.rho_mode = rho_mode
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    out = np.zeros(shape, np.float32)
+    fn = getattr(lib, 'oracle_apply_K_%s' % precision)
+    fn(ctypes.byref(c), _ptr(_f32(rho)), _ptr(u), _ptr(out))
+    return out

@@ -41,7 +41,7 @@ class _Scheme(ctypes.Structure):
 EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_stream', 'sa_set_scheme',
            'sa_set_medium', 'sa_set_sponge', 'sa_set_pml', 'sa_set_sources', 'sa_set_receivers',
            'sa_forward', 'sa_forward_dev', 'sa_adjoint', 'sa_adjoint_dev', 'sa_read_gradient',
-           'sa_adjoint_exact', 'sa_get_wavefield', 'sa_free_saved', 'sa_timing_enable', 'sa_timing_read',
+           'sa_adjoint_exact', 'sa_get_wavefield', 'sa_free_saved', 'sa_set_tile', 'sa_timing_enable', 'sa_timing_read',
            'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize']
 
 _lib = None
@@ -205,6 +205,9 @@ class Solver:
 
     def free_saved(self):
         self._check(self._lib.sa_free_saved(self._h))
+
+    def set_tile(self, rows, chunk):
+        self._check(self._lib.sa_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk)))
 
     def timing(self, enable=None):
         if enable is not None:

@@ -1,17 +1,39 @@
-// Specialised 3D wavefield-update kernels (declarations).
+// Specialised 3D wavefield update for the headline schedule: OT2, no density
+// term, sponge or no boundary, no attenuation. One pass per time step reads
+// u (with halo), the other level, vp, and writes the new level in place of the
+// other level: 16 algorithmic bytes per grid-point update. The every-f-th-step
+// save of p.dt2 (forward) and the vp gradient / preconditioner accumulation
+// (adjoint) are fused into the same pass.
 #pragma once
 #include <hip/hip_runtime.h>
+
 #include "kernels.h"
 
+struct FastTune {
+    int rows;   // rows (axis 1) per thread; the tile is 64 (axis 2) x 4*rows (axis 1)
+    int chunk;  // planes (axis 0) marched per workgroup
+};
+
 struct Fast3D {
-    float *coef = nullptr;  // placeholder until the specialised path lands
+    float *tables = nullptr;  // sponge corner table in use (device)
+    float *corner = nullptr;  // owned corner table, (nabs0+1) x (nabs1+1) x (nabs2+1)
+    int tlen = 0;
+    int boundary = -1;
+    const float *vp = nullptr;
+    float dt = 0.f;
+    FastTune tune = {4, 45};
+    bool prepared = false;
 };
 
 bool fast3d_enabled();
 int fast3d_prepare(Fast3D &f, const Geo &g, const float *vp, const float *tables, int tlen, int boundary, float dt,
                    hipStream_t stream);
+// forward step: out = step(u, o) (out may alias o); optional fused save of p.dt2 into `save` (box layout)
 int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,
                 float idt2, hipStream_t stream);
+// adjoint step with optional fused gradient: grad += ps * u, prec += ps^2 on the interior (box layout ps)
 int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, const float *ps,
                         const Box &box, float *grad, float *prec, hipStream_t stream);
 void fast3d_free(Fast3D &f);
+// tile shape used by later launches (autotuner)
+int fast3d_set_tune(Fast3D &f, int rows, int chunk);

@@ -1,6 +1,328 @@
+// Specialised 3D wavefield update (see fast3d.h).
+//
+// Work decomposition: a workgroup of 4 waves owns a tile of 64 (axis 2, the
+// contiguous axis) x 4*RPT (axis 1) cells and marches along axis 0 over a chunk of
+// planes. Each lane keeps an 11-deep register queue of u along axis 0 for each of
+// its RPT cells, so the axis-0 second derivative costs no memory traffic; the
+// current plane is staged in LDS with a 5-cell halo for the axis-1 / axis-2
+// derivatives. The halo cells are re-read from global memory, where the
+// neighbouring tiles loaded them as their own cells five planes earlier: the
+// block -> tile mapping puts consecutive tiles on the same XCD (blocks are
+// dispatched round-robin over the 8 XCDs), so those re-reads hit that XCD's L2.
+//
+// Per cell and step the kernel reads u (4 B), the other level (4 B), vp (4 B) and
+// writes the new level (4 B). The sponge coefficients are evaluated on the fly
+// from vp and the per-axis layer tables, only in tiles that touch the layer.
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../../include/stride_amd.h"
 #include "fast3d.h"
-bool fast3d_enabled() { return false; }
-int fast3d_prepare(Fast3D &, const Geo &, const float *, const float *, int, int, float, hipStream_t) { return 0; }
-int fast3d_step(Fast3D &, const Geo &, const float *, const float *, float *, float *, const Box &, float, hipStream_t) { return -5; }
-int fast3d_step_adjoint(Fast3D &, const Geo &, const float *, const float *, float *, const float *, const Box &, float *, float *, hipStream_t) { return -5; }
-void fast3d_free(Fast3D &) {}
+
+namespace {
+
+constexpr int kTZ = 64;
+constexpr int kWaves = 4;
+
+struct FastArgs {
+    const float *u;
+    const float *o;
+    float *out;
+    const float *vp;
+    const float *tables;
+    int tlen;
+    float dt, dt2, idt2;
+    float *save;
+    const float *ps;
+    Box box;
+    float *grad;
+    float *prec;
+    int ntile_y, ntile_z, chunk;
+    int nblocks, per_xcd;
+};
+
+// MODE 0: plain step, 1: fused save of p.dt2, 2: fused gradient accumulation
+template <int RPT, int MODE, bool SPONGE>
+__global__ __launch_bounds__(256) void k_fast3d(Geo g, FastArgs a) {
+    constexpr int TY = kWaves * RPT;
+    constexpr int LY = TY + 10, LZ = kTZ + 10;
+    __shared__ float s[LY][LZ];
+
+    // XCD-aware mapping: hardware block b runs on XCD b % 8; give each XCD a contiguous
+    // range of logical tiles
+    const int b = blockIdx.x;
+    const int L = (b & 7) * a.per_xcd + (b >> 3);
+    if (L >= a.nblocks) return;
+    const int tz = L % a.ntile_z;
+    const int rest = L / a.ntile_z;
+    const int ty = rest % a.ntile_y;
+    const int ch = rest / a.ntile_y;
+    const int z0 = tz * kTZ, y0 = ty * TY;
+    const int x0 = ch * a.chunk;
+    const int x1 = min(x0 + a.chunk, g.n[0]);
+
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int z = z0 + lane;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const bool zload = z < nz + SA_RADIUS;
+    const bool zin = z < nz;
+    const long long s0 = g.s0, s1 = g.s1;
+
+    const float w2x[6] = {g.w2[0][0], g.w2[0][1], g.w2[0][2], g.w2[0][3], g.w2[0][4], g.w2[0][5]};
+    const float w2y[6] = {g.w2[1][0], g.w2[1][1], g.w2[1][2], g.w2[1][3], g.w2[1][4], g.w2[1][5]};
+    const float w2z[6] = {g.w2[2][0], g.w2[2][1], g.w2[2][2], g.w2[2][3], g.w2[2][4], g.w2[2][5]};
+
+    int yr[RPT];
+    bool rin[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        yr[r] = y0 + w + kWaves * r;
+        rin[r] = yr[r] < ny && zin;
+    }
+    const bool tile_yz_interior = SPONGE && y0 >= g.nabs[1] && y0 + TY <= ny - g.nabs[1] && z0 >= g.nabs[2] &&
+                                  z0 + kTZ <= nz - g.nabs[2];
+
+    // register queue: q[r][k] = u at plane x - 5 + k
+    float q[RPT][11];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        const int xx = x0 - SA_RADIUS + k;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const long long p = g.base + xx * s0 + yr[r] * s1 + z;
+            q[r][k] = (yr[r] < ny + SA_RADIUS && zload && xx < nx + SA_RADIUS) ? a.u[p] : 0.f;
+        }
+    }
+
+    for (int x = x0; x < x1; ++x) {
+        const long long px = g.base + (long long)x * s0;
+        // front plane x + 5
+#pragma unroll
+        for (int r = 0; r < RPT; ++r)
+            q[r][10] = (yr[r] < ny + SA_RADIUS && zload && x + SA_RADIUS < nx + SA_RADIUS)
+                           ? a.u[px + SA_RADIUS * s0 + yr[r] * s1 + z]
+                           : 0.f;
+        // the pointwise operands of this plane
+        float ov[RPT], vv[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const long long p = px + yr[r] * s1 + z;
+            ov[r] = rin[r] ? a.o[p] : 0.f;
+            vv[r] = rin[r] ? a.vp[p] : 0.f;
+        }
+        // stage plane x: own cells from the queue, halo rows / columns from memory
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) s[w + kWaves * r + 5][lane + 5] = q[r][5];
+        for (int hy = w; hy < 10; hy += kWaves) {
+            const int yy = hy < 5 ? y0 - 5 + hy : y0 + TY + hy - 5;
+            const int ly = hy < 5 ? hy : TY + hy;
+            s[ly][lane + 5] = (yy < ny + SA_RADIUS && zload) ? a.u[px + yy * s1 + z] : 0.f;
+        }
+        for (int i = threadIdx.x; i < 10 * TY; i += 256) {
+            const int row = i / 10, c = i % 10;
+            const int zz = c < 5 ? z0 - 5 + c : z0 + kTZ + c - 5;
+            const int yy = y0 + row;
+            const int lz = c < 5 ? c : kTZ + c;
+            s[row + 5][lz] = (yy < ny + SA_RADIUS && zz < nz + SA_RADIUS) ? a.u[px + yy * s1 + zz] : 0.f;
+        }
+        __syncthreads();
+
+        const bool plane_interior = tile_yz_interior && x >= g.nabs[0] && x < nx - g.nabs[0];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            if (!rin[r]) continue;
+            const int ly = w + kWaves * r + 5, lz = lane + 5;
+            const float c = q[r][5];
+            float vx = w2x[0] * c, vy = w2y[0] * c, vz = w2z[0] * c;
+#pragma unroll
+            for (int k = 1; k <= 5; ++k) {
+                vx += w2x[k] * (q[r][5 + k] + q[r][5 - k]);
+                vy += w2y[k] * (s[ly + k][lz] + s[ly - k][lz]);
+                vz += w2z[k] * (s[ly][lz + k] + s[ly][lz - k]);
+            }
+            const float lap = vx + vy + vz;
+            const float v = vv[r];
+            const float v2 = v * v;
+            float A = 2.f, B = -1.f, G = a.dt2 * v2;
+            if (SPONGE && !plane_interior) {
+                // layer value from the corner table, indexed by the clipped distance to the edge per axis
+                const int M = a.tlen;
+                const int d0 = min(min(x, nx - 1 - x), M);
+                const int d1 = min(min(yr[r], ny - 1 - yr[r]), M);
+                const int d2 = min(min(z, nz - 1 - z), M);
+                const float sm = a.tables[(d0 * (M + 1) + d1) * (M + 1) + d2];
+                if (sm != 0.f) {
+                    const float D = (7.f * (sm * v)) * a.dt;
+                    const float aa = a.dt * v2 * D;
+                    const float e = a.dt * a.dt * v2 * D * D;
+                    const float ia = 1.f / (1.f + aa);
+                    A = (2.f - e) * ia;
+                    B = -(1.f - aa) * ia;
+                    G = (a.dt2 * ia) * v2;
+                }
+            }
+            const float val = A * c + B * ov[r] + G * lap;
+            const long long p = px + yr[r] * s1 + z;
+            a.out[p] = val;
+            if (MODE == 1) {
+                if (sa_inbox(a.box, x, yr[r], z)) a.save[sa_boxidx(a.box, x, yr[r], z)] = (val - 2.f * c + ov[r]) * a.idt2;
+            } else if (MODE == 2) {
+                if (sa_inbox(a.box, x, yr[r], z)) {
+                    const float psv = a.ps[sa_boxidx(a.box, x, yr[r], z)];
+                    a.grad[p] += psv * c;
+                    a.prec[p] += psv * psv;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RPT; ++r)
+#pragma unroll
+            for (int k = 0; k < 10; ++k) q[r][k] = q[r][k + 1];
+    }
+}
+
+template <int RPT, int MODE>
+void launch_rpt(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
+    if (sponge)
+        k_fast3d<RPT, MODE, true><<<grid, 256, 0, st>>>(g, a);
+    else
+        k_fast3d<RPT, MODE, false><<<grid, 256, 0, st>>>(g, a);
+}
+
+template <int MODE>
+int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
+    const int rows = f.tune.rows;
+    const int TY = kWaves * rows;
+    a.ntile_y = (g.n[1] + TY - 1) / TY;
+    a.ntile_z = (g.n[2] + kTZ - 1) / kTZ;
+    a.chunk = f.tune.chunk;
+    const int nchunk = (g.n[0] + a.chunk - 1) / a.chunk;
+    a.nblocks = a.ntile_y * a.ntile_z * nchunk;
+    a.per_xcd = (a.nblocks + 7) / 8;
+    dim3 grid(a.per_xcd * 8);
+    const bool sponge = f.boundary == 0;
+    switch (rows) {
+        case 2: launch_rpt<2, MODE>(sponge, grid, st, g, a); break;
+        case 4: launch_rpt<4, MODE>(sponge, grid, st, g, a); break;
+        case 8: launch_rpt<8, MODE>(sponge, grid, st, g, a); break;
+        default: return SA_E_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : SA_E_HIP;
+}
+
+FastArgs base_args(const Fast3D &f, const float *u, const float *o, float *out) {
+    FastArgs a = {};
+    a.u = u;
+    a.o = o;
+    a.out = out;
+    a.vp = f.vp;
+    a.tables = f.tables;
+    a.tlen = f.tlen;
+    a.dt = f.dt;
+    a.dt2 = f.dt * f.dt;
+    return a;
+}
+
+}  // namespace
+
+bool fast3d_enabled() {
+    const char *e = getenv("STRIDE_AMD_GENERAL_ONLY");
+    return !(e && e[0] == '1');
+}
+
+int fast3d_set_tune(Fast3D &f, int rows, int chunk) {
+    if (!(rows == 2 || rows == 4 || rows == 8) || chunk < 1) return SA_E_ARG;
+    f.tune.rows = rows;
+    f.tune.chunk = chunk;
+    return 0;
+}
+
+int fast3d_prepare(Fast3D &f, const Geo &g, const float *vp, const float *tables, int tlen, int boundary, float dt,
+                   hipStream_t stream) {
+    f.boundary = boundary;
+    f.vp = vp;
+    f.dt = dt;
+    if (boundary == 0) {
+        // corner table: layer sum for distances to the edge (d0, d1, d2) clipped at the widest
+        // layer M, built with the rule of the general kernels (axis d contributes its value at
+        // distance l < nabs[d] when every other axis is at least l from its edges)
+        std::vector<float> t1(3 * (size_t)tlen);
+        if (hipMemcpy(t1.data(), tables, t1.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+            return SA_E_HIP;
+        const int M = std::max(g.nabs[0], std::max(g.nabs[1], g.nabs[2]));
+        const int m0 = M + 1, m1 = M + 1, m2 = M + 1;
+        std::vector<float> corner((size_t)m0 * m1 * m2, 0.f);
+        for (int d0 = 0; d0 < m0; ++d0)
+            for (int d1 = 0; d1 < m1; ++d1)
+                for (int d2 = 0; d2 < m2; ++d2) {
+                    const int dist[3] = {d0, d1, d2};
+                    float s = 0.f;
+                    for (int d = 0; d < 3; ++d) {
+                        if (dist[d] >= g.nabs[d]) continue;
+                        int other = 0x7fffffff;
+                        for (int e = 0; e < 3; ++e)
+                            if (e != d) other = std::min(other, dist[e]);
+                        if (other >= dist[d]) s += t1[(size_t)d * tlen + dist[d]];
+                    }
+                    corner[((size_t)d0 * m1 + d1) * m2 + d2] = s;
+                }
+        if (f.corner) (void)hipFree(f.corner);
+        f.corner = nullptr;
+        if (hipMalloc((void **)&f.corner, corner.size() * sizeof(float)) != hipSuccess) return SA_E_NOMEM;
+        if (hipMemcpyAsync(f.corner, corner.data(), corner.size() * sizeof(float), hipMemcpyHostToDevice, stream) !=
+            hipSuccess)
+            return SA_E_HIP;
+        if (hipStreamSynchronize(stream) != hipSuccess) return SA_E_HIP;
+    }
+    f.tables = f.corner;
+    f.tlen = std::max(g.nabs[0], std::max(g.nabs[1], g.nabs[2]));  // clip distance of the table
+    if (!f.prepared) {
+        // default tile: 64 x 16 cells, chunk sized so that ~8 chunks cover axis 0
+        f.tune.rows = 4;
+        f.tune.chunk = (g.n[0] + 7) / 8;
+        const char *e = getenv("STRIDE_AMD_TILE");
+        if (e) {
+            int r = 0, c = 0;
+            if (sscanf(e, "%d,%d", &r, &c) == 2) fast3d_set_tune(f, r, c);
+        }
+        f.prepared = true;
+    }
+    return 0;
+}
+
+int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,
+                float idt2, hipStream_t stream) {
+    FastArgs a = base_args(f, u, o, out);
+    if (save) {
+        a.save = save;
+        a.box = box;
+        a.idt2 = idt2;
+        return launch<1>(f, g, a, stream);
+    }
+    return launch<0>(f, g, a, stream);
+}
+
+int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, const float *ps,
+                        const Box &box, float *grad, float *prec, hipStream_t stream) {
+    FastArgs a = base_args(f, u, o, out);
+    if (ps && grad && prec) {
+        a.ps = ps;
+        a.box = box;
+        a.grad = grad;
+        a.prec = prec;
+        return launch<2>(f, g, a, stream);
+    }
+    return launch<0>(f, g, a, stream);
+}
+
+void fast3d_free(Fast3D &f) {
+    if (f.corner) (void)hipFree(f.corner);
+    f.corner = nullptr;
+    f.tables = nullptr;
+    f.prepared = false;
+}

@@ -59,7 +59,7 @@ static int fail(int code, const char *fmt, ...) {
 // SO10 weights (Fornberg closed forms)
 static const double kW2[6] = {-5269.0 / 1800.0, 5.0 / 3.0, -5.0 / 21.0, 5.0 / 126.0, -5.0 / 1008.0, 1.0 / 3150.0};
 static const double kW1[6] = {0.0, 5.0 / 6.0, -5.0 / 21.0, 5.0 / 84.0, -5.0 / 504.0, 1.0 / 1260.0};
-static const double kWS[5] = {1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0,
+static const double kWS[5] = {19845.0 / 16384.0, -735.0 / 8192.0, 567.0 / 40960.0, -405.0 / 229376.0,
                               35.0 / 294912.0};
 
 // ------------------------------------------------------------------------------------------
@@ -1280,6 +1280,13 @@ extern "C" int sa_free_saved(sa_solver *s) {
     s->saved = nullptr;
     s->saved_bytes = 0;
     s->saved_mode = 0;
+    return SA_OK;
+}
+
+extern "C" int sa_set_tile(sa_solver *s, int rows, int chunk) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    s->fast.prepared = true;  // an explicit tile overrides the default
+    if (fast3d_set_tune(s->fast, rows, chunk) != 0) return fail(SA_E_ARG, "tile rows must be 2, 4 or 8 and chunk >= 1");
     return SA_OK;
 }
 

@@ -1,0 +1,66 @@
+"""The specialised 3D update against the general kernels and the oracle: awkward
+sizes (not multiples of the 64 x 4*rows tile), unequal layers per axis, every
+tile shape, fused save and gradient."""
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import small_case, solver_for, rel_l2
+from stride_amd.physics.plan import adjoint_source_samples
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(plan, res_s, general, tile=None):
+    os.environ['STRIDE_AMD_GENERAL_ONLY'] = '1' if general else '0'
+    try:
+        s = solver_for(plan, save_mode=1)
+        if tile:
+            s.set_tile(*tile)
+        rec = s.forward(plan.src_data)
+        g = s.adjoint(res_s, want_vp=True)
+        return rec, g
+    finally:
+        os.environ.pop('STRIDE_AMD_GENERAL_ONLY', None)
+
+
+@pytest.mark.parametrize('n,nabs,tile', [
+    (30, 6, (4, 7)), (37, 5, (2, 13)), (41, 7, (8, 41)), (37, 5, (4, 1)), (70, 8, (4, 30)), (66, 6, (8, 9)),
+])
+def test_fast_matches_general_and_oracle(gpu, n, nabs, tile):
+    plan, _ = small_case(dim=3, n=n, nabs=nabs, nt=90, kernel='OT2', grad_vp=True, nrec=20)
+    print('fast3d n=%d nabs=%d tile=%s' % (n, nabs, tile), flush=True)
+    rng = np.random.default_rng(4)
+    res_s = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, plan.nt)))
+    rec_f, g_f = _run(plan, res_s, general=False, tile=tile)
+    rec_g, g_g = _run(plan, res_s, general=True)
+    ref, saved, _ = oracle.forward(plan, save=True)
+    gref = oracle.adjoint_compat(plan, res_s, saved)
+    assert rel_l2(rec_f, rec_g) < 1e-6
+    assert rel_l2(rec_f.T, ref) < 1e-5
+    for key in ('grad_vp', 'prec_vp'):
+        assert rel_l2(g_f[key], g_g[key]) < 1e-5, key
+        assert rel_l2(g_f[key], gref[key]) < 1e-4, key
+
+
+def test_fast_unequal_layers(gpu):
+    from stride_amd.problem import Space, Time
+    from stride_amd.utils import tone_burst, ellipsoidal
+    from stride_amd.physics.plan import build_plan
+    space = Space(shape=(26, 30, 34), spacing=0.5e-3, extra=(4, 7, 9), absorbing=(4, 7, 9))
+    time = Time(start=0., step=0.08e-6, num=90)
+    rng = np.random.default_rng(0)
+    vp = (1500 + 30 * rng.standard_normal(space.extended_shape)).astype(np.float32)
+    c = tuple(l / 2 for l in space.limit)
+    geom = ellipsoidal(16, (4e-3,) * 3, c, threshold=0.5)
+    plan = build_plan(space, time, tone_burst(0.5e6, 3, 90, 0.08e-6)[None], vp, source_coordinates=geom[:1],
+                      receiver_coordinates=geom, kernel='OT2', grad_vp=True)
+    res_s = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, plan.nt)))
+    rec_f, g_f = _run(plan, res_s, general=False, tile=(2, 11))
+    ref, saved, _ = oracle.forward(plan, save=True)
+    assert rel_l2(rec_f.T, ref) < 1e-5
+    gref = oracle.adjoint_compat(plan, res_s, saved)
+    assert rel_l2(g_f['grad_vp'], gref['grad_vp']) < 1e-4

@@ -1,0 +1,99 @@
+"""The drop-in operator (IsoAcousticHIP / IsoAcousticDevito) end to end on the GPU:
+forward returns Traces (n_rec, nt) like after_forward (devito.py:670-671); adjoint
+returns one entry per positional input with inner-grid gradients and .prec
+(problem_type.py:271-321); errors and empty inputs behave as the reference."""
+
+import numpy as np
+import pytest
+
+import oracle
+from stride_amd import IsoAcousticDevito, IsoAcousticHIP, ScalarField, Shot, SubProblem, Traces
+from stride_amd.configs import config1, config2
+from stride_amd.physics.plan import build_plan, adjoint_source_samples
+from stride_amd.physics.gradients import compat_finish
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(scale=0.3, nt=300, **kw):
+    cfg = config1(scale=scale, nt=nt)
+    space, time = cfg['space'], cfg['time']
+    vp = ScalarField('vp', space=space, data=cfg['vp'], needs_grad=True)
+    shot = Shot(3, cfg['sources'], cfg['receivers'], wavelets=cfg['wavelet'][None].astype(np.float32))
+    return cfg, space, time, vp, shot, SubProblem(shot, space, time)
+
+
+def test_forward_adjoint_roundtrip(gpu):
+    cfg, space, time, vp, shot, problem = _setup()
+    pde = IsoAcousticDevito(space=space, time=time)
+    assert isinstance(pde, IsoAcousticHIP)
+    traces = pde(shot.wavelets, vp, problem=problem)
+    assert isinstance(traces, Traces) and traces.data.shape == (shot.num_points_receivers, time.num)
+    assert np.all(traces.data[:, 0] == 0) and np.all(traces.data[:, -1] == 0)
+    plan = build_plan(space, time, cfg['wavelet'][None], cfg['vp'], source_coordinates=cfg['sources'],
+                      receiver_coordinates=cfg['receivers'], grad_vp=True)
+    ref, saved, _ = oracle.forward(plan, save=True)
+    assert np.linalg.norm(traces.data - ref) / np.linalg.norm(ref) < 1e-5
+    res = traces.data * 0.1
+    out = pde.adjoint(Traces('res', data=res), shot.wavelets, vp, problem=problem)
+    assert isinstance(out, tuple) and len(out) == 2 and out[0] is None
+    grad = out[1]
+    assert grad.data.shape == space.shape and grad.prec.data.shape == space.shape
+    acc = oracle.adjoint_compat(plan, adjoint_source_samples(plan, res), saved)
+    g_ref, p_ref = compat_finish(acc['grad_vp'][space.inner], acc['prec_vp'][space.inner], vp.data, vp.transform)
+    assert np.linalg.norm(grad.data - g_ref) / np.linalg.norm(g_ref) < 1e-4
+    assert np.linalg.norm(grad.prec.data - p_ref) / np.linalg.norm(p_ref) < 1e-4
+
+
+def test_variable_density_attenuation_pml_exact(gpu):
+    cfg = config2(scale=0.125, nt=200)
+    space, time = cfg['space'], cfg['time']
+    vp = ScalarField('vp', space=space, data=cfg['vp'], needs_grad=True)
+    rho = ScalarField('rho', space=space, data=cfg['rho'], needs_grad=True)
+    alpha = ScalarField('alpha', space=space, data=cfg['alpha'], needs_grad=True)
+    shot = Shot(0, cfg['sources'][:1], cfg['receivers'], wavelets=cfg['wavelet'][None].astype(np.float32))
+    problem = SubProblem(shot, space, time)
+    pde = IsoAcousticHIP(space=space, time=time)
+    kw = dict(problem=problem, attenuation_power=0, boundary_type='sponge_boundary_2', adjoint_mode='exact')
+    traces = pde.forward(shot.wavelets, vp, rho, alpha, **kw)
+    assert np.isfinite(traces.data).all() and np.abs(traces.data).max() > 0
+    g = pde.adjoint(traces.data * 0.1, shot.wavelets, vp, rho, alpha, **kw)
+    assert g[0] is None and all(x is not None for x in g[1:])
+    for x in g[1:]:
+        assert x.data.shape == space.shape and np.isfinite(x.data).all()
+
+
+def test_reference_errors(gpu):
+    cfg, space, time, vp, shot, problem = _setup()
+    pde = IsoAcousticHIP(space=space, time=time)
+    with pytest.raises(ValueError):
+        pde.forward(shot.wavelets, vp, problem=problem, boundary_type='higdon')
+    with pytest.raises(ValueError):
+        pde.forward(shot.wavelets, vp, problem=problem, interpolation_type='sinc')
+    alpha = ScalarField('alpha', space=space, data=np.zeros(space.extended_shape, np.float32), needs_grad=True)
+    with pytest.raises(ValueError):
+        pde.forward(shot.wavelets, vp, None, alpha, problem=problem, attenuation_power=1)
+    # compat mode has no alpha gradient, like the reference (problem_type.py:226-231)
+    pde.forward(shot.wavelets, vp, None, alpha, problem=problem)
+    with pytest.raises(ValueError):
+        pde.adjoint(np.ones((shot.num_points_receivers, time.num), np.float32), shot.wavelets, vp, None, alpha,
+                    problem=problem)
+
+
+def test_empty_inputs_warn_and_return_zeros(gpu):
+    cfg, space, time, vp, shot, problem = _setup()
+    pde = IsoAcousticHIP(space=space, time=time)
+    empty = Traces('w', data=np.zeros_like(shot.wavelets.data))
+    traces = pde.forward(empty, vp, problem=problem)
+    assert np.all(traces.data == 0)
+    pde.forward(shot.wavelets, vp, problem=problem)
+    g = pde.adjoint(np.zeros((shot.num_points_receivers, time.num), np.float32), shot.wavelets, vp, problem=problem)
+    assert np.all(g[1].data == 0)
+
+
+def test_nan_gradient_returns_none(gpu):
+    cfg, space, time, vp, shot, problem = _setup()
+    pde = IsoAcousticHIP(space=space, time=time)
+    pde.forward(shot.wavelets, vp, problem=problem)
+    res = np.full((shot.num_points_receivers, time.num), np.nan, np.float32)
+    assert pde.adjoint(res, shot.wavelets, vp, problem=problem) is None
