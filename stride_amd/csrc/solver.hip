@@ -303,6 +303,12 @@ __global__ void k_unpack(Geo g, Box box, const float *__restrict__ snap, float *
     dst[p] = sa_inbox(box, i, j, k) ? snap[sa_boxidx(box, i, j, k)] : 0.f;
 }
 
+// padded field -> snapshot (box layout)
+__global__ void k_pack(Geo g, Box box, const float *__restrict__ src, float *__restrict__ snap) {
+    DOMAIN_INDEX
+    if (sa_inbox(box, i, j, k)) snap[sa_boxidx(box, i, j, k)] = src[p];
+}
+
 // compat vp gradient on the interior (prepare_grad_vp, devito.py:985-1024)
 __global__ void k_grad_vp(Geo g, Box box, const float *__restrict__ ps, const float *__restrict__ psm,
                           const float *__restrict__ psp, float fdt2, const float *__restrict__ pa, float *grad,
@@ -429,6 +435,13 @@ struct sa_solver {
     Box box;
     int saved_mode = 0;
     int saved_f = 0;
+    // checkpointed exact adjoint (save_mode 4): pairs (p[jK+1], p[jK]) in `saved`, a K+1 level
+    // segment buffer, forward work buffers and the source samples kept for the recomputation
+    float *segment = nullptr;
+    size_t segment_bytes = 0;
+    float *fb[2] = {};
+    float *src_keep = nullptr;
+    size_t src_keep_n = 0;
     // host staging
     float *d_series = nullptr;
     size_t series_n = 0;
@@ -593,6 +606,10 @@ extern "C" int sa_destroy(sa_solver *s) {
                    s->d_series, s->d_out};
     for (float *p : fs) dfree(p);
     for (float *p : s->gacc) dfree(p);
+    dfree(s->segment);
+    dfree(s->fb[0]);
+    dfree(s->fb[1]);
+    dfree(s->src_keep);
     for (int a = 0; a < 3; ++a) {
         dfree(s->u1[a]); dfree(s->u2[a]); dfree(s->u3[a]); dfree(s->st[a]); dfree(s->st2[a]);
         for (int r = 0; r < 4; ++r) dfree(s->pml[a][r]);
@@ -621,7 +638,7 @@ extern "C" int sa_set_scheme(sa_solver *s, const sa_scheme *sch) {
     if (sch->undersampling < 1) return fail(SA_E_ARG, "undersampling must be >= 1");
     if (sch->tb0 < 0 || sch->tb1 > s->grid.nt || sch->tb1 <= sch->tb0 + 2)
         return fail(SA_E_ARG, "time bounds (%d, %d) invalid for nt %d", sch->tb0, sch->tb1, s->grid.nt);
-    if (sch->save_mode < 0 || sch->save_mode > 3) return fail(SA_E_ARG, "save_mode %d", sch->save_mode);
+    if (sch->save_mode < 0 || sch->save_mode > 4) return fail(SA_E_ARG, "save_mode %d", sch->save_mode);
     if (!s->have_scheme || memcmp(&s->sch, sch, sizeof(*sch)) != 0) s->coef_dirty = true;
     s->sch = *sch;
     s->have_scheme = true;
@@ -955,7 +972,7 @@ static int alloc_saved(sa_solver *s) {
     const sa_scheme &c = s->sch;
     const Geo &g = s->geo;
     int f = c.save_mode == 3 ? 1 : c.undersampling;
-    // vp-only gradient on the fast path needs the interior only; everything else keeps the domain
+    // vp-only gradient needs the interior only; everything else keeps the domain
     Box b;
     for (int a = 0; a < 3; ++a) {
         b.lo[a] = 0;
@@ -968,6 +985,7 @@ static int alloc_saved(sa_solver *s) {
         }
     }
     int nsave = c.tb1 / f + 1;
+    if (c.save_mode == 4) nsave = 2 * ((c.tb1 - 2) / f + 1);  // checkpoint pairs
     size_t per = (size_t)b.n[0] * b.n[1] * b.n[2];
     size_t bytes = per * nsave * sizeof(float);
     if (s->saved && s->saved_bytes >= bytes) {
@@ -987,6 +1005,20 @@ static int alloc_saved(sa_solver *s) {
     s->nsave = nsave;
     s->saved_mode = c.save_mode;
     s->saved_f = f;
+    if (c.save_mode == 4) {
+        size_t sb = per * (size_t)(f + 1) * sizeof(float);
+        if (!s->segment || s->segment_bytes < sb) {
+            dfree(s->segment);
+            s->segment = nullptr;
+            if (hipMalloc((void **)&s->segment, sb) != hipSuccess) {
+                s->segment = nullptr;
+                return fail(SA_E_NOMEM, "cannot allocate %.2f GB for the recomputation segment", sb / 1e9);
+            }
+            s->segment_bytes = sb;
+        }
+        TRY(dalloc(&s->fb[0], s->total));
+        TRY(dalloc(&s->fb[1], s->total));
+    }
     return SA_OK;
 }
 
@@ -1013,13 +1045,19 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
     const size_t per = box_points(s->box);
     const float idt2 = 1.f / (s->dt * s->dt);
     const bool fast = use_fast(s);
+    const bool ckpt = c.save_mode == 4;
+    if (ckpt) {
+        size_t ns = (size_t)s->grid.nt * s->src.npoint;
+        TRY(ensure_staging(s, &s->src_keep, &s->src_keep_n, ns));
+        if (ns) HIP_TRY(hipMemcpyAsync(s->src_keep, src, ns * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+    }
     // levels: buf[t % 2] = p[t]; step t overwrites p[t-1] with p[t+1]
     int tlast = c.tb1 - 2;
     for (int t = 1; t <= tlast; ++t) {
         float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
         float *save = nullptr;
         int mode = 0;
-        if (c.save_mode != 0 && t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave) {
+        if (c.save_mode != 0 && !ckpt && t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave) {
             save = s->saved + (size_t)(t / f) * per;
             mode = c.save_mode == 1 ? 1 : 2;
         }
@@ -1033,8 +1071,40 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
         }
         TRY(inject(s, s->src, src + (size_t)t * s->src.npoint, 1.f, o, mode == 1 ? save : nullptr, idt2));
         TRY(gather(s, s->rec, o, 1.f, rec + (size_t)t * s->rec.npoint));
+        if (ckpt && t % f == 0) {
+            // checkpoint j = t / K holds (p[jK + 1], p[jK])
+            float *slot = s->saved + (size_t)(2 * (t / f)) * per;
+            k_pack<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, s->box, o, slot);
+            k_pack<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, s->box, u, slot + per);
+            CHECK_LAUNCH();
+        }
     }
     s->cur = (tlast + 1) % 2;
+    return SA_OK;
+}
+
+// Recompute forward levels u_{jK} .. u_{min((j+1)K, nt-1)} into the segment buffer from checkpoint j.
+static int recompute_segment(sa_solver *s, int j) {
+    const int K = s->saved_f;
+    const int nt = s->grid.nt;
+    const size_t per = box_points(s->box);
+    const Geo &g = s->geo;
+    const float *pair = s->saved + (size_t)(2 * j) * per;  // (p[jK+1], p[jK])
+    int t0 = j * K;
+    int tend = std::min((j + 1) * K, nt - 1);
+    HIP_TRY(hipMemcpyAsync(s->segment, pair + per, per * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+    if (tend >= t0 + 1)
+        HIP_TRY(hipMemcpyAsync(s->segment + per, pair, per * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+    k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, pair, s->fb[(t0 + 1) % 2]);
+    k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, pair + per, s->fb[t0 % 2]);
+    CHECK_LAUNCH();
+    for (int t = t0 + 1; t < tend; ++t) {
+        float *u = s->fb[t % 2], *o = s->fb[(t + 1) % 2];
+        TRY(step_general(s, u, o, o, 0, nullptr, 0));
+        TRY(inject(s, s->src, s->src_keep + (size_t)t * s->src.npoint, 1.f, o, nullptr, 0.f));
+        k_pack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, o, s->segment + (size_t)(t + 1 - t0) * per);
+        CHECK_LAUNCH();
+    }
     return SA_OK;
 }
 
@@ -1181,8 +1251,9 @@ extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float
     if (c.boundary == 1 || c.att_power == 2 || (c.kernel == 4 && c.rho_mode == 2))
         return fail(SA_E_UNSUPPORTED, "exact adjoint supports sponge/no boundary, attenuation none/0/None, "
                                       "and OT4 only without variable density");
-    if (!s->saved || s->saved_mode != 3)
-        return fail(SA_E_STATE, "the exact adjoint needs a forward run with save_mode 3");
+    if (!s->saved || (s->saved_mode != 3 && s->saved_mode != 4))
+        return fail(SA_E_STATE, "the exact adjoint needs a forward run with save_mode 3 or 4");
+    const bool ckpt = s->saved_mode == 4;
     const int nt = s->grid.nt;
     size_t nr = (size_t)nt * s->rec.npoint;
     size_t nq = (size_t)s->src.npoint * s->src.ntap;
@@ -1208,6 +1279,7 @@ extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float
     const size_t per = box_points(s->box);
     const float c4 = s->dt * s->dt / 12.f;
     int rc = SA_OK;
+    int seg = -1;
     // buf[lv % 2] = mu_lv
     for (int lv = nt - 1; lv >= 1 && rc == SA_OK; --lv) {
         float *m1 = s->buf[(lv + 1) % 2], *m2 = s->buf[lv % 2];  // mu_{lv+1}, mu_{lv+2}
@@ -1218,12 +1290,29 @@ extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float
         }
         int t = lv;
         if (rc != SA_OK || t < 1 || t > nt - 2) continue;
+        const float *ut = nullptr, *utm = nullptr;
+        if (ckpt) {
+            const int K = s->saved_f;
+            const int j = (t - 1) / K;
+            if (j != seg) {
+                // the recomputation uses the forward coefficients of the source taps
+                rc = coef_sparse(s, s->src, 0);
+                if (rc == SA_OK) rc = recompute_segment(s, j);
+                if (rc != SA_OK) break;
+                seg = j;
+            }
+            ut = s->segment + (size_t)(t - j * K) * per;
+            utm = s->segment + (size_t)(t - 1 - j * K) * per;
+        } else {
+            ut = s->saved + (size_t)t * per;
+            utm = s->saved + (size_t)(t - 1) * per;
+        }
         if (nq) {
             k_gather_taps<<<((int)nq + 127) / 128, 128, 0, s->stream>>>(g, (int)nq, s->src.d_idx, m1, s->G,
                                                                          s->d_out + (size_t)t * nq);
         }
-        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, s->saved + (size_t)t * per, s->work0);
-        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, s->saved + (size_t)(t - 1) * per, s->work1);
+        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, ut, s->work0);
+        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, utm, s->work1);
         if ((rc = (hipGetLastError() == hipSuccess ? SA_OK : fail(SA_E_HIP, "launch failed"))) != SA_OK) break;
         if ((rc = apply_K(s, s->work0, nullptr, KuT)) != SA_OK) break;
         if (c.kernel == 4) {

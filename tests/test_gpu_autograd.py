@@ -1,0 +1,75 @@
+"""Exact adjoint behind torch.autograd.Function: checkpointed recomputation equals
+storing every level, and autograd gradients match central finite differences of
+the loss computed through the same forward."""
+
+import numpy as np
+import pytest
+
+from helpers import small_case, solver_for, rel_l2
+from stride_amd.physics.plan import adjoint_source_samples
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize('dim,K', [(2, 7), (2, 16), (3, 5)])
+def test_checkpointed_equals_stored(gpu, dim, K):
+    n, nabs, nt = (48, 10, 200) if dim == 2 else (26, 6, 90)
+    plan, _ = small_case(dim=dim, n=n, nabs=nabs, nt=nt, kernel='OT2', grad_vp=True)
+    print('checkpointed exact adjoint dim=%d K=%d' % (dim, K), flush=True)
+    rng = np.random.default_rng(8)
+    res = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, plan.nt)))
+    s3 = solver_for(plan, save_mode=3)
+    s3.forward(plan.src_data)
+    a3 = s3.adjoint_exact(res)
+    s4 = solver_for(plan, save_mode=4)
+    s4.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source, K, 4,
+                  plan.tb0, plan.tb1, 1, 0, 0)
+    rec4 = s4.forward(plan.src_data)
+    a4 = s4.adjoint_exact(res)
+    for key in ('gA', 'gB', 'gG'):
+        assert rel_l2(a4[key], a3[key]) < 1e-6, key
+    assert rel_l2(a4['lam_src'], a3['lam_src']) < 1e-6
+    del rec4
+
+
+@pytest.mark.parametrize('checkpoint', [None, 9])
+def test_autograd_matches_finite_differences(gpu, checkpoint):
+    import torch
+    from stride_amd.autograd import Geometry, iso_acoustic_traces
+    plan, info = small_case(dim=2, n=48, nabs=10, nt=220, kernel='OT2', rho='var', alpha='p0')
+    print('autograd checkpoint=%s' % checkpoint, flush=True)
+    kw = dict(info['kw'])
+    geom = Geometry(info['space'], info['time'], kw.pop('source_coordinates'), kw.pop('receiver_coordinates'),
+                    checkpoint_every=checkpoint, **kw)
+    vp = torch.tensor(info['vp'], dtype=torch.float32, requires_grad=True)
+    rho = torch.tensor(info['rho'], dtype=torch.float32, requires_grad=True)
+    alpha = torch.tensor(info['alpha'], dtype=torch.float32, requires_grad=True)
+    wav = torch.tensor(info['wavelet'], dtype=torch.float32, requires_grad=True)
+    rng = np.random.default_rng(2)
+    obs = torch.tensor(rng.standard_normal((plan.nrec, plan.nt)).astype(np.float32))
+
+    def loss_of(vp_, rho_, alpha_, wav_):
+        tr = iso_acoustic_traces(vp_, rho_, alpha_, wav_, geom)
+        return 0.5 * torch.sum((tr.double() - obs.double()) ** 2)
+
+    loss = loss_of(vp, rho, alpha, wav)
+    loss.backward()
+    grads = dict(vp=vp.grad.numpy(), rho=rho.grad.numpy(), alpha=alpha.grad.numpy(), wav=wav.grad.numpy())
+    base = dict(vp=vp.detach(), rho=rho.detach(), alpha=alpha.detach(), wav=wav.detach())
+    eps = dict(vp=4.0, rho=4.0, alpha=0.05, wav=2e-2)
+    for name in ('vp', 'rho', 'alpha', 'wav'):
+        d = torch.zeros_like(base[name])
+        if name == 'wav':
+            d[0, 10:80] = torch.tensor(rng.standard_normal(70).astype(np.float32))
+        else:
+            d[14:54, 14:54] = torch.tensor(rng.standard_normal((40, 40)).astype(np.float32))
+        with torch.no_grad():
+            args_p = dict(base)
+            args_m = dict(base)
+            args_p[name] = base[name] + eps[name] * d
+            args_m[name] = base[name] - eps[name] * d
+            fd = (loss_of(args_p['vp'], args_p['rho'], args_p['alpha'], args_p['wav']) -
+                  loss_of(args_m['vp'], args_m['rho'], args_m['alpha'], args_m['wav'])).item() / (2 * eps[name])
+        an = float(np.sum(grads[name].astype(np.float64) * d.numpy()))
+        print('  %s: fd %.6e autograd %.6e' % (name, fd, an))
+        assert abs(fd - an) <= 1e-2 * abs(fd), name

@@ -36,7 +36,8 @@ def test_forward_adjoint_roundtrip(gpu):
     assert np.linalg.norm(traces.data - ref) / np.linalg.norm(ref) < 1e-5
     res = traces.data * 0.1
     out = pde.adjoint(Traces('res', data=res), shot.wavelets, vp, problem=problem)
-    assert isinstance(out, tuple) and len(out) == 2 and out[0] is None
+    # after_adjoint returns get_grad(wavelets, vp, rho, alpha) (devito.py:981)
+    assert isinstance(out, tuple) and len(out) == 4 and out[0] is None and out[2] is None and out[3] is None
     grad = out[1]
     assert grad.data.shape == space.shape and grad.prec.data.shape == space.shape
     acc = oracle.adjoint_compat(plan, adjoint_source_samples(plan, res), saved)
