@@ -1,0 +1,55 @@
+"""Tile autotuner for the specialised 3D update.
+
+For a grid and scheme, times short forward runs of the update kernel (HIP events
+around every launch) for each candidate tile — rows per lane in {2, 4, 8}
+(tile 64 x 4*rows cells) and march chunk lengths that split the slowest axis
+into 2..16 pieces — and returns the fastest. Results are cached per
+(shape, absorbing, boundary) for the life of the process.
+"""
+
+import numpy as np
+
+from ._native import Solver
+
+__all__ = ['candidates', 'autotune']
+
+_CACHE = {}
+
+
+def candidates(shape):
+    nx = shape[0]
+    chunks = sorted({max(1, -(-nx // k)) for k in (2, 3, 4, 6, 8, 12, 16)})
+    return [(rows, chunk) for rows in (2, 4, 8) for chunk in chunks]
+
+
+def autotune(plan, steps=24, device=0, verbose=False, cands=None):
+    """(best (rows, chunk), [(milliseconds per launch, rows, chunk), ...]) for ``plan``'s grid."""
+    key = (plan.shape, plan.absorbing, plan.boundary)
+    if key in _CACHE:
+        return _CACHE[key]
+    nt = steps + 2
+    s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, nt, device=device)
+    s.set_scheme(2, 0, -2, plan.boundary, 0, 4, 0, 0, nt)
+    s.set_medium(plan.vp)
+    if plan.sponge_tables is not None:
+        s.set_sponge(plan.sponge_tables)
+    s.set_sources(plan.src_idx, plan.src_w)
+    s.set_receivers(plan.rec_idx[:1], plan.rec_w[:1])
+    src = np.zeros((nt, plan.nsrc), np.float32)
+    src[1:, :] = 1.0
+    results = []
+    for rows, chunk in (cands or candidates(plan.shape)):
+        s.set_tile(rows, chunk)
+        s.forward(src)  # warm
+        s.timing(True)
+        s.forward(src)
+        ms, n = s.timing()
+        s.timing(False)
+        results.append((ms / max(n, 1), rows, chunk))
+        if verbose:
+            print('tile rows=%d chunk=%d: %.1f us/launch' % (rows, chunk, 1e3 * ms / max(n, 1)), flush=True)
+    s.close()
+    results.sort()
+    best = (results[0][1], results[0][2])
+    _CACHE[key] = (best, results)
+    return best, results
