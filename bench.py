@@ -115,7 +115,8 @@ def main():
     ap.add_argument('--scale', type=float, default=1.0, help='grid scale (1.0 = 320^3 inner)')
     ap.add_argument('--nt', type=int, default=2500)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--tile', type=str, default=None, help='rows,chunk of the 3D update tile')
+    ap.add_argument('--tile', type=str, default=None, help='rows,chunk,version of the 3D update tile')
+    ap.add_argument('--no-autotune', action='store_true')
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
@@ -126,10 +127,19 @@ def main():
     cfg, plan = workload(scale=args.scale, nt=args.nt)
     print('[bench] workload %s ext, nt %d, f %d, nrec %d' % (plan.shape, plan.nt, plan.undersampling, plan.nrec),
           file=sys.stderr, flush=True)
-    s = make_solver(plan)
+    tile = None
     if args.tile:
-        r, c = (int(v) for v in args.tile.split(','))
-        s.set_tile(r, c)
+        tile = tuple(int(v) for v in args.tile.split(','))
+    elif not args.no_autotune:
+        from stride_amd.autotune import autotune
+        t0 = time.perf_counter()
+        tile, table = autotune(plan)
+        print('[bench] autotune %.1f s: best rows=%d chunk=%d version=%d (%.1f us/launch); worst %.1f us'
+              % ((time.perf_counter() - t0,) + tuple(tile) + (table[0][0] * 1e3, table[-1][0] * 1e3)),
+              file=sys.stderr, flush=True)
+    s = make_solver(plan)
+    if tile:
+        s.set_tile(*tile)
     # device-resident inputs
     nbytes_src = plan.src_data.nbytes
     rng = np.random.default_rng(0)
@@ -200,6 +210,7 @@ def main():
             'frac': round(achieved / HBM_PEAK_GBS, 4),
             'traffic': traffic_from_profile(),
             'kernel': 'k_fast3d (wavefield update)',
+            'tile': list(tile) if tile else None,
             'avg_launch_us': round(avg_launch_s * 1e6, 2),
             'launches': launches,
             'algorithmic_bytes_per_launch': int(bytes_per_launch),

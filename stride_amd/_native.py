@@ -206,8 +206,8 @@ class Solver:
     def free_saved(self):
         self._check(self._lib.sa_free_saved(self._h))
 
-    def set_tile(self, rows, chunk):
-        self._check(self._lib.sa_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk)))
+    def set_tile(self, rows, chunk, version=2):
+        self._check(self._lib.sa_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk), ctypes.c_int(version)))
 
     def timing(self, enable=None):
         if enable is not None:

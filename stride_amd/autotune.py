@@ -1,9 +1,9 @@
 """Tile autotuner for the specialised 3D update.
 
 For a grid and scheme, times short forward runs of the update kernel (HIP events
-around every launch) for each candidate tile — rows per lane in {2, 4, 8}
-(tile 64 x 4*rows cells) and march chunk lengths that split the slowest axis
-into 2..16 pieces — and returns the fastest. Results are cached per
+around every launch) for each candidate — rows per lane in {2, 4, 8} (tile
+64 x 4*rows cells), march chunk lengths that split the slowest axis into 2..16
+pieces, kernel version 1 or 2 — and returns the fastest. Results are cached per
 (shape, absorbing, boundary) for the life of the process.
 """
 
@@ -18,12 +18,13 @@ _CACHE = {}
 
 def candidates(shape):
     nx = shape[0]
-    chunks = sorted({max(1, -(-nx // k)) for k in (2, 3, 4, 6, 8, 12, 16)})
-    return [(rows, chunk) for rows in (2, 4, 8) for chunk in chunks]
+    chunks = sorted({max(1, -(-nx // k)) for k in (3, 4, 5, 6, 7, 8, 10, 12, 16)})
+    return [(rows, chunk, version) for version in (1, 2) for rows in (2, 4, 8) for chunk in chunks
+            if not (version == 2 and rows == 8)]
 
 
 def autotune(plan, steps=24, device=0, verbose=False, cands=None):
-    """(best (rows, chunk), [(milliseconds per launch, rows, chunk), ...]) for ``plan``'s grid."""
+    """(best (rows, chunk, version), [(milliseconds per launch, rows, chunk, version), ...]) for ``plan``'s grid."""
     key = (plan.shape, plan.absorbing, plan.boundary)
     if key in _CACHE:
         return _CACHE[key]
@@ -38,18 +39,19 @@ def autotune(plan, steps=24, device=0, verbose=False, cands=None):
     src = np.zeros((nt, plan.nsrc), np.float32)
     src[1:, :] = 1.0
     results = []
-    for rows, chunk in (cands or candidates(plan.shape)):
-        s.set_tile(rows, chunk)
+    for rows, chunk, version in (cands or candidates(plan.shape)):
+        s.set_tile(rows, chunk, version)
         s.forward(src)  # warm
         s.timing(True)
         s.forward(src)
         ms, n = s.timing()
         s.timing(False)
-        results.append((ms / max(n, 1), rows, chunk))
+        results.append((ms / max(n, 1), rows, chunk, version))
         if verbose:
-            print('tile rows=%d chunk=%d: %.1f us/launch' % (rows, chunk, 1e3 * ms / max(n, 1)), flush=True)
+            print('tile rows=%d chunk=%d version=%d: %.1f us/launch' % (rows, chunk, version, 1e3 * ms / max(n, 1)),
+                  flush=True)
     s.close()
     results.sort()
-    best = (results[0][1], results[0][2])
+    best = results[0][1:]
     _CACHE[key] = (best, results)
     return best, results

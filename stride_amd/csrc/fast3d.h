@@ -12,6 +12,7 @@
 struct FastTune {
     int rows;   // rows (axis 1) per thread; the tile is 64 (axis 2) x 4*rows (axis 1)
     int chunk;  // planes (axis 0) marched per workgroup
+    int version;  // 1: shifted register queue, two barriers; 2: rotated queue, prefetch, one barrier
 };
 
 struct Fast3D {
@@ -21,7 +22,7 @@ struct Fast3D {
     int boundary = -1;
     const float *vp = nullptr;
     float dt = 0.f;
-    FastTune tune = {4, 45};
+    FastTune tune = {4, 45, 2};
     bool prepared = false;
 };
 
@@ -36,4 +37,4 @@ int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o,
                         const Box &box, float *grad, float *prec, hipStream_t stream);
 void fast3d_free(Fast3D &f);
 // tile shape used by later launches (autotuner)
-int fast3d_set_tune(Fast3D &f, int rows, int chunk);
+int fast3d_set_tune(Fast3D &f, int rows, int chunk, int version);

@@ -186,8 +186,217 @@ __global__ __launch_bounds__(256) void k_fast3d(Geo g, FastArgs a) {
     }
 }
 
+
+// Second-generation march: the register queue rotates statically (the plane loop is unrolled
+// by 11, so no per-plane register shifts), the LDS plane is double-buffered (one barrier per
+// plane), and the next plane's operands are loaded before the current plane is computed.
+template <int RPT, int MODE, bool SPONGE>
+struct March2 {
+    static constexpr int TY = kWaves * RPT;
+    static constexpr int LY = TY + 10, LZ = kTZ + 10;
+};
+
+template <int RPT, int MODE, bool SPONGE, int J>
+__device__ __forceinline__ void march2_plane(const Geo &g, const FastArgs &a, float (*s)[March2<RPT, MODE, SPONGE>::LY]
+                                             [March2<RPT, MODE, SPONGE>::LZ],
+                                             int &cur, float (&q)[RPT][11], float (&ov)[RPT], float (&vv)[RPT],
+                                             const int x, const int x1, const int (&yr)[RPT], const bool (&rin)[RPT],
+                                             const int y0, const int z0, const int z, const int lane, const int w,
+                                             const bool tile_yz_interior) {
+    constexpr int TY = kWaves * RPT;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const long long s0 = g.s0, s1 = g.s1;
+    const bool zload = z < nz + SA_RADIUS;
+    const bool more = x + 1 < x1;
+    // 1. issue the loads of plane x + 1: front (x + 6), other level, vp, halo cells
+    float qn[RPT], ovn[RPT], vvn[RPT];
+    float hyv[3] = {0.f, 0.f, 0.f};
+    float hzv[(10 * TY + 255) / 256];
+    const long long pn = g.base + (long long)(x + 1) * s0;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const long long p = pn + yr[r] * s1 + z;
+        qn[r] = (more && yr[r] < ny + SA_RADIUS && zload && x + 6 < nx + SA_RADIUS) ? a.u[p + SA_RADIUS * s0] : 0.f;
+        ovn[r] = (more && rin[r]) ? a.o[p] : 0.f;
+        vvn[r] = (more && rin[r]) ? a.vp[p] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int hy = w + kWaves * k;
+        if (hy < 10) {
+            const int yy = hy < 5 ? y0 - 5 + hy : y0 + TY + hy - 5;
+            hyv[k] = (more && yy < ny + SA_RADIUS && zload) ? a.u[pn + yy * s1 + z] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < (10 * TY + 255) / 256; ++k) {
+        const int i = threadIdx.x + 256 * k;
+        hzv[k] = 0.f;
+        if (i < 10 * TY) {
+            const int row = i / 10, c = i % 10;
+            const int zz = c < 5 ? z0 - 5 + c : z0 + kTZ + c - 5;
+            const int yy = y0 + row;
+            hzv[k] = (more && yy < ny + SA_RADIUS && zz < nz + SA_RADIUS) ? a.u[pn + yy * s1 + zz] : 0.f;
+        }
+    }
+    // 2. compute plane x from LDS buffer `cur` and the queue (plane x - 5 + k at (J + k) % 11)
+    const auto sb = s[cur];
+    const bool plane_interior = tile_yz_interior && x >= g.nabs[0] && x < nx - g.nabs[0];
+    const long long px = g.base + (long long)x * s0;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        if (!rin[r]) continue;
+        const int ly = w + kWaves * r + 5, lz = lane + 5;
+        const float c = q[r][(J + 5) % 11];
+        float vx = g.w2[0][0] * c, vy = g.w2[1][0] * c, vz = g.w2[2][0] * c;
+#pragma unroll
+        for (int k = 1; k <= 5; ++k) {
+            vx += g.w2[0][k] * (q[r][(J + 5 + k) % 11] + q[r][(J + 5 - k) % 11]);
+            vy += g.w2[1][k] * (sb[ly + k][lz] + sb[ly - k][lz]);
+            vz += g.w2[2][k] * (sb[ly][lz + k] + sb[ly][lz - k]);
+        }
+        const float lap = vx + vy + vz;
+        const float v = vv[r];
+        const float v2 = v * v;
+        float A = 2.f, B = -1.f, G = a.dt2 * v2;
+        if (SPONGE && !plane_interior) {
+            const int M = a.tlen;
+            const int d0 = min(min(x, nx - 1 - x), M);
+            const int d1 = min(min(yr[r], ny - 1 - yr[r]), M);
+            const int d2 = min(min(z, nz - 1 - z), M);
+            const float sm = a.tables[(d0 * (M + 1) + d1) * (M + 1) + d2];
+            if (sm != 0.f) {
+                const float D = (7.f * (sm * v)) * a.dt;
+                const float aa = a.dt * v2 * D;
+                const float e = a.dt * a.dt * v2 * D * D;
+                const float ia = 1.f / (1.f + aa);
+                A = (2.f - e) * ia;
+                B = -(1.f - aa) * ia;
+                G = (a.dt2 * ia) * v2;
+            }
+        }
+        const float val = A * c + B * ov[r] + G * lap;
+        const long long p = px + yr[r] * s1 + z;
+        a.out[p] = val;
+        if (MODE == 1) {
+            if (sa_inbox(a.box, x, yr[r], z)) a.save[sa_boxidx(a.box, x, yr[r], z)] = (val - 2.f * c + ov[r]) * a.idt2;
+        } else if (MODE == 2) {
+            if (sa_inbox(a.box, x, yr[r], z)) {
+                const float psv = a.ps[sa_boxidx(a.box, x, yr[r], z)];
+                a.grad[p] += psv * c;
+                a.prec[p] += psv * psv;
+            }
+        }
+    }
+    // 3. stage plane x + 1 into the other buffer; rotate the queue (plane x + 6 replaces x - 5)
+    if (more) {
+        const auto sn = s[cur ^ 1];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) sn[w + kWaves * r + 5][lane + 5] = q[r][(J + 6) % 11];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int hy = w + kWaves * k;
+            if (hy < 10) sn[hy < 5 ? hy : TY + hy][lane + 5] = hyv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < (10 * TY + 255) / 256; ++k) {
+            const int i = threadIdx.x + 256 * k;
+            if (i < 10 * TY) {
+                const int row = i / 10, c = i % 10;
+                sn[row + 5][c < 5 ? c : kTZ + c] = hzv[k];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        q[r][J % 11] = qn[r];
+        ov[r] = ovn[r];
+        vv[r] = vvn[r];
+    }
+    __syncthreads();
+    cur ^= 1;
+}
+
+template <int RPT, int MODE, bool SPONGE>
+__global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
+    using M2 = March2<RPT, MODE, SPONGE>;
+    constexpr int TY = M2::TY;
+    __shared__ float s[2][M2::LY][M2::LZ];
+    const int b = blockIdx.x;
+    const int L = (b & 7) * a.per_xcd + (b >> 3);
+    if (L >= a.nblocks) return;
+    const int tz = L % a.ntile_z;
+    const int rest = L / a.ntile_z;
+    const int ty = rest % a.ntile_y;
+    const int ch = rest / a.ntile_y;
+    const int z0 = tz * kTZ, y0 = ty * TY;
+    const int x0 = ch * a.chunk;
+    const int x1 = min(x0 + a.chunk, g.n[0]);
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int z = z0 + lane;
+    const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+    const bool zload = z < nz + SA_RADIUS;
+    const long long s0 = g.s0, s1 = g.s1;
+    int yr[RPT];
+    bool rin[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        yr[r] = y0 + w + kWaves * r;
+        rin[r] = yr[r] < ny && z < nz;
+    }
+    const bool tile_yz_interior = SPONGE && y0 >= g.nabs[1] && y0 + TY <= ny - g.nabs[1] && z0 >= g.nabs[2] &&
+                                  z0 + kTZ <= nz - g.nabs[2];
+    // prologue: queue planes x0 - 5 .. x0 + 5, operands and LDS plane of x0
+    float q[RPT][11], ov[RPT], vv[RPT];
+#pragma unroll
+    for (int k = 0; k < 11; ++k) {
+        const int xx = x0 - SA_RADIUS + k;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const long long p = g.base + xx * s0 + yr[r] * s1 + z;
+            q[r][k] = (yr[r] < ny + SA_RADIUS && zload && xx < nx + SA_RADIUS) ? a.u[p] : 0.f;
+        }
+    }
+    const long long p0 = g.base + (long long)x0 * s0;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const long long p = p0 + yr[r] * s1 + z;
+        ov[r] = rin[r] ? a.o[p] : 0.f;
+        vv[r] = rin[r] ? a.vp[p] : 0.f;
+        s[0][w + kWaves * r + 5][lane + 5] = q[r][5];
+    }
+    for (int hy = w; hy < 10; hy += kWaves) {
+        const int yy = hy < 5 ? y0 - 5 + hy : y0 + TY + hy - 5;
+        s[0][hy < 5 ? hy : TY + hy][lane + 5] = (yy < ny + SA_RADIUS && zload) ? a.u[p0 + yy * s1 + z] : 0.f;
+    }
+    for (int i = threadIdx.x; i < 10 * TY; i += 256) {
+        const int row = i / 10, c = i % 10;
+        const int zz = c < 5 ? z0 - 5 + c : z0 + kTZ + c - 5;
+        const int yy = y0 + row;
+        s[0][row + 5][c < 5 ? c : kTZ + c] = (yy < ny + SA_RADIUS && zz < nz + SA_RADIUS) ? a.u[p0 + yy * s1 + zz]
+                                                                                          : 0.f;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int xb = x0; xb < x1; xb += 11) {
+#define SA_PLANE(J) \
+    if (xb + J < x1) march2_plane<RPT, MODE, SPONGE, J>(g, a, s, cur, q, ov, vv, xb + J, x1, yr, rin, y0, z0, z, lane, w, tile_yz_interior);
+        SA_PLANE(0) SA_PLANE(1) SA_PLANE(2) SA_PLANE(3) SA_PLANE(4) SA_PLANE(5)
+        SA_PLANE(6) SA_PLANE(7) SA_PLANE(8) SA_PLANE(9) SA_PLANE(10)
+#undef SA_PLANE
+    }
+}
+
 template <int RPT, int MODE>
-void launch_rpt(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
+void launch_rpt(int version, bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
+    if (version == 2) {
+        if (sponge)
+            k_fast3d_v2<RPT, MODE, true><<<grid, 256, 0, st>>>(g, a);
+        else
+            k_fast3d_v2<RPT, MODE, false><<<grid, 256, 0, st>>>(g, a);
+        return;
+    }
     if (sponge)
         k_fast3d<RPT, MODE, true><<<grid, 256, 0, st>>>(g, a);
     else
@@ -207,9 +416,9 @@ int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
     dim3 grid(a.per_xcd * 8);
     const bool sponge = f.boundary == 0;
     switch (rows) {
-        case 2: launch_rpt<2, MODE>(sponge, grid, st, g, a); break;
-        case 4: launch_rpt<4, MODE>(sponge, grid, st, g, a); break;
-        case 8: launch_rpt<8, MODE>(sponge, grid, st, g, a); break;
+        case 2: launch_rpt<2, MODE>(f.tune.version, sponge, grid, st, g, a); break;
+        case 4: launch_rpt<4, MODE>(f.tune.version, sponge, grid, st, g, a); break;
+        case 8: launch_rpt<8, MODE>(f.tune.version, sponge, grid, st, g, a); break;
         default: return SA_E_ARG;
     }
     return hipGetLastError() == hipSuccess ? 0 : SA_E_HIP;
@@ -235,10 +444,11 @@ bool fast3d_enabled() {
     return !(e && e[0] == '1');
 }
 
-int fast3d_set_tune(Fast3D &f, int rows, int chunk) {
-    if (!(rows == 2 || rows == 4 || rows == 8) || chunk < 1) return SA_E_ARG;
+int fast3d_set_tune(Fast3D &f, int rows, int chunk, int version) {
+    if (!(rows == 2 || rows == 4 || rows == 8) || chunk < 1 || !(version == 1 || version == 2)) return SA_E_ARG;
     f.tune.rows = rows;
     f.tune.chunk = chunk;
+    f.tune.version = version;
     return 0;
 }
 
@@ -285,10 +495,11 @@ int fast3d_prepare(Fast3D &f, const Geo &g, const float *vp, const float *tables
         // default tile: 64 x 16 cells, chunk sized so that ~8 chunks cover axis 0
         f.tune.rows = 4;
         f.tune.chunk = (g.n[0] + 7) / 8;
+        f.tune.version = 2;
         const char *e = getenv("STRIDE_AMD_TILE");
         if (e) {
-            int r = 0, c = 0;
-            if (sscanf(e, "%d,%d", &r, &c) == 2) fast3d_set_tune(f, r, c);
+            int r = 0, c = 0, v = 2;
+            if (sscanf(e, "%d,%d,%d", &r, &c, &v) >= 2) fast3d_set_tune(f, r, c, v);
         }
         f.prepared = true;
     }

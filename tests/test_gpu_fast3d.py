@@ -28,7 +28,9 @@ def _run(plan, res_s, general, tile=None):
 
 
 @pytest.mark.parametrize('n,nabs,tile', [
-    (30, 6, (4, 7)), (37, 5, (2, 13)), (41, 7, (8, 41)), (37, 5, (4, 1)), (70, 8, (4, 30)), (66, 6, (8, 9)),
+    (30, 6, (4, 7, 1)), (37, 5, (2, 13, 1)), (41, 7, (8, 41, 1)), (37, 5, (4, 1, 1)), (70, 8, (4, 30, 1)),
+    (66, 6, (8, 9, 1)), (30, 6, (4, 7, 2)), (37, 5, (2, 13, 2)), (41, 7, (4, 41, 2)), (37, 5, (4, 1, 2)),
+    (70, 8, (2, 30, 2)), (66, 6, (4, 12, 2)),
 ])
 def test_fast_matches_general_and_oracle(gpu, n, nabs, tile):
     plan, _ = small_case(dim=3, n=n, nabs=nabs, nt=90, kernel='OT2', grad_vp=True, nrec=20)
@@ -59,7 +61,7 @@ def test_fast_unequal_layers(gpu):
     plan = build_plan(space, time, tone_burst(0.5e6, 3, 90, 0.08e-6)[None], vp, source_coordinates=geom[:1],
                       receiver_coordinates=geom, kernel='OT2', grad_vp=True)
     res_s = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, plan.nt)))
-    rec_f, g_f = _run(plan, res_s, general=False, tile=(2, 11))
+    rec_f, g_f = _run(plan, res_s, general=False, tile=(2, 11, 2))
     ref, saved, _ = oracle.forward(plan, save=True)
     assert rel_l2(rec_f.T, ref) < 1e-5
     gref = oracle.adjoint_compat(plan, res_s, saved)
