@@ -187,73 +187,91 @@ __global__ __launch_bounds__(256) void k_fast3d(Geo g, FastArgs a) {
 }
 
 
-// Second-generation march: the register queue rotates statically (the plane loop is unrolled
-// by 11, so no per-plane register shifts), the LDS plane is double-buffered (one barrier per
-// plane), and the next plane's operands are loaded before the current plane is computed.
-template <int RPT, int MODE, bool SPONGE>
-struct March2 {
+// Second-generation march (version 2): ~40 VALU instructions per cell and plane instead of ~100.
+//  - u, the other level, vp, out and the gradient arrays are read / written with raw buffer
+//    instructions: one scalar resource per array, a constant per-lane byte offset computed
+//    once, and the plane offset in a scalar register, so no address arithmetic runs per load;
+//  - out-of-range lanes are clamped onto zero halo cells instead of predicated;
+//  - the register queue rotates statically (the plane loop is unrolled by 11);
+//  - plane x + 1 is loaded before plane x is computed, and the LDS plane is double-buffered,
+//    so there is one barrier per plane.
+// Requires the 3D padded layout (leads 10, 10, 32 cells) and arrays below 2 GiB.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void *p, long long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float bload(rsrc_t r, unsigned voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ void bstore(float v, rsrc_t r, unsigned voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
+template <int RPT>
+struct Lanes {
     static constexpr int TY = kWaves * RPT;
     static constexpr int LY = TY + 10, LZ = kTZ + 10;
+    static constexpr int NHZ = (10 * TY + 255) / 256;  // z-halo cells per lane (upper bound)
+    unsigned row[RPT];   // byte offset of each own cell within a padded plane (clamped)
+    unsigned hy[3];      // y-halo cells
+    unsigned hz[NHZ];    // z-halo cells
+    int hy_lds[3];       // LDS row of each y-halo cell (-1: none)
+    int hz_lds[NHZ];     // flattened LDS index of each z-halo cell (-1: none)
+    unsigned brow[RPT];  // byte offset of each own cell within a box plane (MODE 1/2)
+    bool bin[RPT];       // own cell inside the box along axes 1/2
+};
+
+struct Rsrc {
+    rsrc_t u, o, vp, out, save, ps, grad, prec;
+    int plane_bytes, box_plane_bytes;
 };
 
 template <int RPT, int MODE, bool SPONGE, int J>
-__device__ __forceinline__ void march2_plane(const Geo &g, const FastArgs &a, float (*s)[March2<RPT, MODE, SPONGE>::LY]
-                                             [March2<RPT, MODE, SPONGE>::LZ],
+__device__ __forceinline__ void march2_plane(const Geo &g, const FastArgs &a, const Rsrc &rs, float *__restrict__ lds,
                                              int &cur, float (&q)[RPT][11], float (&ov)[RPT], float (&vv)[RPT],
-                                             const int x, const int x1, const int (&yr)[RPT], const bool (&rin)[RPT],
-                                             const int y0, const int z0, const int z, const int lane, const int w,
+                                             const int x, const int x1, const Lanes<RPT> &ln, const int (&yr)[RPT],
+                                             const bool (&rin)[RPT], const int z, const int lane, const int w,
                                              const bool tile_yz_interior) {
-    constexpr int TY = kWaves * RPT;
+    using L = Lanes<RPT>;
+    constexpr int LZ = L::LZ;
+    constexpr int PLANE = L::LY * L::LZ;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const long long s0 = g.s0, s1 = g.s1;
-    const bool zload = z < nz + SA_RADIUS;
     const bool more = x + 1 < x1;
-    // 1. issue the loads of plane x + 1: front (x + 6), other level, vp, halo cells
-    float qn[RPT], ovn[RPT], vvn[RPT];
-    float hyv[3] = {0.f, 0.f, 0.f};
-    float hzv[(10 * TY + 255) / 256];
-    const long long pn = g.base + (long long)(x + 1) * s0;
+    // 1. loads of plane x + 1 (front plane x + 6 clamped to the last halo plane)
+    float qn[RPT], ovn[RPT], vvn[RPT], hyv[3], hzv[L::NHZ];
+    {
+        const int sf = (min(x + 6, nx + SA_RADIUS - 1) + SA_HALO) * rs.plane_bytes;
+        const int sn = (min(x + 1, nx - 1) + SA_HALO) * rs.plane_bytes;
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-        const long long p = pn + yr[r] * s1 + z;
-        qn[r] = (more && yr[r] < ny + SA_RADIUS && zload && x + 6 < nx + SA_RADIUS) ? a.u[p + SA_RADIUS * s0] : 0.f;
-        ovn[r] = (more && rin[r]) ? a.o[p] : 0.f;
-        vvn[r] = (more && rin[r]) ? a.vp[p] : 0.f;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int hy = w + kWaves * k;
-        if (hy < 10) {
-            const int yy = hy < 5 ? y0 - 5 + hy : y0 + TY + hy - 5;
-            hyv[k] = (more && yy < ny + SA_RADIUS && zload) ? a.u[pn + yy * s1 + z] : 0.f;
+        for (int r = 0; r < RPT; ++r) {
+            qn[r] = bload(rs.u, ln.row[r], sf);
+            ovn[r] = bload(rs.o, ln.row[r], sn);
+            vvn[r] = bload(rs.vp, ln.row[r], sn);
         }
-    }
 #pragma unroll
-    for (int k = 0; k < (10 * TY + 255) / 256; ++k) {
-        const int i = threadIdx.x + 256 * k;
-        hzv[k] = 0.f;
-        if (i < 10 * TY) {
-            const int row = i / 10, c = i % 10;
-            const int zz = c < 5 ? z0 - 5 + c : z0 + kTZ + c - 5;
-            const int yy = y0 + row;
-            hzv[k] = (more && yy < ny + SA_RADIUS && zz < nz + SA_RADIUS) ? a.u[pn + yy * s1 + zz] : 0.f;
-        }
+        for (int k = 0; k < 3; ++k) hyv[k] = bload(rs.u, ln.hy[k], sn);
+#pragma unroll
+        for (int k = 0; k < L::NHZ; ++k) hzv[k] = bload(rs.u, ln.hz[k], sn);
     }
-    // 2. compute plane x from LDS buffer `cur` and the queue (plane x - 5 + k at (J + k) % 11)
-    const auto sb = s[cur];
+    // 2. compute plane x (LDS buffer `cur`; queue slot (J + k) % 11 holds plane x - 5 + k)
+    const float *__restrict__ sb = lds + cur * PLANE;
     const bool plane_interior = tile_yz_interior && x >= g.nabs[0] && x < nx - g.nabs[0];
-    const long long px = g.base + (long long)x * s0;
+    const int sx = (x + SA_HALO) * rs.plane_bytes;
+    const bool xin_box = MODE != 0 && x >= a.box.lo[0] && x < a.box.lo[0] + a.box.n[0];
+    const int sbx = MODE != 0 ? (x - a.box.lo[0]) * rs.box_plane_bytes : 0;
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        if (!rin[r]) continue;
-        const int ly = w + kWaves * r + 5, lz = lane + 5;
+        const int li = (w + kWaves * r + 5) * LZ + lane + 5;
         const float c = q[r][(J + 5) % 11];
         float vx = g.w2[0][0] * c, vy = g.w2[1][0] * c, vz = g.w2[2][0] * c;
 #pragma unroll
         for (int k = 1; k <= 5; ++k) {
             vx += g.w2[0][k] * (q[r][(J + 5 + k) % 11] + q[r][(J + 5 - k) % 11]);
-            vy += g.w2[1][k] * (sb[ly + k][lz] + sb[ly - k][lz]);
-            vz += g.w2[2][k] * (sb[ly][lz + k] + sb[ly][lz - k]);
+            vy += g.w2[1][k] * (sb[li + k * LZ] + sb[li - k * LZ]);
+            vz += g.w2[2][k] * (sb[li + k] + sb[li - k]);
         }
         const float lap = vx + vy + vz;
         const float v = vv[r];
@@ -276,36 +294,30 @@ __device__ __forceinline__ void march2_plane(const Geo &g, const FastArgs &a, fl
             }
         }
         const float val = A * c + B * ov[r] + G * lap;
-        const long long p = px + yr[r] * s1 + z;
-        a.out[p] = val;
-        if (MODE == 1) {
-            if (sa_inbox(a.box, x, yr[r], z)) a.save[sa_boxidx(a.box, x, yr[r], z)] = (val - 2.f * c + ov[r]) * a.idt2;
-        } else if (MODE == 2) {
-            if (sa_inbox(a.box, x, yr[r], z)) {
-                const float psv = a.ps[sa_boxidx(a.box, x, yr[r], z)];
-                a.grad[p] += psv * c;
-                a.prec[p] += psv * psv;
+        if (rin[r]) {
+            bstore(val, rs.out, ln.row[r], sx);
+            if (MODE != 0 && xin_box && ln.bin[r]) {
+                if (MODE == 1) {
+                    bstore((val - 2.f * c + ov[r]) * a.idt2, rs.save, ln.brow[r], sbx);
+                } else {
+                    const float psv = bload(rs.ps, ln.brow[r], sbx);
+                    bstore(bload(rs.grad, ln.row[r], sx) + psv * c, rs.grad, ln.row[r], sx);
+                    bstore(bload(rs.prec, ln.row[r], sx) + psv * psv, rs.prec, ln.row[r], sx);
+                }
             }
         }
     }
-    // 3. stage plane x + 1 into the other buffer; rotate the queue (plane x + 6 replaces x - 5)
+    // 3. stage plane x + 1 in the other buffer, rotate the queue (plane x + 6 replaces x - 5)
     if (more) {
-        const auto sn = s[cur ^ 1];
+        float *__restrict__ sn = lds + (cur ^ 1) * PLANE;
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) sn[w + kWaves * r + 5][lane + 5] = q[r][(J + 6) % 11];
+        for (int r = 0; r < RPT; ++r) sn[(w + kWaves * r + 5) * LZ + lane + 5] = q[r][(J + 6) % 11];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int hy = w + kWaves * k;
-            if (hy < 10) sn[hy < 5 ? hy : TY + hy][lane + 5] = hyv[k];
-        }
+        for (int k = 0; k < 3; ++k)
+            if (ln.hy_lds[k] >= 0) sn[ln.hy_lds[k] * LZ + lane + 5] = hyv[k];
 #pragma unroll
-        for (int k = 0; k < (10 * TY + 255) / 256; ++k) {
-            const int i = threadIdx.x + 256 * k;
-            if (i < 10 * TY) {
-                const int row = i / 10, c = i % 10;
-                sn[row + 5][c < 5 ? c : kTZ + c] = hzv[k];
-            }
-        }
+        for (int k = 0; k < L::NHZ; ++k)
+            if (ln.hz_lds[k] >= 0) sn[ln.hz_lds[k]] = hzv[k];
     }
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
@@ -319,14 +331,15 @@ __device__ __forceinline__ void march2_plane(const Geo &g, const FastArgs &a, fl
 
 template <int RPT, int MODE, bool SPONGE>
 __global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
-    using M2 = March2<RPT, MODE, SPONGE>;
-    constexpr int TY = M2::TY;
-    __shared__ float s[2][M2::LY][M2::LZ];
+    using L = Lanes<RPT>;
+    constexpr int TY = L::TY;
+    constexpr int LZ = L::LZ;
+    __shared__ float lds[2 * L::LY * L::LZ];
     const int b = blockIdx.x;
-    const int L = (b & 7) * a.per_xcd + (b >> 3);
-    if (L >= a.nblocks) return;
-    const int tz = L % a.ntile_z;
-    const int rest = L / a.ntile_z;
+    const int Lg = (b & 7) * a.per_xcd + (b >> 3);
+    if (Lg >= a.nblocks) return;
+    const int tz = Lg % a.ntile_z;
+    const int rest = Lg / a.ntile_z;
     const int ty = rest % a.ntile_y;
     const int ch = rest / a.ntile_y;
     const int z0 = tz * kTZ, y0 = ty * TY;
@@ -336,14 +349,60 @@ __global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
     const int w = threadIdx.x >> 6;
     const int z = z0 + lane;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-    const bool zload = z < nz + SA_RADIUS;
-    const long long s0 = g.s0, s1 = g.s1;
+    const unsigned s1 = (unsigned)g.s1;
+    Rsrc rs;
+    const long long bytes = g.total * 4;
+    rs.u = make_rsrc(a.u, bytes);
+    rs.o = make_rsrc(a.o, bytes);
+    rs.vp = make_rsrc(a.vp, bytes);
+    rs.out = make_rsrc(a.out, bytes);
+    rs.plane_bytes = (int)(g.s0 * 4);
+    rs.box_plane_bytes = a.box.n[1] * a.box.n[2] * 4;
+    const long long box_bytes = (long long)a.box.n[0] * rs.box_plane_bytes;
+    if (MODE == 1) rs.save = make_rsrc(a.save, box_bytes);
+    if (MODE == 2) {
+        rs.ps = make_rsrc(a.ps, box_bytes);
+        rs.grad = make_rsrc(a.grad, bytes);
+        rs.prec = make_rsrc(a.prec, bytes);
+    }
+    // lane offsets (padded coordinates; clamped onto zero halo cells when out of range)
+    const int zc = min(z, nz + SA_RADIUS - 1) + SA_LEAD;
+    L ln;
     int yr[RPT];
     bool rin[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         yr[r] = y0 + w + kWaves * r;
         rin[r] = yr[r] < ny && z < nz;
+        ln.row[r] = ((unsigned)(min(yr[r], ny + SA_RADIUS - 1) + SA_HALO) * s1 + (unsigned)zc) * 4u;
+        ln.bin[r] = yr[r] >= a.box.lo[1] && yr[r] < a.box.lo[1] + a.box.n[1] && z >= a.box.lo[2] &&
+                    z < a.box.lo[2] + a.box.n[2];
+        ln.brow[r] = ln.bin[r] ? (unsigned)((yr[r] - a.box.lo[1]) * a.box.n[2] + (z - a.box.lo[2])) * 4u : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int hy = w + kWaves * k;
+        ln.hy_lds[k] = -1;
+        ln.hy[k] = 0u;
+        if (hy < 10) {
+            const int yy = hy < 5 ? y0 - 5 + hy : y0 + TY + hy - 5;
+            ln.hy_lds[k] = hy < 5 ? hy : TY + hy;
+            ln.hy[k] = ((unsigned)(min(yy, ny + SA_RADIUS - 1) + SA_HALO) * s1 + (unsigned)zc) * 4u;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < L::NHZ; ++k) {
+        const int i = threadIdx.x + 256 * k;
+        ln.hz_lds[k] = -1;
+        ln.hz[k] = 0u;
+        if (i < 10 * TY) {
+            const int row = i / 10, c = i % 10;
+            const int zz = c < 5 ? z0 - 5 + c : z0 + kTZ + c - 5;
+            const int yy = y0 + row;
+            ln.hz_lds[k] = (row + 5) * LZ + (c < 5 ? c : kTZ + c);
+            ln.hz[k] = ((unsigned)(min(yy, ny + SA_RADIUS - 1) + SA_HALO) * s1 +
+                        (unsigned)(min(zz, nz + SA_RADIUS - 1) + SA_LEAD)) * 4u;
+        }
     }
     const bool tile_yz_interior = SPONGE && y0 >= g.nabs[1] && y0 + TY <= ny - g.nabs[1] && z0 >= g.nabs[2] &&
                                   z0 + kTZ <= nz - g.nabs[2];
@@ -351,37 +410,32 @@ __global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
     float q[RPT][11], ov[RPT], vv[RPT];
 #pragma unroll
     for (int k = 0; k < 11; ++k) {
-        const int xx = x0 - SA_RADIUS + k;
+        const int sk = (min(x0 - SA_RADIUS + k, nx + SA_RADIUS - 1) + SA_HALO) * rs.plane_bytes;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) q[r][k] = bload(rs.u, ln.row[r], sk);
+    }
+    {
+        const int s0b = (x0 + SA_HALO) * rs.plane_bytes;
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
-            const long long p = g.base + xx * s0 + yr[r] * s1 + z;
-            q[r][k] = (yr[r] < ny + SA_RADIUS && zload && xx < nx + SA_RADIUS) ? a.u[p] : 0.f;
+            ov[r] = bload(rs.o, ln.row[r], s0b);
+            vv[r] = bload(rs.vp, ln.row[r], s0b);
+            lds[(w + kWaves * r + 5) * LZ + lane + 5] = q[r][5];
         }
-    }
-    const long long p0 = g.base + (long long)x0 * s0;
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-        const long long p = p0 + yr[r] * s1 + z;
-        ov[r] = rin[r] ? a.o[p] : 0.f;
-        vv[r] = rin[r] ? a.vp[p] : 0.f;
-        s[0][w + kWaves * r + 5][lane + 5] = q[r][5];
-    }
-    for (int hy = w; hy < 10; hy += kWaves) {
-        const int yy = hy < 5 ? y0 - 5 + hy : y0 + TY + hy - 5;
-        s[0][hy < 5 ? hy : TY + hy][lane + 5] = (yy < ny + SA_RADIUS && zload) ? a.u[p0 + yy * s1 + z] : 0.f;
-    }
-    for (int i = threadIdx.x; i < 10 * TY; i += 256) {
-        const int row = i / 10, c = i % 10;
-        const int zz = c < 5 ? z0 - 5 + c : z0 + kTZ + c - 5;
-        const int yy = y0 + row;
-        s[0][row + 5][c < 5 ? c : kTZ + c] = (yy < ny + SA_RADIUS && zz < nz + SA_RADIUS) ? a.u[p0 + yy * s1 + zz]
-                                                                                          : 0.f;
+        for (int k = 0; k < 3; ++k)
+            if (ln.hy_lds[k] >= 0) lds[ln.hy_lds[k] * LZ + lane + 5] = bload(rs.u, ln.hy[k], s0b);
+#pragma unroll
+        for (int k = 0; k < L::NHZ; ++k)
+            if (ln.hz_lds[k] >= 0) lds[ln.hz_lds[k]] = bload(rs.u, ln.hz[k], s0b);
     }
     __syncthreads();
     int cur = 0;
     for (int xb = x0; xb < x1; xb += 11) {
-#define SA_PLANE(J) \
-    if (xb + J < x1) march2_plane<RPT, MODE, SPONGE, J>(g, a, s, cur, q, ov, vv, xb + J, x1, yr, rin, y0, z0, z, lane, w, tile_yz_interior);
+#define SA_PLANE(J)                                                                                           \
+    if (xb + J < x1)                                                                                          \
+        march2_plane<RPT, MODE, SPONGE, J>(g, a, rs, lds, cur, q, ov, vv, xb + J, x1, ln, yr, rin, z, lane, w, \
+                                           tile_yz_interior);
         SA_PLANE(0) SA_PLANE(1) SA_PLANE(2) SA_PLANE(3) SA_PLANE(4) SA_PLANE(5)
         SA_PLANE(6) SA_PLANE(7) SA_PLANE(8) SA_PLANE(9) SA_PLANE(10)
 #undef SA_PLANE
@@ -415,10 +469,11 @@ int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
     a.per_xcd = (a.nblocks + 7) / 8;
     dim3 grid(a.per_xcd * 8);
     const bool sponge = f.boundary == 0;
+    const int version = g.total * 4 < (1ll << 31) ? f.tune.version : 1;  // buffer offsets are 32-bit
     switch (rows) {
-        case 2: launch_rpt<2, MODE>(f.tune.version, sponge, grid, st, g, a); break;
-        case 4: launch_rpt<4, MODE>(f.tune.version, sponge, grid, st, g, a); break;
-        case 8: launch_rpt<8, MODE>(f.tune.version, sponge, grid, st, g, a); break;
+        case 2: launch_rpt<2, MODE>(version, sponge, grid, st, g, a); break;
+        case 4: launch_rpt<4, MODE>(version, sponge, grid, st, g, a); break;
+        case 8: launch_rpt<8, MODE>(version, sponge, grid, st, g, a); break;
         default: return SA_E_ARG;
     }
     return hipGetLastError() == hipSuccess ? 0 : SA_E_HIP;
