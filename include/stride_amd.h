@@ -133,8 +133,9 @@ int sa_free_saved(sa_solver *s);
 
 /* Tile of the specialised 3D update (3D, OT2, no density, sponge or no boundary, no
  * attenuation): 64 x 4*rows cells per workgroup (rows in {2, 4, 8}), marching `chunk`
- * planes of the slowest axis; version 1 (shifted register queue) or 2 (rotated queue,
- * prefetched next plane, double-buffered LDS). Used by the autotuner (stride_amd.autotune). */
+ * planes of the slowest axis; version 1 (shifted register queue), 2 (buffer loads, rotated
+ * queue, next plane prefetched, double-buffered LDS) or 3 (as 2, prefetching two planes
+ * ahead). Used by the autotuner (stride_amd.autotune). */
 int sa_set_tile(sa_solver *s, int rows, int chunk, int version);
 
 /* Kernel timing: when enabled, every launch of the wavefield-update kernel is bracketed by

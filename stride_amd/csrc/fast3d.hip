@@ -229,33 +229,46 @@ struct Rsrc {
     int plane_bytes, box_plane_bytes;
 };
 
-template <int RPT, int MODE, bool SPONGE, int J>
+// operands of one plane, loaded ahead of use: front cell of the queue, other level, vp, halo cells
+template <int RPT>
+struct Stage {
+    float q[RPT], o[RPT], v[RPT], hy[3], hz[Lanes<RPT>::NHZ];
+};
+
+template <int RPT>
+__device__ __forceinline__ void load_stage(Stage<RPT> &st, const Rsrc &rs, const Lanes<RPT> &ln, int xf, int xp,
+                                           int nx) {
+    const int sf = (min(xf, nx + SA_RADIUS - 1) + SA_HALO) * rs.plane_bytes;
+    const int sp = (min(xp, nx - 1) + SA_HALO) * rs.plane_bytes;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        st.q[r] = bload(rs.u, ln.row[r], sf);
+        st.o[r] = bload(rs.o, ln.row[r], sp);
+        st.v[r] = bload(rs.vp, ln.row[r], sp);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) st.hy[k] = bload(rs.u, ln.hy[k], sp);
+#pragma unroll
+    for (int k = 0; k < Lanes<RPT>::NHZ; ++k) st.hz[k] = bload(rs.u, ln.hz[k], sp);
+}
+
+// One plane of the march. PD = prefetch distance in planes: with PD = 1 the operands of plane
+// x + 1 are loaded at the top of plane x; with PD = 2 those of plane x + 2 are (and `s1` holds
+// plane x + 1's, loaded one plane earlier).
+template <int RPT, int MODE, bool SPONGE, int PD, int J>
 __device__ __forceinline__ void march2_plane(const Geo &g, const FastArgs &a, const Rsrc &rs, float *__restrict__ lds,
                                              int &cur, float (&q)[RPT][11], float (&ov)[RPT], float (&vv)[RPT],
-                                             const int x, const int x1, const Lanes<RPT> &ln, const int (&yr)[RPT],
-                                             const bool (&rin)[RPT], const int z, const int lane, const int w,
-                                             const bool tile_yz_interior) {
+                                             Stage<RPT> &s1, const int x, const int x1, const Lanes<RPT> &ln,
+                                             const int (&yr)[RPT], const bool (&rin)[RPT], const int z,
+                                             const int lane, const int w, const bool tile_yz_interior) {
     using L = Lanes<RPT>;
     constexpr int LZ = L::LZ;
     constexpr int PLANE = L::LY * L::LZ;
     const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
     const bool more = x + 1 < x1;
-    // 1. loads of plane x + 1 (front plane x + 6 clamped to the last halo plane)
-    float qn[RPT], ovn[RPT], vvn[RPT], hyv[3], hzv[L::NHZ];
-    {
-        const int sf = (min(x + 6, nx + SA_RADIUS - 1) + SA_HALO) * rs.plane_bytes;
-        const int sn = (min(x + 1, nx - 1) + SA_HALO) * rs.plane_bytes;
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            qn[r] = bload(rs.u, ln.row[r], sf);
-            ovn[r] = bload(rs.o, ln.row[r], sn);
-            vvn[r] = bload(rs.vp, ln.row[r], sn);
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) hyv[k] = bload(rs.u, ln.hy[k], sn);
-#pragma unroll
-        for (int k = 0; k < L::NHZ; ++k) hzv[k] = bload(rs.u, ln.hz[k], sn);
-    }
+    // 1. loads PD planes ahead
+    Stage<RPT> sn;
+    load_stage<RPT>(sn, rs, ln, x + 5 + PD, x + PD, nx);
     // 2. compute plane x (LDS buffer `cur`; queue slot (J + k) % 11 holds plane x - 5 + k)
     const float *__restrict__ sb = lds + cur * PLANE;
     const bool plane_interior = tile_yz_interior && x >= g.nabs[0] && x < nx - g.nabs[0];
@@ -308,28 +321,30 @@ __device__ __forceinline__ void march2_plane(const Geo &g, const FastArgs &a, co
         }
     }
     // 3. stage plane x + 1 in the other buffer, rotate the queue (plane x + 6 replaces x - 5)
+    const Stage<RPT> &nx1 = PD == 1 ? sn : s1;
     if (more) {
-        float *__restrict__ sn = lds + (cur ^ 1) * PLANE;
+        float *__restrict__ sw = lds + (cur ^ 1) * PLANE;
 #pragma unroll
-        for (int r = 0; r < RPT; ++r) sn[(w + kWaves * r + 5) * LZ + lane + 5] = q[r][(J + 6) % 11];
+        for (int r = 0; r < RPT; ++r) sw[(w + kWaves * r + 5) * LZ + lane + 5] = q[r][(J + 6) % 11];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-            if (ln.hy_lds[k] >= 0) sn[ln.hy_lds[k] * LZ + lane + 5] = hyv[k];
+            if (ln.hy_lds[k] >= 0) sw[ln.hy_lds[k] * LZ + lane + 5] = nx1.hy[k];
 #pragma unroll
         for (int k = 0; k < L::NHZ; ++k)
-            if (ln.hz_lds[k] >= 0) sn[ln.hz_lds[k]] = hzv[k];
+            if (ln.hz_lds[k] >= 0) sw[ln.hz_lds[k]] = nx1.hz[k];
     }
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        q[r][J % 11] = qn[r];
-        ov[r] = ovn[r];
-        vv[r] = vvn[r];
+        q[r][J % 11] = nx1.q[r];
+        ov[r] = nx1.o[r];
+        vv[r] = nx1.v[r];
     }
+    if (PD == 2) s1 = sn;
     __syncthreads();
     cur ^= 1;
 }
 
-template <int RPT, int MODE, bool SPONGE>
+template <int RPT, int MODE, bool SPONGE, int PD>
 __global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
     using L = Lanes<RPT>;
     constexpr int TY = L::TY;
@@ -406,7 +421,8 @@ __global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
     }
     const bool tile_yz_interior = SPONGE && y0 >= g.nabs[1] && y0 + TY <= ny - g.nabs[1] && z0 >= g.nabs[2] &&
                                   z0 + kTZ <= nz - g.nabs[2];
-    // prologue: queue planes x0 - 5 .. x0 + 5, operands and LDS plane of x0
+    // prologue: queue planes x0 - 5 .. x0 + 5, operands and LDS plane of x0 (and plane x0 + 1's
+    // operands for PD = 2)
     float q[RPT][11], ov[RPT], vv[RPT];
 #pragma unroll
     for (int k = 0; k < 11; ++k) {
@@ -414,28 +430,31 @@ __global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
 #pragma unroll
         for (int r = 0; r < RPT; ++r) q[r][k] = bload(rs.u, ln.row[r], sk);
     }
+    Stage<RPT> s1st;
     {
-        const int s0b = (x0 + SA_HALO) * rs.plane_bytes;
+        Stage<RPT> s0st;
+        load_stage<RPT>(s0st, rs, ln, x0 + 5, x0, nx);
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
-            ov[r] = bload(rs.o, ln.row[r], s0b);
-            vv[r] = bload(rs.vp, ln.row[r], s0b);
+            ov[r] = s0st.o[r];
+            vv[r] = s0st.v[r];
             lds[(w + kWaves * r + 5) * LZ + lane + 5] = q[r][5];
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-            if (ln.hy_lds[k] >= 0) lds[ln.hy_lds[k] * LZ + lane + 5] = bload(rs.u, ln.hy[k], s0b);
+            if (ln.hy_lds[k] >= 0) lds[ln.hy_lds[k] * LZ + lane + 5] = s0st.hy[k];
 #pragma unroll
         for (int k = 0; k < L::NHZ; ++k)
-            if (ln.hz_lds[k] >= 0) lds[ln.hz_lds[k]] = bload(rs.u, ln.hz[k], s0b);
+            if (ln.hz_lds[k] >= 0) lds[ln.hz_lds[k]] = s0st.hz[k];
+        if (PD == 2) load_stage<RPT>(s1st, rs, ln, x0 + 6, x0 + 1, nx);
     }
     __syncthreads();
     int cur = 0;
     for (int xb = x0; xb < x1; xb += 11) {
-#define SA_PLANE(J)                                                                                           \
-    if (xb + J < x1)                                                                                          \
-        march2_plane<RPT, MODE, SPONGE, J>(g, a, rs, lds, cur, q, ov, vv, xb + J, x1, ln, yr, rin, z, lane, w, \
-                                           tile_yz_interior);
+#define SA_PLANE(J)                                                                                            \
+    if (xb + J < x1)                                                                                           \
+        march2_plane<RPT, MODE, SPONGE, PD, J>(g, a, rs, lds, cur, q, ov, vv, s1st, xb + J, x1, ln, yr, rin, z, \
+                                               lane, w, tile_yz_interior);
         SA_PLANE(0) SA_PLANE(1) SA_PLANE(2) SA_PLANE(3) SA_PLANE(4) SA_PLANE(5)
         SA_PLANE(6) SA_PLANE(7) SA_PLANE(8) SA_PLANE(9) SA_PLANE(10)
 #undef SA_PLANE
@@ -444,11 +463,18 @@ __global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
 
 template <int RPT, int MODE>
 void launch_rpt(int version, bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
-    if (version == 2) {
-        if (sponge)
-            k_fast3d_v2<RPT, MODE, true><<<grid, 256, 0, st>>>(g, a);
-        else
-            k_fast3d_v2<RPT, MODE, false><<<grid, 256, 0, st>>>(g, a);
+    if (version == 2 || version == 3) {
+        if (version == 2) {
+            if (sponge)
+                k_fast3d_v2<RPT, MODE, true, 1><<<grid, 256, 0, st>>>(g, a);
+            else
+                k_fast3d_v2<RPT, MODE, false, 1><<<grid, 256, 0, st>>>(g, a);
+        } else {
+            if (sponge)
+                k_fast3d_v2<RPT, MODE, true, 2><<<grid, 256, 0, st>>>(g, a);
+            else
+                k_fast3d_v2<RPT, MODE, false, 2><<<grid, 256, 0, st>>>(g, a);
+        }
         return;
     }
     if (sponge)
@@ -500,7 +526,7 @@ bool fast3d_enabled() {
 }
 
 int fast3d_set_tune(Fast3D &f, int rows, int chunk, int version) {
-    if (!(rows == 2 || rows == 4 || rows == 8) || chunk < 1 || !(version == 1 || version == 2)) return SA_E_ARG;
+    if (!(rows == 2 || rows == 4 || rows == 8) || chunk < 1 || version < 1 || version > 3) return SA_E_ARG;
     f.tune.rows = rows;
     f.tune.chunk = chunk;
     f.tune.version = version;

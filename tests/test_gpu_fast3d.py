@@ -30,7 +30,8 @@ def _run(plan, res_s, general, tile=None):
 @pytest.mark.parametrize('n,nabs,tile', [
     (30, 6, (4, 7, 1)), (37, 5, (2, 13, 1)), (41, 7, (8, 41, 1)), (37, 5, (4, 1, 1)), (70, 8, (4, 30, 1)),
     (66, 6, (8, 9, 1)), (30, 6, (4, 7, 2)), (37, 5, (2, 13, 2)), (41, 7, (4, 41, 2)), (37, 5, (4, 1, 2)),
-    (70, 8, (2, 30, 2)), (66, 6, (4, 12, 2)),
+    (70, 8, (2, 30, 2)), (66, 6, (4, 12, 2)), (37, 5, (2, 13, 3)), (41, 7, (4, 41, 3)), (37, 5, (4, 1, 3)),
+    (66, 6, (2, 12, 3)),
 ])
 def test_fast_matches_general_and_oracle(gpu, n, nabs, tile):
     plan, _ = small_case(dim=3, n=n, nabs=nabs, nt=90, kernel='OT2', grad_vp=True, nrec=20)
