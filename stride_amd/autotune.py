@@ -3,7 +3,7 @@
 For a grid and scheme, times short forward runs of the update kernel (HIP events
 around every launch) for each candidate — rows per lane in {2, 4, 8} (tile
 64 x 4*rows cells), march chunk lengths that split the slowest axis into 2..16
-pieces, kernel version 1, 2 or 3 — and returns the fastest. Results are cached per
+pieces, kernel version 1-4 — and returns the fastest. Results are cached per
 (shape, absorbing, boundary) for the life of the process.
 """
 
@@ -19,8 +19,8 @@ _CACHE = {}
 def candidates(shape):
     nx = shape[0]
     chunks = sorted({max(1, -(-nx // k)) for k in (3, 4, 5, 6, 7, 8, 10, 12, 16)})
-    return [(rows, chunk, version) for version in (1, 2, 3) for rows in (2, 4, 8) for chunk in chunks
-            if not (version >= 2 and rows == 8)]
+    return [(rows, chunk, version) for version in (1, 2, 3, 4) for rows in (2, 4, 8) for chunk in chunks
+            if not (version >= 2 and rows == 8) and not (version == 4 and rows != 2)]
 
 
 def autotune(plan, steps=24, device=0, verbose=False, cands=None):

@@ -344,8 +344,8 @@ __device__ __forceinline__ void march2_plane(const Geo &g, const FastArgs &a, co
     cur ^= 1;
 }
 
-template <int RPT, int MODE, bool SPONGE, int PD>
-__global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
+template <int RPT, int MODE, bool SPONGE, int PD, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fast3d_v2(Geo g, FastArgs a) {
     using L = Lanes<RPT>;
     constexpr int TY = L::TY;
     constexpr int LZ = L::LZ;
@@ -463,17 +463,23 @@ __global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
 
 template <int RPT, int MODE>
 void launch_rpt(int version, bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
-    if (version == 2 || version == 3) {
+    if (version >= 2) {
         if (version == 2) {
             if (sponge)
-                k_fast3d_v2<RPT, MODE, true, 1><<<grid, 256, 0, st>>>(g, a);
+                k_fast3d_v2<RPT, MODE, true, 1, 1><<<grid, 256, 0, st>>>(g, a);
             else
-                k_fast3d_v2<RPT, MODE, false, 1><<<grid, 256, 0, st>>>(g, a);
-        } else {
+                k_fast3d_v2<RPT, MODE, false, 1, 1><<<grid, 256, 0, st>>>(g, a);
+        } else if (version == 3) {
             if (sponge)
-                k_fast3d_v2<RPT, MODE, true, 2><<<grid, 256, 0, st>>>(g, a);
+                k_fast3d_v2<RPT, MODE, true, 2, 1><<<grid, 256, 0, st>>>(g, a);
             else
-                k_fast3d_v2<RPT, MODE, false, 2><<<grid, 256, 0, st>>>(g, a);
+                k_fast3d_v2<RPT, MODE, false, 2, 1><<<grid, 256, 0, st>>>(g, a);
+        } else {
+            // version 4: as 3 with the register budget capped for 6 waves per SIMD
+            if (sponge)
+                k_fast3d_v2<RPT, MODE, true, 2, 6><<<grid, 256, 0, st>>>(g, a);
+            else
+                k_fast3d_v2<RPT, MODE, false, 2, 6><<<grid, 256, 0, st>>>(g, a);
         }
         return;
     }
@@ -526,7 +532,7 @@ bool fast3d_enabled() {
 }
 
 int fast3d_set_tune(Fast3D &f, int rows, int chunk, int version) {
-    if (!(rows == 2 || rows == 4 || rows == 8) || chunk < 1 || version < 1 || version > 3) return SA_E_ARG;
+    if (!(rows == 2 || rows == 4 || rows == 8) || chunk < 1 || version < 1 || version > 4) return SA_E_ARG;
     f.tune.rows = rows;
     f.tune.chunk = chunk;
     f.tune.version = version;
