@@ -132,10 +132,12 @@ int sa_get_wavefield(sa_solver *s, int which, float *out);
 int sa_free_saved(sa_solver *s);
 
 /* Tile of the specialised 3D update (3D, OT2, no density, sponge or no boundary, no
- * attenuation): 64 x 4*rows cells per workgroup (rows in {2, 4, 8}), marching `chunk`
- * planes of the slowest axis; version 1 (shifted register queue), 2 (buffer loads, rotated
- * queue, next plane prefetched, double-buffered LDS) or 3 (as 2, prefetching two planes
- * ahead). Used by the autotuner (stride_amd.autotune). */
+ * attenuation): 64 x waves*rows cells per workgroup, marching `chunk` planes of the slowest
+ * axis. version 1 (shifted register queue; rows 2, 4, 8), 2 (buffer loads, rotated queue,
+ * next plane prefetched, double-buffered LDS; rows 2, 4, 8), 3 (as 2, two planes ahead;
+ * rows 2, 4, 8), 5 (LDS ring of planes filled by direct-to-LDS loads two planes ahead,
+ * 4 waves; rows 2, 4) or 6 (as 5 with 8 waves; rows 1, 2). Used by the autotuner
+ * (stride_amd.autotune). */
 int sa_set_tile(sa_solver *s, int rows, int chunk, int version);
 
 /* Kernel timing: when enabled, every launch of the wavefield-update kernel is bracketed by

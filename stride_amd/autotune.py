@@ -1,9 +1,10 @@
 """Tile autotuner for the specialised 3D update.
 
 For a grid and scheme, times short forward runs of the update kernel (HIP events
-around every launch) for each candidate — rows per lane in {2, 4, 8} (tile
-64 x 4*rows cells), march chunk lengths that split the slowest axis into 2..16
-pieces, kernel version 1-4 — and returns the fastest. Results are cached per
+around every launch) for each candidate — kernel version (1-3: register-staged
+planes, 4 waves; 5/6: LDS plane ring filled by direct-to-LDS loads, 4/8 waves),
+rows per lane (tile 64 x waves*rows cells) and march chunk lengths that split the
+slowest axis into 3..16 pieces — and returns the fastest. Results are cached per
 (shape, absorbing, boundary) for the life of the process.
 """
 
@@ -16,11 +17,14 @@ __all__ = ['candidates', 'autotune']
 _CACHE = {}
 
 
-def candidates(shape):
+def candidates(shape, versions=(1, 2, 3, 5, 6)):
     nx = shape[0]
     chunks = sorted({max(1, -(-nx // k)) for k in (3, 4, 5, 6, 7, 8, 10, 12, 16)})
-    return [(rows, chunk, version) for version in (1, 2, 3, 4) for rows in (2, 4, 8) for chunk in chunks
-            if not (version >= 2 and rows == 8) and not (version == 4 and rows != 2)]
+    # versions 5/6 march whole 12-plane periods: chunk lengths that are multiples of 12
+    chunks12 = sorted({12 * max(1, -(-c // 12)) for c in chunks})
+    rows_of = {1: (2, 4, 8), 2: (2, 4), 3: (2, 4), 5: (2, 4), 6: (1, 2)}
+    return [(rows, chunk, version) for version in versions for rows in rows_of[version]
+            for chunk in (chunks12 if version >= 5 else chunks)]
 
 
 def autotune(plan, steps=24, device=0, verbose=False, cands=None):

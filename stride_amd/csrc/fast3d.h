@@ -12,13 +12,16 @@
 struct FastTune {
     int rows;   // rows (axis 1) per thread; the tile is 64 (axis 2) x 4*rows (axis 1)
     int chunk;  // planes (axis 0) marched per workgroup
-    int version;  // 1: shifted register queue, two barriers; 2: rotated queue, prefetch, one barrier
+    int version;  // 1: shifted register queue, two barriers; 2/3: rotated queue, register prefetch
+                  // 1 or 2 planes ahead; 5/6: LDS plane ring filled by direct-to-LDS loads, 4 or 8 waves
 };
 
 struct Fast3D {
     float *tables = nullptr;  // sponge corner table in use (device)
     float *corner = nullptr;  // owned corner table, (nabs0+1) x (nabs1+1) x (nabs2+1)
     int tlen = 0;
+    const float *axis = nullptr;  // per-axis layer tables [3][alen] (device, owned by the solver)
+    int alen = 0;
     int boundary = -1;
     const float *vp = nullptr;
     float dt = 0.f;

@@ -34,6 +34,8 @@ struct FastArgs {
     const float *vp;
     const float *tables;
     int tlen;
+    const float *axis;  // per-axis layer tables [3][alen] (version 5)
+    int alen;
     float dt, dt2, idt2;
     float *save;
     const float *ps;
@@ -344,8 +346,8 @@ __device__ __forceinline__ void march2_plane(const Geo &g, const FastArgs &a, co
     cur ^= 1;
 }
 
-template <int RPT, int MODE, bool SPONGE, int PD, int OCC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void k_fast3d_v2(Geo g, FastArgs a) {
+template <int RPT, int MODE, bool SPONGE, int PD>
+__global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
     using L = Lanes<RPT>;
     constexpr int TY = L::TY;
     constexpr int LZ = L::LZ;
@@ -461,38 +463,318 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void
     }
 }
 
+// Fifth-generation march (versions 5 and 6): the plane tile with its halo is staged in a ring
+// of three LDS slots by direct-to-LDS buffer loads (16 B per lane, no VGPR staging and no LDS
+// store instructions), issued two planes ahead; the own-column operands (u five planes ahead
+// for the register queue, the other level, vp and, when fused, the gradient operands) are
+// register loads issued two to three planes ahead. Every load and store is unconditional:
+// lanes outside the domain get a byte offset past the end of the buffer, which the buffer
+// range check turns into a zero read or a dropped write, so the plane loop has no divergent
+// branches. The sponge coefficient is evaluated from the per-axis layer tables (two per-lane
+// constants and one scalar per plane) instead of a table gather.
+//
+// Synchronisation per plane: each wave waits for its own direct-to-LDS loads into the slot of
+// the plane (a dummy read of the slot makes the compiler emit exactly that wait), then one
+// workgroup barrier; the slot written next is the one every wave finished reading before that
+// barrier.
+constexpr unsigned kOOB = 0x80000000u;  // byte offset past every buffer (< 2 GiB)
+
+template <int TY>
+struct Ring {
+    static constexpr int ROWS = TY + 10;     // tile rows and a 5-row halo on each side
+    static constexpr int CPR = 20;           // 16-byte chunks per row: axis-2 cells z0 - 8 .. z0 + 71
+    static constexpr int PITCH = 4 * CPR;    // floats per LDS row
+    static constexpr int NCH = ROWS * CPR;   // chunks per plane
+    static constexpr int NI = (NCH + 63) / 64;  // 64-lane load instructions per plane
+};
+
+// every wave issues the same number of direct-to-LDS loads per plane (the surplus ones read
+// past the end of the buffer and fill slot padding with zeros), so that all waves run one
+// straight-line instruction stream and the compiler's wait counts stay exact
+template <int TY, int NW>
+struct RingW : Ring<TY> {
+    static constexpr int KI = (Ring<TY>::NI + NW - 1) / NW;  // loads per wave per plane
+    static constexpr int FLOATS = KI * NW * 256;             // slot size in floats
+};
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ void dma16(rsrc_t r, float *lds, unsigned voff, int soff) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void *)lds, 16, voff, soff, 0, 0);
+}
+
+template <int RPT, int MODE>
+struct Stage5 {
+    float o[RPT], v[RPT];
+    float ps[MODE == 2 ? RPT : 1], gr[MODE == 2 ? RPT : 1], pr[MODE == 2 ? RPT : 1];
+};
+
+template <int RPT, int MODE>
+__device__ __forceinline__ void load_stage5(Stage5<RPT, MODE> &st, const Rsrc &rs, const unsigned (&row)[RPT],
+                                            const unsigned (&grow)[RPT], const unsigned (&brow)[RPT], int x,
+                                            const FastArgs &a) {
+    const int sp = (x + SA_HALO) * rs.plane_bytes;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        st.o[r] = bload(rs.o, row[r], sp);
+        st.v[r] = bload(rs.vp, row[r], sp);
+    }
+    if (MODE == 2) {
+        // box planes outside [lo, lo + n) read as zero (offset past the end)
+        const bool xin = x >= a.box.lo[0] && x < a.box.lo[0] + a.box.n[0];
+        const int sbx = xin ? (x - a.box.lo[0]) * rs.box_plane_bytes : 0;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            st.ps[r] = bload(rs.ps, xin ? brow[r] : kOOB, sbx);
+            st.gr[r] = bload(rs.grad, grow[r], sp);
+            st.pr[r] = bload(rs.prec, grow[r], sp);
+        }
+    }
+}
+
+template <int RPT, int NW, int MODE, bool SPONGE, int J>
+__device__ __forceinline__ void march5_plane(const Geo &g, const FastArgs &a, const Rsrc &rs, float *__restrict__ s0,
+                                             float *__restrict__ s1, float *__restrict__ s2, float (&q)[RPT][12],
+                                             Stage5<RPT, MODE> (&st)[3], const int x, const int x1, const int lane,
+                                             const int w, const unsigned (&row)[RPT], const unsigned (&orow)[RPT],
+                                             const unsigned (&grow)[RPT], const unsigned (&brow)[RPT],
+                                             const unsigned (&dv)[RingW<NW * RPT, NW>::KI], const int (&d1)[RPT],
+                                             const float (&t1v)[RPT], const int d2, const float t2v,
+                                             const bool tile_yz_interior) {
+    using R = RingW<NW * RPT, NW>;
+    constexpr int KI = R::KI;
+    float *__restrict__ cur = J % 3 == 0 ? s0 : (J % 3 == 1 ? s1 : s2);
+    float *__restrict__ nxt = J % 3 == 0 ? s2 : (J % 3 == 1 ? s0 : s1);  // slot of plane x + 2
+    const int nx = g.n[0];
+    // 1. this wave's loads into the slot of plane x have landed; then every wave's have. The
+    // probe read is pinned here (memory clobbers on both sides): hoisted, its wait would
+    // shorten the prefetch distance.
+    asm volatile("" ::: "memory");
+    {
+        const float probe = cur[lane];
+        asm volatile("" ::"v"(probe));
+    }
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    // 2. plane x + 2 into the slot every wave finished reading before the barrier (plane
+    // indices past the last halo plane are clamped: those planes are never stored)
+    const int xlast = nx + SA_HALO - 1;
+    {
+        const int soff = (min(x + 2, xlast) + SA_HALO) * rs.plane_bytes;
+#pragma unroll
+        for (int k = 0; k < KI; ++k) dma16(rs.u, nxt + (w + NW * k) * 256, dv[k], soff);
+    }
+    // 3. compute plane x; queue slot (J + 5 + k) % 12 holds plane x + k
+    Stage5<RPT, MODE> &sx = st[J % 3];
+    const int sxo = (x + SA_HALO) * rs.plane_bytes;
+    // planes past the chunk end (the last 12-plane period) are computed but not stored
+    const bool xin = x < x1;
+    const bool xin_box = MODE != 0 && xin && x >= a.box.lo[0] && x < a.box.lo[0] + a.box.n[0];
+    const int sbx = MODE == 1 && xin_box ? (x - a.box.lo[0]) * rs.box_plane_bytes : 0;
+    const bool plane_interior = tile_yz_interior && x >= g.nabs[0] && x < nx - g.nabs[0];
+    int d0 = 0;
+    float t0v = 0.f;
+    if (SPONGE && !plane_interior) {
+        d0 = max(min(x, nx - 1 - x), 0);
+        t0v = d0 < g.nabs[0] ? a.axis[d0] : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int li = (w + NW * r + 5) * R::PITCH + lane + 8;
+        const float c = q[r][(J + 5) % 12];
+        float vx = g.w2[0][0] * c, vy = g.w2[1][0] * c, vz = g.w2[2][0] * c;
+#pragma unroll
+        for (int k = 1; k <= 5; ++k) {
+            vx += g.w2[0][k] * (q[r][(J + 5 + k) % 12] + q[r][(J + 17 - k) % 12]);
+            vy += g.w2[1][k] * (cur[li + k * R::PITCH] + cur[li - k * R::PITCH]);
+            vz += g.w2[2][k] * (cur[li + k] + cur[li - k]);
+        }
+        const float lap = vx + vy + vz;
+        const float v = sx.v[r];
+        const float v2 = v * v;
+        float A = 2.f, B = -1.f, G = a.dt2 * v2;
+        if (SPONGE && !plane_interior) {
+            // layer rule: axis d contributes its table value at distance d_d from the edge
+            // when no other axis is closer to its own edge
+            float sm = 0.f;
+            if (d0 <= min(d1[r], d2)) sm += t0v;
+            if (d1[r] <= min(d0, d2)) sm += t1v[r];
+            if (d2 <= min(d0, d1[r])) sm += t2v;
+            if (sm != 0.f) {
+                const float D = (7.f * (sm * v)) * a.dt;
+                const float aa = a.dt * v2 * D;
+                const float e = a.dt * a.dt * v2 * D * D;
+                const float ia = 1.f / (1.f + aa);
+                A = (2.f - e) * ia;
+                B = -(1.f - aa) * ia;
+                G = (a.dt2 * ia) * v2;
+            }
+        }
+        const float val = A * c + B * sx.o[r] + G * lap;
+        bstore(val, rs.out, xin ? orow[r] : kOOB, sxo);
+        if (MODE == 1) bstore((val - 2.f * c + sx.o[r]) * a.idt2, rs.save, xin_box ? brow[r] : kOOB, sbx);
+        if (MODE == 2) {
+            const float psv = sx.ps[r];
+            const unsigned go = xin_box ? grow[r] : kOOB;
+            bstore(sx.gr[r] + psv * c, rs.grad, go, sxo);
+            bstore(sx.pr[r] + psv * psv, rs.prec, go, sxo);
+        }
+    }
+    // 4. refill: u of plane x + 7 into the queue slot of plane x - 5, operands of plane x + 3
+    {
+        const int sq = (min(x + 7, xlast) + SA_HALO) * rs.plane_bytes;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) q[r][J % 12] = bload(rs.u, row[r], sq);
+    }
+    load_stage5<RPT, MODE>(sx, rs, row, grow, brow, min(x + 3, xlast), a);
+}
+
+template <int RPT, int NW, int MODE, bool SPONGE>
+__global__ __launch_bounds__(NW * 64) void k_fast3d_v5(Geo g, FastArgs a) {
+    constexpr int TY = NW * RPT;
+    using R = RingW<TY, NW>;
+    constexpr int KI = R::KI;
+    __shared__ __attribute__((aligned(16))) float s0[R::FLOATS];
+    __shared__ __attribute__((aligned(16))) float s1[R::FLOATS];
+    __shared__ __attribute__((aligned(16))) float s2[R::FLOATS];
+    const int b = blockIdx.x;
+    const int Lg = (b & 7) * a.per_xcd + (b >> 3);
+    if (Lg >= a.nblocks) return;
+    const int tz = Lg % a.ntile_z;
+    const int rest = Lg / a.ntile_z;
+    const int ty = rest % a.ntile_y;
+    const int ch = rest / a.ntile_y;
+    const int z0 = tz * kTZ, y0 = ty * TY;
+    const int x0 = ch * a.chunk;
+    const int x1 = min(x0 + a.chunk, g.n[0]);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int z = z0 + lane;
+    const int ny = g.n[1], nz = g.n[2];
+    const unsigned ps1 = (unsigned)g.s1;
+    Rsrc rs;
+    const long long bytes = g.total * 4;
+    rs.u = make_rsrc(a.u, bytes);
+    rs.o = make_rsrc(a.o, bytes);
+    rs.vp = make_rsrc(a.vp, bytes);
+    rs.out = make_rsrc(a.out, bytes);
+    rs.plane_bytes = (int)(g.s0 * 4);
+    rs.box_plane_bytes = a.box.n[1] * a.box.n[2] * 4;
+    const long long box_bytes = (long long)a.box.n[0] * rs.box_plane_bytes;
+    if (MODE == 1) rs.save = make_rsrc(a.save, box_bytes);
+    if (MODE == 2) {
+        rs.ps = make_rsrc(a.ps, box_bytes);
+        rs.grad = make_rsrc(a.grad, bytes);
+        rs.prec = make_rsrc(a.prec, bytes);
+    }
+    // per-lane byte offsets within a plane: loads clamp onto halo cells, stores of lanes
+    // outside the domain (or the box) go past the end of the buffer
+    const unsigned zc = (unsigned)(min(z, nz + SA_RADIUS - 1) + SA_LEAD);
+    unsigned row[RPT], orow[RPT], grow[RPT], brow[RPT];
+    int d1[RPT];
+    float t1v[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int yr = y0 + w + NW * r;
+        const bool rin = yr < ny && z < nz;
+        row[r] = ((unsigned)(min(yr, ny + SA_RADIUS - 1) + SA_HALO) * ps1 + zc) * 4u;
+        orow[r] = rin ? row[r] : kOOB;
+        const bool bin = rin && yr >= a.box.lo[1] && yr < a.box.lo[1] + a.box.n[1] && z >= a.box.lo[2] &&
+                         z < a.box.lo[2] + a.box.n[2];
+        brow[r] = bin ? (unsigned)((yr - a.box.lo[1]) * a.box.n[2] + (z - a.box.lo[2])) * 4u : kOOB;
+        grow[r] = bin ? row[r] : kOOB;
+        d1[r] = min(yr, ny - 1 - yr);
+        t1v[r] = 0.f;
+        if (SPONGE && d1[r] >= 0 && d1[r] < g.nabs[1]) t1v[r] = a.axis[a.alen + d1[r]];
+    }
+    const int d2 = min(z, nz - 1 - z);
+    const float t2v = SPONGE && d2 >= 0 && d2 < g.nabs[2] ? a.axis[2 * a.alen + d2] : 0.f;
+    // direct-to-LDS chunk offsets: instruction w + NW k, lane -> chunk (w + NW k) * 64 + lane
+    unsigned dv[KI];
+#pragma unroll
+    for (int k = 0; k < KI; ++k) {
+        const int c = (w + NW * k) * 64 + lane;
+        dv[k] = kOOB;
+        if (c < R::NCH) {
+            const int rr = c / R::CPR, cc = c % R::CPR;
+            const int yy = min(y0 - 5 + rr, ny + SA_HALO - 1);
+            const int zz = min(z0 + SA_LEAD - 8 + 4 * cc, (int)ps1 - 4);
+            dv[k] = ((unsigned)(yy + SA_HALO) * ps1 + (unsigned)zz) * 4u;
+        }
+    }
+    const bool tile_yz_interior = SPONGE && y0 >= g.nabs[1] && y0 + TY <= ny - g.nabs[1] && z0 >= g.nabs[2] &&
+                                  z0 + kTZ <= nz - g.nabs[2];
+    // prologue: LDS slots of planes x0, x0 + 1; queue planes x0 - 5 .. x0 + 6; operands of
+    // planes x0 .. x0 + 2
+    const int xlast = g.n[0] + SA_HALO - 1;
+#pragma unroll
+    for (int k = 0; k < KI; ++k) {
+        dma16(rs.u, s0 + (w + NW * k) * 256, dv[k], (x0 + SA_HALO) * rs.plane_bytes);
+        dma16(rs.u, s1 + (w + NW * k) * 256, dv[k], (min(x0 + 1, xlast) + SA_HALO) * rs.plane_bytes);
+    }
+    float q[RPT][12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        const int sk = (min(x0 - SA_RADIUS + k, xlast) + SA_HALO) * rs.plane_bytes;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) q[r][k] = bload(rs.u, row[r], sk);
+    }
+    Stage5<RPT, MODE> st[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) load_stage5<RPT, MODE>(st[k], rs, row, grow, brow, min(x0 + k, xlast), a);
+    // whole 12-plane periods, straight-line (no per-plane branch)
+    const int nper = (x1 - x0 + 11) / 12;
+    for (int it = 0; it < nper; ++it) {
+        const int xb = x0 + 12 * it;
+#define SA_PLANE5(J)                                                                                               \
+    march5_plane<RPT, NW, MODE, SPONGE, J>(g, a, rs, s0, s1, s2, q, st, xb + J, x1, lane, w, row, orow, grow, brow, \
+                                           dv, d1, t1v, d2, t2v, tile_yz_interior);
+        SA_PLANE5(0) SA_PLANE5(1) SA_PLANE5(2) SA_PLANE5(3) SA_PLANE5(4) SA_PLANE5(5)
+        SA_PLANE5(6) SA_PLANE5(7) SA_PLANE5(8) SA_PLANE5(9) SA_PLANE5(10) SA_PLANE5(11)
+#undef SA_PLANE5
+    }
+    // no direct-to-LDS load may outlive the workgroup's LDS allocation
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+}
+
 template <int RPT, int MODE>
 void launch_rpt(int version, bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
-    if (version >= 2) {
-        if (version == 2) {
-            if (sponge)
-                k_fast3d_v2<RPT, MODE, true, 1, 1><<<grid, 256, 0, st>>>(g, a);
-            else
-                k_fast3d_v2<RPT, MODE, false, 1, 1><<<grid, 256, 0, st>>>(g, a);
-        } else if (version == 3) {
-            if (sponge)
-                k_fast3d_v2<RPT, MODE, true, 2, 1><<<grid, 256, 0, st>>>(g, a);
-            else
-                k_fast3d_v2<RPT, MODE, false, 2, 1><<<grid, 256, 0, st>>>(g, a);
-        } else {
-            // version 4: as 3 with the register budget capped for 6 waves per SIMD
-            if (sponge)
-                k_fast3d_v2<RPT, MODE, true, 2, 6><<<grid, 256, 0, st>>>(g, a);
-            else
-                k_fast3d_v2<RPT, MODE, false, 2, 6><<<grid, 256, 0, st>>>(g, a);
-        }
-        return;
-    }
-    if (sponge)
+    if (version == 2) {
+        if (sponge)
+            k_fast3d_v2<RPT, MODE, true, 1><<<grid, 256, 0, st>>>(g, a);
+        else
+            k_fast3d_v2<RPT, MODE, false, 1><<<grid, 256, 0, st>>>(g, a);
+    } else if (version == 3) {
+        if (sponge)
+            k_fast3d_v2<RPT, MODE, true, 2><<<grid, 256, 0, st>>>(g, a);
+        else
+            k_fast3d_v2<RPT, MODE, false, 2><<<grid, 256, 0, st>>>(g, a);
+    } else if (sponge) {
         k_fast3d<RPT, MODE, true><<<grid, 256, 0, st>>>(g, a);
-    else
+    } else {
         k_fast3d<RPT, MODE, false><<<grid, 256, 0, st>>>(g, a);
+    }
+}
+
+template <int RPT, int NW, int MODE>
+void launch_v5(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
+    if (sponge)
+        k_fast3d_v5<RPT, NW, MODE, true><<<grid, NW * 64, 0, st>>>(g, a);
+    else
+        k_fast3d_v5<RPT, NW, MODE, false><<<grid, NW * 64, 0, st>>>(g, a);
 }
 
 template <int MODE>
 int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
-    const int rows = f.tune.rows;
-    const int TY = kWaves * rows;
+    int rows = f.tune.rows;
+    int version = f.tune.version;
+    if (g.total * 4 >= (1ll << 31)) {  // buffer offsets are 32-bit: general-addressing version
+        version = 1;
+        rows = std::max(rows, 2);
+    }
+    const int waves = version == 6 ? 8 : kWaves;
+    const int TY = waves * rows;
     a.ntile_y = (g.n[1] + TY - 1) / TY;
     a.ntile_z = (g.n[2] + kTZ - 1) / kTZ;
     a.chunk = f.tune.chunk;
@@ -501,12 +783,25 @@ int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
     a.per_xcd = (a.nblocks + 7) / 8;
     dim3 grid(a.per_xcd * 8);
     const bool sponge = f.boundary == 0;
-    const int version = g.total * 4 < (1ll << 31) ? f.tune.version : 1;  // buffer offsets are 32-bit
-    switch (rows) {
-        case 2: launch_rpt<2, MODE>(version, sponge, grid, st, g, a); break;
-        case 4: launch_rpt<4, MODE>(version, sponge, grid, st, g, a); break;
-        case 8: launch_rpt<8, MODE>(version, sponge, grid, st, g, a); break;
-        default: return SA_E_ARG;
+    if (version == 5) {
+        switch (rows) {
+            case 2: launch_v5<2, 4, MODE>(sponge, grid, st, g, a); break;
+            case 4: launch_v5<4, 4, MODE>(sponge, grid, st, g, a); break;
+            default: return SA_E_ARG;
+        }
+    } else if (version == 6) {
+        switch (rows) {
+            case 1: launch_v5<1, 8, MODE>(sponge, grid, st, g, a); break;
+            case 2: launch_v5<2, 8, MODE>(sponge, grid, st, g, a); break;
+            default: return SA_E_ARG;
+        }
+    } else {
+        switch (rows) {
+            case 2: launch_rpt<2, MODE>(version, sponge, grid, st, g, a); break;
+            case 4: launch_rpt<4, MODE>(version, sponge, grid, st, g, a); break;
+            case 8: launch_rpt<8, MODE>(version, sponge, grid, st, g, a); break;
+            default: return SA_E_ARG;
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : SA_E_HIP;
 }
@@ -519,6 +814,8 @@ FastArgs base_args(const Fast3D &f, const float *u, const float *o, float *out) 
     a.vp = f.vp;
     a.tables = f.tables;
     a.tlen = f.tlen;
+    a.axis = f.axis;
+    a.alen = f.alen;
     a.dt = f.dt;
     a.dt2 = f.dt * f.dt;
     return a;
@@ -532,7 +829,12 @@ bool fast3d_enabled() {
 }
 
 int fast3d_set_tune(Fast3D &f, int rows, int chunk, int version) {
-    if (!(rows == 2 || rows == 4 || rows == 8) || chunk < 1 || version < 1 || version > 4) return SA_E_ARG;
+    if (chunk < 1) return SA_E_ARG;
+    const bool ok = version >= 1 && version <= 3   ? (rows == 2 || rows == 4 || rows == 8)
+                    : version == 5               ? (rows == 2 || rows == 4)
+                    : version == 6               ? (rows == 1 || rows == 2)
+                                                 : false;
+    if (!ok) return SA_E_ARG;
     f.tune.rows = rows;
     f.tune.chunk = chunk;
     f.tune.version = version;
@@ -544,6 +846,8 @@ int fast3d_prepare(Fast3D &f, const Geo &g, const float *vp, const float *tables
     f.boundary = boundary;
     f.vp = vp;
     f.dt = dt;
+    f.axis = tables;
+    f.alen = tlen;
     if (boundary == 0) {
         // corner table: layer sum for distances to the edge (d0, d1, d2) clipped at the widest
         // layer M, built with the rule of the general kernels (axis d contributes its value at

@@ -1,6 +1,6 @@
 """The specialised 3D update against the general kernels and the oracle: awkward
-sizes (not multiples of the 64 x 4*rows tile), unequal layers per axis, every
-tile shape, fused save and gradient."""
+sizes (not multiples of the tile or of the 12-plane period), unequal layers per
+axis, every kernel version and tile shape, fused save and gradient."""
 
 import os
 
@@ -31,7 +31,9 @@ def _run(plan, res_s, general, tile=None):
     (30, 6, (4, 7, 1)), (37, 5, (2, 13, 1)), (41, 7, (8, 41, 1)), (37, 5, (4, 1, 1)), (70, 8, (4, 30, 1)),
     (66, 6, (8, 9, 1)), (30, 6, (4, 7, 2)), (37, 5, (2, 13, 2)), (41, 7, (4, 41, 2)), (37, 5, (4, 1, 2)),
     (70, 8, (2, 30, 2)), (66, 6, (4, 12, 2)), (37, 5, (2, 13, 3)), (41, 7, (4, 41, 3)), (37, 5, (4, 1, 3)),
-    (66, 6, (2, 12, 3)),
+    (66, 6, (2, 12, 3)), (30, 6, (2, 7, 5)), (37, 5, (4, 13, 5)), (41, 7, (2, 41, 5)), (37, 5, (2, 1, 5)),
+    (70, 8, (4, 30, 5)), (66, 6, (2, 12, 5)), (30, 6, (1, 7, 6)), (37, 5, (2, 13, 6)), (41, 7, (1, 41, 6)),
+    (70, 8, (2, 24, 6)), (66, 6, (1, 5, 6)),
 ])
 def test_fast_matches_general_and_oracle(gpu, n, nabs, tile):
     plan, _ = small_case(dim=3, n=n, nabs=nabs, nt=90, kernel='OT2', grad_vp=True, nrec=20)
@@ -49,7 +51,8 @@ def test_fast_matches_general_and_oracle(gpu, n, nabs, tile):
         assert rel_l2(g_f[key], gref[key]) < 1e-4, key
 
 
-def test_fast_unequal_layers(gpu):
+@pytest.mark.parametrize('tile', [(2, 11, 2), (2, 11, 5), (1, 13, 6)])
+def test_fast_unequal_layers(gpu, tile):
     from stride_amd.problem import Space, Time
     from stride_amd.utils import tone_burst, ellipsoidal
     from stride_amd.physics.plan import build_plan
@@ -62,7 +65,8 @@ def test_fast_unequal_layers(gpu):
     plan = build_plan(space, time, tone_burst(0.5e6, 3, 90, 0.08e-6)[None], vp, source_coordinates=geom[:1],
                       receiver_coordinates=geom, kernel='OT2', grad_vp=True)
     res_s = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, plan.nt)))
-    rec_f, g_f = _run(plan, res_s, general=False, tile=(2, 11, 2))
+    print('fast3d unequal layers tile=%s' % (tile,), flush=True)
+    rec_f, g_f = _run(plan, res_s, general=False, tile=tile)
     ref, saved, _ = oracle.forward(plan, save=True)
     assert rel_l2(rec_f.T, ref) < 1e-5
     gref = oracle.adjoint_compat(plan, res_s, saved)
