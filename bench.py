@@ -117,12 +117,29 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--tile', type=str, default=None, help='rows,chunk,version of the 3D update tile')
     ap.add_argument('--no-autotune', action='store_true')
+    ap.add_argument('--copy-check', type=int, default=0, metavar='MB',
+                    help='only run 10 device-to-device copies of MB megabytes (calibrates the byte counters '
+                         'of a profiler against a transfer of known size) and exit')
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one GPU: the solver shards by shot, and multi-GPU runs are out of scope '
               'for this build (see DESIGN.md, sharding).', file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_check:
+        from stride_amd._native import Solver
+        s = Solver(3, (8, 8, 8), (1e-3,) * 3, (0, 0, 0), 1e-7, 3)
+        nbytes = args.copy_check * 1000000
+        a, b = s.device_alloc(nbytes), s.device_alloc(nbytes)
+        for _ in range(10):
+            s.memcpy(b, a, nbytes, 3)
+        s.synchronize()
+        s.device_free(a)
+        s.device_free(b)
+        s.close()
+        print(json.dumps({'copy_check_bytes': nbytes, 'copies': 10}))
+        return
 
     cfg, plan = workload(scale=args.scale, nt=args.nt)
     print('[bench] workload %s ext, nt %d, f %d, nrec %d' % (plan.shape, plan.nt, plan.undersampling, plan.nrec),

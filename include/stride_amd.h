@@ -136,8 +136,9 @@ int sa_free_saved(sa_solver *s);
  * axis. version 1 (shifted register queue; rows 2, 4, 8), 2 (buffer loads, rotated queue,
  * next plane prefetched, double-buffered LDS; rows 2, 4, 8), 3 (as 2, two planes ahead;
  * rows 2, 4, 8), 5 (LDS ring of planes filled by direct-to-LDS loads two planes ahead,
- * 4 waves; rows 2, 4) or 6 (as 5 with 8 waves; rows 1, 2). Used by the autotuner
- * (stride_amd.autotune). */
+ * 4 waves; rows 2, 4), 6 (as 5 with 8 waves; rows 1, 2), 7 (as 5, u read once: axis-0
+ * terms pushed into partial sums; rows 2) or 8 (as 7 with 8 waves; rows 1, 2). Default:
+ * version 7, rows 2, chunk 24. Used by the autotuner (stride_amd.autotune). */
 int sa_set_tile(sa_solver *s, int rows, int chunk, int version);
 
 /* Kernel timing: when enabled, every launch of the wavefield-update kernel is bracketed by

@@ -2,7 +2,8 @@
 
 For a grid and scheme, times short forward runs of the update kernel (HIP events
 around every launch) for each candidate — kernel version (1-3: register-staged
-planes, 4 waves; 5/6: LDS plane ring filled by direct-to-LDS loads, 4/8 waves),
+planes, 4 waves; 5/6: LDS plane ring filled by direct-to-LDS loads, 4/8 waves;
+7/8: as 5/6 with u read once, axis-0 terms pushed into partial sums),
 rows per lane (tile 64 x waves*rows cells) and march chunk lengths that split the
 slowest axis into 3..16 pieces — and returns the fastest. Results are cached per
 (shape, absorbing, boundary) for the life of the process.
@@ -17,12 +18,12 @@ __all__ = ['candidates', 'autotune']
 _CACHE = {}
 
 
-def candidates(shape, versions=(1, 2, 3, 5, 6)):
+def candidates(shape, versions=(3, 5, 6, 7, 8)):
     nx = shape[0]
     chunks = sorted({max(1, -(-nx // k)) for k in (3, 4, 5, 6, 7, 8, 10, 12, 16)})
-    # versions 5/6 march whole 12-plane periods: chunk lengths that are multiples of 12
+    # versions 5-8 march whole 12-plane periods: chunk lengths that are multiples of 12
     chunks12 = sorted({12 * max(1, -(-c // 12)) for c in chunks})
-    rows_of = {1: (2, 4, 8), 2: (2, 4), 3: (2, 4), 5: (2, 4), 6: (1, 2)}
+    rows_of = {1: (2, 4, 8), 2: (2, 4), 3: (2, 4), 5: (2, 4), 6: (1, 2), 7: (2,), 8: (1, 2)}
     return [(rows, chunk, version) for version in versions for rows in rows_of[version]
             for chunk in (chunks12 if version >= 5 else chunks)]
 

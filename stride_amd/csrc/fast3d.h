@@ -25,7 +25,7 @@ struct Fast3D {
     int boundary = -1;
     const float *vp = nullptr;
     float dt = 0.f;
-    FastTune tune = {4, 45, 2};
+    FastTune tune = {2, 24, 7};
     bool prepared = false;
 };
 

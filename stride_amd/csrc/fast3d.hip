@@ -208,6 +208,19 @@ __device__ __forceinline__ float bload(rsrc_t r, unsigned voff, int soff) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
+// Streaming (non-temporal) variants, cache-policy bit "nt": for arrays touched once per step
+// (the other level, vp, the new level, snapshots and gradient accumulators), so that the L2
+// keeps the u planes that neighbouring tiles read as halo.
+constexpr int kNT = 2;
+
+__device__ __forceinline__ float bload_nt(rsrc_t r, unsigned voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, kNT));
+}
+
+__device__ __forceinline__ void bstore_nt(float v, rsrc_t r, unsigned voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, kNT);
+}
+
 __device__ __forceinline__ void bstore(float v, rsrc_t r, unsigned voff, int soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
@@ -516,25 +529,25 @@ __device__ __forceinline__ void load_stage5(Stage5<RPT, MODE> &st, const Rsrc &r
     const int sp = (x + SA_HALO) * rs.plane_bytes;
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        st.o[r] = bload(rs.o, row[r], sp);
-        st.v[r] = bload(rs.vp, row[r], sp);
+        st.o[r] = bload_nt(rs.o, row[r], sp);
+        st.v[r] = bload_nt(rs.vp, row[r], sp);
     }
     if (MODE == 2) {
         // box planes outside [lo, lo + n) read as zero (offset past the end)
-        const bool xin = x >= a.box.lo[0] && x < a.box.lo[0] + a.box.n[0];
+        const bool xin = (x >= a.box.lo[0]) & (x < a.box.lo[0] + a.box.n[0]);
         const int sbx = xin ? (x - a.box.lo[0]) * rs.box_plane_bytes : 0;
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
-            st.ps[r] = bload(rs.ps, xin ? brow[r] : kOOB, sbx);
-            st.gr[r] = bload(rs.grad, grow[r], sp);
-            st.pr[r] = bload(rs.prec, grow[r], sp);
+            st.ps[r] = bload_nt(rs.ps, xin ? brow[r] : kOOB, sbx);
+            st.gr[r] = bload_nt(rs.grad, grow[r], sp);
+            st.pr[r] = bload_nt(rs.prec, grow[r], sp);
         }
     }
 }
 
-template <int RPT, int NW, int MODE, bool SPONGE, int J>
-__device__ __forceinline__ void march5_plane(const Geo &g, const FastArgs &a, const Rsrc &rs, float *__restrict__ s0,
-                                             float *__restrict__ s1, float *__restrict__ s2, float (&q)[RPT][12],
+template <int RPT, int NW, int MODE, bool SPONGE, int NS, int J>
+__device__ __forceinline__ void march5_plane(const Geo &g, const FastArgs &a, const Rsrc &rs, float *const (&sl)[6],
+                                             float (&q)[RPT][12],
                                              Stage5<RPT, MODE> (&st)[3], const int x, const int x1, const int lane,
                                              const int w, const unsigned (&row)[RPT], const unsigned (&orow)[RPT],
                                              const unsigned (&grow)[RPT], const unsigned (&brow)[RPT],
@@ -543,8 +556,8 @@ __device__ __forceinline__ void march5_plane(const Geo &g, const FastArgs &a, co
                                              const bool tile_yz_interior) {
     using R = RingW<NW * RPT, NW>;
     constexpr int KI = R::KI;
-    float *__restrict__ cur = J % 3 == 0 ? s0 : (J % 3 == 1 ? s1 : s2);
-    float *__restrict__ nxt = J % 3 == 0 ? s2 : (J % 3 == 1 ? s0 : s1);  // slot of plane x + 2
+    float *__restrict__ cur = sl[J % NS];              // slot of plane x
+    float *__restrict__ nxt = sl[(J + NS - 1) % NS];   // slot of plane x + NS - 1
     const int nx = g.n[0];
     // 1. this wave's loads into the slot of plane x have landed; then every wave's have. The
     // probe read is pinned here (memory clobbers on both sides): hoisted, its wait would
@@ -557,11 +570,11 @@ __device__ __forceinline__ void march5_plane(const Geo &g, const FastArgs &a, co
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-    // 2. plane x + 2 into the slot every wave finished reading before the barrier (plane
+    // 2. plane x + NS - 1 into the slot every wave finished reading before the barrier (plane
     // indices past the last halo plane are clamped: those planes are never stored)
     const int xlast = nx + SA_HALO - 1;
     {
-        const int soff = (min(x + 2, xlast) + SA_HALO) * rs.plane_bytes;
+        const int soff = (min(x + NS - 1, xlast) + SA_HALO) * rs.plane_bytes;
 #pragma unroll
         for (int k = 0; k < KI; ++k) dma16(rs.u, nxt + (w + NW * k) * 256, dv[k], soff);
     }
@@ -570,7 +583,8 @@ __device__ __forceinline__ void march5_plane(const Geo &g, const FastArgs &a, co
     const int sxo = (x + SA_HALO) * rs.plane_bytes;
     // planes past the chunk end (the last 12-plane period) are computed but not stored
     const bool xin = x < x1;
-    const bool xin_box = MODE != 0 && xin && x >= a.box.lo[0] && x < a.box.lo[0] + a.box.n[0];
+    // bitwise, not short-circuit: a branch here splits the plane's schedule
+    const bool xin_box = MODE != 0 && (xin & (x >= a.box.lo[0]) & (x < a.box.lo[0] + a.box.n[0]));
     const int sbx = MODE == 1 && xin_box ? (x - a.box.lo[0]) * rs.box_plane_bytes : 0;
     const bool plane_interior = tile_yz_interior && x >= g.nabs[0] && x < nx - g.nabs[0];
     int d0 = 0;
@@ -581,7 +595,11 @@ __device__ __forceinline__ void march5_plane(const Geo &g, const FastArgs &a, co
     }
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        const int li = (w + NW * r + 5) * R::PITCH + lane + 8;
+        // the lane offset is made opaque per plane so that the LDS addresses (one base per
+        // 1 KiB window of ds_read2 offsets) are formed here instead of being hoisted out of
+        // the march for every slot, which would hold tens of VGPRs
+        int li = (w + NW * r + 5) * R::PITCH + lane + 8;
+        asm volatile("" : "+v"(li));
         const float c = q[r][(J + 5) % 12];
         float vx = g.w2[0][0] * c, vy = g.w2[1][0] * c, vz = g.w2[2][0] * c;
 #pragma unroll
@@ -612,13 +630,13 @@ __device__ __forceinline__ void march5_plane(const Geo &g, const FastArgs &a, co
             }
         }
         const float val = A * c + B * sx.o[r] + G * lap;
-        bstore(val, rs.out, xin ? orow[r] : kOOB, sxo);
-        if (MODE == 1) bstore((val - 2.f * c + sx.o[r]) * a.idt2, rs.save, xin_box ? brow[r] : kOOB, sbx);
+        bstore_nt(val, rs.out, xin ? orow[r] : kOOB, sxo);
+        if (MODE == 1) bstore_nt((val - 2.f * c + sx.o[r]) * a.idt2, rs.save, xin_box ? brow[r] : kOOB, sbx);
         if (MODE == 2) {
             const float psv = sx.ps[r];
             const unsigned go = xin_box ? grow[r] : kOOB;
-            bstore(sx.gr[r] + psv * c, rs.grad, go, sxo);
-            bstore(sx.pr[r] + psv * psv, rs.prec, go, sxo);
+            bstore_nt(sx.gr[r] + psv * c, rs.grad, go, sxo);
+            bstore_nt(sx.pr[r] + psv * psv, rs.prec, go, sxo);
         }
     }
     // 4. refill: u of plane x + 7 into the queue slot of plane x - 5, operands of plane x + 3
@@ -630,14 +648,25 @@ __device__ __forceinline__ void march5_plane(const Geo &g, const FastArgs &a, co
     load_stage5<RPT, MODE>(sx, rs, row, grow, brow, min(x + 3, xlast), a);
 }
 
-template <int RPT, int NW, int MODE, bool SPONGE>
+// LDS ring of NS plane slots (one array each: the compiler tracks direct-to-LDS loads per
+// array, so a read of one slot waits only for the loads into that slot); plane x + NS - 1 is
+// requested while plane x is computed
+#define SA_RING_SLOTS(NS, FLOATS)                                                   \
+    __shared__ __attribute__((aligned(16))) float s0[FLOATS];                       \
+    __shared__ __attribute__((aligned(16))) float s1[FLOATS];                       \
+    __shared__ __attribute__((aligned(16))) float s2[NS > 2 ? FLOATS : 4];          \
+    __shared__ __attribute__((aligned(16))) float s3[NS > 3 ? FLOATS : 4];          \
+    __shared__ __attribute__((aligned(16))) float s4[NS > 4 ? FLOATS : 4];          \
+    __shared__ __attribute__((aligned(16))) float s5[NS > 5 ? FLOATS : 4];          \
+    float *const sl[6] = {s0, s1, s2, s3, s4, s5};
+
+template <int RPT, int NW, int MODE, bool SPONGE, int NS>
 __global__ __launch_bounds__(NW * 64) void k_fast3d_v5(Geo g, FastArgs a) {
     constexpr int TY = NW * RPT;
     using R = RingW<TY, NW>;
     constexpr int KI = R::KI;
-    __shared__ __attribute__((aligned(16))) float s0[R::FLOATS];
-    __shared__ __attribute__((aligned(16))) float s1[R::FLOATS];
-    __shared__ __attribute__((aligned(16))) float s2[R::FLOATS];
+    static_assert(NS >= 3 && NS <= 6 && 12 % NS == 0, "ring depth: 3, 4 or 6 slots");
+    SA_RING_SLOTS(NS, R::FLOATS)
     const int b = blockIdx.x;
     const int Lg = (b & 7) * a.per_xcd + (b >> 3);
     if (Lg >= a.nblocks) return;
@@ -709,10 +738,10 @@ __global__ __launch_bounds__(NW * 64) void k_fast3d_v5(Geo g, FastArgs a) {
     // planes x0 .. x0 + 2
     const int xlast = g.n[0] + SA_HALO - 1;
 #pragma unroll
-    for (int k = 0; k < KI; ++k) {
-        dma16(rs.u, s0 + (w + NW * k) * 256, dv[k], (x0 + SA_HALO) * rs.plane_bytes);
-        dma16(rs.u, s1 + (w + NW * k) * 256, dv[k], (min(x0 + 1, xlast) + SA_HALO) * rs.plane_bytes);
-    }
+    for (int p = 0; p < NS - 1; ++p)
+#pragma unroll
+        for (int k = 0; k < KI; ++k)
+            dma16(rs.u, sl[p] + (w + NW * k) * 256, dv[k], (min(x0 + p, xlast) + SA_HALO) * rs.plane_bytes);
     float q[RPT][12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
@@ -728,7 +757,7 @@ __global__ __launch_bounds__(NW * 64) void k_fast3d_v5(Geo g, FastArgs a) {
     for (int it = 0; it < nper; ++it) {
         const int xb = x0 + 12 * it;
 #define SA_PLANE5(J)                                                                                               \
-    march5_plane<RPT, NW, MODE, SPONGE, J>(g, a, rs, s0, s1, s2, q, st, xb + J, x1, lane, w, row, orow, grow, brow, \
+    march5_plane<RPT, NW, MODE, SPONGE, NS, J>(g, a, rs, sl, q, st, xb + J, x1, lane, w, row, orow, grow, brow, \
                                            dv, d1, t1v, d2, t2v, tile_yz_interior);
         SA_PLANE5(0) SA_PLANE5(1) SA_PLANE5(2) SA_PLANE5(3) SA_PLANE5(4) SA_PLANE5(5)
         SA_PLANE5(6) SA_PLANE5(7) SA_PLANE5(8) SA_PLANE5(9) SA_PLANE5(10) SA_PLANE5(11)
@@ -736,6 +765,300 @@ __global__ __launch_bounds__(NW * 64) void k_fast3d_v5(Geo g, FastArgs a) {
     }
     // no direct-to-LDS load may outlive the workgroup's LDS allocation
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+}
+
+// Seventh-generation march (versions 7 and 8): like version 5 (LDS plane ring filled by
+// direct-to-LDS loads, branch-free planes), but u is read from memory once. Version 5 keeps
+// a register window of u along axis 0, filled by separate loads five planes ahead of the
+// plane tile that brings the same cells into LDS; with hundreds of tiles in flight per XCD
+// the L2 cannot hold those five planes, so u is fetched from HBM twice. Here the own cell of
+// plane p is read from the LDS tile when plane p arrives, and its axis-0 second-derivative
+// contributions are pushed into a ring of partial sums for planes p - 5 .. p + 5; plane
+// p - 5, now complete, is updated and stored. The first five planes of a chunk get their
+// contributions from own-column loads before the march, the last five are finished after it
+// with own-column loads of the five planes past the chunk.
+template <int RPT, int MODE, bool SPONGE>
+__device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const Rsrc &rs, const int x, const int x0,
+                                        const int x1, const float (&lap)[RPT], const float (&c)[RPT],
+                                        const Stage5<RPT, MODE> &sx, const unsigned (&orow)[RPT],
+                                        const unsigned (&grow)[RPT], const unsigned (&brow)[RPT],
+                                        const int (&d1)[RPT], const float (&t1v)[RPT], const int d2, const float t2v) {
+    const int nx = g.n[0];
+    const int sxo = (x + SA_HALO) * rs.plane_bytes;
+    // planes outside the chunk (before it, or past its end in the last 12-plane period) are
+    // computed but not stored
+    const bool xin = (x >= x0) & (x < x1);
+    // bitwise, not short-circuit: a branch here splits the plane's schedule
+    const bool xin_box = MODE != 0 && (xin & (x >= a.box.lo[0]) & (x < a.box.lo[0] + a.box.n[0]));
+    const int sbx = MODE == 1 && xin_box ? (x - a.box.lo[0]) * rs.box_plane_bytes : 0;
+    int d0 = 0;
+    float t0v = 0.f;
+    if (SPONGE) {
+        d0 = max(min(x, nx - 1 - x), 0);
+        const float tv = a.axis[min(d0, a.alen - 1)];
+        t0v = d0 < g.nabs[0] ? tv : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const float v = sx.v[r];
+        const float v2 = v * v;
+        float A = 2.f, B = -1.f, G = a.dt2 * v2;
+        if (SPONGE) {
+            // branch-free layer rule (axis d contributes its table value at distance d_d from
+            // the edge when no other axis is closer to its own edge); with sm = 0 every
+            // expression reduces exactly to the interior values
+            float sm = d0 <= min(d1[r], d2) ? t0v : 0.f;
+            sm += d1[r] <= min(d0, d2) ? t1v[r] : 0.f;
+            sm += d2 <= min(d0, d1[r]) ? t2v : 0.f;
+            const float D = (7.f * (sm * v)) * a.dt;
+            const float aa = a.dt * v2 * D;
+            const float e = a.dt * a.dt * v2 * D * D;
+            const float ia = 1.f / (1.f + aa);
+            A = (2.f - e) * ia;
+            B = -(1.f - aa) * ia;
+            G = (a.dt2 * ia) * v2;
+        }
+        const float val = A * c[r] + B * sx.o[r] + G * lap[r];
+        bstore_nt(val, rs.out, xin ? orow[r] : kOOB, sxo);
+        if (MODE == 1) bstore_nt((val - 2.f * c[r] + sx.o[r]) * a.idt2, rs.save, xin_box ? brow[r] : kOOB, sbx);
+        if (MODE == 2) {
+            const float psv = sx.ps[r];
+            const unsigned go = xin_box ? grow[r] : kOOB;
+            bstore_nt(sx.gr[r] + psv * c[r], rs.grad, go, sxo);
+            bstore_nt(sx.pr[r] + psv * psv, rs.prec, go, sxo);
+        }
+    }
+}
+
+// One step of the march: plane p (LDS slot J % 3, J = p - x0 mod 12). Partial-sum slot
+// (J + k) % 12 holds plane p + k (k = -5 .. 5); centre slot J % 6 holds u of plane p; operand
+// stage (J + 1) % 3 holds plane p - 5.
+template <int RPT, int NW, int MODE, bool SPONGE, int NS, int J>
+__device__ __forceinline__ void march7_step(const Geo &g, const FastArgs &a, const Rsrc &rs, float *const (&sl)[6],
+                                            float (&acc)[RPT][12],
+                                            float (&cq)[RPT][6], Stage5<RPT, MODE> (&st)[3], const int p,
+                                            const int x0, const int x1, const int lane, const int w,
+                                            const unsigned (&row)[RPT], const unsigned (&orow)[RPT],
+                                            const unsigned (&grow)[RPT], const unsigned (&brow)[RPT],
+                                            const unsigned (&dv)[RingW<NW * RPT, NW>::KI], const int (&d1)[RPT],
+                                            const float (&t1v)[RPT], const int d2, const float t2v) {
+    using R = RingW<NW * RPT, NW>;
+    constexpr int KI = R::KI;
+    float *__restrict__ cur = sl[J % NS];              // slot of plane p
+    float *__restrict__ nxt = sl[(J + NS - 1) % NS];   // slot of plane p + NS - 1
+    const int xlast = g.n[0] + SA_HALO - 1;
+    // 1. this wave's loads into the slot of plane p have landed; then every wave's have (the
+    // probe read is pinned between memory clobbers so that its wait is not hoisted)
+    asm volatile("" ::: "memory");
+    {
+        const float probe = cur[lane];
+        asm volatile("" ::"v"(probe));
+    }
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    // 2. plane p + NS - 1 into the slot every wave finished reading before the barrier
+    {
+        const int soff = (min(p + NS - 1, xlast) + SA_HALO) * rs.plane_bytes;
+#pragma unroll
+        for (int k = 0; k < KI; ++k) dma16(rs.u, nxt + (w + NW * k) * 256, dv[k], soff);
+    }
+    // 3. plane p: axis-1/2 terms, own centre, pushes along axis 0
+    float lap[RPT], c5[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        // the lane offset is made opaque per plane so that the LDS addresses (one base per
+        // 1 KiB window of ds_read2 offsets) are formed here instead of being hoisted out of
+        // the march for every slot, which would hold tens of VGPRs
+        int li = (w + NW * r + 5) * R::PITCH + lane + 8;
+        asm volatile("" : "+v"(li));
+        const float c = cur[li];
+        float vy = g.w2[1][0] * c, vz = g.w2[2][0] * c;
+#pragma unroll
+        for (int k = 1; k <= 5; ++k) {
+            vy += g.w2[1][k] * (cur[li + k * R::PITCH] + cur[li - k * R::PITCH]);
+            vz += g.w2[2][k] * (cur[li + k] + cur[li - k]);
+        }
+        acc[r][(J + 5) % 12] = g.w2[0][5] * c;  // first contribution to plane p + 5
+#pragma unroll
+        for (int k = 1; k <= 4; ++k) acc[r][(J + k) % 12] += g.w2[0][k] * c;
+        acc[r][J % 12] += g.w2[0][0] * c + (vy + vz);
+#pragma unroll
+        for (int k = 1; k <= 5; ++k) acc[r][(J + 12 - k) % 12] += g.w2[0][k] * c;
+        lap[r] = acc[r][(J + 7) % 12];  // plane p - 5 is complete
+        c5[r] = cq[r][(J + 1) % 6];
+        cq[r][J % 6] = c;
+    }
+    // 4. update and store plane p - 5; operands of plane p - 2 into its stage
+    Stage5<RPT, MODE> &sx = st[(J + 1) % 3];
+    finish7<RPT, MODE, SPONGE>(g, a, rs, p - 5, x0, x1, lap, c5, sx, orow, grow, brow, d1, t1v, d2, t2v);
+    load_stage5<RPT, MODE>(st[(J + 1) % 3], rs, row, grow, brow, min(max(p - 2, 0), xlast), a);
+}
+
+// Epilogue step: plane p past the march (own column from memory), finishing plane p - 5.
+template <int RPT, int NW, int MODE, bool SPONGE, int J>
+__device__ __forceinline__ void tail7_step(const Geo &g, const FastArgs &a, const Rsrc &rs, float (&acc)[RPT][12],
+                                           float (&cq)[RPT][6], Stage5<RPT, MODE> (&st)[3], const float (&up)[RPT],
+                                           const int p, const int x0, const int x1, const unsigned (&row)[RPT],
+                                           const unsigned (&orow)[RPT], const unsigned (&grow)[RPT],
+                                           const unsigned (&brow)[RPT], const int (&d1)[RPT],
+                                           const float (&t1v)[RPT], const int d2, const float t2v) {
+    const int xlast = g.n[0] + SA_HALO - 1;
+    float lap[RPT], c5[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        // only planes p - 5 .. p - 1 are still open; J = 0 at the first tail step
+#pragma unroll
+        for (int k = J + 1; k <= 5; ++k) acc[r][(J + 12 - k) % 12] += g.w2[0][k] * up[r];
+        lap[r] = acc[r][(J + 7) % 12];
+        c5[r] = cq[r][(J + 1) % 6];
+    }
+    finish7<RPT, MODE, SPONGE>(g, a, rs, p - 5, x0, x1, lap, c5, st[(J + 1) % 3], orow, grow, brow, d1, t1v, d2,
+                                   t2v);
+    if (J < 2) load_stage5<RPT, MODE>(st[(J + 1) % 3], rs, row, grow, brow, min(p - 2, xlast), a);
+}
+
+// The march of one chunk; SPONGE: the layer coefficients are evaluated on every cell.
+template <int RPT, int NW, int MODE, bool SPONGE, int NS>
+__device__ __forceinline__ void march7(const Geo &g, const FastArgs &a, const Rsrc &rs, float *const (&sl)[6],
+                                       const int x0, const int x1,
+                                       const int lane, const int w, const unsigned (&row)[RPT],
+                                       const unsigned (&orow)[RPT], const unsigned (&grow)[RPT],
+                                       const unsigned (&brow)[RPT], const unsigned (&dv)[RingW<NW * RPT, NW>::KI],
+                                       const int (&d1)[RPT], const float (&t1v)[RPT], const int d2,
+                                       const float t2v) {
+    constexpr int KI = RingW<NW * RPT, NW>::KI;
+    const int xlast = g.n[0] + SA_HALO - 1;
+    // prologue: LDS slots of planes x0 .. x0 + NS - 2; partial sums of planes x0 .. x0 + 4
+    // from the own column of planes x0 - 5 .. x0 - 1
+#pragma unroll
+    for (int p = 0; p < NS - 1; ++p)
+#pragma unroll
+        for (int k = 0; k < KI; ++k)
+            dma16(rs.u, sl[p] + (w + NW * k) * 256, dv[k], (min(x0 + p, xlast) + SA_HALO) * rs.plane_bytes);
+    float acc[RPT][12], cq[RPT][6];
+    Stage5<RPT, MODE> st[3] = {};  // filled inside the march before their first stored use
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[r][k] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cq[r][k] = 0.f;
+#pragma unroll
+        for (int i = 1; i <= 5; ++i) {
+            const float ub = bload(rs.u, row[r], (x0 - i + SA_HALO) * rs.plane_bytes);
+#pragma unroll
+            for (int j = 0; j + i <= 5; ++j) acc[r][j] += g.w2[0][i + j] * ub;
+        }
+    }
+    // whole 12-plane periods, straight-line (no per-plane branch)
+    const int nper = (x1 - x0 + 11) / 12;
+    const int pend = x0 + 12 * nper;
+    for (int it = 0; it < nper; ++it) {
+        const int xb = x0 + 12 * it;
+#define SA_STEP7(J)                                                                                              \
+    march7_step<RPT, NW, MODE, SPONGE, NS, J>(g, a, rs, sl, acc, cq, st, xb + J, x0, x1, lane, w, row, orow, \
+                                          grow, brow, dv, d1, t1v, d2, t2v);
+        SA_STEP7(0) SA_STEP7(1) SA_STEP7(2) SA_STEP7(3) SA_STEP7(4) SA_STEP7(5)
+        SA_STEP7(6) SA_STEP7(7) SA_STEP7(8) SA_STEP7(9) SA_STEP7(10) SA_STEP7(11)
+#undef SA_STEP7
+    }
+    // epilogue: planes pend - 5 .. pend - 1 from the own column of planes pend .. pend + 4
+    float up[5][RPT];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) up[j][r] = bload(rs.u, row[r], (min(pend + j, xlast) + SA_HALO) * rs.plane_bytes);
+#define SA_TAIL7(J)                                                                                                 \
+    tail7_step<RPT, NW, MODE, SPONGE, J>(g, a, rs, acc, cq, st, up[J], pend + J, x0, x1, row, orow, grow, brow, d1, \
+                                         t1v, d2, t2v);
+    SA_TAIL7(0) SA_TAIL7(1) SA_TAIL7(2) SA_TAIL7(3) SA_TAIL7(4)
+#undef SA_TAIL7
+    // no direct-to-LDS load may outlive the workgroup's LDS allocation
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+}
+
+// register budget with one row per lane: 80 VGPRs (6 waves per SIMD);
+// without it the scheduler may trade occupancy for latency hiding that this
+// bandwidth-bound loop does not need
+template <int RPT, int NW, int MODE, bool SPONGE, int NS>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(RPT == 1 ? 6 : 1))) void k_fast3d_v7(
+    Geo g, FastArgs a) {
+    constexpr int TY = NW * RPT;
+    using R = RingW<TY, NW>;
+    constexpr int KI = R::KI;
+    static_assert(NS >= 3 && NS <= 6 && 12 % NS == 0, "ring depth: 3, 4 or 6 slots");
+    SA_RING_SLOTS(NS, R::FLOATS)
+    const int b = blockIdx.x;
+    const int Lg = (b & 7) * a.per_xcd + (b >> 3);
+    if (Lg >= a.nblocks) return;
+    const int tz = Lg % a.ntile_z;
+    const int rest = Lg / a.ntile_z;
+    const int ty = rest % a.ntile_y;
+    const int ch = rest / a.ntile_y;
+    const int z0 = tz * kTZ, y0 = ty * TY;
+    const int x0 = ch * a.chunk;
+    const int x1 = min(x0 + a.chunk, g.n[0]);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int z = z0 + lane;
+    const int ny = g.n[1], nz = g.n[2];
+    const unsigned ps1 = (unsigned)g.s1;
+    Rsrc rs;
+    const long long bytes = g.total * 4;
+    rs.u = make_rsrc(a.u, bytes);
+    rs.o = make_rsrc(a.o, bytes);
+    rs.vp = make_rsrc(a.vp, bytes);
+    rs.out = make_rsrc(a.out, bytes);
+    rs.plane_bytes = (int)(g.s0 * 4);
+    rs.box_plane_bytes = a.box.n[1] * a.box.n[2] * 4;
+    const long long box_bytes = (long long)a.box.n[0] * rs.box_plane_bytes;
+    if (MODE == 1) rs.save = make_rsrc(a.save, box_bytes);
+    if (MODE == 2) {
+        rs.ps = make_rsrc(a.ps, box_bytes);
+        rs.grad = make_rsrc(a.grad, bytes);
+        rs.prec = make_rsrc(a.prec, bytes);
+    }
+    const unsigned zc = (unsigned)(min(z, nz + SA_RADIUS - 1) + SA_LEAD);
+    unsigned row[RPT], orow[RPT], grow[RPT], brow[RPT];
+    int d1[RPT];
+    float t1v[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int yr = y0 + w + NW * r;
+        const bool rin = yr < ny && z < nz;
+        row[r] = ((unsigned)(min(yr, ny + SA_RADIUS - 1) + SA_HALO) * ps1 + zc) * 4u;
+        orow[r] = rin ? row[r] : kOOB;
+        const bool bin = rin && yr >= a.box.lo[1] && yr < a.box.lo[1] + a.box.n[1] && z >= a.box.lo[2] &&
+                         z < a.box.lo[2] + a.box.n[2];
+        brow[r] = bin ? (unsigned)((yr - a.box.lo[1]) * a.box.n[2] + (z - a.box.lo[2])) * 4u : kOOB;
+        grow[r] = bin ? row[r] : kOOB;
+        d1[r] = min(yr, ny - 1 - yr);
+        t1v[r] = 0.f;
+        if (SPONGE && d1[r] >= 0 && d1[r] < g.nabs[1]) t1v[r] = a.axis[a.alen + d1[r]];
+    }
+    const int d2 = min(z, nz - 1 - z);
+    const float t2v = SPONGE && d2 >= 0 && d2 < g.nabs[2] ? a.axis[2 * a.alen + d2] : 0.f;
+    unsigned dv[KI];
+#pragma unroll
+    for (int k = 0; k < KI; ++k) {
+        const int c = (w + NW * k) * 64 + lane;
+        dv[k] = kOOB;
+        if (c < R::NCH) {
+            const int rr = c / R::CPR, cc = c % R::CPR;
+            const int yy = min(y0 - 5 + rr, ny + SA_HALO - 1);
+            const int zz = min(z0 + SA_LEAD - 8 + 4 * cc, (int)ps1 - 4);
+            dv[k] = ((unsigned)(yy + SA_HALO) * ps1 + (unsigned)zz) * 4u;
+        }
+    }
+    // tiles whose cells (axes 1/2) and chunk planes all lie inside the layers take the plain
+    // update; the others evaluate the layer coefficients on every cell, branch-free
+    const bool interior = y0 >= g.nabs[1] && y0 + TY <= ny - g.nabs[1] && z0 >= g.nabs[2] &&
+                          z0 + kTZ <= nz - g.nabs[2] && x0 >= g.nabs[0] && x1 <= g.n[0] - g.nabs[0];
+    if (SPONGE && !interior)
+        march7<RPT, NW, MODE, true, NS>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2, t2v);
+    else
+        march7<RPT, NW, MODE, false, NS>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2, t2v);
 }
 
 template <int RPT, int MODE>
@@ -757,12 +1080,20 @@ void launch_rpt(int version, bool sponge, dim3 grid, hipStream_t st, const Geo &
     }
 }
 
-template <int RPT, int NW, int MODE>
+template <int RPT, int NW, int MODE, int NS>
 void launch_v5(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
     if (sponge)
-        k_fast3d_v5<RPT, NW, MODE, true><<<grid, NW * 64, 0, st>>>(g, a);
+        k_fast3d_v5<RPT, NW, MODE, true, NS><<<grid, NW * 64, 0, st>>>(g, a);
     else
-        k_fast3d_v5<RPT, NW, MODE, false><<<grid, NW * 64, 0, st>>>(g, a);
+        k_fast3d_v5<RPT, NW, MODE, false, NS><<<grid, NW * 64, 0, st>>>(g, a);
+}
+
+template <int RPT, int NW, int MODE, int NS>
+void launch_v7(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
+    if (sponge)
+        k_fast3d_v7<RPT, NW, MODE, true, NS><<<grid, NW * 64, 0, st>>>(g, a);
+    else
+        k_fast3d_v7<RPT, NW, MODE, false, NS><<<grid, NW * 64, 0, st>>>(g, a);
 }
 
 template <int MODE>
@@ -773,7 +1104,7 @@ int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
         version = 1;
         rows = std::max(rows, 2);
     }
-    const int waves = version == 6 ? 8 : kWaves;
+    const int waves = version == 6 || version == 8 ? 8 : kWaves;
     const int TY = waves * rows;
     a.ntile_y = (g.n[1] + TY - 1) / TY;
     a.ntile_z = (g.n[2] + kTZ - 1) / kTZ;
@@ -785,14 +1116,25 @@ int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
     const bool sponge = f.boundary == 0;
     if (version == 5) {
         switch (rows) {
-            case 2: launch_v5<2, 4, MODE>(sponge, grid, st, g, a); break;
-            case 4: launch_v5<4, 4, MODE>(sponge, grid, st, g, a); break;
+            case 2: launch_v5<2, 4, MODE, 3>(sponge, grid, st, g, a); break;
+            case 4: launch_v5<4, 4, MODE, 3>(sponge, grid, st, g, a); break;
             default: return SA_E_ARG;
         }
     } else if (version == 6) {
         switch (rows) {
-            case 1: launch_v5<1, 8, MODE>(sponge, grid, st, g, a); break;
-            case 2: launch_v5<2, 8, MODE>(sponge, grid, st, g, a); break;
+            case 1: launch_v5<1, 8, MODE, 3>(sponge, grid, st, g, a); break;
+            case 2: launch_v5<2, 8, MODE, 3>(sponge, grid, st, g, a); break;
+            default: return SA_E_ARG;
+        }
+    } else if (version == 7) {
+        switch (rows) {
+            case 2: launch_v7<2, 4, MODE, 3>(sponge, grid, st, g, a); break;
+            default: return SA_E_ARG;
+        }
+    } else if (version == 8) {
+        switch (rows) {
+            case 1: launch_v7<1, 8, MODE, 3>(sponge, grid, st, g, a); break;
+            case 2: launch_v7<2, 8, MODE, 3>(sponge, grid, st, g, a); break;
             default: return SA_E_ARG;
         }
     } else {
@@ -832,7 +1174,8 @@ int fast3d_set_tune(Fast3D &f, int rows, int chunk, int version) {
     if (chunk < 1) return SA_E_ARG;
     const bool ok = version >= 1 && version <= 3   ? (rows == 2 || rows == 4 || rows == 8)
                     : version == 5               ? (rows == 2 || rows == 4)
-                    : version == 6               ? (rows == 1 || rows == 2)
+                    : version == 7               ? rows == 2
+                    : version == 6 || version == 8 ? (rows == 1 || rows == 2)
                                                  : false;
     if (!ok) return SA_E_ARG;
     f.tune.rows = rows;
@@ -883,10 +1226,10 @@ int fast3d_prepare(Fast3D &f, const Geo &g, const float *vp, const float *tables
     f.tables = f.corner;
     f.tlen = std::max(g.nabs[0], std::max(g.nabs[1], g.nabs[2]));  // clip distance of the table
     if (!f.prepared) {
-        // default tile: 64 x 16 cells, chunk sized so that ~8 chunks cover axis 0
-        f.tune.rows = 4;
-        f.tune.chunk = (g.n[0] + 7) / 8;
-        f.tune.version = 2;
+        // default: version 7, tile 64 x 8 cells, 24-plane chunks (the autotuner refines it)
+        f.tune.rows = 2;
+        f.tune.chunk = 24;
+        f.tune.version = 7;
         const char *e = getenv("STRIDE_AMD_TILE");
         if (e) {
             int r = 0, c = 0, v = 2;

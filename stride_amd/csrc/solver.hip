@@ -1376,7 +1376,8 @@ extern "C" int sa_set_tile(sa_solver *s, int rows, int chunk, int version) {
     if (!s) return fail(SA_E_ARG, "null solver");
     s->fast.prepared = true;  // an explicit tile overrides the default
     if (fast3d_set_tune(s->fast, rows, chunk, version) != 0)
-        return fail(SA_E_ARG, "tile: chunk >= 1 and version 1-3 with rows 2/4/8, 5 with rows 2/4, 6 with rows 1/2");
+        return fail(SA_E_ARG, "tile: chunk >= 1 and version 1-3 with rows 2/4/8, 5 with rows 2/4, 6 or 8 with rows "
+                                 "1/2, 7 with rows 2");
     return SA_OK;
 }
 

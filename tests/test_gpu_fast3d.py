@@ -33,7 +33,9 @@ def _run(plan, res_s, general, tile=None):
     (70, 8, (2, 30, 2)), (66, 6, (4, 12, 2)), (37, 5, (2, 13, 3)), (41, 7, (4, 41, 3)), (37, 5, (4, 1, 3)),
     (66, 6, (2, 12, 3)), (30, 6, (2, 7, 5)), (37, 5, (4, 13, 5)), (41, 7, (2, 41, 5)), (37, 5, (2, 1, 5)),
     (70, 8, (4, 30, 5)), (66, 6, (2, 12, 5)), (30, 6, (1, 7, 6)), (37, 5, (2, 13, 6)), (41, 7, (1, 41, 6)),
-    (70, 8, (2, 24, 6)), (66, 6, (1, 5, 6)),
+    (70, 8, (2, 24, 6)), (66, 6, (1, 5, 6)), (30, 6, (2, 7, 7)), (37, 5, (2, 13, 7)), (41, 7, (2, 41, 7)),
+    (37, 5, (2, 1, 7)), (70, 8, (2, 24, 7)), (30, 6, (1, 7, 8)), (37, 5, (2, 13, 8)), (41, 7, (1, 41, 8)),
+    (37, 5, (1, 1, 8)), (70, 8, (1, 36, 8)), (66, 6, (1, 12, 8)), (66, 6, (2, 5, 8)),
 ])
 def test_fast_matches_general_and_oracle(gpu, n, nabs, tile):
     plan, _ = small_case(dim=3, n=n, nabs=nabs, nt=90, kernel='OT2', grad_vp=True, nrec=20)
@@ -51,7 +53,7 @@ def test_fast_matches_general_and_oracle(gpu, n, nabs, tile):
         assert rel_l2(g_f[key], gref[key]) < 1e-4, key
 
 
-@pytest.mark.parametrize('tile', [(2, 11, 2), (2, 11, 5), (1, 13, 6)])
+@pytest.mark.parametrize('tile', [(2, 11, 2), (2, 11, 5), (1, 13, 6), (2, 11, 7), (1, 13, 8)])
 def test_fast_unequal_layers(gpu, tile):
     from stride_amd.problem import Space, Time
     from stride_amd.utils import tone_burst, ellipsoidal
