@@ -387,7 +387,7 @@ __global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
     rs.vp = make_rsrc(a.vp, bytes);
     rs.out = make_rsrc(a.out, bytes);
     rs.plane_bytes = (int)(g.s0 * 4);
-    rs.box_plane_bytes = a.box.n[1] * a.box.n[2] * 4;
+    rs.box_plane_bytes = a.box.n[1] * a.box.p2 * 4;
     const long long box_bytes = (long long)a.box.n[0] * rs.box_plane_bytes;
     if (MODE == 1) rs.save = make_rsrc(a.save, box_bytes);
     if (MODE == 2) {
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void k_fast3d_v2(Geo g, FastArgs a) {
         ln.row[r] = ((unsigned)(min(yr[r], ny + SA_RADIUS - 1) + SA_HALO) * s1 + (unsigned)zc) * 4u;
         ln.bin[r] = yr[r] >= a.box.lo[1] && yr[r] < a.box.lo[1] + a.box.n[1] && z >= a.box.lo[2] &&
                     z < a.box.lo[2] + a.box.n[2];
-        ln.brow[r] = ln.bin[r] ? (unsigned)((yr[r] - a.box.lo[1]) * a.box.n[2] + (z - a.box.lo[2])) * 4u : 0u;
+        ln.brow[r] = ln.bin[r] ? (unsigned)((yr[r] - a.box.lo[1]) * a.box.p2 + a.box.off2 + (z - a.box.lo[2])) * 4u : 0u;
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(NW * 64) void k_fast3d_v5(Geo g, FastArgs a) {
     rs.vp = make_rsrc(a.vp, bytes);
     rs.out = make_rsrc(a.out, bytes);
     rs.plane_bytes = (int)(g.s0 * 4);
-    rs.box_plane_bytes = a.box.n[1] * a.box.n[2] * 4;
+    rs.box_plane_bytes = a.box.n[1] * a.box.p2 * 4;
     const long long box_bytes = (long long)a.box.n[0] * rs.box_plane_bytes;
     if (MODE == 1) rs.save = make_rsrc(a.save, box_bytes);
     if (MODE == 2) {
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(NW * 64) void k_fast3d_v5(Geo g, FastArgs a) {
         orow[r] = rin ? row[r] : kOOB;
         const bool bin = rin && yr >= a.box.lo[1] && yr < a.box.lo[1] + a.box.n[1] && z >= a.box.lo[2] &&
                          z < a.box.lo[2] + a.box.n[2];
-        brow[r] = bin ? (unsigned)((yr - a.box.lo[1]) * a.box.n[2] + (z - a.box.lo[2])) * 4u : kOOB;
+        brow[r] = bin ? (unsigned)((yr - a.box.lo[1]) * a.box.p2 + a.box.off2 + (z - a.box.lo[2])) * 4u : kOOB;
         grow[r] = bin ? row[r] : kOOB;
         d1[r] = min(yr, ny - 1 - yr);
         t1v[r] = 0.f;
@@ -1011,7 +1011,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(RPT == 
     rs.vp = make_rsrc(a.vp, bytes);
     rs.out = make_rsrc(a.out, bytes);
     rs.plane_bytes = (int)(g.s0 * 4);
-    rs.box_plane_bytes = a.box.n[1] * a.box.n[2] * 4;
+    rs.box_plane_bytes = a.box.n[1] * a.box.p2 * 4;
     const long long box_bytes = (long long)a.box.n[0] * rs.box_plane_bytes;
     if (MODE == 1) rs.save = make_rsrc(a.save, box_bytes);
     if (MODE == 2) {
@@ -1031,7 +1031,7 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(RPT == 
         orow[r] = rin ? row[r] : kOOB;
         const bool bin = rin && yr >= a.box.lo[1] && yr < a.box.lo[1] + a.box.n[1] && z >= a.box.lo[2] &&
                          z < a.box.lo[2] + a.box.n[2];
-        brow[r] = bin ? (unsigned)((yr - a.box.lo[1]) * a.box.n[2] + (z - a.box.lo[2])) * 4u : kOOB;
+        brow[r] = bin ? (unsigned)((yr - a.box.lo[1]) * a.box.p2 + a.box.off2 + (z - a.box.lo[2])) * 4u : kOOB;
         grow[r] = bin ? row[r] : kOOB;
         d1[r] = min(yr, ny - 1 - yr);
         t1v[r] = 0.f;

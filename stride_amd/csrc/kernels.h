@@ -91,12 +91,17 @@ __device__ __forceinline__ float sa_dminus(const Geo &g, const float *__restrict
     return v;
 }
 
-struct Box {  // sub-box of the domain, C order; snapshots are stored in this layout
+// Sub-box of the domain, C order; snapshots are stored in this layout. Rows (axis 2) have a
+// pitch p2 that is a multiple of 32 floats and start off2 = lo[2] % 32 cells in, so that a
+// cell sits at the same 128-byte line offset as in the padded field layout and a 64-cell
+// segment of a row is two whole cache lines in both.
+struct Box {
     int lo[3], n[3];
+    int p2, off2;
 };
 
 __device__ __forceinline__ long long sa_boxidx(const Box &b, int i, int j, int k) {
-    return ((long long)(i - b.lo[0]) * b.n[1] + (j - b.lo[1])) * b.n[2] + (k - b.lo[2]);
+    return ((long long)(i - b.lo[0]) * b.n[1] + (j - b.lo[1])) * b.p2 + b.off2 + (k - b.lo[2]);
 }
 
 __device__ __forceinline__ bool sa_inbox(const Box &b, int i, int j, int k) {

@@ -233,9 +233,10 @@ __global__ void k_scale_copy(Geo g, const float *__restrict__ src, const float *
 }
 
 // sparse injection, deterministic: one thread per destination cell sums its contributions in
-// a fixed order. dest: unpadded linear index; coef per contribution; values[point].
-__global__ void k_inject(Geo g, int ndest, const long long *__restrict__ dest, const int *__restrict__ off,
-                         const int *__restrict__ point, const float *__restrict__ coef,
+// a fixed order. dest: padded index of each destination cell, ijk: its domain coordinates
+// (3 ints); coef per contribution; values[point].
+__global__ void k_inject(int ndest, const long long *__restrict__ dest, const int *__restrict__ ijk,
+                         const int *__restrict__ off, const int *__restrict__ point, const float *__restrict__ coef,
                          const float *__restrict__ values, float sign, float *field, float *save, Box box,
                          float save_scale) {
     int d = blockIdx.x * blockDim.x + threadIdx.x;
@@ -243,54 +244,45 @@ __global__ void k_inject(Geo g, int ndest, const long long *__restrict__ dest, c
     float acc = 0.f;
     for (int c = off[d]; c < off[d + 1]; ++c) acc += coef[c] * values[point[c]];
     acc *= sign;
-    long long lin = dest[d];
-    long long p = sa_lin2p(g, lin);
-    field[p] += acc;
+    field[dest[d]] += acc;
     if (save) {
-        int k = (int)(lin % g.n[2]);
-        long long r = lin / g.n[2];
-        int j = (int)(r % g.n[1]);
-        int i = (int)(r / g.n[1]);
+        const int i = ijk[3 * d], j = ijk[3 * d + 1], k = ijk[3 * d + 2];
         if (sa_inbox(box, i, j, k)) save[sa_boxidx(box, i, j, k)] += acc * save_scale;
     }
 }
 
-// rec[r] = sign * sum_q w * field (receiver interpolation, wavelet gradient)
-__global__ void k_gather(Geo g, int npt, int ntap, const long long *__restrict__ idx, const float *__restrict__ w,
+// rec[r] = sign * sum_q w * field (receiver interpolation, wavelet gradient); idx: padded
+// index per tap, negative for none
+__global__ void k_gather(int npt, int ntap, const long long *__restrict__ idx, const float *__restrict__ w,
                          const float *__restrict__ field, float sign, float *__restrict__ out) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= npt) return;
     float acc = 0.f;
     for (int q = 0; q < ntap; ++q) {
-        long long lin = idx[(long long)r * ntap + q];
-        if (lin < 0) continue;
-        acc += w[(long long)r * ntap + q] * field[sa_lin2p(g, lin)];
+        const long long p = idx[(long long)r * ntap + q];
+        if (p < 0) continue;
+        acc += w[(long long)r * ntap + q] * field[p];
     }
     out[r] = sign * acc;
 }
 
-// per-tap values field / G (exact adjoint: lambda at each source tap)
-__global__ void k_gather_taps(Geo g, int nq, const long long *__restrict__ idx, const float *__restrict__ field,
+// per-tap values field / G (exact adjoint: lambda at each source tap); idx padded
+__global__ void k_gather_taps(int nq, const long long *__restrict__ idx, const float *__restrict__ field,
                               const float *__restrict__ G, float *__restrict__ out) {
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
-    long long lin = idx[q];
-    if (lin < 0) {
-        out[q] = 0.f;
-        return;
-    }
-    long long p = sa_lin2p(g, lin);
-    out[q] = field[p] / G[p];
+    const long long p = idx[q];
+    out[q] = p < 0 ? 0.f : field[p] / G[p];
 }
 
 // injection coefficients per contribution: 0 source (w 2 dt^k vp / h), 1 compat adjoint
-// (w dt^2 vp^2), 2 exact adjoint (w G)
-__global__ void k_tap_coef(Geo g, int nnz, const long long *__restrict__ lin, const float *__restrict__ w,
+// (w dt^2 vp^2), 2 exact adjoint (w G); pidx: padded index per contribution
+__global__ void k_tap_coef(int nnz, const long long *__restrict__ pidx, const float *__restrict__ w,
                            const float *__restrict__ vp, const float *__restrict__ G, int kind, float scale,
                            float dt2, float *__restrict__ out) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nnz) return;
-    long long p = sa_lin2p(g, lin[c]);
+    const long long p = pidx[c];
     float v = vp[p];
     if (kind == 0) out[c] = w[c] * scale * v;
     else if (kind == 1) out[c] = w[c] * dt2 * v * v;
@@ -385,13 +377,14 @@ struct DevBuf {
 
 struct Sparse {
     int npoint = 0, ntap = 0;
-    std::vector<long long> idx;  // host copy
+    std::vector<long long> idx;  // host copy (extended-grid linear indices, -1: none)
     std::vector<float> w;
-    long long *d_idx = nullptr;
+    long long *d_idx = nullptr;  // padded indices per tap (-1: none)
     float *d_w = nullptr;
     // deterministic scatter: destinations sorted, CSR of contributions
     int ndest = 0, nnz = 0;
-    long long *d_dest = nullptr, *d_lin = nullptr;
+    long long *d_dest = nullptr, *d_lin = nullptr;  // padded index per destination / contribution
+    int *d_ijk = nullptr;                           // domain coordinates per destination
     int *d_off = nullptr, *d_point = nullptr;
     float *d_cw = nullptr;    // raw weights per contribution
     float *d_coef = nullptr;  // scaled coefficients per contribution
@@ -593,6 +586,7 @@ extern "C" int sa_create(const sa_grid *grid, sa_solver **out) {
 
 static void free_sparse(Sparse &sp) {
     dfree(sp.d_idx); dfree(sp.d_w); dfree(sp.d_dest); dfree(sp.d_lin); dfree(sp.d_off); dfree(sp.d_point);
+    dfree(sp.d_ijk);
     dfree(sp.d_cw); dfree(sp.d_coef);
     sp = Sparse();
 }
@@ -712,9 +706,18 @@ static int set_sparse(sa_solver *s, Sparse &sp, int npoint, int ntap, const int6
     for (size_t q = 0; q < nq; ++q)
         if (sp.idx[q] >= s->npts) return fail(SA_E_ARG, "tap index %lld outside the grid", (long long)sp.idx[q]);
     if (nq == 0) return SA_OK;
+    // padded indices, computed once here so that the per-step kernels do no 64-bit division
+    const Geo &g = s->geo;
+    auto padded = [&](long long lin) -> long long {
+        if (lin < 0) return -1;
+        const long long k = lin % g.n[2], r = lin / g.n[2];
+        return g.base + (r / g.n[1]) * g.s0 + (r % g.n[1]) * g.s1 + k;
+    };
+    std::vector<long long> pidx(nq);
+    for (size_t q = 0; q < nq; ++q) pidx[q] = padded(sp.idx[q]);
     HIP_TRY(hipMalloc((void **)&sp.d_idx, nq * sizeof(long long)));
     HIP_TRY(hipMalloc((void **)&sp.d_w, nq * sizeof(float)));
-    HIP_TRY(hipMemcpy(sp.d_idx, sp.idx.data(), nq * sizeof(long long), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sp.d_idx, pidx.data(), nq * sizeof(long long), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(sp.d_w, sp.w.data(), nq * sizeof(float), hipMemcpyHostToDevice));
     // destination-sorted CSR for the deterministic scatter (stable order of contributions)
     std::vector<size_t> ord;
@@ -722,16 +725,20 @@ static int set_sparse(sa_solver *s, Sparse &sp, int npoint, int ntap, const int6
         if (sp.idx[q] >= 0) ord.push_back(q);
     std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return sp.idx[a] < sp.idx[b]; });
     std::vector<long long> dest, lin;
-    std::vector<int> offv, point;
+    std::vector<int> offv, point, ijk;
     std::vector<float> cw;
     for (size_t c = 0; c < ord.size(); ++c) {
         size_t q = ord[c];
         if (c == 0 || sp.idx[q] != sp.idx[ord[c - 1]]) {
-            dest.push_back(sp.idx[q]);
+            const long long l = sp.idx[q];
+            dest.push_back(pidx[q]);
+            ijk.push_back((int)(l / ((long long)g.n[1] * g.n[2])));
+            ijk.push_back((int)((l / g.n[2]) % g.n[1]));
+            ijk.push_back((int)(l % g.n[2]));
             offv.push_back((int)c);
         }
         point.push_back((int)(q / ntap));
-        lin.push_back(sp.idx[q]);
+        lin.push_back(pidx[q]);
         cw.push_back(sp.w[q]);
     }
     offv.push_back((int)ord.size());
@@ -744,6 +751,8 @@ static int set_sparse(sa_solver *s, Sparse &sp, int npoint, int ntap, const int6
     HIP_TRY(hipMalloc((void **)&sp.d_point, point.size() * sizeof(int)));
     HIP_TRY(hipMalloc((void **)&sp.d_cw, cw.size() * sizeof(float)));
     HIP_TRY(hipMalloc((void **)&sp.d_coef, cw.size() * sizeof(float)));
+    HIP_TRY(hipMalloc((void **)&sp.d_ijk, ijk.size() * sizeof(int)));
+    HIP_TRY(hipMemcpy(sp.d_ijk, ijk.data(), ijk.size() * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(sp.d_dest, dest.data(), dest.size() * sizeof(long long), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(sp.d_lin, lin.data(), lin.size() * sizeof(long long), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(sp.d_off, offv.data(), offv.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -934,7 +943,7 @@ static int coef_sparse(sa_solver *s, Sparse &sp, int kind) {
     if (sp.nnz == 0) return SA_OK;
     float scale = s->sch.diff_source ? (float)(2.0 * s->grid.dt * s->grid.dt / s->hmax)
                                      : (float)(2.0 * s->grid.dt / s->hmax);
-    k_tap_coef<<<(sp.nnz + 255) / 256, 256, 0, s->stream>>>(s->geo, sp.nnz, sp.d_lin, sp.d_cw, s->vp, s->G, kind,
+    k_tap_coef<<<(sp.nnz + 255) / 256, 256, 0, s->stream>>>(sp.nnz, sp.d_lin, sp.d_cw, s->vp, s->G, kind,
                                                             scale, s->dt * s->dt, sp.d_coef);
     CHECK_LAUNCH();
     return SA_OK;
@@ -943,7 +952,7 @@ static int coef_sparse(sa_solver *s, Sparse &sp, int kind) {
 static int inject(sa_solver *s, Sparse &sp, const float *values, float sign, float *field, float *save,
                   float save_scale) {
     if (sp.ndest == 0) return SA_OK;
-    k_inject<<<(sp.ndest + 127) / 128, 128, 0, s->stream>>>(s->geo, sp.ndest, sp.d_dest, sp.d_off, sp.d_point,
+    k_inject<<<(sp.ndest + 127) / 128, 128, 0, s->stream>>>(sp.ndest, sp.d_dest, sp.d_ijk, sp.d_off, sp.d_point,
                                                             sp.d_coef, values, sign, field, save, s->box, save_scale);
     CHECK_LAUNCH();
     return SA_OK;
@@ -951,7 +960,7 @@ static int inject(sa_solver *s, Sparse &sp, const float *values, float sign, flo
 
 static int gather(sa_solver *s, Sparse &sp, const float *field, float sign, float *out) {
     if (sp.npoint == 0) return SA_OK;
-    k_gather<<<(sp.npoint + 127) / 128, 128, 0, s->stream>>>(s->geo, sp.npoint, sp.ntap, sp.d_idx, sp.d_w, field,
+    k_gather<<<(sp.npoint + 127) / 128, 128, 0, s->stream>>>(sp.npoint, sp.ntap, sp.d_idx, sp.d_w, field,
                                                              sign, out);
     CHECK_LAUNCH();
     return SA_OK;
@@ -984,13 +993,13 @@ static int alloc_saved(sa_solver *s) {
             b.n[a] = g.n[a] - 2 * g.nabs[a];
         }
     }
+    b.off2 = b.lo[2] % 32;
+    b.p2 = (b.off2 + b.n[2] + 31) / 32 * 32;
     int nsave = c.tb1 / f + 1;
     if (c.save_mode == 4) nsave = 2 * ((c.tb1 - 2) / f + 1);  // checkpoint pairs
-    size_t per = (size_t)b.n[0] * b.n[1] * b.n[2];
+    size_t per = (size_t)b.n[0] * b.n[1] * b.p2;
     size_t bytes = per * nsave * sizeof(float);
-    if (s->saved && s->saved_bytes >= bytes) {
-        HIP_TRY(hipMemsetAsync(s->saved, 0, bytes, s->stream));
-    } else {
+    if (!s->saved || s->saved_bytes < bytes) {
         dfree(s->saved);
         s->saved = nullptr;
         s->saved_bytes = 0;
@@ -999,6 +1008,23 @@ static int alloc_saved(sa_solver *s) {
             return fail(SA_E_NOMEM, "cannot allocate %.2f GB for the saved wavefield", bytes / 1e9);
         }
         s->saved_bytes = bytes;
+    }
+    // Slots the forward run never writes read as zero (as the reference's zero-initialised save
+    // buffer). Every-f saves (modes 1, 2) write every box point of the slots t / f for
+    // t % f == 0, max(1, tb0) <= t <= tb1 - 2, so only the slots outside that run are cleared;
+    // the other modes clear the whole buffer.
+    if (c.save_mode == 1 || c.save_mode == 2) {
+        const int first = (std::max(1, c.tb0) + f - 1) / f;
+        const int last = std::min((c.tb1 - 2) / f, nsave - 1);
+        if (first > last) {
+            HIP_TRY(hipMemsetAsync(s->saved, 0, bytes, s->stream));
+        } else {
+            if (first > 0) HIP_TRY(hipMemsetAsync(s->saved, 0, per * first * sizeof(float), s->stream));
+            if (last + 1 < nsave)
+                HIP_TRY(hipMemsetAsync(s->saved + per * (size_t)(last + 1), 0, per * (nsave - 1 - last) * sizeof(float),
+                                       s->stream));
+        }
+    } else {
         HIP_TRY(hipMemsetAsync(s->saved, 0, bytes, s->stream));
     }
     s->box = b;
@@ -1022,7 +1048,8 @@ static int alloc_saved(sa_solver *s) {
     return SA_OK;
 }
 
-static size_t box_points(const Box &b) { return (size_t)b.n[0] * b.n[1] * b.n[2]; }
+// floats per snapshot slot (rows padded to the pitch)
+static size_t box_points(const Box &b) { return (size_t)b.n[0] * b.n[1] * b.p2; }
 
 extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
     if (!s) return fail(SA_E_ARG, "null solver");
@@ -1308,7 +1335,7 @@ extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float
             utm = s->saved + (size_t)(t - 1) * per;
         }
         if (nq) {
-            k_gather_taps<<<((int)nq + 127) / 128, 128, 0, s->stream>>>(g, (int)nq, s->src.d_idx, m1, s->G,
+            k_gather_taps<<<((int)nq + 127) / 128, 128, 0, s->stream>>>((int)nq, s->src.d_idx, m1, s->G,
                                                                          s->d_out + (size_t)t * nq);
         }
         k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, ut, s->work0);
