@@ -1,11 +1,12 @@
 """Benchmark: 3D 320^3 (360^3 extended) SO10 OT2 forward + adjoint vp gradient.
 
-One step = one shot: the state run over nt time steps saving p.dt2 every 4th step
-(interior), then the as-written adjoint accumulating the vp gradient and
-preconditioner on the guarded steps -- the hot path of IsoAcousticDevito's
-forward/adjoint with vp.needs_grad. Inputs (medium, taps, source samples,
-residual) are resident in HBM before the timed region; nothing crosses PCIe
-inside it.
+One step = one batch of shots (default 2, `--shots 1` for one): the state run over nt
+time steps saving p.dt2 every 4th step (interior), then the as-written adjoint
+accumulating the vp gradient and preconditioner on the guarded steps -- the hot path of
+IsoAcousticDevito's forward/adjoint with vp.needs_grad. Two shots advance together
+through one update kernel (interleaved wavefields, one vp read, one gradient
+read-modify-write for both). Inputs (medium, taps, source samples, residuals) are
+resident in HBM before the timed region; nothing crosses PCIe inside it.
 
 Prints one JSON line (rank 0) with the headline Gpts/s (extended-grid point
 updates per second, forward and adjoint updates counted separately), the
@@ -37,6 +38,30 @@ def workload(scale=1.0, nt=2500, nrec=1024, undersampling=4):
     return cfg, plan
 
 
+def second_shot(cfg, plan):
+    """The second shot of a two-shot step: the source at the receiver diametrically across the
+    hemisphere from the first source, the same 1024 receivers."""
+    from stride_amd.physics.plan import build_plan
+    rec = cfg['receivers']
+    d = np.linalg.norm(rec - cfg['sources'][0], axis=1)
+    return build_plan(cfg['space'], cfg['time'], cfg['wavelet'][None], cfg['vp'],
+                      source_coordinates=rec[int(np.argmax(d))][None], receiver_coordinates=rec,
+                      kernel='OT2', save_undersampling=plan.undersampling, grad_vp=True)
+
+
+def make_batch_solver(plan, plan1):
+    from stride_amd._native import Solver
+    s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, plan.nt)
+    s.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
+                 plan.undersampling, 1, plan.tb0, plan.tb1, 1, 0, 0)
+    s.set_medium(plan.vp)
+    s.set_sponge(plan.sponge_tables)
+    s.batch_create()
+    for k, p in enumerate((plan, plan1)):
+        s.batch_set_taps(k, p.src_idx, p.src_w, p.rec_idx, p.rec_w)
+    return s
+
+
 def make_solver(plan):
     from stride_amd._native import Solver
     s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, plan.nt)
@@ -49,11 +74,13 @@ def make_solver(plan):
     return s
 
 
-def algorithmic_bytes(plan):
-    """Bytes the update kernel must move per shot: 16 B per point update (read u, other
-    level, vp; write the new level); +4 B per interior point on save steps (the p.dt2
-    snapshot); +20 B per interior point on gradient steps (read the snapshot, read and
-    write grad and prec)."""
+def algorithmic_bytes(plan, shots=1):
+    """Bytes the update kernel must move per step of `shots` shots advanced together.
+    One shot: 16 B per point update (read u, other level, vp; write the new level); +4 B per
+    interior point on save steps (the p.dt2 snapshot); +20 B per interior point on gradient
+    steps (read the snapshot, read and write grad and prec). Two interleaved shots: 28 B per
+    point and pass (u, other level, new level of both shots, one vp); +8 B on save steps;
+    +24 B on gradient steps (both snapshots, one grad and one prec read-modify-write)."""
     n = int(np.prod(plan.shape))
     interior = int(np.prod([s - 2 * a for s, a in zip(plan.shape, plan.absorbing)]))
     f = plan.undersampling
@@ -61,7 +88,10 @@ def algorithmic_bytes(plan):
     adj_steps = plan.tb1 - 1 - f - plan.tb0
     saves = sum(1 for t in range(1, plan.tb1 - 1) if t % f == 0)
     grads = sum(1 for t in range(plan.tb1 - 1 - f, plan.tb0, -1) if t % f == 0)
-    total = 16 * n * (fwd_steps + adj_steps) + 4 * interior * saves + 20 * interior * grads
+    if shots == 1:
+        total = 16 * n * (fwd_steps + adj_steps) + 4 * interior * saves + 20 * interior * grads
+    else:
+        total = 28 * n * (fwd_steps + adj_steps) + 8 * interior * saves + 24 * interior * grads
     return total, fwd_steps, adj_steps, n, interior
 
 
@@ -97,8 +127,8 @@ def cpu_baseline(plan_full, seconds_budget=20.0):
                        'as-written adjoint steps with vp gradient, %.1f s' % (fwd, adj, t2 - t0))
 
 
-def traffic_from_profile():
-    path = os.path.join(ROOT, 'profiles', 'fast3d_traffic.json')
+def traffic_from_profile(shots=1):
+    path = os.path.join(ROOT, 'profiles', 'fast3d_traffic.json' if shots == 1 else 'fast3d_b2_traffic.json')
     if not os.path.exists(path):
         return None
     try:
@@ -115,7 +145,10 @@ def main():
     ap.add_argument('--scale', type=float, default=1.0, help='grid scale (1.0 = 320^3 inner)')
     ap.add_argument('--nt', type=int, default=2500)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--tile', type=str, default=None, help='rows,chunk,version of the 3D update tile')
+    ap.add_argument('--tile', type=str, default=None,
+                    help='rows,chunk,version of the 3D update tile (rows,chunk,waves with --shots 2)')
+    ap.add_argument('--shots', type=int, default=2, choices=(1, 2),
+                    help='shots advanced together per step (2: interleaved wavefields, shared vp reads)')
     ap.add_argument('--no-autotune', action='store_true')
     ap.add_argument('--copy-check', type=int, default=0, metavar='MB',
                     help='only run 10 device-to-device copies of MB megabytes (calibrates the byte counters '
@@ -144,49 +177,61 @@ def main():
     cfg, plan = workload(scale=args.scale, nt=args.nt)
     print('[bench] workload %s ext, nt %d, f %d, nrec %d' % (plan.shape, plan.nt, plan.undersampling, plan.nrec),
           file=sys.stderr, flush=True)
+    shots = args.shots
+    plan1 = second_shot(cfg, plan) if shots == 2 else None
     tile = None
     if args.tile:
         tile = tuple(int(v) for v in args.tile.split(','))
     elif not args.no_autotune:
-        from stride_amd.autotune import autotune
+        from stride_amd.autotune import autotune, autotune_batch
         t0 = time.perf_counter()
-        tile, table = autotune(plan)
-        print('[bench] autotune %.1f s: best rows=%d chunk=%d version=%d (%.1f us/launch); worst %.1f us'
-              % ((time.perf_counter() - t0,) + tuple(tile) + (table[0][0] * 1e3, table[-1][0] * 1e3)),
-              file=sys.stderr, flush=True)
-    s = make_solver(plan)
-    if tile:
-        s.set_tile(*tile)
-    # device-resident inputs
-    nbytes_src = plan.src_data.nbytes
+        tile, table = autotune_batch(plan, plan1) if shots == 2 else autotune(plan)
+        print('[bench] autotune %.1f s: best %s (%.1f us/launch); worst %.1f us'
+              % (time.perf_counter() - t0, tile, table[0][0] * 1e3, table[-1][0] * 1e3), file=sys.stderr, flush=True)
     rng = np.random.default_rng(0)
-    res = rng.standard_normal((plan.nt, plan.nrec)).astype(np.float32) * 1e-3
-    d_src = s.device_alloc(nbytes_src)
-    d_rec = s.device_alloc(plan.nt * plan.nrec * 4)
-    d_res = s.device_alloc(res.nbytes)
-    s.memcpy(d_src, plan.src_data.ctypes.data, nbytes_src, 1)
-    s.memcpy(d_res, res.ctypes.data, res.nbytes, 1)
+    if shots == 2:
+        s = make_batch_solver(plan, plan1)
+        if tile:
+            s.batch_set_tile(*tile)
+    else:
+        s = make_solver(plan)
+        if tile:
+            s.set_tile(*tile)
+    # device-resident inputs: source samples and residuals of each shot
+    plans = (plan, plan1) if shots == 2 else (plan,)
+    d_src, d_rec, d_res = [], [], []
+    for p in plans:
+        res = rng.standard_normal((p.nt, p.nrec)).astype(np.float32) * 1e-3
+        d_src.append(s.device_alloc(p.src_data.nbytes))
+        d_rec.append(s.device_alloc(p.nt * p.nrec * 4))
+        d_res.append(s.device_alloc(res.nbytes))
+        s.memcpy(d_src[-1], p.src_data.ctypes.data, p.src_data.nbytes, 1)
+        s.memcpy(d_res[-1], res.ctypes.data, res.nbytes, 1)
 
-    def one_shot():
-        s.forward_dev(d_src, d_rec)
-        s.adjoint_dev(d_res, want=1)
+    def one_step():
+        if shots == 2:
+            s.batch_forward_dev(d_src, d_rec)
+            s.batch_adjoint_dev(d_res)
+        else:
+            s.forward_dev(d_src[0], d_rec[0])
+            s.adjoint_dev(d_res[0], want=1)
 
     for _ in range(args.warmup):
-        one_shot()
+        one_step()
     s.synchronize()
     s.timing(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
-        one_shot()
+        one_step()
         print('[bench] step %d issued' % k, file=sys.stderr, flush=True)
     s.synchronize()
     t1 = time.perf_counter()
     kernel_ms, launches = s.timing()
     s.timing(False)
 
-    total_bytes, fwd_steps, adj_steps, n, interior = algorithmic_bytes(plan)
+    total_bytes, fwd_steps, adj_steps, n, interior = algorithmic_bytes(plan, shots)
     wall = (t1 - t0) / args.steps
-    updates = n * (fwd_steps + adj_steps)
+    updates = shots * n * (fwd_steps + adj_steps)
     gpts = updates / wall / 1e9
     avg_launch_s = kernel_ms / 1e3 / max(launches, 1)
     bytes_per_launch = total_bytes / (fwd_steps + adj_steps)
@@ -209,14 +254,20 @@ def main():
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': 'f32',
-        'data': 'synthetic (smooth random vp in a 55 mm hemisphere, seeds 3001/3002)',
+        'data': 'synthetic (smooth random vp in a 55 mm hemisphere, seeds 3001/3002; second shot sourced at the '
+                'receiver farthest from the first source)',
         'config': {
             'workload': '3D %dx%dx%d inner (%dx%dx%d extended), dx 0.4 mm, SO10 OT2, sponge %d, nt %d, '
-                        'fwd with p.dt2 save every %d + as-written adjoint with vp gradient, 1 source, %d receivers, '
-                        'one shot per step' % (tuple(cfg['space'].shape) + tuple(plan.shape) +
-                                               (plan.absorbing[0], plan.nt, plan.undersampling, plan.nrec)),
+                        'fwd with p.dt2 save every %d + as-written adjoint with vp gradient, 1 source and %d '
+                        'receivers per shot, %s' % (
+                            tuple(cfg['space'].shape) + tuple(plan.shape) +
+                            (plan.absorbing[0], plan.nt, plan.undersampling, plan.nrec,
+                             'one shot per step' if shots == 1 else
+                             '2 shots per step advanced together (interleaved wavefields, shared vp reads, '
+                             'summed vp gradient)')),
+            'shots_per_step': shots,
             'point_updates_per_step': updates,
-            'inner_gpts': round(interior * (fwd_steps + adj_steps) / wall / 1e9, 3),
+            'inner_gpts': round(shots * interior * (fwd_steps + adj_steps) / wall / 1e9, 3),
             'hbm_roofline_frac_of_value': round(gpts * 1e9 * (total_bytes / updates) / 1e9 / HBM_PEAK_GBS, 4),
         },
         'roofline': {
@@ -225,8 +276,8 @@ def main():
             'peak': HBM_PEAK_GBS,
             'unit': 'GB/s',
             'frac': round(achieved / HBM_PEAK_GBS, 4),
-            'traffic': traffic_from_profile(),
-            'kernel': 'k_fast3d (wavefield update)',
+            'traffic': traffic_from_profile(shots),
+            'kernel': 'k_fast3d_b2 (two-shot wavefield update)' if shots == 2 else 'k_fast3d (wavefield update)',
             'tile': list(tile) if tile else None,
             'avg_launch_us': round(avg_launch_s * 1e6, 2),
             'launches': launches,
@@ -238,7 +289,7 @@ def main():
             out['cpu_baseline'] = cpu_baseline(plan)
         except Exception as exc:  # the baseline is reported, never required
             out['cpu_baseline'] = {'error': repr(exc)}
-    for p in (d_src, d_rec, d_res):
+    for p in d_src + d_rec + d_res:
         s.device_free(p)
     s.close()
     print(json.dumps(out))

@@ -153,6 +153,27 @@ int sa_device_free(sa_solver *s, void *ptr);
 int sa_memcpy(sa_solver *s, void *dst, const void *src, int64_t bytes, int kind /* 1 h2d, 2 d2h, 3 d2d */);
 int sa_synchronize(sa_solver *s);
 
+/* Two shots per pass. No counterpart in the reference, which runs one shot per worker call
+ * (`IsoAcousticDevito.forward/adjoint`, stride/physics/iso_acoustic/devito.py:515-581, 875-920;
+ * shots dispatched by stride/__init__.py:110-141, 325-334) and sums per-shot gradients in the
+ * optimisation loop (stride/core.py:565-633). Two shots on the same grid, scheme, medium and
+ * boundary advance together: their wavefields are interleaved per cell so one pass of the
+ * specialised 3D update (3D, OT2, no density, sponge or no boundary, no attenuation) serves
+ * both with one read of vp, and the adjoint accumulates the SUM of the two shots' vp
+ * gradients and preconditioners (read with sa_read_gradient 0 / 1). Per-shot taps, samples,
+ * traces and residuals. Other schemes return SA_E_UNSUPPORTED. */
+int sa_batch_create(sa_solver *s, int nshot /* 2 */);
+int sa_batch_set_taps(sa_solver *s, int shot, int nsrc, int src_ntap, const int64_t *src_index,
+                      const float *src_weight, int nrec, int rec_ntap, const int64_t *rec_index,
+                      const float *rec_weight);
+/* src[k]: device (nt, nsrc_k) source samples of shot k; rec[k]: device (nt, nrec_k) traces */
+int sa_batch_forward_dev(sa_solver *s, const float *const *src, float *const *rec);
+/* res[k]: device (nt, nrec_k) adjoint source of shot k, as for sa_adjoint_dev */
+int sa_batch_adjoint_dev(sa_solver *s, const float *const *res);
+int sa_batch_get_wavefield(sa_solver *s, int shot, int which, float *out);
+/* tile of the two-shot update: waves 8 with rows 1 or 2, or waves 4 with rows 2 */
+int sa_batch_set_tile(sa_solver *s, int rows, int chunk, int waves);
+
 #ifdef __cplusplus
 }
 #endif

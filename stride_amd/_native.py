@@ -42,7 +42,9 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_set_medium', 'sa_set_sponge', 'sa_set_pml', 'sa_set_sources', 'sa_set_receivers',
            'sa_forward', 'sa_forward_dev', 'sa_adjoint', 'sa_adjoint_dev', 'sa_read_gradient',
            'sa_adjoint_exact', 'sa_get_wavefield', 'sa_free_saved', 'sa_set_tile', 'sa_timing_enable', 'sa_timing_read',
-           'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize']
+           'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize', 'sa_batch_create',
+           'sa_batch_set_taps', 'sa_batch_forward_dev', 'sa_batch_adjoint_dev', 'sa_batch_get_wavefield',
+           'sa_batch_set_tile']
 
 _lib = None
 
@@ -205,6 +207,70 @@ class Solver:
 
     def free_saved(self):
         self._check(self._lib.sa_free_saved(self._h))
+
+    # ---- two shots per pass (interleaved wavefields, shared vp, summed vp gradient) ----
+    def batch_create(self):
+        self._check(self._lib.sa_batch_create(self._h, ctypes.c_int(2)))
+        self._batch = [dict(), dict()]
+
+    def batch_set_taps(self, shot, src_idx, src_w, rec_idx, rec_w):
+        si = np.ascontiguousarray(src_idx, dtype=np.int64)
+        ri = np.ascontiguousarray(rec_idx, dtype=np.int64)
+        sw, rw = _f32(src_w), _f32(rec_w)
+        self._check(self._lib.sa_batch_set_taps(self._h, ctypes.c_int(shot), ctypes.c_int(si.shape[0]),
+                                                ctypes.c_int(si.shape[1]), _p(si), _p(sw), ctypes.c_int(ri.shape[0]),
+                                                ctypes.c_int(ri.shape[1]), _p(ri), _p(rw)))
+        self._batch[shot] = dict(nsrc=si.shape[0], nrec=ri.shape[0])
+
+    def batch_set_tile(self, rows, chunk, waves):
+        self._check(self._lib.sa_batch_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk), ctypes.c_int(waves)))
+
+    def batch_forward_dev(self, src_ptrs, rec_ptrs):
+        sp = (ctypes.c_void_p * 2)(*src_ptrs)
+        rp = (ctypes.c_void_p * 2)(*rec_ptrs)
+        self._check(self._lib.sa_batch_forward_dev(self._h, sp, rp))
+
+    def batch_adjoint_dev(self, res_ptrs):
+        rp = (ctypes.c_void_p * 2)(*res_ptrs)
+        self._check(self._lib.sa_batch_adjoint_dev(self._h, rp))
+
+    def _to_device(self, a):
+        a = _f32(a)
+        d = self.device_alloc(max(a.nbytes, 4))
+        self.memcpy(d, a.ctypes.data, a.nbytes, 1)
+        return d
+
+    def batch_forward(self, srcs):
+        """Host convenience: source samples (nt, nsrc_k) of the two shots -> their traces (nt, nrec_k)."""
+        ds = [self._to_device(srcs[k]) for k in range(2)]
+        recs = [np.zeros((self.nt, self._batch[k]['nrec']), np.float32) for k in range(2)]
+        dr = [self.device_alloc(max(r.nbytes, 4)) for r in recs]
+        try:
+            self.batch_forward_dev(ds, dr)
+            self.synchronize()
+            for k in range(2):
+                self.memcpy(recs[k].ctypes.data, dr[k], recs[k].nbytes, 2)
+        finally:
+            for d in ds + dr:
+                self.device_free(d)
+        return recs
+
+    def batch_adjoint(self, residuals):
+        """Host convenience: adjoint sources (nt, nrec_k) of the two shots -> summed vp gradient
+        and preconditioner (extended grid, as read_gradient(0) / read_gradient(1))."""
+        dres = [self._to_device(residuals[k]) for k in range(2)]
+        try:
+            self.batch_adjoint_dev(dres)
+            self.synchronize()
+        finally:
+            for d in dres:
+                self.device_free(d)
+        return dict(grad_vp=self.read_gradient(0), prec_vp=self.read_gradient(1))
+
+    def batch_wavefield(self, shot, which=0):
+        out = np.zeros(self.shape, np.float32)
+        self._check(self._lib.sa_batch_get_wavefield(self._h, ctypes.c_int(shot), ctypes.c_int(which), _p(out)))
+        return out
 
     def set_tile(self, rows, chunk, version=2):
         self._check(self._lib.sa_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk), ctypes.c_int(version)))

@@ -13,7 +13,7 @@ import numpy as np
 
 from ._native import Solver
 
-__all__ = ['candidates', 'autotune']
+__all__ = ['candidates', 'autotune', 'candidates_batch', 'autotune_batch']
 
 _CACHE = {}
 
@@ -55,6 +55,46 @@ def autotune(plan, steps=24, device=0, verbose=False, cands=None):
         if verbose:
             print('tile rows=%d chunk=%d version=%d: %.1f us/launch' % (rows, chunk, version, 1e3 * ms / max(n, 1)),
                   flush=True)
+    s.close()
+    results.sort()
+    best = results[0][1:]
+    _CACHE[key] = (best, results)
+    return best, results
+
+
+def candidates_batch(shape):
+    nx = shape[0]
+    chunks = sorted({12 * max(1, -(-max(1, -(-nx // k)) // 12)) for k in (3, 4, 5, 6, 8, 10, 12, 16)})
+    return [(rows, chunk, waves) for (rows, waves) in ((1, 8), (2, 8), (2, 4)) for chunk in chunks]
+
+
+def autotune_batch(plan, plan1, steps=24, device=0, verbose=False, cands=None):
+    """As ``autotune`` for the two-shot update: (best (rows, chunk, waves), results)."""
+    key = ('b2', plan.shape, plan.absorbing, plan.boundary)
+    if key in _CACHE:
+        return _CACHE[key]
+    nt = steps + 2
+    s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, nt, device=device)
+    s.set_scheme(2, 0, -2, plan.boundary, 0, 4, 0, 0, nt)
+    s.set_medium(plan.vp)
+    if plan.sponge_tables is not None:
+        s.set_sponge(plan.sponge_tables)
+    s.batch_create()
+    for k, p in enumerate((plan, plan1)):
+        s.batch_set_taps(k, p.src_idx, p.src_w, p.rec_idx[:1], p.rec_w[:1])
+    src = [np.ones((nt, p.nsrc), np.float32) for p in (plan, plan1)]
+    results = []
+    for rows, chunk, waves in (cands or candidates_batch(plan.shape)):
+        s.batch_set_tile(rows, chunk, waves)
+        s.batch_forward(src)  # warm
+        s.timing(True)
+        s.batch_forward(src)
+        ms, n = s.timing()
+        s.timing(False)
+        results.append((ms / max(n, 1), rows, chunk, waves))
+        if verbose:
+            print('two-shot tile rows=%d chunk=%d waves=%d: %.1f us/launch' % (rows, chunk, waves,
+                                                                              1e3 * ms / max(n, 1)), flush=True)
     s.close()
     results.sort()
     best = results[0][1:]

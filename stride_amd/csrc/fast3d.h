@@ -26,6 +26,7 @@ struct Fast3D {
     const float *vp = nullptr;
     float dt = 0.f;
     FastTune tune = {2, 24, 7};
+    FastTune tune2 = {2, 24, 8};  // two-shot kernel: rows, chunk, waves (4 or 8)
     bool prepared = false;
 };
 
@@ -41,3 +42,10 @@ int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o,
 void fast3d_free(Fast3D &f);
 // tile shape used by later launches (autotuner)
 int fast3d_set_tune(Fast3D &f, int rows, int chunk, int version);
+// Two shots per pass: u, o, out, save and ps hold two interleaved shots per cell (8 bytes per
+// padded cell / box cell); vp, grad and prec are shared (the gradient sums both shots).
+int fast3d_step_b2(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,
+                   float idt2, hipStream_t stream);
+int fast3d_step_adjoint_b2(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, const float *ps,
+                           const Box &box, float *grad, float *prec, hipStream_t stream);
+int fast3d_set_tune_b2(Fast3D &f, int rows, int chunk, int waves);

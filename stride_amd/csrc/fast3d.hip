@@ -1061,6 +1061,349 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(RPT == 
         march7<RPT, NW, MODE, false, NS>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2, t2v);
 }
 
+// ---------------------------------------------------------------------------------------
+// Two shots per pass (the batched entry points): the two shots' wavefields are interleaved
+// per cell (8 bytes per padded cell, shot-minor), vp and the gradient / preconditioner are
+// shared. Structure as version 7 (LDS plane ring by direct-to-LDS loads, u read once, axis-0
+// terms pushed into partial sums), with every per-cell quantity a pair. Per update this
+// moves 14 B instead of 16 (the vp read is shared), and on gradient steps the 16 B of
+// gradient and preconditioner read-modify-write are shared by the two shots.
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ f2 bload2(rsrc_t r, unsigned voff, int soff) {
+    return __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ f2 bload2_nt(rsrc_t r, unsigned voff, int soff) {
+    return __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, kNT));
+}
+
+__device__ __forceinline__ void bstore2_nt(f2 v, rsrc_t r, unsigned voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, v), r, voff, soff, kNT);
+}
+
+template <int TY>
+struct Ring2 {                               // LDS plane slot of pairs
+    static constexpr int ROWS = TY + 10;
+    static constexpr int CPR = 40;           // 16-byte chunks per row: cells z0 - 8 .. z0 + 71
+    static constexpr int PITCH = 80;         // cells (pairs) per LDS row
+    static constexpr int NCH = ROWS * CPR;
+    static constexpr int NI = (NCH + 63) / 64;
+};
+
+template <int TY, int NW>
+struct Ring2W : Ring2<TY> {
+    static constexpr int KI = (Ring2<TY>::NI + NW - 1) / NW;
+    static constexpr int FLOATS = KI * NW * 256;
+};
+
+template <int RPT, int MODE>
+struct StageB {
+    f2 o[RPT];
+    float v[RPT];
+    f2 ps[MODE == 2 ? RPT : 1];
+    float gr[MODE == 2 ? RPT : 1], pr[MODE == 2 ? RPT : 1];
+};
+
+template <int RPT, int MODE>
+__device__ __forceinline__ void load_stageB(StageB<RPT, MODE> &st, const Rsrc &rs, const unsigned (&row)[RPT],
+                                            const unsigned (&grow)[RPT], const unsigned (&brow)[RPT], int x,
+                                            const FastArgs &a) {
+    const int sp = (x + SA_HALO) * rs.plane_bytes;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        st.o[r] = bload2_nt(rs.o, 2 * row[r], 2 * sp);
+        st.v[r] = bload_nt(rs.vp, row[r], sp);
+    }
+    if (MODE == 2) {
+        const bool xin = (x >= a.box.lo[0]) & (x < a.box.lo[0] + a.box.n[0]);
+        const int sbx = xin ? (x - a.box.lo[0]) * rs.box_plane_bytes : 0;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            st.ps[r] = bload2_nt(rs.ps, xin ? 2 * brow[r] : kOOB, 2 * sbx);
+            st.gr[r] = bload_nt(rs.grad, grow[r], sp);
+            st.pr[r] = bload_nt(rs.prec, grow[r], sp);
+        }
+    }
+}
+
+template <int RPT, int MODE, bool SPONGE>
+__device__ __forceinline__ void finishB(const Geo &g, const FastArgs &a, const Rsrc &rs, const int x, const int x0,
+                                        const int x1, const f2 (&lap)[RPT], const f2 (&c)[RPT],
+                                        const StageB<RPT, MODE> &sx, const unsigned (&orow)[RPT],
+                                        const unsigned (&grow)[RPT], const unsigned (&brow)[RPT],
+                                        const int (&d1)[RPT], const float (&t1v)[RPT], const int d2, const float t2v) {
+    const int nx = g.n[0];
+    const int sxo = (x + SA_HALO) * rs.plane_bytes;
+    const bool xin = (x >= x0) & (x < x1);
+    const bool xin_box = MODE != 0 && (xin & (x >= a.box.lo[0]) & (x < a.box.lo[0] + a.box.n[0]));
+    const int sbx = MODE == 1 && xin_box ? (x - a.box.lo[0]) * rs.box_plane_bytes : 0;
+    int d0 = 0;
+    float t0v = 0.f;
+    if (SPONGE) {
+        d0 = max(min(x, nx - 1 - x), 0);
+        const float tv = a.axis[min(d0, a.alen - 1)];
+        t0v = d0 < g.nabs[0] ? tv : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const float v = sx.v[r];
+        const float v2 = v * v;
+        float A = 2.f, B = -1.f, G = a.dt2 * v2;
+        if (SPONGE) {
+            float sm = d0 <= min(d1[r], d2) ? t0v : 0.f;
+            sm += d1[r] <= min(d0, d2) ? t1v[r] : 0.f;
+            sm += d2 <= min(d0, d1[r]) ? t2v : 0.f;
+            const float D = (7.f * (sm * v)) * a.dt;
+            const float aa = a.dt * v2 * D;
+            const float e = a.dt * a.dt * v2 * D * D;
+            const float ia = 1.f / (1.f + aa);
+            A = (2.f - e) * ia;
+            B = -(1.f - aa) * ia;
+            G = (a.dt2 * ia) * v2;
+        }
+        f2 val;
+        val.x = A * c[r].x + B * sx.o[r].x + G * lap[r].x;
+        val.y = A * c[r].y + B * sx.o[r].y + G * lap[r].y;
+        bstore2_nt(val, rs.out, xin ? 2 * orow[r] : kOOB, 2 * sxo);
+        if (MODE == 1) {
+            f2 ps;
+            ps.x = (val.x - 2.f * c[r].x + sx.o[r].x) * a.idt2;
+            ps.y = (val.y - 2.f * c[r].y + sx.o[r].y) * a.idt2;
+            bstore2_nt(ps, rs.save, xin_box ? 2 * brow[r] : kOOB, 2 * sbx);
+        }
+        if (MODE == 2) {
+            const f2 psv = sx.ps[r];
+            const unsigned go = xin_box ? grow[r] : kOOB;
+            bstore_nt((sx.gr[r] + psv.x * c[r].x) + psv.y * c[r].y, rs.grad, go, sxo);
+            bstore_nt((sx.pr[r] + psv.x * psv.x) + psv.y * psv.y, rs.prec, go, sxo);
+        }
+    }
+}
+
+template <int RPT, int NW, int MODE, bool SPONGE, int J>
+__device__ __forceinline__ void marchB_step(const Geo &g, const FastArgs &a, const Rsrc &rs, float *const (&sl)[6],
+                                            f2 (&acc)[RPT][12], f2 (&cq)[RPT][6], StageB<RPT, MODE> (&st)[3],
+                                            const int p, const int x0, const int x1, const int lane, const int w,
+                                            const unsigned (&row)[RPT], const unsigned (&orow)[RPT],
+                                            const unsigned (&grow)[RPT], const unsigned (&brow)[RPT],
+                                            const unsigned (&dv)[Ring2W<NW * RPT, NW>::KI], const int (&d1)[RPT],
+                                            const float (&t1v)[RPT], const int d2, const float t2v) {
+    using R = Ring2W<NW * RPT, NW>;
+    constexpr int KI = R::KI;
+    float *__restrict__ cur = sl[J % 3];
+    float *__restrict__ nxt = sl[(J + 2) % 3];
+    const int xlast = g.n[0] + SA_HALO - 1;
+    asm volatile("" ::: "memory");
+    {
+        const float probe = cur[lane];
+        asm volatile("" ::"v"(probe));
+    }
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    {
+        const int soff = 2 * (min(p + 2, xlast) + SA_HALO) * rs.plane_bytes;
+#pragma unroll
+        for (int k = 0; k < KI; ++k) dma16(rs.u, nxt + (w + NW * k) * 256, dv[k], soff);
+    }
+    const f2 *__restrict__ pc = reinterpret_cast<const f2 *>(cur);
+    f2 lap[RPT], c5[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        int li = (w + NW * r + 5) * R::PITCH + lane + 8;  // in pairs; opaque per plane (see version 5)
+        asm volatile("" : "+v"(li));
+        const f2 c = pc[li];
+        f2 vy = g.w2[1][0] * c, vz = g.w2[2][0] * c;
+#pragma unroll
+        for (int k = 1; k <= 5; ++k) {
+            const f2 sy = pc[li + k * R::PITCH] + pc[li - k * R::PITCH];
+            const f2 sz = pc[li + k] + pc[li - k];
+            vy.x += g.w2[1][k] * sy.x;
+            vy.y += g.w2[1][k] * sy.y;
+            vz.x += g.w2[2][k] * sz.x;
+            vz.y += g.w2[2][k] * sz.y;
+        }
+        acc[r][(J + 5) % 12] = g.w2[0][5] * c;
+#pragma unroll
+        for (int k = 1; k <= 4; ++k) {
+            acc[r][(J + k) % 12].x += g.w2[0][k] * c.x;
+            acc[r][(J + k) % 12].y += g.w2[0][k] * c.y;
+        }
+        acc[r][J % 12].x += g.w2[0][0] * c.x + (vy.x + vz.x);
+        acc[r][J % 12].y += g.w2[0][0] * c.y + (vy.y + vz.y);
+#pragma unroll
+        for (int k = 1; k <= 5; ++k) {
+            acc[r][(J + 12 - k) % 12].x += g.w2[0][k] * c.x;
+            acc[r][(J + 12 - k) % 12].y += g.w2[0][k] * c.y;
+        }
+        lap[r] = acc[r][(J + 7) % 12];
+        c5[r] = cq[r][(J + 1) % 6];
+        cq[r][J % 6] = c;
+    }
+    finishB<RPT, MODE, SPONGE>(g, a, rs, p - 5, x0, x1, lap, c5, st[(J + 1) % 3], orow, grow, brow, d1, t1v, d2, t2v);
+    load_stageB<RPT, MODE>(st[(J + 1) % 3], rs, row, grow, brow, min(max(p - 2, 0), xlast), a);
+}
+
+template <int RPT, int NW, int MODE, bool SPONGE, int J>
+__device__ __forceinline__ void tailB_step(const Geo &g, const FastArgs &a, const Rsrc &rs, f2 (&acc)[RPT][12],
+                                           f2 (&cq)[RPT][6], StageB<RPT, MODE> (&st)[3], const f2 (&up)[RPT],
+                                           const int p, const int x0, const int x1, const unsigned (&row)[RPT],
+                                           const unsigned (&orow)[RPT], const unsigned (&grow)[RPT],
+                                           const unsigned (&brow)[RPT], const int (&d1)[RPT],
+                                           const float (&t1v)[RPT], const int d2, const float t2v) {
+    const int xlast = g.n[0] + SA_HALO - 1;
+    f2 lap[RPT], c5[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int k = J + 1; k <= 5; ++k) {
+            acc[r][(J + 12 - k) % 12].x += g.w2[0][k] * up[r].x;
+            acc[r][(J + 12 - k) % 12].y += g.w2[0][k] * up[r].y;
+        }
+        lap[r] = acc[r][(J + 7) % 12];
+        c5[r] = cq[r][(J + 1) % 6];
+    }
+    finishB<RPT, MODE, SPONGE>(g, a, rs, p - 5, x0, x1, lap, c5, st[(J + 1) % 3], orow, grow, brow, d1, t1v, d2, t2v);
+    if (J < 2) load_stageB<RPT, MODE>(st[(J + 1) % 3], rs, row, grow, brow, min(p - 2, xlast), a);
+}
+
+template <int RPT, int NW, int MODE, bool SPONGE>
+__device__ __forceinline__ void marchB(const Geo &g, const FastArgs &a, const Rsrc &rs, float *const (&sl)[6],
+                                       const int x0, const int x1, const int lane, const int w,
+                                       const unsigned (&row)[RPT], const unsigned (&orow)[RPT],
+                                       const unsigned (&grow)[RPT], const unsigned (&brow)[RPT],
+                                       const unsigned (&dv)[Ring2W<NW * RPT, NW>::KI], const int (&d1)[RPT],
+                                       const float (&t1v)[RPT], const int d2, const float t2v) {
+    constexpr int KI = Ring2W<NW * RPT, NW>::KI;
+    const int xlast = g.n[0] + SA_HALO - 1;
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int k = 0; k < KI; ++k)
+            dma16(rs.u, sl[q] + (w + NW * k) * 256, dv[k], 2 * (min(x0 + q, xlast) + SA_HALO) * rs.plane_bytes);
+    f2 acc[RPT][12], cq[RPT][6];
+    StageB<RPT, MODE> st[3] = {};
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[r][k] = f2{0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cq[r][k] = f2{0.f, 0.f};
+#pragma unroll
+        for (int i = 1; i <= 5; ++i) {
+            const f2 ub = bload2(rs.u, 2 * row[r], 2 * (x0 - i + SA_HALO) * rs.plane_bytes);
+#pragma unroll
+            for (int j = 0; j + i <= 5; ++j) {
+                acc[r][j].x += g.w2[0][i + j] * ub.x;
+                acc[r][j].y += g.w2[0][i + j] * ub.y;
+            }
+        }
+    }
+    const int nper = (x1 - x0 + 11) / 12;
+    const int pend = x0 + 12 * nper;
+    for (int it = 0; it < nper; ++it) {
+        const int xb = x0 + 12 * it;
+#define SA_STEPB(J)                                                                                            \
+    marchB_step<RPT, NW, MODE, SPONGE, J>(g, a, rs, sl, acc, cq, st, xb + J, x0, x1, lane, w, row, orow, grow, \
+                                          brow, dv, d1, t1v, d2, t2v);
+        SA_STEPB(0) SA_STEPB(1) SA_STEPB(2) SA_STEPB(3) SA_STEPB(4) SA_STEPB(5)
+        SA_STEPB(6) SA_STEPB(7) SA_STEPB(8) SA_STEPB(9) SA_STEPB(10) SA_STEPB(11)
+#undef SA_STEPB
+    }
+    f2 up[5][RPT];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+        for (int r = 0; r < RPT; ++r)
+            up[j][r] = bload2(rs.u, 2 * row[r], 2 * (min(pend + j, xlast) + SA_HALO) * rs.plane_bytes);
+#define SA_TAILB(J)                                                                                                \
+    tailB_step<RPT, NW, MODE, SPONGE, J>(g, a, rs, acc, cq, st, up[J], pend + J, x0, x1, row, orow, grow, brow, d1, \
+                                         t1v, d2, t2v);
+    SA_TAILB(0) SA_TAILB(1) SA_TAILB(2) SA_TAILB(3) SA_TAILB(4)
+#undef SA_TAILB
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): no direct-to-LDS load outlives the workgroup
+}
+
+template <int RPT, int NW, int MODE, bool SPONGE>
+__global__ __launch_bounds__(NW * 64) void k_fast3d_b2(Geo g, FastArgs a) {
+    constexpr int TY = NW * RPT;
+    using R = Ring2W<TY, NW>;
+    constexpr int KI = R::KI;
+    SA_RING_SLOTS(3, R::FLOATS)
+    const int b = blockIdx.x;
+    const int Lg = (b & 7) * a.per_xcd + (b >> 3);
+    if (Lg >= a.nblocks) return;
+    const int tz = Lg % a.ntile_z;
+    const int rest = Lg / a.ntile_z;
+    const int ty = rest % a.ntile_y;
+    const int ch = rest / a.ntile_y;
+    const int z0 = tz * kTZ, y0 = ty * TY;
+    const int x0 = ch * a.chunk;
+    const int x1 = min(x0 + a.chunk, g.n[0]);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int z = z0 + lane;
+    const int ny = g.n[1], nz = g.n[2];
+    const unsigned ps1 = (unsigned)g.s1;
+    Rsrc rs;
+    const long long bytes = g.total * 4;
+    rs.u = make_rsrc(a.u, 2 * bytes);
+    rs.o = make_rsrc(a.o, 2 * bytes);
+    rs.vp = make_rsrc(a.vp, bytes);
+    rs.out = make_rsrc(a.out, 2 * bytes);
+    rs.plane_bytes = (int)(g.s0 * 4);  // of a one-shot field; pair fields use twice the offsets
+    rs.box_plane_bytes = a.box.n[1] * a.box.p2 * 4;
+    const long long box_bytes = (long long)a.box.n[0] * rs.box_plane_bytes;
+    if (MODE == 1) rs.save = make_rsrc(a.save, 2 * box_bytes);
+    if (MODE == 2) {
+        rs.ps = make_rsrc(a.ps, 2 * box_bytes);
+        rs.grad = make_rsrc(a.grad, bytes);
+        rs.prec = make_rsrc(a.prec, bytes);
+    }
+    // offsets are in one-shot bytes (pair arrays double them at the use)
+    const unsigned zc = (unsigned)(min(z, nz + SA_RADIUS - 1) + SA_LEAD);
+    unsigned row[RPT], orow[RPT], grow[RPT], brow[RPT];
+    int d1[RPT];
+    float t1v[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int yr = y0 + w + NW * r;
+        const bool rin = yr < ny && z < nz;
+        row[r] = ((unsigned)(min(yr, ny + SA_RADIUS - 1) + SA_HALO) * ps1 + zc) * 4u;
+        orow[r] = rin ? row[r] : kOOB / 2;
+        const bool bin = rin && yr >= a.box.lo[1] && yr < a.box.lo[1] + a.box.n[1] && z >= a.box.lo[2] &&
+                         z < a.box.lo[2] + a.box.n[2];
+        brow[r] = bin ? (unsigned)((yr - a.box.lo[1]) * a.box.p2 + a.box.off2 + (z - a.box.lo[2])) * 4u : kOOB / 2;
+        grow[r] = bin ? row[r] : kOOB;
+        d1[r] = min(yr, ny - 1 - yr);
+        t1v[r] = 0.f;
+        if (SPONGE && d1[r] >= 0 && d1[r] < g.nabs[1]) t1v[r] = a.axis[a.alen + d1[r]];
+    }
+    const int d2 = min(z, nz - 1 - z);
+    const float t2v = SPONGE && d2 >= 0 && d2 < g.nabs[2] ? a.axis[2 * a.alen + d2] : 0.f;
+    // direct-to-LDS chunk offsets (pair bytes): chunk c -> row c / 40, cells 2 (c % 40) .. + 1
+    unsigned dv[KI];
+#pragma unroll
+    for (int k = 0; k < KI; ++k) {
+        const int c = (w + NW * k) * 64 + lane;
+        dv[k] = kOOB;
+        if (c < R::NCH) {
+            const int rr = c / R::CPR, cc = c % R::CPR;
+            const int yy = min(y0 - 5 + rr, ny + SA_HALO - 1);
+            const int zz = min(z0 + SA_LEAD - 8 + 2 * cc, (int)ps1 - 2);
+            dv[k] = ((unsigned)(yy + SA_HALO) * ps1 + (unsigned)zz) * 8u;
+        }
+    }
+    const bool interior = y0 >= g.nabs[1] && y0 + TY <= ny - g.nabs[1] && z0 >= g.nabs[2] &&
+                          z0 + kTZ <= nz - g.nabs[2] && x0 >= g.nabs[0] && x1 <= g.n[0] - g.nabs[0];
+    if (SPONGE && !interior)
+        marchB<RPT, NW, MODE, true>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2, t2v);
+    else
+        marchB<RPT, NW, MODE, false>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2, t2v);
+}
+
 template <int RPT, int MODE>
 void launch_rpt(int version, bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
     if (version == 2) {
@@ -1145,6 +1488,38 @@ int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
             default: return SA_E_ARG;
         }
     }
+    return hipGetLastError() == hipSuccess ? 0 : SA_E_HIP;
+}
+
+template <int RPT, int NW, int MODE>
+void launch_b2_rpt(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
+    if (sponge)
+        k_fast3d_b2<RPT, NW, MODE, true><<<grid, NW * 64, 0, st>>>(g, a);
+    else
+        k_fast3d_b2<RPT, NW, MODE, false><<<grid, NW * 64, 0, st>>>(g, a);
+}
+
+template <int MODE>
+int launch_b2(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
+    if (g.total * 8 >= (1ll << 31)) return SA_E_UNSUPPORTED;  // pair buffers need 32-bit offsets
+    const int rows = f.tune2.rows, waves = f.tune2.version;
+    const int TY = waves * rows;
+    a.ntile_y = (g.n[1] + TY - 1) / TY;
+    a.ntile_z = (g.n[2] + kTZ - 1) / kTZ;
+    a.chunk = f.tune2.chunk;
+    const int nchunk = (g.n[0] + a.chunk - 1) / a.chunk;
+    a.nblocks = a.ntile_y * a.ntile_z * nchunk;
+    a.per_xcd = (a.nblocks + 7) / 8;
+    dim3 grid(a.per_xcd * 8);
+    const bool sponge = f.boundary == 0;
+    if (waves == 8 && rows == 1)
+        launch_b2_rpt<1, 8, MODE>(sponge, grid, st, g, a);
+    else if (waves == 8 && rows == 2)
+        launch_b2_rpt<2, 8, MODE>(sponge, grid, st, g, a);
+    else if (waves == 4 && rows == 2)
+        launch_b2_rpt<2, 4, MODE>(sponge, grid, st, g, a);
+    else
+        return SA_E_ARG;
     return hipGetLastError() == hipSuccess ? 0 : SA_E_HIP;
 }
 
@@ -1263,6 +1638,39 @@ int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o,
         return launch<2>(f, g, a, stream);
     }
     return launch<0>(f, g, a, stream);
+}
+
+int fast3d_step_b2(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,
+                   float idt2, hipStream_t stream) {
+    FastArgs a = base_args(f, u, o, out);
+    if (save) {
+        a.save = save;
+        a.box = box;
+        a.idt2 = idt2;
+        return launch_b2<1>(f, g, a, stream);
+    }
+    return launch_b2<0>(f, g, a, stream);
+}
+
+int fast3d_step_adjoint_b2(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, const float *ps,
+                           const Box &box, float *grad, float *prec, hipStream_t stream) {
+    FastArgs a = base_args(f, u, o, out);
+    if (ps && grad && prec) {
+        a.ps = ps;
+        a.box = box;
+        a.grad = grad;
+        a.prec = prec;
+        return launch_b2<2>(f, g, a, stream);
+    }
+    return launch_b2<0>(f, g, a, stream);
+}
+
+int fast3d_set_tune_b2(Fast3D &f, int rows, int chunk, int waves) {
+    if (chunk < 1 || !((waves == 8 && (rows == 1 || rows == 2)) || (waves == 4 && rows == 2))) return SA_E_ARG;
+    f.tune2.rows = rows;
+    f.tune2.chunk = chunk;
+    f.tune2.version = waves;
+    return 0;
 }
 
 void fast3d_free(Fast3D &f) {

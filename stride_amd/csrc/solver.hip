@@ -235,33 +235,34 @@ __global__ void k_scale_copy(Geo g, const float *__restrict__ src, const float *
 // sparse injection, deterministic: one thread per destination cell sums its contributions in
 // a fixed order. dest: padded index of each destination cell, ijk: its domain coordinates
 // (3 ints); coef per contribution; values[point].
+// Fields of S interleaved shots hold shot sh of cell p at p * S + sh (S = 1: one shot).
 __global__ void k_inject(int ndest, const long long *__restrict__ dest, const int *__restrict__ ijk,
                          const int *__restrict__ off, const int *__restrict__ point, const float *__restrict__ coef,
                          const float *__restrict__ values, float sign, float *field, float *save, Box box,
-                         float save_scale) {
+                         float save_scale, int S, int sh) {
     int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ndest) return;
     float acc = 0.f;
     for (int c = off[d]; c < off[d + 1]; ++c) acc += coef[c] * values[point[c]];
     acc *= sign;
-    field[dest[d]] += acc;
+    field[dest[d] * S + sh] += acc;
     if (save) {
         const int i = ijk[3 * d], j = ijk[3 * d + 1], k = ijk[3 * d + 2];
-        if (sa_inbox(box, i, j, k)) save[sa_boxidx(box, i, j, k)] += acc * save_scale;
+        if (sa_inbox(box, i, j, k)) save[sa_boxidx(box, i, j, k) * S + sh] += acc * save_scale;
     }
 }
 
 // rec[r] = sign * sum_q w * field (receiver interpolation, wavelet gradient); idx: padded
 // index per tap, negative for none
 __global__ void k_gather(int npt, int ntap, const long long *__restrict__ idx, const float *__restrict__ w,
-                         const float *__restrict__ field, float sign, float *__restrict__ out) {
+                         const float *__restrict__ field, float sign, float *__restrict__ out, int S, int sh) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= npt) return;
     float acc = 0.f;
     for (int q = 0; q < ntap; ++q) {
         const long long p = idx[(long long)r * ntap + q];
         if (p < 0) continue;
-        acc += w[(long long)r * ntap + q] * field[p];
+        acc += w[(long long)r * ntap + q] * field[p * S + sh];
     }
     out[r] = sign * acc;
 }
@@ -450,6 +451,14 @@ struct sa_solver {
     float *gacc[4] = {};
     // fast 3D path
     Fast3D fast;
+    // two shots per pass (sa_batch_*): interleaved levels, per-shot taps, interleaved snapshots
+    int nshot = 0;
+    float *bbuf[2] = {};
+    Sparse bsrc[2], brec[2];
+    float *bsaved = nullptr;
+    size_t bsaved_bytes = 0;
+    int bnsave = 0, bf = 0;
+    Box bbox;
 };
 
 static int dalloc(float **p, size_t n) {
@@ -480,6 +489,11 @@ static const dim3 kBlk(64, 4, 1);
 
 static int zero_field(sa_solver *s, float *p) {
     HIP_TRY(hipMemsetAsync(p, 0, s->total * sizeof(float), s->stream));
+    return SA_OK;
+}
+
+static int zero_field2(sa_solver *s, float *p) {  // two interleaved shots
+    HIP_TRY(hipMemsetAsync(p, 0, 2 * s->total * sizeof(float), s->stream));
     return SA_OK;
 }
 
@@ -610,6 +624,12 @@ extern "C" int sa_destroy(sa_solver *s) {
     }
     free_sparse(s->src);
     free_sparse(s->rec);
+    for (int k = 0; k < 2; ++k) {
+        free_sparse(s->bsrc[k]);
+        free_sparse(s->brec[k]);
+        dfree(s->bbuf[k]);
+    }
+    dfree(s->bsaved);
     fast3d_free(s->fast);
     for (auto e : s->ev) (void)hipEventDestroy(e);
     delete s;
@@ -950,18 +970,19 @@ static int coef_sparse(sa_solver *s, Sparse &sp, int kind) {
 }
 
 static int inject(sa_solver *s, Sparse &sp, const float *values, float sign, float *field, float *save,
-                  float save_scale) {
+                  float save_scale, const Box &box, int S = 1, int sh = 0) {
     if (sp.ndest == 0) return SA_OK;
     k_inject<<<(sp.ndest + 127) / 128, 128, 0, s->stream>>>(sp.ndest, sp.d_dest, sp.d_ijk, sp.d_off, sp.d_point,
-                                                            sp.d_coef, values, sign, field, save, s->box, save_scale);
+                                                            sp.d_coef, values, sign, field, save, box, save_scale, S,
+                                                            sh);
     CHECK_LAUNCH();
     return SA_OK;
 }
 
-static int gather(sa_solver *s, Sparse &sp, const float *field, float sign, float *out) {
+static int gather(sa_solver *s, Sparse &sp, const float *field, float sign, float *out, int S = 1, int sh = 0) {
     if (sp.npoint == 0) return SA_OK;
     k_gather<<<(sp.npoint + 127) / 128, 128, 0, s->stream>>>(sp.npoint, sp.ntap, sp.d_idx, sp.d_w, field,
-                                                             sign, out);
+                                                             sign, out, S, sh);
     CHECK_LAUNCH();
     return SA_OK;
 }
@@ -1096,7 +1117,7 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
         } else {
             TRY(step_general(s, u, o, o, 0, save, mode));
         }
-        TRY(inject(s, s->src, src + (size_t)t * s->src.npoint, 1.f, o, mode == 1 ? save : nullptr, idt2));
+        TRY(inject(s, s->src, src + (size_t)t * s->src.npoint, 1.f, o, mode == 1 ? save : nullptr, idt2, s->box));
         TRY(gather(s, s->rec, o, 1.f, rec + (size_t)t * s->rec.npoint));
         if (ckpt && t % f == 0) {
             // checkpoint j = t / K holds (p[jK + 1], p[jK])
@@ -1128,7 +1149,7 @@ static int recompute_segment(sa_solver *s, int j) {
     for (int t = t0 + 1; t < tend; ++t) {
         float *u = s->fb[t % 2], *o = s->fb[(t + 1) % 2];
         TRY(step_general(s, u, o, o, 0, nullptr, 0));
-        TRY(inject(s, s->src, s->src_keep + (size_t)t * s->src.npoint, 1.f, o, nullptr, 0.f));
+        TRY(inject(s, s->src, s->src_keep + (size_t)t * s->src.npoint, 1.f, o, nullptr, 0.f, s->box));
         k_pack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, o, s->segment + (size_t)(t + 1 - t0) * per);
         CHECK_LAUNCH();
     }
@@ -1193,7 +1214,7 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
         } else {
             TRY(step_general(s, u, o, o, 1, nullptr, 0));
         }
-        TRY(inject(s, s->rec, res + (size_t)t * s->rec.npoint, -1.f, o, nullptr, 0.f));
+        TRY(inject(s, s->rec, res + (size_t)t * s->rec.npoint, -1.f, o, nullptr, 0.f, s->box));
         if (gsrc) TRY(gather(s, s->src, u, -1.f, gsrc + (size_t)t * s->src.npoint));
         if (!guard) continue;
         if (gv && !(fast && s->saved_mode == 1)) {
@@ -1313,7 +1334,7 @@ extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float
         if (lv >= 2) {
             rc = step_general(s, m1, m2, m2, 1, nullptr, 0);  // mu_lv over mu_{lv+2}
             if (rc == SA_OK && lv - 1 >= 1 && lv - 1 <= nt - 2)
-                rc = inject(s, s->rec, s->d_series + (size_t)(lv - 1) * s->rec.npoint, 1.f, m2, nullptr, 0.f);
+                rc = inject(s, s->rec, s->d_series + (size_t)(lv - 1) * s->rec.npoint, 1.f, m2, nullptr, 0.f, s->box);
         }
         int t = lv;
         if (rc != SA_OK || t < 1 || t > nt - 2) continue;
@@ -1452,4 +1473,172 @@ extern "C" int sa_synchronize(sa_solver *s) {
     if (!s) return fail(SA_E_ARG, "null solver");
     HIP_TRY(hipStreamSynchronize(s->stream));
     return drain_timing(s);
+}
+
+// ---------------------------------------------------------------------------------------
+// Two shots per pass. The shots share the grid, scheme, medium and boundary; each has its own
+// source / receiver taps, source samples, traces and residual. Their wavefields are stored
+// interleaved per cell so that one pass of the specialised 3D update advances both with one
+// read of vp, and the adjoint accumulates the sum of the two shots' vp gradients and
+// preconditioners (the quantity an FWI iteration uses) with one read-modify-write of each.
+
+static bool batch_ok(const sa_solver *s) {
+    const sa_scheme &c = s->sch;
+    return use_fast(s) && (c.save_mode == 0 || c.save_mode == 1) && s->nshot == 2;
+}
+
+extern "C" int sa_batch_create(sa_solver *s, int nshot) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    if (nshot != 2) return fail(SA_E_ARG, "batches hold 2 shots");
+    if (s->total * 8 >= (1ll << 31)) return fail(SA_E_UNSUPPORTED, "grid too large for the two-shot kernel");
+    HIP_TRY(hipSetDevice(s->device));
+    for (int k = 0; k < 2; ++k)
+        if (!s->bbuf[k]) TRY(dalloc(&s->bbuf[k], 2 * s->total));
+    s->nshot = 2;
+    return SA_OK;
+}
+
+extern "C" int sa_batch_set_taps(sa_solver *s, int shot, int nsrc, int src_ntap, const int64_t *src_index,
+                                 const float *src_weight, int nrec, int rec_ntap, const int64_t *rec_index,
+                                 const float *rec_weight) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    if (s->nshot != 2 || shot < 0 || shot > 1) return fail(SA_E_ARG, "shot must be 0 or 1 of a batch");
+    TRY(set_sparse(s, s->bsrc[shot], nsrc, src_ntap, src_index, src_weight));
+    return set_sparse(s, s->brec[shot], nrec, rec_ntap, rec_index, rec_weight);
+}
+
+extern "C" int sa_batch_set_tile(sa_solver *s, int rows, int chunk, int waves) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    if (fast3d_set_tune_b2(s->fast, rows, chunk, waves) != 0)
+        return fail(SA_E_ARG, "two-shot tile: chunk >= 1, waves 8 with rows 1 or 2, or waves 4 with rows 2");
+    return SA_OK;
+}
+
+static int batch_alloc_saved(sa_solver *s) {
+    const sa_scheme &c = s->sch;
+    const Geo &g = s->geo;
+    const int f = c.undersampling;
+    Box b;
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = g.nabs[a];
+        b.n[a] = g.n[a] - 2 * g.nabs[a];
+    }
+    b.off2 = b.lo[2] % 32;
+    b.p2 = (b.off2 + b.n[2] + 31) / 32 * 32;
+    const int nsave = c.tb1 / f + 1;
+    const size_t per = 2 * box_points(b);  // two shots per cell
+    const size_t bytes = per * nsave * sizeof(float);
+    if (!s->bsaved || s->bsaved_bytes < bytes) {
+        dfree(s->bsaved);
+        s->bsaved = nullptr;
+        s->bsaved_bytes = 0;
+        if (hipMalloc((void **)&s->bsaved, bytes) != hipSuccess) {
+            s->bsaved = nullptr;
+            return fail(SA_E_NOMEM, "cannot allocate %.2f GB for the saved wavefields", bytes / 1e9);
+        }
+        s->bsaved_bytes = bytes;
+    }
+    // slots the forward run never writes read as zero (see alloc_saved)
+    const int first = (std::max(1, c.tb0) + f - 1) / f;
+    const int last = std::min((c.tb1 - 2) / f, nsave - 1);
+    if (first > last) {
+        HIP_TRY(hipMemsetAsync(s->bsaved, 0, bytes, s->stream));
+    } else {
+        if (first > 0) HIP_TRY(hipMemsetAsync(s->bsaved, 0, per * first * sizeof(float), s->stream));
+        if (last + 1 < nsave)
+            HIP_TRY(hipMemsetAsync(s->bsaved + per * (size_t)(last + 1), 0, per * (nsave - 1 - last) * sizeof(float),
+                                   s->stream));
+    }
+    s->bbox = b;
+    s->bnsave = nsave;
+    s->bf = f;
+    return SA_OK;
+}
+
+// src[k]: device (nt, nsrc_k) samples of shot k; rec[k]: device (nt, nrec_k) traces of shot k
+extern "C" int sa_batch_forward_dev(sa_solver *s, const float *const *src, float *const *rec) {
+    if (!s || !src || !rec) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    TRY(prepare(s));
+    if (!batch_ok(s))
+        return fail(SA_E_UNSUPPORTED, "two-shot runs need sa_batch_create and the specialised 3D scheme (OT2, no "
+                                      "density, sponge or no boundary, no attenuation, save mode 0 or 1)");
+    const sa_scheme &c = s->sch;
+    for (int k = 0; k < 2; ++k) {
+        TRY(coef_sparse(s, s->bsrc[k], 0));
+        TRY(zero_field2(s, s->bbuf[k]));
+        if (s->brec[k].npoint > 0)
+            HIP_TRY(hipMemsetAsync(rec[k], 0, sizeof(float) * (size_t)s->grid.nt * s->brec[k].npoint, s->stream));
+    }
+    if (c.save_mode == 1) TRY(batch_alloc_saved(s));
+    const int f = c.undersampling;
+    const size_t per = 2 * box_points(s->bbox);
+    const float idt2 = 1.f / (s->dt * s->dt);
+    const int tlast = c.tb1 - 2;
+    for (int t = 1; t <= tlast; ++t) {
+        float *u = s->bbuf[t % 2], *o = s->bbuf[(t + 1) % 2];
+        float *save = nullptr;
+        if (c.save_mode == 1 && t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->bnsave)
+            save = s->bsaved + (size_t)(t / f) * per;
+        hipEvent_t e0;
+        TRY(timed_begin(s, &e0));
+        TRY(fast3d_step_b2(s->fast, s->geo, u, o, o, save, s->bbox, idt2, s->stream));
+        TRY(timed_end(s));
+        for (int k = 0; k < 2; ++k) {
+            TRY(inject(s, s->bsrc[k], src[k] + (size_t)t * s->bsrc[k].npoint, 1.f, o, save, idt2, s->bbox, 2, k));
+            TRY(gather(s, s->brec[k], o, 1.f, rec[k] + (size_t)t * s->brec[k].npoint, 2, k));
+        }
+    }
+    s->cur = (tlast + 1) % 2;
+    return SA_OK;
+}
+
+// res[k]: device (nt, nrec_k) adjoint source of shot k (windowed as for sa_adjoint_dev); the
+// vp gradient and preconditioner of both shots are summed (sa_read_gradient 0 / 1)
+extern "C" int sa_batch_adjoint_dev(sa_solver *s, const float *const *res) {
+    if (!s || !res) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    TRY(prepare(s));
+    if (!batch_ok(s)) return fail(SA_E_UNSUPPORTED, "two-shot runs need sa_batch_create and the specialised 3D scheme");
+    if (!s->bsaved || s->sch.save_mode != 1)
+        return fail(SA_E_STATE, "the two-shot adjoint needs a two-shot forward run with save mode 1");
+    const sa_scheme &c = s->sch;
+    TRY(grad_outputs(s, 2, s->gacc));
+    for (int k = 0; k < 2; ++k) {
+        TRY(coef_sparse(s, s->brec[k], 1));
+        TRY(zero_field2(s, s->bbuf[k]));
+    }
+    const int f = s->bf;
+    const size_t per = 2 * box_points(s->bbox);
+    const int tstart = c.tb1 - 1 - f, tend = c.tb0 + 1;
+    for (int t = tstart; t >= tend; --t) {
+        float *u = s->bbuf[t % 2], *o = s->bbuf[(t + 1) % 2];
+        const bool guard = t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->bnsave;
+        const float *ps = guard ? s->bsaved + (size_t)(t / f) * per : nullptr;
+        hipEvent_t e0;
+        TRY(timed_begin(s, &e0));
+        TRY(fast3d_step_adjoint_b2(s->fast, s->geo, u, o, o, ps, s->bbox, s->gacc[0], s->gacc[1], s->stream));
+        TRY(timed_end(s));
+        for (int k = 0; k < 2; ++k)
+            TRY(inject(s, s->brec[k], res[k] + (size_t)t * s->brec[k].npoint, -1.f, o, nullptr, 0.f, s->bbox, 2, k));
+    }
+    s->cur = (tend - 1 + 2) % 2;
+    return SA_OK;
+}
+
+// wavefield `which` (0: last level, 1: the one before) of shot `shot` of a batch, unpadded
+extern "C" int sa_batch_get_wavefield(sa_solver *s, int shot, int which, float *out) {
+    if (!s || !out || shot < 0 || shot > 1 || s->nshot != 2) return fail(SA_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(s->device));
+    const Geo &g = s->geo;
+    std::vector<float> tmp(2 * (size_t)s->total);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(tmp.data(), s->bbuf[which == 0 ? s->cur : 1 - s->cur], tmp.size() * sizeof(float),
+                      hipMemcpyDeviceToHost));
+    for (int i = 0; i < g.n[0]; ++i)
+        for (int j = 0; j < g.n[1]; ++j)
+            for (int k = 0; k < g.n[2]; ++k)
+                out[((long long)i * g.n[1] + j) * g.n[2] + k] =
+                    tmp[2 * (g.base + (long long)i * g.s0 + (long long)j * g.s1 + k) + shot];
+    return SA_OK;
 }
