@@ -174,6 +174,63 @@ int sa_batch_get_wavefield(sa_solver *s, int shot, int which, float *out);
 /* tile of the two-shot update: waves 8 with rows 1 or 2, or waves 4 with rows 2 */
 int sa_batch_set_tile(sa_solver *s, int rows, int chunk, int waves);
 
+/* ---- Steps either side of the solver, on device arrays (the solver supplies device and
+ * stream). Traces are (ntr, nt) float32 rows; fields are extended-grid float32 arrays in C
+ * order. Reductions are deterministic. sa_field_axpb, sa_field_div_nonzero and
+ * sa_field_descent are queued on the solver's stream and return at once (sa_synchronize waits
+ * for them); the other calls return with their work finished. */
+
+/* CheckTraces (stride/optimisation/pipelines/steps/check_traces.py:52-83): flags rows with a
+ * NaN / Inf (bad_rows: host array of ntr ints, may be NULL), counts them, zeroes them if
+ * zero_bad. */
+int sa_traces_check(sa_solver *s, float *d, int ntr, int nt, int zero_bad, int *bad_rows, int *nbad);
+/* MuteTraces (mute_traces.py:26-60): mod *= conv(obs != 0, table) (centred, zero outside;
+ * table: host, odd length, e.g. the cosine low-pass of stride/utils/filters.py:401-465). */
+int sa_traces_mute(sa_solver *s, float *mod, const float *obs, int ntr, int nt, int len, const float *table);
+/* FIR along time: out[i] = sum_j w[j] d[i + j - off]; adjoint: sum_j w[j] d[i - j + off]
+ * (cosine low-pass of FilterTraces, filters.py:414-465; table: host). In place. */
+int sa_traces_fir(sa_solver *s, float *d, int ntr, int nt, int len, const float *table, int off, int adjoint);
+/* Butterworth second-order sections as scipy.signal.sosfilt (filters.py:14-175 with
+ * zero_phase False, filter_traces.py:88); adjoint runs backwards in time. sos: host, nsec x 6
+ * (b0 b1 b2 1 a1 a2). In place. */
+int sa_traces_sosfilt(sa_solver *s, float *d, int ntr, int nt, int nsec, const double *sos, int adjoint);
+/* NormPerShot / NormPerTrace (norm_traces.py:41-86): d /= sqrt(mean over rows of sum d^2)
+ * + 1e-31, or per row sqrt(sum d^2) + 1e-31; norm: the per-shot value. In place. */
+int sa_traces_norm(sa_solver *s, float *d, int ntr, int nt, int per_trace, double *norm);
+/* L2DistanceLoss (stride/optimisation/loss/l2_distance.py:36-66): res = mod - obs,
+ * fun = 0.5 sum res^2. */
+int sa_traces_l2(sa_solver *s, const float *mod, const float *obs, float *res, int ntr, int nt, double *fun);
+/* out[c, r] = in[r, c] x col_scale[c] (col_scale: host, length cols, or null). Converts the
+ * solver's time-major receiver traces (nt, nrec) to Traces rows (nrec, nt) (data.py:1423-1430),
+ * and back for an adjoint source with the Tukey window of devito.py:863-869 as col_scale. */
+int sa_traces_transpose(sa_solver *s, const float *in, float *out, int rows, int cols, const float *col_scale);
+/* MaskField (mask_field.py:8-67): f *= [lo <= c < hi] x the ramp-off profile of width ramp. */
+int sa_field_mask(sa_solver *s, float *f, int dim, const int *shape, const int *lo, const int *hi, int ramp);
+/* SmoothField / apply_prec smoothing (smooth_field.py:28-47, data.py:409-446):
+ * scipy.ndimage.gaussian_filter (truncate 4), mode 0 nearest, 1 reflect. In place. */
+int sa_field_gaussian(sa_solver *s, float *f, int dim, const int *shape, const double *sigma, int mode);
+/* min, max and sum |f| (NormField, norm_field.py:24-52; apply_prec's normalisation). */
+int sa_field_stats(sa_solver *s, const float *f, int64_t n, double *vmin, double *vmax, double *abssum);
+int sa_field_axpb(sa_solver *s, float *f, int64_t n, double a, double b); /* f = a f + b */
+int sa_field_div_nonzero(sa_solver *s, float *g, const float *prec, int64_t n); /* g /= prec where != 0 */
+/* GradientDescent step in the variable's transformed space (optimiser.py:242-330) with Clip
+ * (clip.py): transform 0 identity, 1 slowness (x -> 1/x). */
+int sa_field_descent(sa_solver *s, float *x, const float *dir, int64_t n, double step, int transform, int clip,
+                     double lo, double hi);
+/* Box [lo, hi) of a C-order grid of `shape` copied out to a dense array (crop), or a dense
+ * array copied into that box of the grid (embed; the rest of the grid is left as it is): the
+ * inner / extended views of a ScalarField (data.py:284-293). Queued on the stream. */
+int sa_field_crop(sa_solver *s, const float *grid, float *box, int dim, const int *shape, const int *lo,
+                  const int *hi);
+int sa_field_embed(sa_solver *s, const float *box, float *grid, int dim, const int *shape, const int *lo,
+                   const int *hi);
+/* get_grad_vp's post-scale (devito.py:1076-1081): slowness 1 (vp's default transform):
+ * grad /= vp, prec /= vp^2; else grad *= -2 / vp^3, prec *= (2 / vp)^9 (as written). grad or
+ * prec may be NULL. Queued on the stream. */
+int sa_field_grad_vp_finish(sa_solver *s, float *grad, float *prec, const float *vp, int64_t n, int slowness);
+/* Gradient accumulator `which` (as sa_read_gradient) copied to an unpadded device array. */
+int sa_read_gradient_dev(sa_solver *s, int which, float *dev_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,7 +44,10 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_adjoint_exact', 'sa_get_wavefield', 'sa_free_saved', 'sa_set_tile', 'sa_timing_enable', 'sa_timing_read',
            'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize', 'sa_batch_create',
            'sa_batch_set_taps', 'sa_batch_forward_dev', 'sa_batch_adjoint_dev', 'sa_batch_get_wavefield',
-           'sa_batch_set_tile']
+           'sa_batch_set_tile', 'sa_traces_check', 'sa_traces_mute', 'sa_traces_fir', 'sa_traces_sosfilt',
+           'sa_traces_norm', 'sa_traces_l2', 'sa_field_mask', 'sa_field_gaussian', 'sa_field_stats', 'sa_field_axpb',
+           'sa_field_div_nonzero', 'sa_field_descent', 'sa_read_gradient_dev',
+           'sa_traces_transpose', 'sa_field_crop', 'sa_field_embed', 'sa_field_grad_vp_finish']
 
 _lib = None
 
@@ -93,6 +96,7 @@ class Solver:
         self.key = (dim, tuple(shape), tuple(spacing), tuple(absorbing), float(dt), int(nt), int(device))
         self.shape = tuple(shape)
         self.nt = nt
+        self.device = int(device)
         self._h = ctypes.c_void_p()
         self._check(self._lib.sa_create(ctypes.byref(g), ctypes.byref(self._h)))
         self.nsrc = self.nrec = 0
@@ -271,6 +275,106 @@ class Solver:
         out = np.zeros(self.shape, np.float32)
         self._check(self._lib.sa_batch_get_wavefield(self._h, ctypes.c_int(shot), ctypes.c_int(which), _p(out)))
         return out
+
+    # ---- steps around the solver on device arrays (pointers) ----
+    def traces_check(self, ptr, ntr, nt, zero_bad=False):
+        bad = np.zeros(max(ntr, 1), np.int32)
+        nbad = ctypes.c_int()
+        self._check(self._lib.sa_traces_check(self._h, ctypes.c_void_p(ptr), ctypes.c_int(ntr), ctypes.c_int(nt),
+                                              ctypes.c_int(int(zero_bad)), _p(bad), ctypes.byref(nbad)))
+        return bad[:ntr].astype(bool), nbad.value
+
+    def traces_mute(self, mod_ptr, obs_ptr, ntr, nt, table):
+        t = _f32(table)
+        self._check(self._lib.sa_traces_mute(self._h, ctypes.c_void_p(mod_ptr), ctypes.c_void_p(obs_ptr),
+                                             ctypes.c_int(ntr), ctypes.c_int(nt), ctypes.c_int(t.size), _p(t)))
+
+    def traces_fir(self, ptr, ntr, nt, table, off, adjoint=False):
+        t = _f32(table)
+        self._check(self._lib.sa_traces_fir(self._h, ctypes.c_void_p(ptr), ctypes.c_int(ntr), ctypes.c_int(nt),
+                                            ctypes.c_int(t.size), _p(t), ctypes.c_int(off),
+                                            ctypes.c_int(int(adjoint))))
+
+    def traces_sosfilt(self, ptr, ntr, nt, sos, adjoint=False):
+        sos = np.ascontiguousarray(sos, dtype=np.float64)
+        self._check(self._lib.sa_traces_sosfilt(self._h, ctypes.c_void_p(ptr), ctypes.c_int(ntr), ctypes.c_int(nt),
+                                                ctypes.c_int(sos.shape[0]), _p(sos), ctypes.c_int(int(adjoint))))
+
+    def traces_norm(self, ptr, ntr, nt, per_trace=False):
+        norm = ctypes.c_double()
+        self._check(self._lib.sa_traces_norm(self._h, ctypes.c_void_p(ptr), ctypes.c_int(ntr), ctypes.c_int(nt),
+                                             ctypes.c_int(int(per_trace)), ctypes.byref(norm)))
+        return norm.value
+
+    def traces_l2(self, mod_ptr, obs_ptr, res_ptr, ntr, nt):
+        fun = ctypes.c_double()
+        self._check(self._lib.sa_traces_l2(self._h, ctypes.c_void_p(mod_ptr), ctypes.c_void_p(obs_ptr),
+                                           ctypes.c_void_p(res_ptr), ctypes.c_int(ntr), ctypes.c_int(nt),
+                                           ctypes.byref(fun)))
+        return fun.value
+
+    def traces_transpose(self, in_ptr, out_ptr, rows, cols, col_scale=None):
+        w = _f32(col_scale)
+        if w is not None and w.size != cols:
+            raise ValueError('column scale of length %d, expected %d' % (w.size, cols))
+        self._check(self._lib.sa_traces_transpose(self._h, ctypes.c_void_p(in_ptr), ctypes.c_void_p(out_ptr),
+                                                  ctypes.c_int(rows), ctypes.c_int(cols), _p(w)))
+
+    def field_mask(self, ptr, shape, lo, hi, ramp):
+        d = len(shape)
+        arr = lambda v: (ctypes.c_int * d)(*[int(x) for x in v])
+        self._check(self._lib.sa_field_mask(self._h, ctypes.c_void_p(ptr), ctypes.c_int(d), arr(shape), arr(lo),
+                                            arr(hi), ctypes.c_int(ramp)))
+
+    def field_gaussian(self, ptr, shape, sigma, mode='nearest'):
+        d = len(shape)
+        sg = sigma if np.iterable(sigma) else (sigma,) * d
+        self._check(self._lib.sa_field_gaussian(self._h, ctypes.c_void_p(ptr), ctypes.c_int(d),
+                                                (ctypes.c_int * d)(*[int(x) for x in shape]),
+                                                (ctypes.c_double * d)(*[float(x) for x in sg]),
+                                                ctypes.c_int({'nearest': 0, 'reflect': 1}[mode])))
+
+    def field_stats(self, ptr, n):
+        lo, hi, ab = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self._lib.sa_field_stats(self._h, ctypes.c_void_p(ptr), ctypes.c_int64(n), ctypes.byref(lo),
+                                             ctypes.byref(hi), ctypes.byref(ab)))
+        return lo.value, hi.value, ab.value
+
+    def field_axpb(self, ptr, n, a, b):
+        self._check(self._lib.sa_field_axpb(self._h, ctypes.c_void_p(ptr), ctypes.c_int64(n), ctypes.c_double(a),
+                                            ctypes.c_double(b)))
+
+    def field_div_nonzero(self, g_ptr, prec_ptr, n):
+        self._check(self._lib.sa_field_div_nonzero(self._h, ctypes.c_void_p(g_ptr), ctypes.c_void_p(prec_ptr),
+                                                   ctypes.c_int64(n)))
+
+    def field_descent(self, x_ptr, dir_ptr, n, step, slowness=True, lo=None, hi=None):
+        clip = lo is not None or hi is not None
+        self._check(self._lib.sa_field_descent(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(dir_ptr),
+                                               ctypes.c_int64(n), ctypes.c_double(step), ctypes.c_int(int(slowness)),
+                                               ctypes.c_int(int(clip)), ctypes.c_double(-np.inf if lo is None else lo),
+                                               ctypes.c_double(np.inf if hi is None else hi)))
+
+    def _box(self, fn, a_ptr, b_ptr, shape, lo, hi):
+        d = len(shape)
+        arr = lambda v: (ctypes.c_int * d)(*[int(x) for x in v])
+        self._check(fn(self._h, ctypes.c_void_p(a_ptr), ctypes.c_void_p(b_ptr), ctypes.c_int(d), arr(shape), arr(lo),
+                       arr(hi)))
+
+    def field_crop(self, grid_ptr, box_ptr, shape, lo, hi):
+        self._box(self._lib.sa_field_crop, grid_ptr, box_ptr, shape, lo, hi)
+
+    def field_embed(self, box_ptr, grid_ptr, shape, lo, hi):
+        self._box(self._lib.sa_field_embed, box_ptr, grid_ptr, shape, lo, hi)
+
+    def field_grad_vp_finish(self, grad_ptr, prec_ptr, vp_ptr, n, slowness=True):
+        self._check(self._lib.sa_field_grad_vp_finish(self._h, ctypes.c_void_p(grad_ptr) if grad_ptr else None,
+                                                      ctypes.c_void_p(prec_ptr) if prec_ptr else None,
+                                                      ctypes.c_void_p(vp_ptr), ctypes.c_int64(n),
+                                                      ctypes.c_int(int(slowness))))
+
+    def read_gradient_dev(self, which, dev_ptr):
+        self._check(self._lib.sa_read_gradient_dev(self._h, ctypes.c_int(which), ctypes.c_void_p(dev_ptr)))
 
     def set_tile(self, rows, chunk, version=2):
         self._check(self._lib.sa_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk), ctypes.c_int(version)))

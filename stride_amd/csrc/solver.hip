@@ -226,6 +226,12 @@ __global__ void k_pml(Geo g, int a, const float *__restrict__ F, float *u1, floa
     u1[p] = v1;
 }
 
+// padded field -> unpadded C-order array of the domain
+__global__ void k_unpad(Geo g, const float *__restrict__ src, float *__restrict__ dst) {
+    DOMAIN_INDEX
+    dst[((long long)i * g.n[1] + j) * g.n[2] + k] = src[p];
+}
+
 __global__ void k_scale_copy(Geo g, const float *__restrict__ src, const float *__restrict__ scale,
                              float *__restrict__ dst) {
     DOMAIN_INDEX
@@ -1642,3 +1648,17 @@ extern "C" int sa_batch_get_wavefield(sa_solver *s, int shot, int which, float *
                     tmp[2 * (g.base + (long long)i * g.s0 + (long long)j * g.s1 + k) + shot];
     return SA_OK;
 }
+
+extern "C" int sa_read_gradient_dev(sa_solver *s, int which, float *dev_out) {
+    if (!s || !dev_out || which < 0 || which > 3) return fail(SA_E_ARG, "bad arguments");
+    if (!s->gacc[which]) return fail(SA_E_STATE, "gradient %d was not computed", which);
+    HIP_TRY(hipSetDevice(s->device));
+    k_unpad<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, s->gacc[which], dev_out);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
+// accessors for the processing kernels (csrc/process.hip)
+hipStream_t sa_solver_stream(sa_solver *s) { return s->stream; }
+int sa_solver_device(sa_solver *s) { return s->device; }
+int sa_fail(int code, const char *msg) { return fail(code, "%s", msg); }
