@@ -137,6 +137,67 @@ def traffic_from_profile(shots=1):
         return None
 
 
+def bench_pipelines(plan, reps=5):
+    """Wall time of each device step either side of the solver at the full workload: traces of
+    nrec x nt (Traces rows), fields on the extended grid (gradient steps on the inner box). Each
+    step returns with its work finished; `gbs` is the step's algorithmic bytes (inputs read once,
+    outputs written once) over its time."""
+    import torch
+    from stride_amd import pipelines as pl
+    from stride_amd._native import Solver
+    s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, plan.nt)
+    dt = plan.dt
+    rng = np.random.default_rng(1)
+    ntr, nt = plan.nrec, plan.nt
+    mod = torch.from_numpy(rng.standard_normal((ntr, nt)).astype(np.float32)).cuda()
+    obs = torch.from_numpy(rng.standard_normal((ntr, nt)).astype(np.float32)).cuda()
+    obs[:, :200] = 0
+    tm = torch.from_numpy(rng.standard_normal((nt, ntr)).astype(np.float32)).cuda()
+    window = np.ones(nt, np.float32)
+    ext = tuple(plan.shape)
+    inner = tuple((a, n - a) for a, n in zip(plan.absorbing, ext))
+    ishape = tuple(b - a for a, b in inner)
+    g_ext = torch.from_numpy(rng.standard_normal(ext).astype(np.float32)).cuda()
+    g_in = torch.from_numpy(rng.standard_normal(ishape).astype(np.float32)).cuda()
+    p_in = torch.from_numpy(rng.random(ishape).astype(np.float32)).cuda()
+    vp_in = (1500 + 20 * torch.rand(ishape, device='cuda')).contiguous()
+    d_in = (1e-6 * torch.randn(ishape, device='cuda')).contiguous()
+    band = pl.FilterTraces(f_min=0.2e6, f_max=0.8e6)
+    low = pl.FilterTraces(f_max=0.8e6)
+    fir = pl.FilterTraces(f_min=0.2e6, f_max=0.8e6, filter_type='fir')
+    T, F = ntr * nt * 4, int(np.prod(ext)) * 4
+    I = int(np.prod(ishape)) * 4
+    steps = [
+        ('check_traces', lambda: pl.CheckTraces().forward(s, mod), 2 * T),
+        ('mute_traces', lambda: pl.MuteTraces(f_max=0.8e6).forward(s, mod, obs, dt), 4 * T),
+        ('filter butterworth band (8 sections)', lambda: band.forward(s, mod, dt=dt), 2 * T),
+        ('filter butterworth band adjoint', lambda: band.adjoint(s, mod, dt=dt), 2 * T),
+        ('filter cos low-pass', lambda: low.forward(s, mod, dt=dt), 2 * T),
+        ('filter fir band', lambda: fir.forward(s, mod, dt=dt), 2 * T),
+        ('norm_per_shot', lambda: pl.NormPerShot().forward(s, mod), 2 * T),
+        ('l2 misfit + residual', lambda: pl.L2DistanceLoss().forward(s, mod, obs), 3 * T),
+        ('traces transpose', lambda: pl.traces_from_solver(s, tm), 2 * T),
+        ('mask_field (extended)', lambda: pl.MaskField().forward(s, g_ext, inner), 2 * F),
+        ('smooth_field sigma 0.25 (extended)', lambda: pl.SmoothField().forward(s, g_ext), 2 * F),
+        ('norm_field (extended)', lambda: pl.NormField().forward(s, g_ext), 3 * F),
+        ('apply_prec sigma 1 (inner)', lambda: pl.apply_prec(s, g_in, p_in), 3 * I),
+        ('descent + clip (inner)', lambda: pl.GradientDescent(min=1400., max=1600.).step(s, vp_in, d_in), 3 * I),
+    ]
+    out = {}
+    for name, fn, nbytes in steps:
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        out[name] = {'ms': round(ms, 3), 'gbs': round(nbytes / ms / 1e6, 1)}
+        print('[pipelines] %-40s %9.3f ms %8.1f GB/s' % (name, ms, nbytes / ms / 1e6), file=sys.stderr, flush=True)
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -153,6 +214,9 @@ def main():
     ap.add_argument('--copy-check', type=int, default=0, metavar='MB',
                     help='only run 10 device-to-device copies of MB megabytes (calibrates the byte counters '
                          'of a profiler against a transfer of known size) and exit')
+    ap.add_argument('--pipelines', action='store_true',
+                    help='only time the device steps either side of the solver (trace processing, misfit, '
+                         'gradient post-processing, model update) at the full workload and exit')
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
@@ -175,6 +239,11 @@ def main():
         return
 
     cfg, plan = workload(scale=args.scale, nt=args.nt)
+    if args.pipelines:
+        res = bench_pipelines(plan)
+        print(json.dumps({'pipelines': res, 'traces': [plan.nrec, plan.nt], 'grid': list(plan.shape),
+                          'peak_gbs': HBM_PEAK_GBS}))
+        return
     print('[bench] workload %s ext, nt %d, f %d, nrec %d' % (plan.shape, plan.nt, plan.undersampling, plan.nrec),
           file=sys.stderr, flush=True)
     shots = args.shots

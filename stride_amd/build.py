@@ -25,25 +25,41 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith('.hip'))
 
 
-def build(verbose=False):
+def _headers():
+    inc = os.path.join(HERE, '..', 'include')
+    return [os.path.join(d, f) for d in (CSRC, inc) for f in os.listdir(d) if f.endswith('.h')]
+
+
+def build(verbose=False, objdir=None):
+    """Compile every source and link the library. With ``objdir``, objects are kept there and
+    a source is recompiled only when it or a header is newer than its object."""
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     os.makedirs(OUT_DIR, exist_ok=True)
+    keep = objdir is not None
+    objdir = objdir or OUT_DIR
+    os.makedirs(objdir, exist_ok=True)
+    newest_header = max(os.path.getmtime(h) for h in _headers())
     objs = []
     for src in sources():
-        obj = os.path.join(OUT_DIR, os.path.basename(src)[:-4] + '.o')
-        cmd = [hipcc, '-c', src, '-o', obj] + FLAGS
-        if verbose:
-            print(' '.join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        obj = os.path.join(objdir, os.path.basename(src)[:-4] + '.o')
+        fresh = keep and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(src), newest_header)
+        if not fresh:
+            cmd = [hipcc, '-c', src, '-o', obj] + FLAGS
+            if verbose:
+                print(' '.join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
         objs.append(obj)
     tmp = LIB + '.tmp'
     cmd = [hipcc, '-shared', '-o', tmp] + objs + ['--offload-arch=%s' % ARCH, '-fPIC']
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
-    for obj in objs:
-        os.remove(obj)
+    if not keep:
+        for obj in objs:
+            os.remove(obj)
     return LIB
 
 
 if __name__ == '__main__':
-    print(build(verbose='-v' in sys.argv))
+    # --objdir DIR: keep objects in DIR and rebuild only what changed
+    objdir = sys.argv[sys.argv.index('--objdir') + 1] if '--objdir' in sys.argv else None
+    print(build(verbose='-v' in sys.argv, objdir=objdir))

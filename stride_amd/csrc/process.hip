@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -62,12 +63,13 @@ __global__ void k_row_sumsq(const float *__restrict__ a, const float *__restrict
     if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+// sum of n doubles in a fixed order (thread i sums i, i + kT, ... then a fixed tree)
 __global__ void k_ordered_sum(const double *__restrict__ v, int n, double *__restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double s = 0.0;
-        for (int i = 0; i < n; ++i) s += v[i];
-        *out = s;
-    }
+    __shared__ double sh[kT];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kT) acc += v[i];
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) *out = t;
 }
 
 __global__ void k_row_bad(const float *__restrict__ d, int nt, int *__restrict__ bad) {
@@ -127,26 +129,95 @@ __global__ void k_mute(float *__restrict__ mod, const float *__restrict__ obs, i
     mod[base + i] = (float)(m * mod[base + i]);
 }
 
-// cascaded second-order sections (b0 b1 b2 a0 a1 a2, a0 = 1), transposed direct form II, in
-// double precision; one lane per row, forward in time or (adjoint) backward
-__global__ void k_sosfilt(float *__restrict__ d, int ntr, int nt, const double *__restrict__ sos, int nsec,
-                          int adjoint) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= ntr) return;
-    double z1[32], z2[32];
-    for (int k = 0; k < nsec; ++k) z1[k] = z2[k] = 0.0;
-    const long long base = (long long)r * nt;
-    for (int n = 0; n < nt; ++n) {
-        const int t = adjoint ? nt - 1 - n : n;
-        double x = d[base + t];
-        for (int k = 0; k < nsec; ++k) {
-            const double *c = sos + 6 * k;
-            const double y = c[0] * x + z1[k];
-            z1[k] = c[1] * x - c[4] * y + z2[k];
-            z2[k] = c[2] * x - c[5] * y;
-            x = y;
+// Cascaded second-order sections (b0 b1 b2 a0 a1 a2, a0 = 1), transposed direct form II, in
+// double precision, forward in time or (adjoint) backward. Systolic: a group of G lanes
+// filters one row, lane k running section k (lanes k >= nsec pass their input through); at
+// iteration n lane k works on sample n - k, taking lane k-1's output of iteration n-1, so the
+// per-sample chain is one section instead of nsec (the hand-over is a DPP lane shift; the
+// shift across group boundaries reaches lane 0 of the next group, which ignores it). Lane 0
+// reads the row 16 samples ahead in
+// registers; lane G-1 writes sample n - G + 1 (already read: the filter runs in place). Each
+// section sees the same operations in the same order as a sequential cascade.
+// value of the lane one below (DPP wave_shr:1; lane 0 gets 0), both halves of a double
+__device__ __forceinline__ double wave_shr1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Cascaded second-order sections (b0 b1 b2 a0 a1 a2, a0 = 1), transposed direct form II, in
+// double precision, forward in time or (adjoint) backward. Systolic: a group of G lanes
+// filters one row, lane k running section k (lanes k >= nsec pass their input through); at
+// iteration n lane k works on sample n - k, taking lane k-1's output of iteration n-1 (a DPP
+// lane shift; across a group boundary it reaches lane 0 of the next group, which ignores it),
+// so the per-sample chain is one section instead of nsec. The block stages its rows through
+// LDS in chunks of C samples, loading chunk c+1 with coalesced reads while it filters chunk c;
+// lane G-1 writes sample n - G + 1, already staged (the filter runs in place). Each section
+// sees the same operations in the same order as a sequential cascade.
+template <int G>
+__global__ void __launch_bounds__(kT) k_sosfilt(float *__restrict__ d, int ntr, int nt,
+                                                const double *__restrict__ sos, int nsec, int adjoint) {
+    constexpr int RB = kT / G;        // rows per block
+    constexpr int C = 128;            // samples per chunk
+    constexpr int PER = RB * C / kT;  // staged samples per thread and chunk
+    __shared__ float buf[2][RB][C + 1];
+    const int k = threadIdx.x % G;
+    const int lr = threadIdx.x / G;
+    const int r = blockIdx.x * RB + lr;
+    const bool live = r < ntr;
+    const bool pass = k >= nsec;
+    double b0 = 0, b1 = 0, b2 = 0, a1 = 0, a2 = 0;
+    if (!pass) {
+        const double *c = sos + 6 * k;
+        b0 = c[0], b1 = c[1], b2 = c[2], a1 = c[4], a2 = c[5];
+    }
+    auto at = [&](int row, int t) { return (long long)row * nt + (adjoint ? nt - 1 - t : t); };
+    float stage[PER];
+    auto fetch = [&](int ch) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = threadIdx.x + kT * j, row = blockIdx.x * RB + e / C, t = ch * C + e % C;
+            stage[j] = ((row < ntr) & (t < nt)) ? d[at(row, t)] : 0.f;
         }
-        d[base + t] = (float)x;
+    };
+    auto put = [&](int b) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = threadIdx.x + kT * j;
+            buf[b][e / C][e % C] = stage[j];
+        }
+    };
+    fetch(0);
+    put(0);
+    __syncthreads();
+    double z1 = 0.0, z2 = 0.0, y = 0.0;
+    const int nch = (nt + G - 1 + C - 1) / C;
+    for (int ch = 0; ch < nch; ++ch) {
+        const int cb = ch & 1;
+        fetch(ch + 1);
+        for (int i0 = 0; i0 < C; i0 += 8) {
+            float xs[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) xs[u] = buf[cb][lr][i0 + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int smp = ch * C + i0 + u - k;
+                const double prev = wave_shr1(y);
+                const double x = k == 0 ? (double)xs[u] : prev;
+                const double o = b0 * x + z1;
+                const double nz1 = b1 * x - a1 * o + z2;
+                const double nz2 = b2 * x - a2 * o;
+                const bool in = (smp >= 0) & (smp < nt);
+                const bool upd = in & !pass;
+                z1 = upd ? nz1 : z1;
+                z2 = upd ? nz2 : z2;
+                y = in ? (pass ? x : o) : y;
+                if (live & (k == G - 1) & in) d[at(r, smp)] = (float)y;
+            }
+        }
+        put(cb ^ 1);
+        __syncthreads();
     }
 }
 
@@ -174,81 +245,148 @@ struct Shape {
     int n[3];
 };
 
-__device__ __forceinline__ void unflat(const Shape &s, long long p, int (&c)[3]) {
-    c[2] = (int)(p % s.n[2]);
-    const long long q = p / s.n[2];
-    c[1] = (int)(q % s.n[1]);
-    c[0] = (int)(q / s.n[1]);
+
+// Field kernels take one thread per cell in C order (32-bit indices: fields below 2^31 cells).
+
+// Mask of stride's MaskField (mask_field.py:8-67): 1 inside [lo, hi) x the ramp-off profile.
+// The reference writes, per axis d (ascending) and index i < ramp, tab[i] on the two faces
+// c[d] = i and c[d] = n_d - 1 - i restricted to i <= c[e] <= n_e - i on the other axes; later
+// writes win. Axes with 2 ramp > n_d are skipped. A cell matches at most one index per axis,
+// i = min(c[d], n_d - 1 - c[d]), so its value is tab[i] of the last axis whose write covers
+// it, or 1.
+constexpr int kMaskRamp = 1024;  // longest ramp-off (cells)
+
+constexpr int kEl = 4;  // cells per thread in the field kernels below (strided by kT)
+
+__device__ __forceinline__ void cell_coords(const Shape &s, unsigned p, int (&c)[3]) {
+    const unsigned q = p / (unsigned)s.n[2];
+    c[0] = (int)(q / (unsigned)s.n[1]);
+    c[1] = (int)(q % (unsigned)s.n[1]);
+    c[2] = (int)(p - q * (unsigned)s.n[2]);
 }
 
-// mask of stride's MaskField (mask_field.py:8-67): 1 inside [lo, hi) x ramp-off profile
-// assigned per axis, index and side in the reference's loop order (later writes win)
-__global__ void k_mask(float *__restrict__ f, Shape s, int dim, Shape lo, Shape hi, int ramp) {
-    const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
-    const long long p = (long long)blockIdx.x * kT + threadIdx.x;
-    if (p >= n) return;
-    int c[3];
-    unflat(s, p, c);
+__global__ void k_mask(float *__restrict__ f, Shape s, int dim, Shape lo, Shape hi, int ramp,
+                       const float *__restrict__ tab, unsigned n) {
+    __shared__ float lt[kMaskRamp];
+    for (int i = threadIdx.x; i < ramp; i += kT) lt[i] = tab[i];
+    const unsigned p0 = blockIdx.x * (kT * kEl) + threadIdx.x;
+    float v[kEl];
+#pragma unroll
+    for (int u = 0; u < kEl; ++u) v[u] = p0 + u * kT < n ? f[p0 + u * kT] : 0.f;
+    __syncthreads();
     const int off = 3 - dim;  // leading unit axes of a lower-dimensional field
-    bool inner = true;
-    for (int a = off; a < 3; ++a) inner = inner && c[a] >= lo.n[a] && c[a] < hi.n[a];
-    float m = 1.f;
-    for (int d = off; d < 3; ++d) {
-        if (2 * ramp > s.n[d]) continue;
-        for (int index = 0; index < ramp; ++index) {
-            const float pos = fabsf((float)(ramp - index - 1) / (float)(ramp - 1));
-            const float val = 1.f - cosf(3.14159265358979f / 2.f * (1.f - pos));
-            bool others = true;
-            for (int e = off; e < 3; ++e)
-                if (e != d) others = others && c[e] >= index && c[e] <= s.n[e] - index;
-            if (!others) continue;
-            if (c[d] == index) m = val;
-            if (c[d] == s.n[d] - index - 1) m = val;
+#pragma unroll
+    for (int u = 0; u < kEl; ++u) {
+        const unsigned p = p0 + u * kT;
+        if (p >= n) break;
+        int c[3];
+        cell_coords(s, p, c);
+        bool inner = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) inner = inner & ((a < off) | ((c[a] >= lo.n[a]) & (c[a] < hi.n[a])));
+        float m = 1.f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int i = min(c[d], s.n[d] - 1 - c[d]);
+            bool hit = (d >= off) & (2 * ramp <= s.n[d]) & (i < ramp);
+#pragma unroll
+            for (int e = 0; e < 3; ++e)
+                if (e != d) hit = hit & ((e < off) | ((c[e] >= i) & (c[e] <= s.n[e] - i)));
+            if (hit) m = lt[i];
         }
+        f[p] = v[u] * (inner ? m : 0.f);
     }
-    f[p] *= inner ? m : 0.f;
 }
 
-// one axis of a separable correlation with weights w[0 .. 2R] (centre R), boundary mode
-// 0 nearest (edge value), 1 reflect (d c b a | a b c d | d c b a)
+// One axis of a separable correlation with weights w[0 .. 2R] (centre R), boundary mode
+// 0 nearest (edge value), 1 reflect (d c b a | a b c d | d c b a); the boundary index map is
+// evaluated only within R of an edge. Blocks take contiguous runs of kT cells (whole cache
+// lines, whatever the row length). Along axis 0 the 2R + 1 planes a block reads are shared with
+// the blocks at the same place in the neighbouring planes: block b runs on XCD b % 8, and each
+// XCD takes a contiguous run of (plane-fastest) chunks, so those blocks share its L2.
+template <int RM, int EL>  // RM > 0: R <= RM, taps unrolled; RM = 0: any R. EL cells per thread
 __global__ void k_axis_filter(const float *__restrict__ in, float *__restrict__ out, Shape s, int axis,
-                              const double *__restrict__ w, int R, int mode) {
-    const long long n = (long long)s.n[0] * s.n[1] * s.n[2];
-    const long long p = (long long)blockIdx.x * kT + threadIdx.x;
-    if (p >= n) return;
-    int c[3];
-    unflat(s, p, c);
-    const long long stride = axis == 2 ? 1 : (axis == 1 ? s.n[2] : (long long)s.n[1] * s.n[2]);
-    const int len = s.n[axis];
-    double acc = 0.0;
-    for (int j = -R; j <= R; ++j) {
-        int t = c[axis] + j;
-        if (mode == 0) {
-            t = t < 0 ? 0 : (t >= len ? len - 1 : t);
-        } else {
-            const int period = 2 * len;
-            t = ((t % period) + period) % period;
-            if (t >= len) t = period - 1 - t;
-        }
-        acc += w[j + R] * in[p + (long long)(t - c[axis]) * stride];
+                              const double *__restrict__ w, int R, int mode, unsigned cpp, unsigned per_xcd) {
+    constexpr int CH = kT * EL;  // cells per block
+    const unsigned P = (unsigned)s.n[1] * (unsigned)s.n[2];
+    const unsigned n = P * (unsigned)s.n[0];
+    unsigned p0, lim;  // first cell of this thread, end of its run
+    if (axis == 0) {
+        const unsigned b = blockIdx.x;
+        const unsigned L = (b & 7) * per_xcd + (b >> 3);
+        const unsigned z = L % (unsigned)s.n[0], j = L / (unsigned)s.n[0];
+        if (j >= cpp) return;
+        p0 = z * P + j * CH + threadIdx.x;
+        lim = z * P + P;
+    } else {
+        p0 = blockIdx.x * CH + threadIdx.x;
+        lim = n;
     }
-    out[p] = (float)acc;
+    const unsigned stride = axis == 2 ? 1u : (axis == 1 ? (unsigned)s.n[2] : P);
+    const int len = s.n[axis];
+    auto src = [&](int t) {  // boundary map of the line index t
+        if (mode == 0) return t < 0 ? 0 : (t >= len ? len - 1 : t);
+        const int period = 2 * len;
+        t = ((t % period) + period) % period;
+        return t >= len ? period - 1 - t : t;
+    };
+    if (RM > 0) {
+        constexpr int NT = RM > 0 ? 2 * RM + 1 : 1;
+        float v[EL][NT];
+#pragma unroll
+        for (int u = 0; u < EL; ++u) {  // every load of the thread first, then the sums
+            const unsigned p = p0 + u * kT;
+            const bool live = p < lim;
+            int c[3];
+            cell_coords(s, live ? p : 0u, c);
+            const int ca = c[axis];
+            const float *q0 = in + ((live ? p : 0u) - (unsigned)ca * stride);
+            const bool inside = (ca >= R) & (ca + R < len);
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const int t = ca - R + j;
+                v[u][j] = (live & (j <= 2 * R)) ? q0[(unsigned)(inside ? t : src(t)) * stride] : 0.f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < EL; ++u) {
+            const unsigned p = p0 + u * kT;
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                if (j <= 2 * R) acc += w[j] * v[u][j];
+            if (p < lim) out[p] = (float)acc;
+        }
+    } else {
+        for (int u = 0; u < EL; ++u) {
+            const unsigned p = p0 + u * kT;
+            if (p >= lim) break;
+            int c[3];
+            cell_coords(s, p, c);
+            const int ca = c[axis];
+            const float *q0 = in + (p - (unsigned)ca * stride);
+            double acc = 0.0;
+            for (int j = -R; j <= R; ++j) acc += w[j + R] * q0[(unsigned)src(ca + j) * stride];
+            out[p] = (float)acc;
+        }
+    }
 }
 
-// box of a grid: to_box 1: box[b] = grid[lo + b]; 0: grid[lo + b] = box[b]
+// box of a grid, one thread per box cell in order: to_box 1: box[b] = grid[lo + b]; 0:
+// grid[lo + b] = box[b]
 __global__ void k_box_copy(float *__restrict__ grid, float *__restrict__ box, Shape s, Shape lo, Shape bn,
-                           int to_box) {
-    const long long n = (long long)bn.n[0] * bn.n[1] * bn.n[2];
-    const long long b = (long long)blockIdx.x * kT + threadIdx.x;
+                           unsigned n, int to_box) {
+    const unsigned b = blockIdx.x * kT + threadIdx.x;
     if (b >= n) return;
-    int c[3];
-    unflat(bn, b, c);
+    const unsigned q = b / (unsigned)bn.n[2];
+    const int c[3] = {(int)(q / (unsigned)bn.n[1]), (int)(q % (unsigned)bn.n[1]), (int)(b - q * (unsigned)bn.n[2])};
     const long long g = ((long long)(c[0] + lo.n[0]) * s.n[1] + (c[1] + lo.n[1])) * s.n[2] + (c[2] + lo.n[2]);
     if (to_box)
         box[b] = grid[g];
     else
         grid[g] = box[b];
 }
+
 
 // compat-mode vp gradient post-scale: slowness transform: grad /= vp, prec /= vp^2; else
 // grad *= -2 / vp^3, prec *= (2 / vp)^9
@@ -341,7 +479,7 @@ int rows_sumsq(sa_solver *s, const float *a, const float *b, float *res, int ntr
                double *total) {
     hipStream_t st = sa_solver_stream(s);
     k_row_sumsq<<<ntr, kT, 0, st>>>(a, b, res, nt, rows);
-    k_ordered_sum<<<1, 1, 0, st>>>(rows, ntr, total);
+    k_ordered_sum<<<1, kT, 0, st>>>(rows, ntr, total);
     PTRY(hipGetLastError());
     return SA_OK;
 }
@@ -418,7 +556,15 @@ extern "C" int sa_traces_sosfilt(sa_solver *s, float *d, int ntr, int nt, int ns
     if ((rc = sc.alloc(6 * nsec * sizeof(double)))) return rc;
     hipStream_t st = sa_solver_stream(s);
     PTRY(hipMemcpyAsync(sc.p, sos, 6 * nsec * sizeof(double), hipMemcpyHostToDevice, st));
-    k_sosfilt<<<(ntr + 63) / 64, 64, 0, st>>>(d, ntr, nt, (const double *)sc.p, nsec, adjoint);
+    const double *c = (const double *)sc.p;
+    const int G = nsec <= 8 ? 8 : (nsec <= 16 ? 16 : 32);  // lanes per row
+    const dim3 g((ntr + kT / G - 1) / (kT / G));
+    if (G == 8)
+        k_sosfilt<8><<<g, kT, 0, st>>>(d, ntr, nt, c, nsec, adjoint);
+    else if (G == 16)
+        k_sosfilt<16><<<g, kT, 0, st>>>(d, ntr, nt, c, nsec, adjoint);
+    else
+        k_sosfilt<32><<<g, kT, 0, st>>>(d, ntr, nt, c, nsec, adjoint);
     PTRY(hipGetLastError());
     PTRY(hipStreamSynchronize(st));
     return SA_OK;
@@ -491,8 +637,20 @@ extern "C" int sa_field_mask(sa_solver *s, float *f, int dim, const int *shape, 
         l.n[3 - dim + a] = lo[a];
         h.n[3 - dim + a] = hi[a];
     }
+    if (ramp > kMaskRamp) return sa_fail(SA_E_ARG, "mask ramp longer than 1024 cells");
+    if (ramp < 2) ramp = 0;  // one-cell ramps divide by zero in the reference: no ramp
+    std::vector<float> tab(std::max(ramp, 1), 1.f);
+    for (int i = 0; i < ramp; ++i) {  // mask_field.py:25-27 in double, stored as float32
+        const double pos = std::fabs((double)(ramp - i - 1) / (double)(ramp - 1));
+        tab[i] = (float)(1.0 - std::cos(M_PI / 2.0 * (1.0 - pos)));
+    }
+    Scratch sc;
+    if ((rc = sc.alloc(tab.size() * sizeof(float)))) return rc;
+    hipStream_t st = sa_solver_stream(s);
+    PTRY(hipMemcpyAsync(sc.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, st));
     const long long n = (long long)sh.n[0] * sh.n[1] * sh.n[2];
-    k_mask<<<blocks(n), kT, 0, sa_solver_stream(s)>>>(f, sh, dim, l, h, ramp < 2 ? 0 : ramp);
+    if (n >= (1LL << 31)) return sa_fail(SA_E_ARG, "field larger than 2^31 cells");
+    k_mask<<<(n + kT * kEl - 1) / (kT * kEl), kT, 0, st>>>(f, sh, dim, l, h, ramp, (const float *)sc.p, (unsigned)n);
     PTRY(hipGetLastError());
     PTRY(hipStreamSynchronize(sa_solver_stream(s)));
     return SA_OK;
@@ -511,6 +669,7 @@ static int box_args(int dim, const int *shape, const int *lo, const int *hi, Sha
         l.n[k] = lo[a];
         bn.n[k] = hi[a] - lo[a];
     }
+    if ((long long)bn.n[0] * bn.n[1] * bn.n[2] >= (1LL << 31)) return sa_fail(SA_E_ARG, "box larger than 2^31 cells");
     return SA_OK;
 }
 
@@ -522,7 +681,7 @@ extern "C" int sa_field_crop(sa_solver *s, const float *grid, float *box, int di
     if (!grid || !box) return sa_fail(SA_E_ARG, "null array");
     if ((rc = box_args(dim, shape, lo, hi, sh, l, bn))) return rc;
     const long long n = (long long)bn.n[0] * bn.n[1] * bn.n[2];
-    if (n) k_box_copy<<<blocks(n), kT, 0, sa_solver_stream(s)>>>((float *)grid, box, sh, l, bn, 1);
+    if (n) k_box_copy<<<blocks(n), kT, 0, sa_solver_stream(s)>>>((float *)grid, box, sh, l, bn, (unsigned)n, 1);
     PTRY(hipGetLastError());
     return SA_OK;
 }
@@ -535,7 +694,7 @@ extern "C" int sa_field_embed(sa_solver *s, const float *box, float *grid, int d
     if (!grid || !box) return sa_fail(SA_E_ARG, "null array");
     if ((rc = box_args(dim, shape, lo, hi, sh, l, bn))) return rc;
     const long long n = (long long)bn.n[0] * bn.n[1] * bn.n[2];
-    if (n) k_box_copy<<<blocks(n), kT, 0, sa_solver_stream(s)>>>(grid, (float *)box, sh, l, bn, 0);
+    if (n) k_box_copy<<<blocks(n), kT, 0, sa_solver_stream(s)>>>(grid, (float *)box, sh, l, bn, (unsigned)n, 0);
     PTRY(hipGetLastError());
     return SA_OK;
 }
@@ -551,36 +710,64 @@ extern "C" int sa_field_grad_vp_finish(sa_solver *s, float *grad, float *prec, c
 }
 
 // separable Gaussian smoothing as scipy.ndimage.gaussian_filter (truncate 4): per axis the
-// weights exp(-x^2 / (2 sigma^2)), x = -R .. R, R = int(4 sigma + 0.5), normalised
+// weights exp(-x^2 / (2 sigma^2)), x = -R .. R, R = int(4 sigma + 0.5), normalised. Axis
+// passes alternate between the field and a scratch copy.
 extern "C" int sa_field_gaussian(sa_solver *s, float *f, int dim, const int *shape, const double *sigma, int mode) {
     int rc = prep(s);
     if (rc) return rc;
     if (!f || !shape || !sigma || dim < 1 || dim > 3 || mode < 0 || mode > 1)
         return sa_fail(SA_E_ARG, "bad smoothing arguments");
     Shape sh{{1, 1, 1}};
-    for (int a = 0; a < dim; ++a) sh.n[3 - dim + a] = shape[a];
+    for (int a = 0; a < dim; ++a) {
+        if (shape[a] < 1) return sa_fail(SA_E_ARG, "bad smoothing arguments");
+        sh.n[3 - dim + a] = shape[a];
+    }
     const long long n = (long long)sh.n[0] * sh.n[1] * sh.n[2];
-    Scratch sc;
-    const long long nf = (n + 1) / 2 * 2;  // keeps the weights 8-byte aligned
-    if ((rc = sc.alloc(nf * sizeof(float) + 4096 * sizeof(double)))) return rc;
-    float *tmp = (float *)sc.p;
-    double *w = (double *)(tmp + nf);
-    hipStream_t st = sa_solver_stream(s);
+    if (n >= (1LL << 31)) return sa_fail(SA_E_ARG, "field larger than 2^31 cells");
+    std::vector<double> h;  // weights of every smoothed axis, back to back
+    int axis[3], R[3], at[3], npass = 0;
     for (int a = 0; a < dim; ++a) {
         const double sg = sigma[a];
         if (!(sg > 0)) continue;
-        const int R = (int)(4.0 * sg + 0.5);
-        if (2 * R + 1 > 4096) return sa_fail(SA_E_ARG, "smoothing radius too large");
-        std::vector<double> h(2 * R + 1);
+        const int r = (int)(4.0 * sg + 0.5);
+        if (2 * r + 1 > 4096) return sa_fail(SA_E_ARG, "smoothing radius too large");
+        at[npass] = (int)h.size();
         double sum = 0.0;
-        for (int j = -R; j <= R; ++j) sum += (h[j + R] = std::exp(-0.5 * (double)j * j / (sg * sg)));
-        for (double &v : h) v /= sum;
-        PTRY(hipMemcpyAsync(w, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
-        k_axis_filter<<<blocks(n), kT, 0, st>>>(f, tmp, sh, 3 - dim + a, w, R, mode);
-        PTRY(hipMemcpyAsync(f, tmp, n * sizeof(float), hipMemcpyDeviceToDevice, st));
-        PTRY(hipStreamSynchronize(st));  // the host weights are reused for the next axis
+        for (int j = -r; j <= r; ++j) {
+            h.push_back(std::exp(-0.5 * (double)j * j / (sg * sg)));
+            sum += h.back();
+        }
+        for (int j = at[npass]; j < (int)h.size(); ++j) h[j] /= sum;
+        axis[npass] = 3 - dim + a;
+        R[npass++] = r;
+    }
+    if (!npass) return SA_OK;
+    const long long nf = (n + 1) / 2 * 2;  // keeps the weights 8-byte aligned
+    Scratch sc;
+    if ((rc = sc.alloc(nf * sizeof(float) + h.size() * sizeof(double)))) return rc;
+    float *tmp = (float *)sc.p;
+    double *w = (double *)(tmp + nf);
+    hipStream_t st = sa_solver_stream(s);
+    PTRY(hipMemcpyAsync(w, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    float *src = f, *dst = tmp;
+    for (int k = 0; k < npass; ++k) {
+        const int el = R[k] <= 2 ? 4 : (R[k] <= 4 ? 2 : 1);  // cells per thread
+        const unsigned P = (unsigned)sh.n[1] * sh.n[2], cpp = (P + kT * el - 1) / (kT * el);
+        const unsigned nblk = axis[k] == 0 ? cpp * sh.n[0] : (unsigned)((n + kT * el - 1) / (kT * el));
+        const unsigned per_xcd = (nblk + 7) / 8;
+        const unsigned nb = axis[k] == 0 ? per_xcd * 8 : nblk;
+        const double *wk = w + at[k];
+        if (R[k] <= 2)
+            k_axis_filter<2, 4><<<nb, kT, 0, st>>>(src, dst, sh, axis[k], wk, R[k], mode, cpp, per_xcd);
+        else if (R[k] <= 4)
+            k_axis_filter<4, 2><<<nb, kT, 0, st>>>(src, dst, sh, axis[k], wk, R[k], mode, cpp, per_xcd);
+        else
+            k_axis_filter<0, 1><<<nb, kT, 0, st>>>(src, dst, sh, axis[k], wk, R[k], mode, cpp, per_xcd);
+        std::swap(src, dst);
     }
     PTRY(hipGetLastError());
+    if (src != f) PTRY(hipMemcpyAsync(f, src, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    PTRY(hipStreamSynchronize(st));  // the host weights and the scratch live until here
     return SA_OK;
 }
 
@@ -589,7 +776,7 @@ extern "C" int sa_field_stats(sa_solver *s, const float *f, int64_t n, double *v
     int rc = prep(s);
     if (rc) return rc;
     if (!f || n < 1) return sa_fail(SA_E_ARG, "bad field arguments");
-    const int nb = (int)std::min<int64_t>(1024, (n + kT - 1) / kT);
+    const int nb = (int)std::min<int64_t>(4096, (n + kT - 1) / kT);
     Scratch sc;
     if ((rc = sc.alloc(nb * (2 * sizeof(float) + sizeof(double))))) return rc;
     double *pabs = (double *)sc.p;

@@ -273,6 +273,50 @@ __global__ void k_gather(int npt, int ntap, const long long *__restrict__ idx, c
     out[r] = sign * acc;
 }
 
+// both shots of a two-shot pass in one launch (blockIdx.y = shot; interleaved fields, S = 2)
+struct InjectSet {
+    int ndest;
+    const long long *dest;
+    const int *ijk, *off, *point;
+    const float *coef, *values;
+};
+struct GatherSet {
+    int npt, ntap;
+    const long long *idx;
+    const float *w;
+    float *out;
+};
+
+__global__ void k_inject2(InjectSet s0, InjectSet s1, float sign, float *field, float *save, Box box,
+                          float save_scale) {
+    const int sh = blockIdx.y;
+    const InjectSet q = sh ? s1 : s0;
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= q.ndest) return;
+    float acc = 0.f;
+    for (int c = q.off[d]; c < q.off[d + 1]; ++c) acc += q.coef[c] * q.values[q.point[c]];
+    acc *= sign;
+    field[q.dest[d] * 2 + sh] += acc;
+    if (save) {
+        const int i = q.ijk[3 * d], j = q.ijk[3 * d + 1], k = q.ijk[3 * d + 2];
+        if (sa_inbox(box, i, j, k)) save[sa_boxidx(box, i, j, k) * 2 + sh] += acc * save_scale;
+    }
+}
+
+__global__ void k_gather2(GatherSet s0, GatherSet s1, const float *__restrict__ field, float sign) {
+    const int sh = blockIdx.y;
+    const GatherSet q = sh ? s1 : s0;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= q.npt) return;
+    float acc = 0.f;
+    for (int t = 0; t < q.ntap; ++t) {
+        const long long p = q.idx[(long long)r * q.ntap + t];
+        if (p < 0) continue;
+        acc += q.w[(long long)r * q.ntap + t] * field[p * 2 + sh];
+    }
+    q.out[r] = sign * acc;
+}
+
 // per-tap values field / G (exact adjoint: lambda at each source tap); idx padded
 __global__ void k_gather_taps(int nq, const long long *__restrict__ idx, const float *__restrict__ field,
                               const float *__restrict__ G, float *__restrict__ out) {
@@ -993,6 +1037,29 @@ static int gather(sa_solver *s, Sparse &sp, const float *field, float sign, floa
     return SA_OK;
 }
 
+// injections of both shots of a two-shot pass (values[k]: shot k's samples of this step)
+static int inject2(sa_solver *s, Sparse (&sp)[2], const float *const (&values)[2], float sign, float *field,
+                   float *save, float save_scale, const Box &box) {
+    const int n = std::max(sp[0].ndest, sp[1].ndest);
+    if (n == 0) return SA_OK;
+    InjectSet q[2];
+    for (int k = 0; k < 2; ++k)
+        q[k] = InjectSet{sp[k].ndest, sp[k].d_dest, sp[k].d_ijk, sp[k].d_off, sp[k].d_point, sp[k].d_coef, values[k]};
+    k_inject2<<<dim3((n + 127) / 128, 2), 128, 0, s->stream>>>(q[0], q[1], sign, field, save, box, save_scale);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
+static int gather2(sa_solver *s, Sparse (&sp)[2], const float *field, float sign, float *const (&out)[2]) {
+    const int n = std::max(sp[0].npoint, sp[1].npoint);
+    if (n == 0) return SA_OK;
+    GatherSet q[2];
+    for (int k = 0; k < 2; ++k) q[k] = GatherSet{sp[k].npoint, sp[k].ntap, sp[k].d_idx, sp[k].d_w, out[k]};
+    k_gather2<<<dim3((n + 127) / 128, 2), 128, 0, s->stream>>>(q[0], q[1], field, sign);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
 static int ensure_staging(sa_solver *s, float **p, size_t *have, size_t n) {
     if (*have >= n && *p) return SA_OK;
     dfree(*p);
@@ -1590,10 +1657,10 @@ extern "C" int sa_batch_forward_dev(sa_solver *s, const float *const *src, float
         TRY(timed_begin(s, &e0));
         TRY(fast3d_step_b2(s->fast, s->geo, u, o, o, save, s->bbox, idt2, s->stream));
         TRY(timed_end(s));
-        for (int k = 0; k < 2; ++k) {
-            TRY(inject(s, s->bsrc[k], src[k] + (size_t)t * s->bsrc[k].npoint, 1.f, o, save, idt2, s->bbox, 2, k));
-            TRY(gather(s, s->brec[k], o, 1.f, rec[k] + (size_t)t * s->brec[k].npoint, 2, k));
-        }
+        const float *const sv[2] = {src[0] + (size_t)t * s->bsrc[0].npoint, src[1] + (size_t)t * s->bsrc[1].npoint};
+        float *const rv[2] = {rec[0] + (size_t)t * s->brec[0].npoint, rec[1] + (size_t)t * s->brec[1].npoint};
+        TRY(inject2(s, s->bsrc, sv, 1.f, o, save, idt2, s->bbox));
+        TRY(gather2(s, s->brec, o, 1.f, rv));
     }
     s->cur = (tlast + 1) % 2;
     return SA_OK;
@@ -1625,8 +1692,8 @@ extern "C" int sa_batch_adjoint_dev(sa_solver *s, const float *const *res) {
         TRY(timed_begin(s, &e0));
         TRY(fast3d_step_adjoint_b2(s->fast, s->geo, u, o, o, ps, s->bbox, s->gacc[0], s->gacc[1], s->stream));
         TRY(timed_end(s));
-        for (int k = 0; k < 2; ++k)
-            TRY(inject(s, s->brec[k], res[k] + (size_t)t * s->brec[k].npoint, -1.f, o, nullptr, 0.f, s->bbox, 2, k));
+        const float *const rv[2] = {res[0] + (size_t)t * s->brec[0].npoint, res[1] + (size_t)t * s->brec[1].npoint};
+        TRY(inject2(s, s->brec, rv, -1.f, o, nullptr, 0.f, s->bbox));
     }
     s->cur = (tend - 1 + 2) % 2;
     return SA_OK;
