@@ -198,6 +198,68 @@ def bench_pipelines(plan, reps=5):
     return out
 
 
+def bench_schedule(tile=None, num_shots=32):
+    """Batched 3D forward over uneven shots: the 360^3 grid and hemisphere medium, 32 shots whose
+    durations (1500-3000 steps) and receiver counts (256-1024 hemisphere points) come from the
+    seeded schedule. Shots are paired by duration and each pair advances together through the
+    two-shot update (interleaved wavefields, one vp read), the pair running for its longer shot's
+    steps. Counted work: each shot's own forward updates, (nt_i - 2) x grid points."""
+    from stride_amd.configs import config3, shot_schedule
+    from stride_amd.physics.plan import build_plan
+    from stride_amd.utils import ellipsoidal
+    from stride_amd._native import Solver
+    nts, nrecs = shot_schedule(num_shots)
+    nt_max = int(nts.max())
+    cfg = config3(scale=1.0, nt=nt_max, nrec=1024)
+    space, time_ = cfg['space'], cfg['time']
+    c = tuple(l / 2 for l in space.limit)
+    plans = []
+    for i in range(num_shots):
+        rec = ellipsoidal(int(nrecs[i]), (55e-3,) * 3, c, threshold=0.5)
+        wav = cfg['wavelet'].copy()
+        wav[int(nts[i]):] = 0  # the shot ends at its own duration
+        plans.append(build_plan(space, time_, wav[None], cfg['vp'], source_coordinates=rec[(7 * i) % len(rec)][None],
+                                receiver_coordinates=rec, kernel='OT2', save_undersampling=4))
+    order = np.argsort(nts, kind='stable')
+    pairs = [(int(order[2 * k]), int(order[2 * k + 1])) for k in range(num_shots // 2)]
+    p0 = plans[0]
+    s = Solver(p0.dim, p0.shape, p0.spacing, p0.absorbing, p0.dt, p0.nt)
+    s.set_medium(p0.vp)
+    s.set_sponge(p0.sponge_tables)
+    s.set_scheme(p0.kernel_order, p0.rho_mode, p0.att_power, p0.boundary, p0.diff_source, p0.undersampling, 0,
+                 0, nt_max, 0, 0, 0)
+    s.batch_create()
+    if tile is None:
+        from stride_amd.autotune import autotune_batch
+        tile, _ = autotune_batch(plans[pairs[0][0]], plans[pairs[0][1]])
+    s.batch_set_tile(*tile)
+    d_src = [s._to_device(p.src_data) for p in plans]
+    d_rec = [s.device_alloc(p.nt * p.nrec * 4) for p in plans]
+    waste = 0
+    t0 = time.perf_counter()
+    for a, b in pairs:
+        nt_pair = int(max(nts[a], nts[b]))
+        waste += 2 * nt_pair - int(nts[a]) - int(nts[b])
+        s.set_scheme(p0.kernel_order, p0.rho_mode, p0.att_power, p0.boundary, p0.diff_source, p0.undersampling, 0,
+                     0, nt_pair, 0, 0, 0)
+        for k, i in enumerate((a, b)):
+            s.batch_set_taps(k, plans[i].src_idx, plans[i].src_w, plans[i].rec_idx, plans[i].rec_w)
+        s.batch_forward_dev([d_src[a], d_src[b]], [d_rec[a], d_rec[b]])
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    n = int(np.prod(p0.shape))
+    updates = int(sum(int(t) - 2 for t in nts)) * n
+    for d in d_src + d_rec:
+        s.device_free(d)
+    s.close()
+    return dict(metric='Gpts/s batched 3D forward, %d uneven shots' % num_shots, value=round(updates / wall / 1e9, 3),
+                unit='Gpts/s', seconds=round(wall, 3), shots=num_shots, pairs=len(pairs),
+                nt_range=[int(nts.min()), int(nts.max())], nrec_range=[int(nrecs.min()), int(nrecs.max())],
+                point_updates=updates, padded_steps_fraction=round(waste / float(nts.sum()), 4), tile=list(tile),
+                config={'workload': '3D 320x320x320 inner (360x360x360 extended), dx 0.4 mm, SO10 OT2, sponge 20, '
+                                    'forward only, shots paired by duration through the two-shot update'})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -214,6 +276,9 @@ def main():
     ap.add_argument('--copy-check', type=int, default=0, metavar='MB',
                     help='only run 10 device-to-device copies of MB megabytes (calibrates the byte counters '
                          'of a profiler against a transfer of known size) and exit')
+    ap.add_argument('--schedule', action='store_true',
+                    help='only run the batched 3D forward over 32 shots of uneven duration and receiver count '
+                         '(seeded schedule) and exit')
     ap.add_argument('--pipelines', action='store_true',
                     help='only time the device steps either side of the solver (trace processing, misfit, '
                          'gradient post-processing, model update) at the full workload and exit')
@@ -238,6 +303,10 @@ def main():
         print(json.dumps({'copy_check_bytes': nbytes, 'copies': 10}))
         return
 
+    if args.schedule:
+        tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
+        print(json.dumps(bench_schedule(tile)))
+        return
     cfg, plan = workload(scale=args.scale, nt=args.nt)
     if args.pipelines:
         res = bench_pipelines(plan)

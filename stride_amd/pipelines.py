@@ -303,6 +303,7 @@ class L2DistanceLoss:
 
     def __init__(self, **kwargs):
         self.residual = None
+        self._solver = None
 
     def forward(self, solver, modelled, observed, **kwargs):
         _check_dev(modelled, 'modelled', solver)
@@ -314,14 +315,24 @@ class L2DistanceLoss:
         ntr, nt = _rows(res)
         _enter(solver)
         fun = solver.traces_l2(modelled.data_ptr(), observed.data_ptr(), res.data_ptr(), ntr, nt)
-        self.residual = res
+        self.residual, self._solver = res, solver
         return fun, res
 
     def adjoint(self, d_fun=1.0, modelled_needs_grad=True, observed_needs_grad=False):
-        res, self.residual = self.residual, None
-        grad_mod = res if d_fun == 1.0 else d_fun * res
-        grad_obs = -d_fun * res if observed_needs_grad else None
-        return (grad_mod if modelled_needs_grad else None), grad_obs
+        """(+d_fun residual, -d_fun residual) as new arrays (l2_distance.py:55-66)."""
+        res, s = self.residual, self._solver
+        self.residual = self._solver = None
+        if res is None:
+            raise RuntimeError('L2DistanceLoss.adjoint called before forward')
+
+        def scaled(a):
+            out = res.clone()
+            s.field_axpb(out.data_ptr(), out.numel(), a, 0.0)
+            return out
+        grad_mod = scaled(float(d_fun)) if modelled_needs_grad else None
+        grad_obs = scaled(-float(d_fun)) if observed_needs_grad else None
+        s.synchronize()
+        return grad_mod, grad_obs
 
 
 def traces_from_solver(solver, rec_time_major):
