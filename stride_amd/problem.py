@@ -93,6 +93,50 @@ def inv_transform(x):
     return 1 / x
 
 
+def _inner_desc(inner):
+    return [[str(sl.start), str(sl.stop), str(sl.step)] for sl in inner]
+
+
+def _inner_from_desc(desc):
+    out = []
+    for each in desc:
+        each = [e.decode() if isinstance(e, bytes) else e for e in each]
+        out.append(slice(*[int(e) if e != 'None' else None for e in each]))
+    return tuple(out)
+
+
+def _structured_desc(obj):
+    """StructuredData.__get_desc__ (stride/problem/data.py:725-752), without compression."""
+    return {'shape': tuple(obj.shape), 'extended_shape': tuple(obj.extended_shape),
+            'inner': _inner_desc(obj._inner), 'dtype': 'float32', 'data': np.ascontiguousarray(obj.data),
+            'compression': False, 'step_size': None}
+
+
+def _dump(obj, description, args, kwargs):
+    """GriddedSaved.dump (stride/problem/base.py:186-204): grid description + the object's."""
+    from . import h5
+    desc = {}
+    space = getattr(obj, 'space', None)
+    if space is not None:
+        desc['space'] = {'shape': tuple(space.shape), 'spacing': tuple(space.spacing), 'extra': tuple(space.extra),
+                         'absorbing': tuple(space.absorbing)}
+    desc.update(description)
+    kwargs['parameter'] = kwargs.get('parameter', obj.name)
+    with h5.HDF5(*args, mode='w', **kwargs) as f:
+        f.dump(desc)
+        return f.filename
+
+
+def _load(obj, args, kwargs):
+    from . import h5
+    kwargs['parameter'] = kwargs.get('parameter', obj.name)
+    with h5.HDF5(*args, mode='r', **kwargs) as f:
+        desc = f.load()
+    if desc.get('compression'):
+        raise ValueError('compressed data (%s) is not supported' % desc['compression'])
+    return desc
+
+
 class ScalarField:
     """Extended-shape float32 field with an inner view.
 
@@ -158,6 +202,31 @@ class ScalarField:
     def pad(self):
         self._data[...] = self.pad_data(np.array(self.data))
 
+    def dump(self, *args, **kwargs):
+        """Write as Stride does (Mosaic HDF5 protocol 0.1; base.py:186-204, data.py:725-752,
+        1163-1167): ``path``, ``project_name``, ``parameter`` (default: the name), ``version``
+        or ``filename``. Returns the file name."""
+        desc = _structured_desc(self)
+        desc.update(time_dependent=False, slow_time_dependent=False)
+        return _dump(self, desc, args, kwargs)
+
+    def load(self, *args, **kwargs):
+        """Read a file written by ``dump`` or by Stride (base.py:232-276, data.py:754-781): the
+        inner data is edge-padded into the extended shape; the space is taken from the file
+        when this field has none."""
+        desc = _load(self, args, kwargs)
+        if self.space is None and 'space' in desc:
+            sp = desc['space']
+            self.space = Space(shape=sp['shape'], spacing=sp['spacing'], extra=sp['extra'],
+                               absorbing=sp['absorbing'])
+        self._shape = tuple(desc['shape'])
+        self._extended_shape = tuple(desc['extended_shape'])
+        self._inner = _inner_from_desc(desc['inner'])
+        data = np.asarray(desc['data'], dtype=np.float32)
+        self._data = np.zeros(self._extended_shape, dtype=np.float32)
+        self._data[...] = data if data.shape == self._extended_shape else self.pad_data(data)
+        return self
+
     def alike(self, name=None, data=None, shape=None, extended_shape=None, inner=None, **kwargs):
         shape = shape if shape is not None else self._shape
         extended_shape = extended_shape if extended_shape is not None else (
@@ -173,7 +242,7 @@ class Traces:
     def __init__(self, name, data=None, shape=None, needs_grad=False, transducer_ids=None):
         self.name = name
         if data is None:
-            data = np.zeros(shape, dtype=np.float32)
+            data = np.zeros(shape if shape is not None else (0, 0), dtype=np.float32)
         self._data = np.ascontiguousarray(data, dtype=np.float32)
         self.needs_grad = needs_grad
         self.transducer_ids = transducer_ids
@@ -192,8 +261,29 @@ class Traces:
         return self._data.shape
 
     @property
+    def extended_shape(self):
+        return self._data.shape
+
+    @property
     def num_transducers(self):
         return self._data.shape[0]
+
+    _inner = property(lambda self: tuple(slice(0, n, None) for n in self._data.shape))
+
+    def dump(self, *args, **kwargs):
+        """As ``ScalarField.dump`` (the Traces description, data.py:1631-1636)."""
+        desc = _structured_desc(self)
+        desc.update(num_transducers=self.num_transducers,
+                    transducer_ids=(np.asarray(self.transducer_ids) if self.transducer_ids is not None
+                                    else np.arange(self.num_transducers)))
+        return _dump(self, desc, args, kwargs)
+
+    def load(self, *args, **kwargs):
+        desc = _load(self, args, kwargs)
+        self._data = np.ascontiguousarray(desc['data'], dtype=np.float32)
+        if desc.get('transducer_ids') is not None:
+            self.transducer_ids = np.asarray(desc['transducer_ids'])
+        return self
 
     def alike(self, name=None, data=None, **kwargs):
         return Traces(name or self.name, data=data if data is not None else np.zeros_like(self._data),
