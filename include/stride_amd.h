@@ -171,8 +171,8 @@ int sa_batch_forward_dev(sa_solver *s, const float *const *src, float *const *re
 /* res[k]: device (nt, nrec_k) adjoint source of shot k, as for sa_adjoint_dev */
 int sa_batch_adjoint_dev(sa_solver *s, const float *const *res);
 int sa_batch_get_wavefield(sa_solver *s, int shot, int which, float *out);
-/* tile of the two-shot update: waves 8 with rows 1 or 2, or waves 4 with rows 2 (default rows 1,
- * chunk 24, waves 8) */
+/* tile of the two-shot update: waves 8 with rows 1 or 2, or waves 4 with rows 2 (default rows 2,
+ * chunk 24, waves 4) */
 int sa_batch_set_tile(sa_solver *s, int rows, int chunk, int waves);
 
 /* ---- Steps either side of the solver, on device arrays (the solver supplies device and

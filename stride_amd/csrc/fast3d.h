@@ -26,7 +26,7 @@ struct Fast3D {
     const float *vp = nullptr;
     float dt = 0.f;
     FastTune tune = {2, 24, 7};
-    FastTune tune2 = {1, 24, 8};  // two-shot kernel: rows, chunk, waves (4 or 8)
+    FastTune tune2 = {2, 24, 4};  // two-shot kernel: rows, chunk, waves (4 or 8)
     bool prepared = false;
 };
 

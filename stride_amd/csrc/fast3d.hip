@@ -1182,7 +1182,7 @@ __device__ __forceinline__ void finishB(const Geo &g, const FastArgs &a, const R
     }
 }
 
-template <int RPT, int NW, int MODE, bool SPONGE, int J>
+template <int RPT, int NW, int MODE, bool SPONGE, bool ISO, int J>
 __device__ __forceinline__ void marchB_step(const Geo &g, const FastArgs &a, const Rsrc &rs, float *const (&sl)[6],
                                             f2 (&acc)[RPT][12], f2 (&cq)[RPT][6], StageB<RPT, MODE> (&st)[3],
                                             const int p, const int x0, const int x1, const int lane, const int w,
@@ -1215,15 +1215,28 @@ __device__ __forceinline__ void marchB_step(const Geo &g, const FastArgs &a, con
         int li = (w + NW * r + 5) * R::PITCH + lane + 8;  // in pairs; opaque per plane (see version 5)
         asm volatile("" : "+v"(li));
         const f2 c = pc[li];
-        f2 vy = g.w2[1][0] * c, vz = g.w2[2][0] * c;
+        f2 vy, vz;
+        if (ISO) {  // one spacing on every axis: the y and z neighbours share the weights of x
+            vy = (2.f * g.w2[0][0]) * c;
+            vz = f2{0.f, 0.f};
 #pragma unroll
-        for (int k = 1; k <= 5; ++k) {
-            const f2 sy = pc[li + k * R::PITCH] + pc[li - k * R::PITCH];
-            const f2 sz = pc[li + k] + pc[li - k];
-            vy.x += g.w2[1][k] * sy.x;
-            vy.y += g.w2[1][k] * sy.y;
-            vz.x += g.w2[2][k] * sz.x;
-            vz.y += g.w2[2][k] * sz.y;
+            for (int k = 1; k <= 5; ++k) {
+                const f2 syz = (pc[li + k * R::PITCH] + pc[li - k * R::PITCH]) + (pc[li + k] + pc[li - k]);
+                vy.x += g.w2[0][k] * syz.x;
+                vy.y += g.w2[0][k] * syz.y;
+            }
+        } else {
+            vy = g.w2[1][0] * c;
+            vz = g.w2[2][0] * c;
+#pragma unroll
+            for (int k = 1; k <= 5; ++k) {
+                const f2 sy = pc[li + k * R::PITCH] + pc[li - k * R::PITCH];
+                const f2 sz = pc[li + k] + pc[li - k];
+                vy.x += g.w2[1][k] * sy.x;
+                vy.y += g.w2[1][k] * sy.y;
+                vz.x += g.w2[2][k] * sz.x;
+                vz.y += g.w2[2][k] * sz.y;
+            }
         }
         acc[r][(J + 5) % 12] = g.w2[0][5] * c;
 #pragma unroll
@@ -1269,7 +1282,7 @@ __device__ __forceinline__ void tailB_step(const Geo &g, const FastArgs &a, cons
     if (J < 2) load_stageB<RPT, MODE>(st[(J + 1) % 3], rs, row, grow, brow, min(p - 2, xlast), a);
 }
 
-template <int RPT, int NW, int MODE, bool SPONGE>
+template <int RPT, int NW, int MODE, bool SPONGE, bool ISO>
 __device__ __forceinline__ void marchB(const Geo &g, const FastArgs &a, const Rsrc &rs, float *const (&sl)[6],
                                        const int x0, const int x1, const int lane, const int w,
                                        const unsigned (&row)[RPT], const unsigned (&orow)[RPT],
@@ -1306,7 +1319,7 @@ __device__ __forceinline__ void marchB(const Geo &g, const FastArgs &a, const Rs
     for (int it = 0; it < nper; ++it) {
         const int xb = x0 + 12 * it;
 #define SA_STEPB(J)                                                                                            \
-    marchB_step<RPT, NW, MODE, SPONGE, J>(g, a, rs, sl, acc, cq, st, xb + J, x0, x1, lane, w, row, orow, grow, \
+    marchB_step<RPT, NW, MODE, SPONGE, ISO, J>(g, a, rs, sl, acc, cq, st, xb + J, x0, x1, lane, w, row, orow, grow, \
                                           brow, dv, d1, t1v, d2, t2v);
         SA_STEPB(0) SA_STEPB(1) SA_STEPB(2) SA_STEPB(3) SA_STEPB(4) SA_STEPB(5)
         SA_STEPB(6) SA_STEPB(7) SA_STEPB(8) SA_STEPB(9) SA_STEPB(10) SA_STEPB(11)
@@ -1326,7 +1339,7 @@ __device__ __forceinline__ void marchB(const Geo &g, const FastArgs &a, const Rs
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): no direct-to-LDS load outlives the workgroup
 }
 
-template <int RPT, int NW, int MODE, bool SPONGE>
+template <int RPT, int NW, int MODE, bool SPONGE, bool ISO>
 __global__ __launch_bounds__(NW * 64) void k_fast3d_b2(Geo g, FastArgs a) {
     constexpr int TY = NW * RPT;
     using R = Ring2W<TY, NW>;
@@ -1399,9 +1412,9 @@ __global__ __launch_bounds__(NW * 64) void k_fast3d_b2(Geo g, FastArgs a) {
     const bool interior = y0 >= g.nabs[1] && y0 + TY <= ny - g.nabs[1] && z0 >= g.nabs[2] &&
                           z0 + kTZ <= nz - g.nabs[2] && x0 >= g.nabs[0] && x1 <= g.n[0] - g.nabs[0];
     if (SPONGE && !interior)
-        marchB<RPT, NW, MODE, true>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2, t2v);
+        marchB<RPT, NW, MODE, true, ISO>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2, t2v);
     else
-        marchB<RPT, NW, MODE, false>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2, t2v);
+        marchB<RPT, NW, MODE, false, ISO>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2, t2v);
 }
 
 template <int RPT, int MODE>
@@ -1493,10 +1506,16 @@ int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
 
 template <int RPT, int NW, int MODE>
 void launch_b2_rpt(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
-    if (sponge)
-        k_fast3d_b2<RPT, NW, MODE, true><<<grid, NW * 64, 0, st>>>(g, a);
+    bool iso = true;  // the same weights on every axis (equal spacing)
+    for (int k = 0; k < 6; ++k) iso = iso && g.w2[1][k] == g.w2[0][k] && g.w2[2][k] == g.w2[0][k];
+    if (sponge && iso)
+        k_fast3d_b2<RPT, NW, MODE, true, true><<<grid, NW * 64, 0, st>>>(g, a);
+    else if (sponge)
+        k_fast3d_b2<RPT, NW, MODE, true, false><<<grid, NW * 64, 0, st>>>(g, a);
+    else if (iso)
+        k_fast3d_b2<RPT, NW, MODE, false, true><<<grid, NW * 64, 0, st>>>(g, a);
     else
-        k_fast3d_b2<RPT, NW, MODE, false><<<grid, NW * 64, 0, st>>>(g, a);
+        k_fast3d_b2<RPT, NW, MODE, false, false><<<grid, NW * 64, 0, st>>>(g, a);
 }
 
 template <int MODE>

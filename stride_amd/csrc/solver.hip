@@ -173,6 +173,7 @@ struct UpdateArgs {
     const float *u, *other;
     float *out;
     const float *A, *B, *G, *Ku, *KmKu, *Cbeta, *Cpml;
+    const float *lapother;  // Laplacian of `other`, taken before this launch overwrites it
     const float *u1[3];
     float c4;  // dt^2 / 12 (OT4)
     float *save;
@@ -187,7 +188,7 @@ __global__ void k_update(Geo g, UpdateArgs x) {
     float q = x.Ku[p];
     if (x.KmKu) q += x.c4 * x.KmKu[p];
     float acc = x.G[p] * q;
-    if (x.Cbeta) acc += x.Cbeta[p] * (sa_lap(g, x.u, p) - sa_lap(g, x.other, p));
+    if (x.Cbeta) acc += x.Cbeta[p] * (sa_lap(g, x.u, p) - x.lapother[p]);
     if (x.Cpml) {
         float s1 = 0.f;
         for (int a = 0; a < 3; ++a)
@@ -466,6 +467,7 @@ struct sa_solver {
     // wavefields and work
     float *buf[2] = {};
     float *Ku = nullptr, *L2 = nullptr, *KmKu = nullptr, *lapo = nullptr;
+    float *lapother = nullptr;  // Laplacian of the other level (power-2 attenuation)
     float *st[3] = {};
     float *st2[3] = {};
     float *work0 = nullptr, *work1 = nullptr;
@@ -660,7 +662,7 @@ extern "C" int sa_destroy(sa_solver *s) {
     (void)hipSetDevice(s->device);
     (void)hipDeviceSynchronize();
     float *fs[] = {s->vp, s->rho, s->b, s->alpha, s->A, s->B, s->G, s->M, s->Cbeta, s->Cpml, s->tables,
-                   s->buf[0], s->buf[1], s->Ku, s->L2, s->KmKu, s->lapo, s->work0, s->work1, s->saved,
+                   s->buf[0], s->buf[1], s->Ku, s->L2, s->KmKu, s->lapo, s->lapother, s->work0, s->work1, s->saved,
                    s->d_series, s->d_out};
     for (float *p : fs) dfree(p);
     for (float *p : s->gacc) dfree(p);
@@ -865,6 +867,7 @@ static int prepare(sa_solver *s) {
     TRY(dalloc(&s->G, s->total));
     TRY(dalloc(&s->M, s->total));
     if (c.att_power == 2) TRY(dalloc(&s->Cbeta, s->total));
+    if (c.att_power == 2) TRY(dalloc(&s->lapother, s->total));
     if (c.boundary == 1) TRY(dalloc(&s->Cpml, s->total));
     const Geo &g = s->geo;
     k_coefficients<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(
@@ -995,6 +998,13 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
     x.KmKu = c.kernel == 4 ? s->KmKu : nullptr;
     x.c4 = s->dt * s->dt / 12.f;
     x.Cbeta = c.att_power == 2 ? s->Cbeta : nullptr;
+    if (c.att_power == 2) {
+        // out may alias other (the new level overwrites the oldest in place): its Laplacian
+        // is taken by a launch of its own before the update reads neighbours of it
+        k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, other, nullptr, s->lapother);
+        CHECK_LAUNCH();
+        x.lapother = s->lapother;
+    }
     x.Cpml = c.boundary == 1 ? s->Cpml : nullptr;
     for (int a = 0; a < 3; ++a) x.u1[a] = c.boundary == 1 ? s->u1[a] : nullptr;
     x.save = save;

@@ -88,3 +88,35 @@ def test_two_shot_api_errors(gpu):
     with pytest.raises(SolverError):
         s.batch_adjoint([np.zeros((plan0.nt, 1), np.float32)] * 2)  # no two-shot forward yet
     s.close()
+
+
+def test_two_shots_anisotropic_spacing(gpu):
+    """Unequal spacings take the general-weights path of the two-shot update (equal spacings
+    share one weight set across the axes)."""
+    from stride_amd.problem import Space, Time
+    from stride_amd.utils import tone_burst, ellipsoidal
+    space = Space(shape=(30, 34, 38), spacing=(0.5e-3, 0.45e-3, 0.4e-3), extra=(6, 6, 6), absorbing=(6, 6, 6))
+    time = Time(start=0., step=0.06e-6, num=140)
+    rng = np.random.default_rng(5)
+    vp = (1500 + 30 * rng.standard_normal(space.extended_shape)).astype(np.float32)
+    c = tuple(l / 2 for l in space.limit)
+    geom = ellipsoidal(16, (4e-3,) * 3, c, threshold=0.5)
+    wav = tone_burst(0.5e6, 3, 140, 0.06e-6)[None]
+    plans = [build_plan(space, time, wav, vp, source_coordinates=geom[k:k + 1], receiver_coordinates=geom,
+                        kernel='OT2', grad_vp=True) for k in (0, 9)]
+    print('batch anisotropic spacing', flush=True)
+    res = [adjoint_source_samples(p, rng.standard_normal((p.nrec, p.nt))) for p in plans]
+    single = []
+    for p, r in zip(plans, res):
+        s = solver_for(p, save_mode=1)
+        rec = s.forward(p.src_data)
+        single.append((rec, s.adjoint(r, want_vp=True)))
+        s.close()
+    recs, g = _batched(plans[0], plans[1], (1, 12, 8), res)
+    for k, p in enumerate(plans):
+        assert rel_l2(recs[k], single[k][0]) < 1e-6, k
+        ref, _, _ = oracle.forward(p, save=False)
+        assert rel_l2(recs[k].T, ref) < 1e-5, k
+    for key in ('grad_vp', 'prec_vp'):
+        want = single[0][1][key].astype(np.float64) + single[1][1][key]
+        assert rel_l2(g[key], want) < 1e-5, key

@@ -73,3 +73,25 @@ def test_fast_unequal_layers(gpu, tile):
     assert rel_l2(rec_f.T, ref) < 1e-5
     gref = oracle.adjoint_compat(plan, res_s, saved)
     assert rel_l2(g_f['grad_vp'], gref['grad_vp']) < 1e-4
+
+
+@pytest.mark.parametrize('tile', [(2, 24, 7), (1, 13, 8)])
+def test_fast_anisotropic_spacing(gpu, tile):
+    from stride_amd.problem import Space, Time
+    from stride_amd.utils import tone_burst, ellipsoidal
+    from stride_amd.physics.plan import build_plan
+    space = Space(shape=(30, 34, 38), spacing=(0.5e-3, 0.45e-3, 0.4e-3), extra=(6, 6, 6), absorbing=(6, 6, 6))
+    time = Time(start=0., step=0.06e-6, num=140)
+    rng = np.random.default_rng(6)
+    vp = (1500 + 30 * rng.standard_normal(space.extended_shape)).astype(np.float32)
+    c = tuple(l / 2 for l in space.limit)
+    geom = ellipsoidal(16, (4e-3,) * 3, c, threshold=0.5)
+    plan = build_plan(space, time, tone_burst(0.5e6, 3, 140, 0.06e-6)[None], vp, source_coordinates=geom[:1],
+                      receiver_coordinates=geom, kernel='OT2', grad_vp=True)
+    res_s = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, plan.nt)))
+    print('fast3d anisotropic spacing tile=%s' % (tile,), flush=True)
+    rec_f, g_f = _run(plan, res_s, general=False, tile=tile)
+    ref, saved, _ = oracle.forward(plan, save=True)
+    assert rel_l2(rec_f.T, ref) < 1e-5
+    gref = oracle.adjoint_compat(plan, res_s, saved)
+    assert rel_l2(g_f['grad_vp'], gref['grad_vp']) < 1e-4

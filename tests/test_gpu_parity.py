@@ -160,3 +160,22 @@ def test_exact_adjoint_finite_difference(gpu):
     an = float(np.sum(grad * d))
     print('FD %.6e analytic %.6e' % (fd, an))
     assert abs(fd - an) <= 2e-3 * abs(fd)
+
+
+@pytest.mark.parametrize('dim', [2, 3])
+def test_power2_attenuation_runs_are_identical(gpu, dim):
+    """The power-2 attenuation term takes the Laplacian of the level the update overwrites in
+    place; that Laplacian is taken by a launch of its own, so repeated runs agree bit for bit
+    (a race between the two would make them differ)."""
+    plan, _ = _case(dim, dict(kernel='OT2', rho='var', alpha='p2', attenuation_power=2,
+                              boundary_type='complex_frequency_shift_PML_2'))
+    print('power-2 attenuation determinism %dD' % dim, flush=True)
+    runs = []
+    for _ in range(3):
+        s = solver_for(plan)
+        runs.append(s.forward(plan.src_data))
+        s.close()
+    for r in runs[1:]:
+        np.testing.assert_array_equal(r, runs[0])
+    ref, _, _ = oracle.forward(plan)
+    assert rel_l2(runs[0].T, ref) < 1e-5
