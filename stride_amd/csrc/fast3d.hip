@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_fast3d(Geo g, FastArgs a) {
                     const float D = (7.f * (sm * v)) * a.dt;
                     const float aa = a.dt * v2 * D;
                     const float e = a.dt * a.dt * v2 * D * D;
-                    const float ia = 1.f / (1.f + aa);
+                    const float ia = __builtin_amdgcn_rcpf(1.f + aa);  // v_rcp_f32 (1 ulp), no divide sequence
                     A = (2.f - e) * ia;
                     B = -(1.f - aa) * ia;
                     G = (a.dt2 * ia) * v2;
@@ -315,7 +315,7 @@ __device__ __forceinline__ void march2_plane(const Geo &g, const FastArgs &a, co
                 const float D = (7.f * (sm * v)) * a.dt;
                 const float aa = a.dt * v2 * D;
                 const float e = a.dt * a.dt * v2 * D * D;
-                const float ia = 1.f / (1.f + aa);
+                const float ia = __builtin_amdgcn_rcpf(1.f + aa);  // v_rcp_f32 (1 ulp), no divide sequence
                 A = (2.f - e) * ia;
                 B = -(1.f - aa) * ia;
                 G = (a.dt2 * ia) * v2;
@@ -623,7 +623,7 @@ __device__ __forceinline__ void march5_plane(const Geo &g, const FastArgs &a, co
                 const float D = (7.f * (sm * v)) * a.dt;
                 const float aa = a.dt * v2 * D;
                 const float e = a.dt * a.dt * v2 * D * D;
-                const float ia = 1.f / (1.f + aa);
+                const float ia = __builtin_amdgcn_rcpf(1.f + aa);  // v_rcp_f32 (1 ulp), no divide sequence
                 A = (2.f - e) * ia;
                 B = -(1.f - aa) * ia;
                 G = (a.dt2 * ia) * v2;
@@ -813,7 +813,7 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
             const float D = (7.f * (sm * v)) * a.dt;
             const float aa = a.dt * v2 * D;
             const float e = a.dt * a.dt * v2 * D * D;
-            const float ia = 1.f / (1.f + aa);
+            const float ia = __builtin_amdgcn_rcpf(1.f + aa);  // v_rcp_f32 (1 ulp), no divide sequence
             A = (2.f - e) * ia;
             B = -(1.f - aa) * ia;
             G = (a.dt2 * ia) * v2;
@@ -1158,7 +1158,7 @@ __device__ __forceinline__ void finishB(const Geo &g, const FastArgs &a, const R
             const float D = (7.f * (sm * v)) * a.dt;
             const float aa = a.dt * v2 * D;
             const float e = a.dt * a.dt * v2 * D * D;
-            const float ia = 1.f / (1.f + aa);
+            const float ia = __builtin_amdgcn_rcpf(1.f + aa);  // v_rcp_f32 (1 ulp), no divide sequence
             A = (2.f - e) * ia;
             B = -(1.f - aa) * ia;
             G = (a.dt2 * ia) * v2;
