@@ -141,6 +141,15 @@ int sa_free_saved(sa_solver *s);
  * version 7, rows 2, chunk 24. Used by the autotuner (stride_amd.autotune). */
 int sa_set_tile(sa_solver *s, int rows, int chunk, int version);
 
+/* Tile of the fused 3D update for the div(b grad) forms (constant or variable density; OT2,
+ * sponge or no boundary, attenuation power 0 / None / off), which replaces the staggered
+ * derivative, divergence and update passes of the operator built by
+ * stride/physics/iso_acoustic/devito.py:1466-1608, 1646-1666 for those schemes. The tile is
+ * 64 (axis 2) x waves*rows (axis 1) cells marched over `chunk` planes of axis 0; rows 1 or 2,
+ * waves 4 or 8. rows = 0 switches the fused update off (the separate general passes run
+ * instead). Default: rows 1, chunk 40, waves 8. */
+int sa_set_tile_density(sa_solver *s, int rows, int chunk, int waves);
+
 /* Kernel timing: when enabled, every launch of the wavefield-update kernel is bracketed by
  * HIP events on the solver stream. sa_timing_read returns the summed kernel milliseconds
  * and launch count since the last reset. */

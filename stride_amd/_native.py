@@ -41,7 +41,8 @@ class _Scheme(ctypes.Structure):
 EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_stream', 'sa_set_scheme',
            'sa_set_medium', 'sa_set_sponge', 'sa_set_pml', 'sa_set_sources', 'sa_set_receivers',
            'sa_forward', 'sa_forward_dev', 'sa_adjoint', 'sa_adjoint_dev', 'sa_read_gradient',
-           'sa_adjoint_exact', 'sa_get_wavefield', 'sa_free_saved', 'sa_set_tile', 'sa_timing_enable', 'sa_timing_read',
+           'sa_adjoint_exact', 'sa_get_wavefield', 'sa_free_saved', 'sa_set_tile', 'sa_set_tile_density',
+           'sa_timing_enable', 'sa_timing_read',
            'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize', 'sa_batch_create',
            'sa_batch_set_taps', 'sa_batch_forward_dev', 'sa_batch_adjoint_dev', 'sa_batch_get_wavefield',
            'sa_batch_set_tile', 'sa_traces_check', 'sa_traces_mute', 'sa_traces_fir', 'sa_traces_sosfilt',
@@ -378,6 +379,11 @@ class Solver:
 
     def set_tile(self, rows, chunk, version=2):
         self._check(self._lib.sa_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk), ctypes.c_int(version)))
+
+    def set_tile_density(self, rows, chunk=40, waves=8):
+        """Tile of the fused 3D density update; rows=0 runs the separate general passes."""
+        self._check(self._lib.sa_set_tile_density(self._h, ctypes.c_int(rows), ctypes.c_int(chunk),
+                                                  ctypes.c_int(waves)))
 
     def timing(self, enable=None):
         if enable is not None:

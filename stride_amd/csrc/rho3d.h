@@ -1,0 +1,31 @@
+// Fused 3D wavefield update for the div(b grad) forms of the operator (constant or variable
+// density), OT2, sponge or no boundary, attenuation power 0 / None / off: one pass per time
+// step in place of three staggered-derivative passes, a divergence pass and the update pass.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+struct Rho3DTune {
+    int rows = 1;    // rows (axis 1) per thread; the tile is 64 (axis 2) x waves*rows (axis 1)
+    int chunk = 40;  // planes (axis 0) per workgroup
+    int waves = 8;   // 4 or 8
+    bool enabled = true;
+};
+
+struct Rho3DStep {
+    const float *u, *other;  // current level and the level it replaces
+    float *out;              // new level (may alias other)
+    const float *b;          // buoyancy (variable density) or null (constant density: b = 1)
+    const float *A, *B, *G;  // per-point update coefficients
+    float *save;             // optional fused save into `box` (save_mode 1: p.dt2, 2: raw u)
+    int save_mode;
+    Box box;
+    float idt2;
+};
+
+// true when the layout fits the kernel's 32-bit buffer offsets
+bool rho3d_supported(const Geo &g);
+int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_t stream);
+// rows 1/2, waves 4/8, chunk >= 1
+bool rho3d_valid_tune(int rows, int chunk, int waves);

@@ -1,0 +1,332 @@
+// Fused 3D update for the div(b grad) operator (see rho3d.h).
+//
+// The operator is K u = sum_d D-_d (bbar_d D+_d u): D+_d the SO10 staggered derivative from
+// nodes to the half points i + 1/2, bbar_d the mean of the buoyancy at i and i + 1 (1 for
+// constant density), D-_d the staggered derivative from half points back to nodes
+// (stride/physics/iso_acoustic/devito.py:1646-1666). The step is
+//   out = A u + B other + G K u
+// with the per-point coefficients A, B, G of the general kernels (sponge and attenuation
+// folded in), i.e. the same update as k_stagger x 3 + k_divergence + k_update in one pass.
+//
+// Work decomposition: a workgroup of NW waves owns a tile of 64 (axis 2) x NW*RPT (axis 1)
+// cells and marches along axis 0 over a chunk of planes. For every input plane m:
+//  - the plane's u with a 9-cell halo (rows y0-9 .. y0+TY+8, columns z0-12 .. z0+75) and b
+//    with a 5-cell halo are staged in LDS (loaded one plane ahead into registers);
+//  - the in-plane fluxes bbar D+ u along axes 1 and 2 are formed once per half point into LDS
+//    tiles, and their D- divergence is added to the node accumulator of plane m;
+//  - the own cell of plane m is pushed into a ring of axis-0 D+ partial sums for the half
+//    planes m-5 .. m+4; half plane m-5 is then complete, is scaled by its bbar and pushed
+//    into the node accumulators of planes m-9 .. m; node plane m-9 is then complete, is
+//    updated and stored.
+// So u, b, the other level and the coefficients are each read once per step (plus the
+// 18 planes of lead-in and lead-out per chunk), and the new level is written once.
+// Every load is unconditional: cells beyond the padded rows read zero through the buffer
+// range check.
+
+#include "rho3d.h"
+#include "bufio.h"
+
+namespace {
+
+using namespace bufio;
+
+constexpr int kTZ = 64;
+
+struct RhoArgs {
+    const float *u, *o, *b, *A, *B, *G;
+    float *out, *save;
+    long long bytes;
+    int plane_bytes;
+    int chunk, ntz, nty, ntiles, per_xcd;
+    int save_mode;
+    float idt2;
+    Box box;
+};
+
+template <int RPT, int NW, bool VARB>
+struct Tile {
+    static constexpr int NT = NW * 64, TY = NW * RPT;
+    static constexpr int UY = TY + 18, UZ = 88, UC = UZ / 4;  // u: rows y0-9.., columns z0-12..
+    static constexpr int BY = TY + 10, BZ = 80, BC = BZ / 4;  // b: rows y0-5.., columns z0-8..
+    static constexpr int FY = TY + 9;                         // axis-1 fluxes, half rows y0-5 .. y0+TY+3
+    static constexpr int FZN = 73, FZP = 76;                  // axis-2 fluxes, half columns z0-5 .. z0+67
+    static constexpr int NLU = (UY * UC + NT - 1) / NT;
+    static constexpr int NLB = VARB ? (BY * BC + NT - 1) / NT : 1;
+};
+
+template <int RPT, int NW, bool VARB>
+struct State {
+    using T = Tile<RPT, NW, VARB>;
+    float dp[RPT][10];  // axis-0 D+ partial sums; slot of half plane q: (q - mbeg) % 10
+    float nd[RPT][10];  // node accumulators (K u); slot of plane p: (p - mbeg) % 10
+    float uq[RPT][10];  // own u; slot of plane p
+    float hb[RPT][5];   // axis-0 bbar of half plane q; slot (q - mbeg) % 5
+    float bprev[RPT];   // own b of the previous plane
+    float4 su[T::NLU];  // next plane's u tile, staged in registers
+    float4 sb[T::NLB];
+    unsigned uoff[T::NLU], boff[T::NLB];  // in-plane byte offsets of the tile chunks (kOOB: none)
+    int ulds[T::NLU], blds[T::NLB];       // their LDS float offsets (-1: none)
+    unsigned own[RPT];                    // own cells' in-plane byte offsets (kOOB outside the domain)
+};
+
+struct Res {
+    rsrc_t u, o, b, A, B, G, out;
+};
+
+template <int RPT, int NW, bool VARB>
+__device__ __forceinline__ void load_tile(State<RPT, NW, VARB> &st, const Res &rs, unsigned poff) {
+    using T = Tile<RPT, NW, VARB>;
+#pragma unroll
+    for (int i = 0; i < T::NLU; ++i) st.su[i] = load4(rs.u, st.uoff[i] + poff);
+    if (VARB) {
+#pragma unroll
+        for (int i = 0; i < T::NLB; ++i) st.sb[i] = load4(rs.b, st.boff[i] + poff);
+    }
+}
+
+template <int RPT, int NW, bool VARB, bool SAVE, int J>
+__device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res &rs, State<RPT, NW, VARB> &st,
+                                      float *sU, float *sB, float *sFy, float *sFz, const int m, const int x0,
+                                      const int x1, const int y0, const int z0, const int lane, const int w) {
+    using T = Tile<RPT, NW, VARB>;
+    // slots: plane / half plane m -> J, m - 9 -> (J + 1) % 10, half plane m - 5 -> (J + 5) % 10
+    __syncthreads();  // every wave is done with the previous plane's LDS tiles
+#pragma unroll
+    for (int i = 0; i < T::NLU; ++i)
+        if (st.ulds[i] >= 0) *reinterpret_cast<float4 *>(sU + st.ulds[i]) = st.su[i];
+    if (VARB) {
+#pragma unroll
+        for (int i = 0; i < T::NLB; ++i)
+            if (st.blds[i] >= 0) *reinterpret_cast<float4 *>(sB + st.blds[i]) = st.sb[i];
+    }
+    if (m + 1 <= x1 + 8) load_tile(st, rs, (unsigned)((m + 1) * a.plane_bytes));
+    const int p = m - 9;
+    const bool emit = p >= x0 && p < x1;
+    float ov[RPT], Av[RPT], Bv[RPT], Gv[RPT];
+    if (emit) {
+        const unsigned po = (unsigned)(p * a.plane_bytes);
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            ov[r] = load_nt(rs.o, st.own[r] + po);
+            Av[r] = load_nt(rs.A, st.own[r] + po);
+            Bv[r] = load_nt(rs.B, st.own[r] + po);
+            Gv[r] = load_nt(rs.G, st.own[r] + po);
+        }
+    }
+    __syncthreads();  // plane m's tiles are in LDS
+    const bool inplane = m >= x0 && m < x1;
+    if (inplane) {
+        // axis-1 fluxes at half rows q = y0 - 5 + fr, column z0 + lane
+        for (int fr = w; fr < T::FY; fr += NW) {
+            const float *c = sU + (fr + 4) * T::UZ + lane + 12;  // u at row q
+            float d = 0.f;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) d += g.ws[1][k] * (c[(k + 1) * T::UZ] - c[-k * T::UZ]);
+            if (VARB) d *= 0.5f * (sB[fr * T::BZ + lane + 8] + sB[(fr + 1) * T::BZ + lane + 8]);
+            sFy[fr * kTZ + lane] = d;
+        }
+        // axis-2 fluxes at half columns q = z0 - 5 + fz, row y0 + row
+        for (int e = w * 64 + lane; e < T::TY * T::FZN; e += T::NT) {
+            const int row = e / T::FZN, fz = e - row * T::FZN;
+            const float *c = sU + (row + 9) * T::UZ + fz + 7;  // u at column q
+            float d = 0.f;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) d += g.ws[2][k] * (c[k + 1] - c[-k]);
+            if (VARB) d *= 0.5f * (sB[(row + 5) * T::BZ + fz + 3] + sB[(row + 5) * T::BZ + fz + 4]);
+            sFz[row * T::FZP + fz] = d;
+        }
+    }
+    // axis 0: own cell of plane m into the D+ partial sums of half planes m-5 .. m+4
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int ry = w * RPT + r;
+        const float uc = sU[(ry + 9) * T::UZ + lane + 12];
+        st.uq[r][J] = uc;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            st.dp[r][(J + 9 - k) % 10] += g.ws[0][k] * uc;  // half plane m - 1 - k
+            st.dp[r][(J + k) % 10] -= g.ws[0][k] * uc;      // half plane m + k
+        }
+        if (VARB) {
+            const float bc = sB[(ry + 5) * T::BZ + lane + 8];
+            st.hb[r][(J + 4) % 5] = 0.5f * (st.bprev[r] + bc);  // half plane m - 1
+            st.bprev[r] = bc;
+        }
+    }
+    if (inplane) {
+        __syncthreads();  // fluxes are in LDS
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const int ry = w * RPT + r;
+            const float *fy = sFy + (ry + 5) * kTZ + lane;  // flux at half row y
+            const float *fz = sFz + ry * T::FZP + lane + 5;  // flux at half column z
+            float d = 0.f;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) d += g.ws[1][k] * (fy[k * kTZ] - fy[-(k + 1) * kTZ]);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) d += g.ws[2][k] * (fz[k] - fz[-(k + 1)]);
+            st.nd[r][J] += d;
+        }
+    }
+    // half plane m - 5 is complete: its flux goes to the nodes m-9 .. m
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        float f = st.dp[r][(J + 5) % 10];
+        st.dp[r][(J + 5) % 10] = 0.f;
+        if (VARB) f *= st.hb[r][J % 5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            st.nd[r][(J + 5 + 10 - k) % 10] += g.ws[0][k] * f;  // node m - 5 - k
+            st.nd[r][(J + 6 + k) % 10] -= g.ws[0][k] * f;       // node m - 4 + k
+        }
+    }
+    // node plane m - 9 is complete
+    if (emit) {
+        const unsigned po = (unsigned)(p * a.plane_bytes);
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const float u = st.uq[r][(J + 1) % 10];
+            const float out = Av[r] * u + Bv[r] * ov[r] + Gv[r] * st.nd[r][(J + 1) % 10];
+            store_nt(out, rs.out, st.own[r] + po);
+            if (SAVE) {
+                const int y = y0 + w * RPT + r, z = z0 + lane;
+                if (sa_inbox(a.box, p, y, z))
+                    a.save[sa_boxidx(a.box, p, y, z)] = a.save_mode == 1 ? (out - 2.f * u + ov[r]) * a.idt2 : u;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) st.nd[r][(J + 1) % 10] = 0.f;
+}
+
+template <int RPT, int NW, bool VARB, bool SAVE>
+__global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
+    using T = Tile<RPT, NW, VARB>;
+    __shared__ float4 sU4[T::UY * T::UZ / 4];
+    __shared__ float4 sB4[VARB ? T::BY * T::BZ / 4 : 1];
+    __shared__ float sFy[T::FY * kTZ];
+    __shared__ float sFz[T::TY * T::FZP];
+    float *sU = reinterpret_cast<float *>(sU4), *sB = reinterpret_cast<float *>(sB4);
+
+    // XCD-aware tile order: hardware block b runs on XCD b % 8; each XCD takes a contiguous
+    // run of tiles (axis 2 fastest, then axis 1, then chunks) so halo re-reads hit its L2
+    const int L = (blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
+    if (L >= a.ntiles) return;
+    const int tz = L % a.ntz, rest = L / a.ntz;
+    const int ty = rest % a.nty, ch = rest / a.nty;
+    const int z0 = tz * kTZ, y0 = ty * T::TY, x0 = ch * a.chunk;
+    const int x1 = min(x0 + a.chunk, g.n[0]);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+
+    Res rs;
+    rs.u = make(a.u, a.bytes);
+    rs.o = make(a.o, a.bytes);
+    rs.b = make(VARB ? a.b : a.u, a.bytes);
+    rs.A = make(a.A, a.bytes);
+    rs.B = make(a.B, a.bytes);
+    rs.G = make(a.G, a.bytes);
+    rs.out = make(a.out, a.bytes);
+
+    State<RPT, NW, VARB> st;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) st.dp[r][k] = st.nd[r][k] = st.uq[r][k] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) st.hb[r][k] = 0.f;
+        st.bprev[r] = 0.f;
+        const int y = y0 + w * RPT + r, z = z0 + lane;
+        st.own[r] = (y < g.n[1] && z < g.n[2]) ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
+    }
+    // rows past the padded plane (y > n1 + 9) would read the next plane: they read zero
+#pragma unroll
+    for (int i = 0; i < T::NLU; ++i) {
+        const int e = t + T::NT * i;
+        const int row = e / T::UC, c = e - row * T::UC;
+        const int y = y0 - 9 + row, z = z0 - 12 + 4 * c;
+        const bool ok = e < T::UY * T::UC;
+        st.ulds[i] = ok ? row * T::UZ + 4 * c : -1;
+        st.uoff[i] = ok && y <= g.n[1] + 9 ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
+    }
+#pragma unroll
+    for (int i = 0; i < T::NLB; ++i) {
+        const int e = t + T::NT * i;
+        const int row = e / T::BC, c = e - row * T::BC;
+        const int y = y0 - 5 + row, z = z0 - 8 + 4 * c;
+        const bool ok = VARB && e < T::BY * T::BC;
+        st.blds[i] = ok ? row * T::BZ + 4 * c : -1;
+        st.boff[i] = ok && y <= g.n[1] + 9 ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
+    }
+
+    const int mbeg = x0 - 9;
+    load_tile(st, rs, (unsigned)(mbeg * a.plane_bytes));
+    // whole periods of 10 planes (compile-time ring slots); planes past x1 + 8 store nothing
+    const int nper = (x1 - x0 + 18 + 9) / 10;
+    for (int q = 0; q < nper; ++q) {
+        const int m = mbeg + 10 * q;
+#define SA_RHO_PLANE(J) plane<RPT, NW, VARB, SAVE, J>(g, a, rs, st, sU, sB, sFy, sFz, m + J, x0, x1, y0, z0, lane, w)
+        SA_RHO_PLANE(0);
+        SA_RHO_PLANE(1);
+        SA_RHO_PLANE(2);
+        SA_RHO_PLANE(3);
+        SA_RHO_PLANE(4);
+        SA_RHO_PLANE(5);
+        SA_RHO_PLANE(6);
+        SA_RHO_PLANE(7);
+        SA_RHO_PLANE(8);
+        SA_RHO_PLANE(9);
+#undef SA_RHO_PLANE
+    }
+}
+
+template <int RPT, int NW>
+int launch(const Geo &g, RhoArgs a, bool varb, bool save, hipStream_t stream) {
+    constexpr int TY = RPT * NW;
+    a.nty = (g.n[1] + TY - 1) / TY;
+    a.ntiles = a.ntz * a.nty * ((g.n[0] + a.chunk - 1) / a.chunk);
+    a.per_xcd = (a.ntiles + 7) / 8;
+    const dim3 grid(8 * a.per_xcd), block(NW * 64);
+    if (varb && save) k_rho3d<RPT, NW, true, true><<<grid, block, 0, stream>>>(g, a);
+    else if (varb) k_rho3d<RPT, NW, true, false><<<grid, block, 0, stream>>>(g, a);
+    else if (save) k_rho3d<RPT, NW, false, true><<<grid, block, 0, stream>>>(g, a);
+    else k_rho3d<RPT, NW, false, false><<<grid, block, 0, stream>>>(g, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace
+
+bool rho3d_supported(const Geo &g) {
+    if (!(g.act[0] && g.act[1] && g.act[2])) return false;
+    // every byte offset (plus the plane offsets of the lead-in/out planes) stays below kOOB
+    return g.total * 4 + 32 * g.s0 * 4 < (long long)bufio::kOOB;
+}
+
+bool rho3d_valid_tune(int rows, int chunk, int waves) {
+    if (chunk < 1) return false;
+    return (waves == 4 || waves == 8) && (rows == 1 || rows == 2);
+}
+
+int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_t stream) {
+    RhoArgs a;
+    a.u = st.u;
+    a.o = st.other;
+    a.b = st.b;
+    a.A = st.A;
+    a.B = st.B;
+    a.G = st.G;
+    a.out = st.out;
+    a.save = st.save;
+    a.bytes = g.total * 4;
+    a.plane_bytes = (int)(g.s0 * 4);
+    a.chunk = t.chunk;
+    a.ntz = (g.n[2] + kTZ - 1) / kTZ;
+    a.save_mode = st.save_mode;
+    a.idt2 = st.idt2;
+    a.box = st.box;
+    const bool varb = st.b != nullptr, save = st.save != nullptr;
+    if (t.waves == 4 && t.rows == 1) return launch<1, 4>(g, a, varb, save, stream);
+    if (t.waves == 4 && t.rows == 2) return launch<2, 4>(g, a, varb, save, stream);
+    if (t.waves == 8 && t.rows == 1) return launch<1, 8>(g, a, varb, save, stream);
+    if (t.waves == 8 && t.rows == 2) return launch<2, 8>(g, a, varb, save, stream);
+    return -1;
+}

@@ -35,6 +35,7 @@
 #include "../../include/stride_amd.h"
 #include "kernels.h"
 #include "fast3d.h"
+#include "rho3d.h"
 
 static thread_local std::string g_err;
 
@@ -503,6 +504,7 @@ struct sa_solver {
     float *gacc[4] = {};
     // fast 3D path
     Fast3D fast;
+    Rho3DTune rho3d;  // fused div(b grad) update (3D, OT2, no PML, no power-2 attenuation)
     // two shots per pass (sa_batch_*): interleaved levels, per-shot taps, interleaved snapshots
     int nshot = 0;
     float *bbuf[2] = {};
@@ -843,6 +845,12 @@ extern "C" int sa_set_receivers(sa_solver *s, int npoint, int ntap, const int64_
     return set_sparse(s, s->rec, npoint, ntap, index, weight);
 }
 
+static bool use_rho3d(const sa_solver *s) {
+    const sa_scheme &c = s->sch;
+    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode != 0 && c.att_power != 2 && c.boundary != 1 &&
+           s->rho3d.enabled && rho3d_supported(s->geo);
+}
+
 static bool use_fast(const sa_solver *s) {
     const sa_scheme &c = s->sch;
     return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 && c.att_power == -2 && c.boundary != 1 &&
@@ -968,6 +976,24 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
     const sa_scheme &c = s->sch;
     hipEvent_t e0;
     TRY(timed_begin(s, &e0));
+    if (use_rho3d(s)) {
+        Rho3DStep st;
+        st.u = u;
+        st.other = other;
+        st.out = out;
+        st.b = c.rho_mode == 2 ? s->b : nullptr;
+        st.A = s->A;
+        st.B = s->B;
+        st.G = s->G;
+        st.save = save;
+        st.save_mode = save_mode;
+        st.box = s->box;
+        st.idt2 = 1.f / (s->dt * s->dt);
+        if (rho3d_step(s->rho3d, g, st, s->stream) != 0)
+            return fail(SA_E_HIP, "fused density update launch: %s", hipGetErrorString(hipGetLastError()));
+        TRY(timed_end(s));
+        return SA_OK;
+    }
     TRY(apply_K(s, u, nullptr, s->Ku));
     const float *F = u;
     if (c.kernel == 4) {
@@ -1509,6 +1535,21 @@ extern "C" int sa_set_tile(sa_solver *s, int rows, int chunk, int version) {
     if (fast3d_set_tune(s->fast, rows, chunk, version) != 0)
         return fail(SA_E_ARG, "tile: chunk >= 1 and version 1-3 with rows 2/4/8, 5 with rows 2/4, 6 or 8 with rows "
                                  "1/2, 7 with rows 2");
+    return SA_OK;
+}
+
+extern "C" int sa_set_tile_density(sa_solver *s, int rows, int chunk, int waves) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    if (rows == 0) {
+        s->rho3d.enabled = false;
+        return SA_OK;
+    }
+    if (!rho3d_valid_tune(rows, chunk, waves))
+        return fail(SA_E_ARG, "density tile: chunk >= 1, rows 1 or 2, waves 4 or 8");
+    s->rho3d.rows = rows;
+    s->rho3d.chunk = chunk;
+    s->rho3d.waves = waves;
+    s->rho3d.enabled = true;
     return SA_OK;
 }
 
