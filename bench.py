@@ -260,6 +260,66 @@ def bench_schedule(tile=None, num_shots=32):
                                     'forward only, shots paired by duration through the two-shot update'})
 
 
+def bench_density(nt=2500, tile=None):
+    """One 3D shot on the variable vp / rho / alpha medium (360^3 extended, attenuation power 0,
+    sponge 20, OT2, 1024 receivers): forward with the raw wavefield saved every 4th step over the
+    whole domain, then the as-written adjoint with the vp and rho gradients. The update runs as the
+    fused density kernel; `roofline` covers it (algorithmic bytes: read u, the other level, b, A, B,
+    G, write the new level = 28 B per point update, + 4 B per point on save steps)."""
+    from stride_amd.configs import config4
+    from stride_amd.physics.plan import build_plan
+    from stride_amd._native import Solver
+    cfg = config4(scale=1.0, nt=nt, nrec=1024)
+    space, time_ = cfg['space'], cfg['time']
+    plan = build_plan(space, time_, cfg['wavelet'][None], cfg['vp'], rho=cfg['rho'], alpha=cfg['alpha'],
+                      source_coordinates=cfg['sources'], receiver_coordinates=cfg['receivers'], kernel='OT2',
+                      save_undersampling=4, grad_vp=True, grad_rho=True)
+    s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, plan.nt)
+    s.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
+                 plan.undersampling, 2, plan.tb0, plan.tb1, 1, 1, 0)
+    s.set_medium(plan.vp, plan.rho, plan.alpha)
+    s.set_sponge(plan.sponge_tables)
+    s.set_sources(plan.src_idx, plan.src_w)
+    s.set_receivers(plan.rec_idx, plan.rec_w)
+    if tile:
+        s.set_tile_density(*tile)
+    rng = np.random.default_rng(0)
+    d_src = s._to_device(plan.src_data)
+    d_rec = s.device_alloc(plan.nt * plan.nrec * 4)
+    d_res = s._to_device(rng.standard_normal((plan.nt, plan.nrec)).astype(np.float32) * 1e-3)
+    want = 3  # vp and rho gradients
+
+    def shot():
+        s.forward_dev(d_src, d_rec)
+        s.adjoint_dev(d_res, want)
+        s.synchronize()
+
+    shot()  # allocations (saved wavefield, gradient accumulators)
+    s.timing(True)
+    t0 = time.perf_counter()
+    shot()
+    wall = time.perf_counter() - t0
+    ms, launches = s.timing()
+    s.timing(False)
+    n = int(np.prod(plan.shape))
+    nsave = len(range(0, plan.nt - 1, plan.undersampling))
+    alg = launches * 28.0 * n + nsave * 4.0 * n
+    achieved = alg / (ms * 1e-3) / 1e9
+    for d in (d_src, d_rec, d_res):
+        s.device_free(d)
+    s.close()
+    return dict(metric='Gpts/s fwd+adjoint 3D 320^3 SO10, variable vp/rho/alpha, vp+rho gradients',
+                value=round(launches * n / wall / 1e9, 3), unit='Gpts/s', seconds=round(wall, 3),
+                update_launches=int(launches), update_kernel_us=round(ms * 1e3 / launches, 1),
+                update_kernel_share=round(ms * 1e-3 / wall, 3),
+                roofline={'bound': 'hbm', 'kernel': 'k_rho3d', 'achieved': round(achieved, 1),
+                          'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4)},
+                dtype='f32', data='synthetic',
+                config={'workload': '3D 320x320x320 inner (360x360x360 extended), dx 0.4 mm, SO10 OT2, sponge 20, '
+                                    'variable vp / rho (Gaussian-filtered noise in the hemisphere) and alpha 0.5 '
+                                    '(power 0), one shot, nt %d, raw saves every 4th step, 1024 receivers' % plan.nt})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -279,6 +339,9 @@ def main():
     ap.add_argument('--schedule', action='store_true',
                     help='only run the batched 3D forward over 32 shots of uneven duration and receiver count '
                          '(seeded schedule) and exit')
+    ap.add_argument('--density', action='store_true',
+                    help='only run one 3D shot on the variable vp / rho / alpha medium (fused density update, '
+                         'vp and rho gradients) and exit')
     ap.add_argument('--pipelines', action='store_true',
                     help='only time the device steps either side of the solver (trace processing, misfit, '
                          'gradient post-processing, model update) at the full workload and exit')
@@ -303,6 +366,10 @@ def main():
         print(json.dumps({'copy_check_bytes': nbytes, 'copies': 10}))
         return
 
+    if args.density:
+        tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
+        print(json.dumps(bench_density(args.nt, tile)))
+        return
     if args.schedule:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
         print(json.dumps(bench_schedule(tile)))

@@ -8,7 +8,7 @@
 
 struct Rho3DTune {
     int rows = 1;    // rows (axis 1) per thread; the tile is 64 (axis 2) x waves*rows (axis 1)
-    int chunk = 40;  // planes (axis 0) per workgroup
+    int chunk = 60;  // planes (axis 0) per workgroup
     int waves = 8;   // 4 or 8
     bool enabled = true;
 };
@@ -27,5 +27,5 @@ struct Rho3DStep {
 // true when the layout fits the kernel's 32-bit buffer offsets
 bool rho3d_supported(const Geo &g);
 int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_t stream);
-// rows 1/2, waves 4/8, chunk >= 1
+// rows 1/2 with waves 4/8, or rows 1 with waves 16; chunk >= 1
 bool rho3d_valid_tune(int rows, int chunk, int waves);

@@ -11,13 +11,13 @@
 // Work decomposition: a workgroup of NW waves owns a tile of 64 (axis 2) x NW*RPT (axis 1)
 // cells and marches along axis 0 over a chunk of planes. For every input plane m:
 //  - the plane's u with a 9-cell halo (rows y0-9 .. y0+TY+8, columns z0-12 .. z0+75) and b
-//    with a 5-cell halo are staged in LDS (loaded one plane ahead into registers);
+//    with a 5-cell halo are staged in LDS (requested two planes ahead into registers);
 //  - the in-plane fluxes bbar D+ u along axes 1 and 2 are formed once per half point into LDS
 //    tiles, and their D- divergence is added to the node accumulator of plane m;
-//  - the own cell of plane m is pushed into a ring of axis-0 D+ partial sums for the half
-//    planes m-5 .. m+4; half plane m-5 is then complete, is scaled by its bbar and pushed
-//    into the node accumulators of planes m-9 .. m; node plane m-9 is then complete, is
-//    updated and stored.
+//  - the own cell of plane m enters a register ring of u (planes m-9 .. m), from which the
+//    axis-0 flux of half plane m-5 is formed, scaled by its bbar and pushed into the node
+//    accumulators of planes m-9 .. m; node plane m-9 is then complete, is updated (its other
+//    level and coefficients were requested one plane earlier) and stored.
 // So u, b, the other level and the coefficients are each read once per step (plus the
 // 18 planes of lead-in and lead-out per chunk), and the new level is written once.
 // Every load is unconditional: cells beyond the padded rows read zero through the buffer
@@ -57,30 +57,49 @@ struct Tile {
 template <int RPT, int NW, bool VARB>
 struct State {
     using T = Tile<RPT, NW, VARB>;
-    float dp[RPT][10];  // axis-0 D+ partial sums; slot of half plane q: (q - mbeg) % 10
     float nd[RPT][10];  // node accumulators (K u); slot of plane p: (p - mbeg) % 10
     float uq[RPT][10];  // own u; slot of plane p
     float hb[RPT][5];   // axis-0 bbar of half plane q; slot (q - mbeg) % 5
     float bprev[RPT];   // own b of the previous plane
-    float4 su[T::NLU];  // next plane's u tile, staged in registers
-    float4 sb[T::NLB];
+    float4 su[2][T::NLU];  // u tiles of the next two planes, staged in registers (slot: plane % 2)
+    float4 sb[2][T::NLB];
+    float ov[2][RPT], Av[2][RPT], Bv[2][RPT], Gv[2][RPT];  // own operands of the next node plane
     unsigned uoff[T::NLU], boff[T::NLB];  // in-plane byte offsets of the tile chunks (kOOB: none)
-    int ulds[T::NLU], blds[T::NLB];       // their LDS float offsets (-1: none)
+    int ulds[T::NLU], blds[T::NLB];       // their LDS float offsets (the dump slot: none)
     unsigned own[RPT];                    // own cells' in-plane byte offsets (kOOB outside the domain)
 };
+
+// Workgroup barrier that orders LDS accesses only: a __syncthreads() would also wait for every
+// outstanding global load and store (the prefetched planes) before the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 struct Res {
     rsrc_t u, o, b, A, B, G, out;
 };
 
-template <int RPT, int NW, bool VARB>
+template <int S, int RPT, int NW, bool VARB>
 __device__ __forceinline__ void load_tile(State<RPT, NW, VARB> &st, const Res &rs, unsigned poff) {
     using T = Tile<RPT, NW, VARB>;
 #pragma unroll
-    for (int i = 0; i < T::NLU; ++i) st.su[i] = load4(rs.u, st.uoff[i] + poff);
+    for (int i = 0; i < T::NLU; ++i) st.su[S][i] = load4(rs.u, st.uoff[i] + poff);
     if (VARB) {
 #pragma unroll
-        for (int i = 0; i < T::NLB; ++i) st.sb[i] = load4(rs.b, st.boff[i] + poff);
+        for (int i = 0; i < T::NLB; ++i) st.sb[S][i] = load4(rs.b, st.boff[i] + poff);
+    }
+}
+
+template <int S, int RPT, int NW, bool VARB>
+__device__ __forceinline__ void load_own(State<RPT, NW, VARB> &st, const Res &rs, unsigned poff) {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        st.ov[S][r] = load_nt(rs.o, st.own[r] + poff);
+        st.Av[S][r] = load_nt(rs.A, st.own[r] + poff);
+        st.Bv[S][r] = load_nt(rs.B, st.own[r] + poff);
+        st.Gv[S][r] = load_nt(rs.G, st.own[r] + poff);
     }
 }
 
@@ -89,64 +108,56 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
                                       float *sU, float *sB, float *sFy, float *sFz, const int m, const int x0,
                                       const int x1, const int y0, const int z0, const int lane, const int w) {
     using T = Tile<RPT, NW, VARB>;
-    // slots: plane / half plane m -> J, m - 9 -> (J + 1) % 10, half plane m - 5 -> (J + 5) % 10
-    __syncthreads();  // every wave is done with the previous plane's LDS tiles
+    // slots: plane m -> J, m - 9 -> (J + 1) % 10; staging slot of plane m: J % 2 (m - mbeg = J mod 10)
+    constexpr int S = J % 2;
+    lds_barrier();  // every wave is done with the previous plane's LDS tiles
 #pragma unroll
-    for (int i = 0; i < T::NLU; ++i)
-        if (st.ulds[i] >= 0) *reinterpret_cast<float4 *>(sU + st.ulds[i]) = st.su[i];
+    for (int i = 0; i < T::NLU; ++i) *reinterpret_cast<float4 *>(sU + st.ulds[i]) = st.su[S][i];
     if (VARB) {
 #pragma unroll
-        for (int i = 0; i < T::NLB; ++i)
-            if (st.blds[i] >= 0) *reinterpret_cast<float4 *>(sB + st.blds[i]) = st.sb[i];
+        for (int i = 0; i < T::NLB; ++i) *reinterpret_cast<float4 *>(sB + st.blds[i]) = st.sb[S][i];
     }
-    if (m + 1 <= x1 + 8) load_tile(st, rs, (unsigned)((m + 1) * a.plane_bytes));
+    // requests two planes ahead for the tiles, one node plane ahead for the own operands; all
+    // unconditional (planes outside the chunk are read but never stored; past the array they
+    // read zero), so the wait counts before each use are exact
+    load_tile<S>(st, rs, (unsigned)((m + 2) * a.plane_bytes));
     const int p = m - 9;
     const bool emit = p >= x0 && p < x1;
-    float ov[RPT], Av[RPT], Bv[RPT], Gv[RPT];
-    if (emit) {
-        const unsigned po = (unsigned)(p * a.plane_bytes);
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            ov[r] = load_nt(rs.o, st.own[r] + po);
-            Av[r] = load_nt(rs.A, st.own[r] + po);
-            Bv[r] = load_nt(rs.B, st.own[r] + po);
-            Gv[r] = load_nt(rs.G, st.own[r] + po);
-        }
-    }
-    __syncthreads();  // plane m's tiles are in LDS
+    load_own<1 - S>(st, rs, (unsigned)((p + 1) * a.plane_bytes));
+    lds_barrier();  // plane m's tiles are in LDS
     const bool inplane = m >= x0 && m < x1;
     if (inplane) {
-        // axis-1 fluxes at half rows q = y0 - 5 + fr, column z0 + lane
-        for (int fr = w; fr < T::FY; fr += NW) {
-            const float *c = sU + (fr + 4) * T::UZ + lane + 12;  // u at row q
-            float d = 0.f;
+        // one index space over both flux tiles, so that every wave gets the same share: items
+        // [0, FY*64) are axis-1 fluxes (whole waves, since FY*64 is a multiple of 64), the rest
+        // axis-2 fluxes
+        for (int e = w * 64 + lane; e < T::FY * kTZ + T::TY * T::FZN; e += T::NT) {
+            if (e < T::FY * kTZ) {
+                // axis-1 flux at half row q = y0 - 5 + fr, column z0 + lane
+                const int fr = e >> 6;
+                const float *c = sU + (fr + 4) * T::UZ + lane + 12;  // u at row q
+                float d = 0.f;
 #pragma unroll
-            for (int k = 0; k < 5; ++k) d += g.ws[1][k] * (c[(k + 1) * T::UZ] - c[-k * T::UZ]);
-            if (VARB) d *= 0.5f * (sB[fr * T::BZ + lane + 8] + sB[(fr + 1) * T::BZ + lane + 8]);
-            sFy[fr * kTZ + lane] = d;
-        }
-        // axis-2 fluxes at half columns q = z0 - 5 + fz, row y0 + row
-        for (int e = w * 64 + lane; e < T::TY * T::FZN; e += T::NT) {
-            const int row = e / T::FZN, fz = e - row * T::FZN;
-            const float *c = sU + (row + 9) * T::UZ + fz + 7;  // u at column q
-            float d = 0.f;
+                for (int k = 0; k < 5; ++k) d += g.ws[1][k] * (c[(k + 1) * T::UZ] - c[-k * T::UZ]);
+                if (VARB) d *= 0.5f * (sB[fr * T::BZ + lane + 8] + sB[(fr + 1) * T::BZ + lane + 8]);
+                sFy[fr * kTZ + lane] = d;
+            } else {
+                // axis-2 flux at half column q = z0 - 5 + fz, row y0 + row
+                const int ez = e - T::FY * kTZ;
+                const int row = ez / T::FZN, fz = ez - row * T::FZN;
+                const float *c = sU + (row + 9) * T::UZ + fz + 7;  // u at column q
+                float d = 0.f;
 #pragma unroll
-            for (int k = 0; k < 5; ++k) d += g.ws[2][k] * (c[k + 1] - c[-k]);
-            if (VARB) d *= 0.5f * (sB[(row + 5) * T::BZ + fz + 3] + sB[(row + 5) * T::BZ + fz + 4]);
-            sFz[row * T::FZP + fz] = d;
+                for (int k = 0; k < 5; ++k) d += g.ws[2][k] * (c[k + 1] - c[-k]);
+                if (VARB) d *= 0.5f * (sB[(row + 5) * T::BZ + fz + 3] + sB[(row + 5) * T::BZ + fz + 4]);
+                sFz[row * T::FZP + fz] = d;
+            }
         }
     }
-    // axis 0: own cell of plane m into the D+ partial sums of half planes m-5 .. m+4
+    // axis 0: own cell of plane m into the u ring (planes m-9 .. m)
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         const int ry = w * RPT + r;
-        const float uc = sU[(ry + 9) * T::UZ + lane + 12];
-        st.uq[r][J] = uc;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            st.dp[r][(J + 9 - k) % 10] += g.ws[0][k] * uc;  // half plane m - 1 - k
-            st.dp[r][(J + k) % 10] -= g.ws[0][k] * uc;      // half plane m + k
-        }
+        st.uq[r][J] = sU[(ry + 9) * T::UZ + lane + 12];
         if (VARB) {
             const float bc = sB[(ry + 5) * T::BZ + lane + 8];
             st.hb[r][(J + 4) % 5] = 0.5f * (st.bprev[r] + bc);  // half plane m - 1
@@ -154,7 +165,7 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
         }
     }
     if (inplane) {
-        __syncthreads();  // fluxes are in LDS
+        lds_barrier();  // fluxes are in LDS
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const int ry = w * RPT + r;
@@ -168,11 +179,12 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
             st.nd[r][J] += d;
         }
     }
-    // half plane m - 5 is complete: its flux goes to the nodes m-9 .. m
+    // axis-0 flux at half plane m - 5 (u of planes m-9 .. m) goes to the nodes m-9 .. m
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        float f = st.dp[r][(J + 5) % 10];
-        st.dp[r][(J + 5) % 10] = 0.f;
+        float f = 0.f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) f += g.ws[0][k] * (st.uq[r][(J + 6 + k) % 10] - st.uq[r][(J + 15 - k) % 10]);
         if (VARB) f *= st.hb[r][J % 5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
@@ -180,18 +192,18 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
             st.nd[r][(J + 6 + k) % 10] -= g.ws[0][k] * f;       // node m - 4 + k
         }
     }
-    // node plane m - 9 is complete
-    if (emit) {
+    // node plane m - 9 is complete (stored only inside the chunk)
+    {
         const unsigned po = (unsigned)(p * a.plane_bytes);
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const float u = st.uq[r][(J + 1) % 10];
-            const float out = Av[r] * u + Bv[r] * ov[r] + Gv[r] * st.nd[r][(J + 1) % 10];
-            store_nt(out, rs.out, st.own[r] + po);
-            if (SAVE) {
+            const float out = st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][r] + st.Gv[S][r] * st.nd[r][(J + 1) % 10];
+            store_nt(out, rs.out, emit ? st.own[r] + po : kOOB);
+            if (SAVE && emit) {
                 const int y = y0 + w * RPT + r, z = z0 + lane;
                 if (sa_inbox(a.box, p, y, z))
-                    a.save[sa_boxidx(a.box, p, y, z)] = a.save_mode == 1 ? (out - 2.f * u + ov[r]) * a.idt2 : u;
+                    a.save[sa_boxidx(a.box, p, y, z)] = a.save_mode == 1 ? (out - 2.f * u + st.ov[S][r]) * a.idt2 : u;
             }
         }
     }
@@ -202,8 +214,8 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
 template <int RPT, int NW, bool VARB, bool SAVE>
 __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     using T = Tile<RPT, NW, VARB>;
-    __shared__ float4 sU4[T::UY * T::UZ / 4];
-    __shared__ float4 sB4[VARB ? T::BY * T::BZ / 4 : 1];
+    __shared__ float4 sU4[T::UY * T::UZ / 4 + 1];  // + a dump slot for threads without a chunk
+    __shared__ float4 sB4[VARB ? T::BY * T::BZ / 4 + 1 : 1];
     __shared__ float sFy[T::FY * kTZ];
     __shared__ float sFz[T::TY * T::FZP];
     float *sU = reinterpret_cast<float *>(sU4), *sB = reinterpret_cast<float *>(sB4);
@@ -231,7 +243,7 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
 #pragma unroll
-        for (int k = 0; k < 10; ++k) st.dp[r][k] = st.nd[r][k] = st.uq[r][k] = 0.f;
+        for (int k = 0; k < 10; ++k) st.nd[r][k] = st.uq[r][k] = 0.f;
 #pragma unroll
         for (int k = 0; k < 5; ++k) st.hb[r][k] = 0.f;
         st.bprev[r] = 0.f;
@@ -245,7 +257,7 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
         const int row = e / T::UC, c = e - row * T::UC;
         const int y = y0 - 9 + row, z = z0 - 12 + 4 * c;
         const bool ok = e < T::UY * T::UC;
-        st.ulds[i] = ok ? row * T::UZ + 4 * c : -1;
+        st.ulds[i] = ok ? row * T::UZ + 4 * c : T::UY * T::UZ;
         st.uoff[i] = ok && y <= g.n[1] + 9 ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
     }
 #pragma unroll
@@ -254,12 +266,13 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
         const int row = e / T::BC, c = e - row * T::BC;
         const int y = y0 - 5 + row, z = z0 - 8 + 4 * c;
         const bool ok = VARB && e < T::BY * T::BC;
-        st.blds[i] = ok ? row * T::BZ + 4 * c : -1;
+        st.blds[i] = ok ? row * T::BZ + 4 * c : T::BY * T::BZ;
         st.boff[i] = ok && y <= g.n[1] + 9 ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
     }
 
     const int mbeg = x0 - 9;
-    load_tile(st, rs, (unsigned)(mbeg * a.plane_bytes));
+    load_tile<0>(st, rs, (unsigned)(mbeg * a.plane_bytes));
+    load_tile<1>(st, rs, (unsigned)((mbeg + 1) * a.plane_bytes));
     // whole periods of 10 planes (compile-time ring slots); planes past x1 + 8 store nothing
     const int nper = (x1 - x0 + 18 + 9) / 10;
     for (int q = 0; q < nper; ++q) {
@@ -303,7 +316,7 @@ bool rho3d_supported(const Geo &g) {
 
 bool rho3d_valid_tune(int rows, int chunk, int waves) {
     if (chunk < 1) return false;
-    return (waves == 4 || waves == 8) && (rows == 1 || rows == 2);
+    return ((waves == 4 || waves == 8) && (rows == 1 || rows == 2)) || (waves == 16 && rows == 1);
 }
 
 int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_t stream) {
@@ -328,5 +341,6 @@ int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_
     if (t.waves == 4 && t.rows == 2) return launch<2, 4>(g, a, varb, save, stream);
     if (t.waves == 8 && t.rows == 1) return launch<1, 8>(g, a, varb, save, stream);
     if (t.waves == 8 && t.rows == 2) return launch<2, 8>(g, a, varb, save, stream);
+    if (t.waves == 16 && t.rows == 1) return launch<1, 16>(g, a, varb, save, stream);
     return -1;
 }

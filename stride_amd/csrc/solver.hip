@@ -1545,7 +1545,7 @@ extern "C" int sa_set_tile_density(sa_solver *s, int rows, int chunk, int waves)
         return SA_OK;
     }
     if (!rho3d_valid_tune(rows, chunk, waves))
-        return fail(SA_E_ARG, "density tile: chunk >= 1, rows 1 or 2, waves 4 or 8");
+        return fail(SA_E_ARG, "density tile: chunk >= 1, rows 1 or 2 with 4 or 8 waves, rows 1 with 16 waves");
     s->rho3d.rows = rows;
     s->rho3d.chunk = chunk;
     s->rho3d.waves = waves;

@@ -58,7 +58,7 @@ def test_fused_density_large_grid_and_tiles(gpu):
     s.set_tile_density(0)
     general = s.forward(plan.src_data).T
     assert rel_l2(general, ref) <= 1e-5
-    for rows, chunk, waves in ((1, 40, 8), (2, 40, 8), (1, 7, 4), (2, 13, 4), (1, 86, 8)):
+    for rows, chunk, waves in ((1, 40, 8), (2, 40, 8), (1, 7, 4), (2, 13, 4), (1, 86, 8), (1, 23, 16)):
         print('  tile', rows, chunk, waves, flush=True)
         s.set_tile_density(rows, chunk, waves)
         got = s.forward(plan.src_data).T
