@@ -88,3 +88,26 @@ def test_fused_density_compat_adjoint(gpu, opts):
         print('  %s rel L2 %.3e' % (key, err))
         assert np.abs(ref[key]).max() > 0
         assert err <= 1e-4, key
+
+
+def test_fused_density_exact_adjoint(gpu):
+    """The exact adjoint (the transpose of the forward map, vp / rho / alpha sensitivities) runs
+    its backward steps through the fused update as well: against the oracle's exact adjoint."""
+    from stride_amd.physics.gradients import exact_gradients
+    plan, _ = _case(rho='var', alpha='p0', grad_vp=True)
+    print('density3d exact adjoint', plan.shape, flush=True)
+    rng = np.random.default_rng(13)
+    res = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, plan.nt)).astype(np.float32))
+    ref = oracle.adjoint_exact(plan, res, oracle.forward_states(plan))
+    s = solver_for(plan, save_mode=3)
+    s.forward(plan.src_data)
+    got = s.adjoint_exact(res, want_b=True)
+    s.close()
+    g_ref = exact_gradients(plan, ref, ref['lam_src'])
+    g_got = exact_gradients(plan, got, got['lam_src'])
+    assert set(g_ref) >= {'vp', 'rho', 'alpha'}
+    for key in g_ref:
+        err = rel_l2(g_got[key], g_ref[key])
+        print('  d/d%s rel L2 %.3e' % (key, err))
+        assert np.abs(g_ref[key]).max() > 0
+        assert err <= 1e-4, key
