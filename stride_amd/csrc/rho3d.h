@@ -29,3 +29,10 @@ bool rho3d_supported(const Geo &g);
 int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_t stream);
 // rows 1/2 with waves 4/8, or rows 1 with waves 16; chunk >= 1
 bool rho3d_valid_tune(int rows, int chunk, int waves);
+
+// Density gradient step of the as-written adjoint on the interior, read from the raw snapshots
+// (whole-domain box): grad_rho / prec_rho and, when gv / pv are given, grad_vp / prec_vp from the
+// second time difference of the snapshots t - f, t, t + f (psm / psp null: zero).
+bool rho3d_grad_supported(const Geo &g, const Box &box);
+int rho3d_grad(const Geo &g, const Box &box, const float *ps, const float *psm, const float *psp, float fdt2,
+               const float *b, const float *pa, float *gv, float *pv, float *gr, float *pr, hipStream_t stream);

@@ -306,6 +306,214 @@ int launch(const Geo &g, RhoArgs a, bool varb, bool save, hipStream_t stream) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+
+// ------------------------------------------------------------------------------------------
+// Density gradient step (as-written adjoint, every f-th step): on the interior,
+//   grad_rho += f pa,  prec_rho += f^2,  f = -sum_d D-_d(b) D-_d(p) - b lap(p)
+// (stride/physics/iso_acoustic/devito.py:1115-1150) with p the raw saved wavefield, and
+//   grad_vp += pdt pa,  prec_vp += pdt^2,  pdt = (p[t+f] - 2 p[t] + p[t-f]) / (f dt)^2
+// (common/devito.py:541-550), the two read from the snapshots directly (whole-domain box
+// layout), so no unpacking pass. Same march as the update: the plane tiles of p and b with a
+// 5-cell halo in LDS, register rings of the own column of p and b along axis 0 (planes p-5 ..
+// p+5) and of the in-plane partial terms; node plane m-5 is finished when plane m arrives.
+
+struct GradArgs {
+    const float *ps, *psm, *psp;  // snapshots t, t - f, t + f (psm / psp null: zero)
+    const float *b, *pa;          // buoyancy, adjoint level (padded layout)
+    float *gv, *pv, *gr, *pr;     // accumulators (padded layout); gv / pv null: no vp gradient
+    long long bytes, sbytes;
+    int plane_bytes, splane_bytes, srow;  // srow: snapshot row pitch in floats
+    float fdt2;
+    int lo[3], hi[3];  // interior [lo, hi)
+    int chunk, ntz, nty, ntiles, per_xcd;
+};
+
+template <int NW>
+struct GTile {
+    static constexpr int NT = NW * 64, TY = NW;
+    static constexpr int LY = TY + 10, LZ = 80, LC = LZ / 4;  // rows y0-5 .., columns z0-8 ..
+    static constexpr int NL = (LY * LC + NT - 1) / NT;
+};
+
+template <int NW>
+struct GState {
+    using T = GTile<NW>;
+    float pr_[12], br_[12], ir_[12], lr_[12];  // own p, own b, in-plane products, in-plane Laplacian
+    float4 sp[2][T::NL], sb[2][T::NL];
+    unsigned poff[T::NL], boff[T::NL];
+    int lds[T::NL];
+    unsigned own, sown;  // own cell: padded-layout and snapshot byte offsets within a plane
+    float pa[2], pm[2], pp[2], gv[2], pv[2], gr[2], prr[2];
+};
+
+struct GRes {
+    rsrc_t ps, psm, psp, b, pa, gv, pv, gr, pr;
+};
+
+template <int S, int NW>
+__device__ __forceinline__ void gload_tile(GState<NW> &st, const GRes &rs, const GradArgs &a, int m) {
+    using T = GTile<NW>;
+    const unsigned sp = (unsigned)(m * a.splane_bytes), pp = (unsigned)(m * a.plane_bytes);
+#pragma unroll
+    for (int i = 0; i < T::NL; ++i) {
+        st.sp[S][i] = load4(rs.ps, st.poff[i] + sp);
+        st.sb[S][i] = load4(rs.b, st.boff[i] + pp);
+    }
+}
+
+template <int S, int NW>
+__device__ __forceinline__ void gload_own(GState<NW> &st, const GRes &rs, const GradArgs &a, int p) {
+    const unsigned po = st.own + (unsigned)(p * a.plane_bytes), so = st.sown + (unsigned)(p * a.splane_bytes);
+    st.pa[S] = load_nt(rs.pa, po);
+    st.pm[S] = load_nt(rs.psm, so);
+    st.pp[S] = load_nt(rs.psp, so);
+    st.gr[S] = load_nt(rs.gr, po);
+    st.prr[S] = load_nt(rs.pr, po);
+    st.gv[S] = load_nt(rs.gv, po);
+    st.pv[S] = load_nt(rs.pv, po);
+}
+
+template <int NW, int J>
+__device__ __forceinline__ void gplane(const Geo &g, const GradArgs &a, const GRes &rs, GState<NW> &st, float *sP,
+                                       float *sB, const int m, const int x0, const int x1, const int lane,
+                                       const int w) {
+    using T = GTile<NW>;
+    constexpr int S = J % 2;
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < T::NL; ++i) {
+        *reinterpret_cast<float4 *>(sP + st.lds[i]) = st.sp[S][i];
+        *reinterpret_cast<float4 *>(sB + st.lds[i]) = st.sb[S][i];
+    }
+    gload_tile<S>(st, rs, a, m + 2);
+    const int p = m - 5;
+    gload_own<1 - S>(st, rs, a, p + 1);
+    lds_barrier();
+    const int c = (w + 5) * T::LZ + lane + 8;  // own cell in the tiles
+    st.pr_[J] = sP[c];
+    st.br_[J] = sB[c];
+    if (m >= x0 && m < x1) {
+        float prod = 0.f, lap = 0.f;
+        // axis 1 (rows)
+        {
+            float db = 0.f, dp = 0.f, l = g.w2[1][0] * sP[c];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                db += g.ws[1][r] * (sB[c + r * T::LZ] - sB[c - (r + 1) * T::LZ]);
+                dp += g.ws[1][r] * (sP[c + r * T::LZ] - sP[c - (r + 1) * T::LZ]);
+            }
+#pragma unroll
+            for (int r = 1; r <= 5; ++r) l += g.w2[1][r] * (sP[c + r * T::LZ] + sP[c - r * T::LZ]);
+            prod += db * dp;
+            lap += l;
+        }
+        // axis 2 (columns)
+        {
+            float db = 0.f, dp = 0.f, l = g.w2[2][0] * sP[c];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                db += g.ws[2][r] * (sB[c + r] - sB[c - (r + 1)]);
+                dp += g.ws[2][r] * (sP[c + r] - sP[c - (r + 1)]);
+            }
+#pragma unroll
+            for (int r = 1; r <= 5; ++r) l += g.w2[2][r] * (sP[c + r] + sP[c - r]);
+            prod += db * dp;
+            lap += l;
+        }
+        st.ir_[J] = prod;
+        st.lr_[J] = lap;
+    }
+    // node plane p = m - 5 (ring slot (J + 7) % 12): axis 0 from the rings of planes p-5 .. p+5
+    {
+        constexpr int C = (J + 7) % 12;
+        float db = 0.f, dp = 0.f, l = g.w2[0][0] * st.pr_[C];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            db += g.ws[0][r] * (st.br_[(J + 7 + r) % 12] - st.br_[(J + 6 - r) % 12]);
+            dp += g.ws[0][r] * (st.pr_[(J + 7 + r) % 12] - st.pr_[(J + 6 - r) % 12]);
+        }
+#pragma unroll
+        for (int r = 1; r <= 5; ++r) l += g.w2[0][r] * (st.pr_[(J + 7 + r) % 12] + st.pr_[(J + 7 - r) % 12]);
+        const float f = -(st.ir_[C] + db * dp) - st.br_[C] * (st.lr_[C] + l);
+        const bool emit = p >= x0 && p < x1;
+        const unsigned po = emit ? st.own + (unsigned)(p * a.plane_bytes) : kOOB;
+        const float pav = st.pa[S];
+        store_nt(st.gr[S] + f * pav, rs.gr, po);
+        store_nt(st.prr[S] + f * f, rs.pr, po);
+        if (a.gv) {
+            const float pdt = (st.pp[S] - 2.f * st.pr_[C] + st.pm[S]) * a.fdt2;
+            store_nt(st.gv[S] + pdt * pav, rs.gv, po);
+            store_nt(st.pv[S] + pdt * pdt, rs.pv, po);
+        }
+    }
+}
+
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_grad_density(Geo g, GradArgs a) {
+    using T = GTile<NW>;
+    __shared__ float4 sP4[T::LY * T::LZ / 4 + 1];  // + a dump slot for threads without a chunk
+    __shared__ float4 sB4[T::LY * T::LZ / 4 + 1];
+    float *sP = reinterpret_cast<float *>(sP4), *sB = reinterpret_cast<float *>(sB4);
+    const int L = (blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
+    if (L >= a.ntiles) return;
+    const int tz = L % a.ntz, rest = L / a.ntz;
+    const int ty = rest % a.nty, ch = rest / a.nty;
+    const int z0 = a.lo[2] + tz * kTZ, y0 = a.lo[1] + ty * T::TY, x0 = a.lo[0] + ch * a.chunk;
+    const int x1 = min(x0 + a.chunk, a.hi[0]);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    GRes rs;
+    rs.ps = make(a.ps, a.sbytes);
+    rs.psm = make(a.psm ? a.psm : a.ps, a.psm ? a.sbytes : 0);  // null: every read is zero
+    rs.psp = make(a.psp ? a.psp : a.ps, a.psp ? a.sbytes : 0);
+    rs.b = make(a.b, a.bytes);
+    rs.pa = make(a.pa, a.bytes);
+    rs.gr = make(a.gr, a.bytes);
+    rs.pr = make(a.pr, a.bytes);
+    rs.gv = make(a.gv ? a.gv : a.gr, a.gv ? a.bytes : 0);
+    rs.pv = make(a.pv ? a.pv : a.pr, a.pv ? a.bytes : 0);
+    GState<NW> st;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) st.pr_[k] = st.br_[k] = st.ir_[k] = st.lr_[k] = 0.f;
+    {
+        const int y = y0 + w, z = z0 + lane;
+        const bool ok = y < a.hi[1] && z < a.hi[2];
+        st.own = ok ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
+        st.sown = ok ? (unsigned)(((long long)y * a.srow + z) * 4) : kOOB;
+    }
+#pragma unroll
+    for (int i = 0; i < T::NL; ++i) {
+        const int e = t + T::NT * i;
+        const int row = e / T::LC, c = e - row * T::LC;
+        const int y = y0 - 5 + row, z = z0 - 8 + 4 * c;
+        const bool ok = e < T::LY * T::LC && y < g.n[1];
+        st.lds[i] = e < T::LY * T::LC ? row * T::LZ + 4 * c : T::LY * T::LZ;
+        st.poff[i] = ok ? (unsigned)(((long long)y * a.srow + z) * 4) : kOOB;
+        st.boff[i] = ok ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
+    }
+    const int mbeg = x0 - 5;
+    gload_tile<0>(st, rs, a, mbeg);
+    gload_tile<1>(st, rs, a, mbeg + 1);
+    gload_own<0>(st, rs, a, mbeg - 5);
+    const int nper = (x1 - x0 + 10 + 11) / 12;
+    for (int q = 0; q < nper; ++q) {
+        const int m = mbeg + 12 * q;
+#define SA_GRAD_PLANE(J) gplane<NW, J>(g, a, rs, st, sP, sB, m + J, x0, x1, lane, w)
+        SA_GRAD_PLANE(0);
+        SA_GRAD_PLANE(1);
+        SA_GRAD_PLANE(2);
+        SA_GRAD_PLANE(3);
+        SA_GRAD_PLANE(4);
+        SA_GRAD_PLANE(5);
+        SA_GRAD_PLANE(6);
+        SA_GRAD_PLANE(7);
+        SA_GRAD_PLANE(8);
+        SA_GRAD_PLANE(9);
+        SA_GRAD_PLANE(10);
+        SA_GRAD_PLANE(11);
+#undef SA_GRAD_PLANE
+    }
+}
+
 }  // namespace
 
 bool rho3d_supported(const Geo &g) {
@@ -343,4 +551,44 @@ int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_
     if (t.waves == 8 && t.rows == 2) return launch<2, 8>(g, a, varb, save, stream);
     if (t.waves == 16 && t.rows == 1) return launch<1, 16>(g, a, varb, save, stream);
     return -1;
+}
+
+bool rho3d_grad_supported(const Geo &g, const Box &box) {
+    if (!rho3d_supported(g)) return false;
+    for (int d = 0; d < 3; ++d)
+        if (g.nabs[d] < 8 || 2 * g.nabs[d] >= g.n[d] || box.lo[d] != 0 || box.n[d] != g.n[d]) return false;
+    const long long sbytes = (long long)box.n[0] * box.n[1] * box.p2 * 4;
+    return box.off2 == 0 && sbytes + 32LL * box.n[1] * box.p2 * 4 < (long long)bufio::kOOB;
+}
+
+int rho3d_grad(const Geo &g, const Box &box, const float *ps, const float *psm, const float *psp, float fdt2,
+               const float *b, const float *pa, float *gv, float *pv, float *gr, float *pr, hipStream_t stream) {
+    constexpr int NW = 8;
+    GradArgs a;
+    a.ps = ps;
+    a.psm = psm;
+    a.psp = psp;
+    a.b = b;
+    a.pa = pa;
+    a.gv = gv;
+    a.pv = pv;
+    a.gr = gr;
+    a.pr = pr;
+    a.bytes = g.total * 4;
+    a.sbytes = (long long)box.n[0] * box.n[1] * box.p2 * 4;
+    a.plane_bytes = (int)(g.s0 * 4);
+    a.splane_bytes = box.n[1] * box.p2 * 4;
+    a.srow = box.p2;
+    a.fdt2 = fdt2;
+    for (int d = 0; d < 3; ++d) {
+        a.lo[d] = g.nabs[d];
+        a.hi[d] = g.n[d] - g.nabs[d];
+    }
+    a.chunk = 40;
+    a.ntz = (a.hi[2] - a.lo[2] + kTZ - 1) / kTZ;
+    a.nty = (a.hi[1] - a.lo[1] + NW - 1) / NW;
+    a.ntiles = a.ntz * a.nty * ((a.hi[0] - a.lo[0] + a.chunk - 1) / a.chunk);
+    a.per_xcd = (a.ntiles + 7) / 8;
+    k_grad_density<NW><<<dim3(8 * a.per_xcd), dim3(NW * 64), 0, stream>>>(g, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -1326,6 +1326,14 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
         TRY(inject(s, s->rec, res + (size_t)t * s->rec.npoint, -1.f, o, nullptr, 0.f, s->box));
         if (gsrc) TRY(gather(s, s->src, u, -1.f, gsrc + (size_t)t * s->src.npoint));
         if (!guard) continue;
+        if (gr && s->saved_mode == 2 && s->b && use_rho3d(s) && rho3d_grad_supported(g, s->box)) {
+            // vp and rho gradients in one pass over the raw snapshots
+            const float *psm = t / f >= 1 ? s->saved + (size_t)(t / f - 1) * per : nullptr;
+            const float *psp = t / f + 1 < s->nsave ? s->saved + (size_t)(t / f + 1) * per : nullptr;
+            if (rho3d_grad(g, s->box, ps, psm, psp, fdt2, s->b, u, gv, pv, gr, pr, s->stream) != 0)
+                return fail(SA_E_HIP, "density gradient launch: %s", hipGetErrorString(hipGetLastError()));
+            continue;
+        }
         if (gv && !(fast && s->saved_mode == 1)) {
             const float *psm = nullptr, *psp = nullptr;
             if (s->saved_mode == 2) {
