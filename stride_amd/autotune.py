@@ -6,14 +6,18 @@ planes, 4 waves; 5/6: LDS plane ring filled by direct-to-LDS loads, 4/8 waves;
 7/8: as 5/6 with u read once, axis-0 terms pushed into partial sums),
 rows per lane (tile 64 x waves*rows cells) and march chunk lengths that split the
 slowest axis into 3..16 pieces — and returns the fastest. Results are cached per
-(shape, absorbing, boundary) for the life of the process.
+(shape, absorbing, boundary) for the life of the process. ``autotune_batch`` does the same for
+the two-shot update and ``autotune_density`` for the fused update of the density schemes
+(tile rows x waves and chunk length); ``tune_solver`` applies the tuned tile of whichever
+update a scheme runs.
 """
 
 import numpy as np
 
 from ._native import Solver
 
-__all__ = ['candidates', 'autotune', 'candidates_batch', 'autotune_batch']
+__all__ = ['candidates', 'autotune', 'candidates_batch', 'autotune_batch', 'candidates_density',
+           'autotune_density', 'uses_fast_update', 'uses_density_update', 'tune_solver']
 
 _CACHE = {}
 
@@ -100,3 +104,71 @@ def autotune_batch(plan, plan1, steps=24, device=0, verbose=False, cands=None):
     best = results[0][1:]
     _CACHE[key] = (best, results)
     return best, results
+
+
+def candidates_density(shape):
+    nx = shape[0]
+    chunks = sorted({max(1, -(-nx // k)) for k in (3, 4, 5, 6, 8)})
+    return [(rows, chunk, waves) for (rows, waves) in ((1, 8), (1, 4), (2, 4)) for chunk in chunks]
+
+
+def autotune_density(plan, steps=24, device=0, verbose=False, cands=None):
+    """As ``autotune`` for the fused density update (``plan`` with rho): (best (rows, chunk, waves), results)."""
+    key = ('rho', plan.shape, plan.absorbing, plan.boundary, plan.rho_mode, plan.att_power)
+    if key in _CACHE:
+        return _CACHE[key]
+    nt = steps + 2
+    s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, nt, device=device)
+    s.set_scheme(2, plan.rho_mode, plan.att_power, plan.boundary, 0, 4, 0, 0, nt, 0, 0, 0)
+    s.set_medium(plan.vp, plan.rho, plan.alpha)
+    if plan.sponge_tables is not None:
+        s.set_sponge(plan.sponge_tables)
+    s.set_sources(plan.src_idx, plan.src_w)
+    s.set_receivers(plan.rec_idx[:1], plan.rec_w[:1])
+    src = np.zeros((nt, plan.nsrc), np.float32)
+    src[1:, :] = 1.0
+    results = []
+    for rows, chunk, waves in (cands or candidates_density(plan.shape)):
+        s.set_tile_density(rows, chunk, waves)
+        s.forward(src)  # warm
+        s.timing(True)
+        s.forward(src)
+        ms, n = s.timing()
+        s.timing(False)
+        results.append((ms / max(n, 1), rows, chunk, waves))
+        if verbose:
+            print('density tile rows=%d chunk=%d waves=%d: %.1f us/launch' % (rows, chunk, waves,
+                                                                              1e3 * ms / max(n, 1)), flush=True)
+    s.close()
+    results.sort()
+    best = results[0][1:]
+    _CACHE[key] = (best, results)
+    return best, results
+
+
+def uses_fast_update(plan, save_mode):
+    """The scheme runs on the specialised 3D update (3D, OT2, no density, no attenuation, no PML)."""
+    return (plan.dim == 3 and plan.kernel_order == 2 and plan.rho_mode == 0 and plan.att_power == -2
+            and plan.boundary != 1 and save_mode in (0, 1))
+
+
+def uses_density_update(plan):
+    """The scheme runs on the fused 3D density update (3D, OT2, constant or variable density, no
+    PML, no power-2 attenuation)."""
+    return (plan.dim == 3 and plan.kernel_order == 2 and plan.rho_mode != 0 and plan.att_power != 2
+            and plan.boundary != 1)
+
+
+def tune_solver(solver, plan, save_mode, device=0):
+    """Time the candidate tiles of the update ``plan``'s scheme runs on (cached per grid and
+    scheme) and set the fastest on ``solver``. Returns the tile, or None for schemes on the
+    general kernels."""
+    if uses_fast_update(plan, save_mode):
+        tile, _ = autotune(plan, device=device)
+        solver.set_tile(*tile)
+        return tile
+    if uses_density_update(plan):
+        tile, _ = autotune_density(plan, device=device)
+        solver.set_tile_density(*tile)
+        return tile
+    return None

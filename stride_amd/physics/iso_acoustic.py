@@ -17,7 +17,10 @@ attenuation_power, save_wavefield, save_undersampling, time_bounds,
 diff_source, platform, deviceid, deallocate). ``adjoint_mode='exact'`` selects
 the exact discrete adjoint (every forward level stored) instead of the
 reference's as-written adjoint; the default, ``'compat'``, reproduces the
-reference.
+reference. ``autotune`` (True / False; default: on for 3D grids of at least 2^24
+points) times the candidate tiles of the 3D update the scheme runs on, once per grid
+and scheme, and uses the fastest, as Devito's runtime autotuning picks block sizes
+(stride/physics/common/devito.py:1061-1078).
 
 The forward state (solver, saved wavefield, plan) is kept on the instance
 between ``forward`` and ``adjoint`` of a shot, as the reference keeps
@@ -104,6 +107,10 @@ class ProblemTypeBase:
                 return None
             grads.append(grad)
         return tuple(grads)
+
+
+# grids from this size up are autotuned by default (the tuning runs take about a second)
+_AUTOTUNE_MIN_POINTS = 1 << 24
 
 
 def _needs_grad(*variables):
@@ -217,6 +224,14 @@ class IsoAcousticHIP(ProblemTypeBase):
             solver.set_pml(plan.pml)
         solver.set_sources(plan.src_idx, plan.src_w)
         solver.set_receivers(plan.rec_idx, plan.rec_w)
+        tune = kwargs.get('autotune')
+        if tune is None:
+            tune = plan.dim == 3 and int(np.prod(plan.shape)) >= _AUTOTUNE_MIN_POINTS
+        if tune:
+            from ..autotune import tune_solver
+            tile = tune_solver(solver, plan, save_mode, device=device)
+            if tile is not None:
+                logger.debug('(ShotID %d) update tile %s', shot.id, tile)
         self._plan = plan
 
     def run_forward(self, wavelets, vp, rho=None, alpha=None, **kwargs):
