@@ -5,7 +5,9 @@
 - 2D gradient (572^2 extended, 2500 steps, CFS-PML 30, variable density, power-2 attenuation)
   traces and vp / rho gradients against the oracle;
 - the 3D grid (360^3 extended, the hemisphere medium) on a short run: the fast 3D update's
-  traces and vp gradient against the oracle, and a two-shot pass against two one-shot runs.
+  traces and vp gradient against the oracle, and a two-shot pass against two one-shot runs;
+- the 3D grid with variable vp / rho / alpha on a short run: the fused density update's traces
+  and the vp / rho gradients against the oracle.
 
 Tolerances: traces 1e-5, gradients 1e-4 relative L2 (float32 kernels against the float32
 oracle)."""
@@ -129,3 +131,35 @@ def test_config3_grid_two_shots(gpu):
     for key in ('grad_vp', 'prec_vp'):
         want = single[0][1][key].astype(np.float64) + single[1][1][key]
         assert rel_l2(g[key], want) < 1e-5, key
+
+
+def test_config4_grid_density_update(gpu):
+    """The 3D grid with config 4's variable vp / rho / alpha (power 0) medium on a short run:
+    the fused density update's traces and the one-pass vp / rho gradients against the oracle."""
+    from stride_amd.configs import config4
+    from stride_amd.problem import Time
+    from stride_amd.utils import ellipsoidal, tone_burst
+    nt = 48
+    cfg = config4(scale=1.0, nt=200)
+    space = cfg['space']
+    dt = cfg['time'].step
+    cen = np.array([l / 2 for l in space.limit])
+    src = cen + 0.8 * (cfg['sources'][0] - cen)
+    rec = ellipsoidal(64, (3e-3,) * 3, tuple(src))
+    wav = tone_burst(400e3, 3, 200, dt)[:nt]
+    plan = build_plan(space, Time(start=0., step=dt, num=nt), wav[None], cfg['vp'], rho=cfg['rho'],
+                      alpha=cfg['alpha'], source_coordinates=src[None], receiver_coordinates=rec, kernel='OT2',
+                      save_undersampling=4, grad_vp=True, grad_rho=True)
+    print('3D full grid density %s nt %d' % (plan.shape, plan.nt), flush=True)
+    s = solver_for(plan, save_mode=2)
+    got = s.forward(plan.src_data)
+    ref, saved, _ = oracle.forward(plan, save=True)
+    assert np.linalg.norm(ref) > 0
+    assert rel_l2(got.T, ref) < 1e-5
+    res = adjoint_source_samples(plan, 0.1 * ref)
+    g = s.adjoint(res, want_vp=True, want_rho=True)
+    s.close()
+    acc = oracle.adjoint_compat(plan, res, saved)
+    for key in ('grad_vp', 'prec_vp', 'grad_rho', 'prec_rho'):
+        assert np.linalg.norm(acc[key]) > 0
+        assert rel_l2(g[key], acc[key]) < 1e-4, key
