@@ -1,0 +1,28 @@
+// Fused 2D wavefield update: OT2, any density form, sponge or CFS-PML, attenuation off / power
+// 0 / power None / power 2, in one launch per step (plus the Laplacian of the replaced level for
+// power 2) instead of the general kernels' separate passes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+struct Flat2DStep {
+    const float *u, *other;  // current level and the level it replaces
+    float *out;              // new level (may alias other)
+    const float *b;          // buoyancy (variable density) or null
+    const float *A, *B, *G;
+    const float *Cbeta, *lapother;  // power-2 attenuation (null: off); lapother = lap(other)
+    const float *Cpml;              // CFS-PML (null: off)
+    float *u1[3], *u2[3], *u3[3];   // PML auxiliary fields per axis
+    const float *pml[3][4];         // PML profiles per axis: sigma, alpha, dsigma, dalpha
+    float dt;
+    int backward;
+    float *save;  // optional fused save into `box` (save_mode 1: p.dt2, 2: raw u)
+    int save_mode;
+    Box box;
+    float idt2;
+};
+
+// true for the 2D padded layout within the kernel's 32-bit buffer offsets
+bool flat2d_supported(const Geo &g);
+int flat2d_step(const Geo &g, int rho_mode, const Flat2DStep &x, hipStream_t stream);

@@ -36,6 +36,7 @@
 #include "kernels.h"
 #include "fast3d.h"
 #include "rho3d.h"
+#include "flat2d.h"
 
 static thread_local std::string g_err;
 
@@ -851,6 +852,10 @@ static bool use_rho3d(const sa_solver *s) {
            s->rho3d.enabled && rho3d_supported(s->geo);
 }
 
+static bool use_flat2d(const sa_solver *s) {
+    return s->grid.dim == 2 && s->sch.kernel == 2 && s->rho3d.enabled && flat2d_supported(s->geo);
+}
+
 static bool use_fast(const sa_solver *s) {
     const sa_scheme &c = s->sch;
     return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 && c.att_power == -2 && c.boundary != 1 &&
@@ -991,6 +996,43 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
         st.idt2 = 1.f / (s->dt * s->dt);
         if (rho3d_step(s->rho3d, g, st, s->stream) != 0)
             return fail(SA_E_HIP, "fused density update launch: %s", hipGetErrorString(hipGetLastError()));
+        TRY(timed_end(s));
+        return SA_OK;
+    }
+    if (use_flat2d(s)) {
+        Flat2DStep x;
+        memset(&x, 0, sizeof(x));
+        x.u = u;
+        x.other = other;
+        x.out = out;
+        x.b = c.rho_mode == 2 ? s->b : nullptr;
+        x.A = s->A;
+        x.B = s->B;
+        x.G = s->G;
+        if (c.att_power == 2) {
+            // `other` is overwritten in place: its Laplacian is taken by a launch of its own first
+            k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, other, nullptr, s->lapother);
+            CHECK_LAUNCH();
+            x.Cbeta = s->Cbeta;
+            x.lapother = s->lapother;
+        }
+        if (c.boundary == 1) {
+            x.Cpml = s->Cpml;
+            for (int a = 0; a < 3; ++a) {
+                x.u1[a] = s->u1[a];
+                x.u2[a] = s->u2[a];
+                x.u3[a] = s->u3[a];
+                for (int r = 0; r < 4; ++r) x.pml[a][r] = s->pml[a][r];
+            }
+        }
+        x.dt = s->dt;
+        x.backward = backward;
+        x.save = save;
+        x.save_mode = save_mode;
+        x.box = s->box;
+        x.idt2 = 1.f / (s->dt * s->dt);
+        if (flat2d_step(g, c.rho_mode, x, s->stream) != 0)
+            return fail(SA_E_HIP, "fused 2D update launch: %s", hipGetErrorString(hipGetLastError()));
         TRY(timed_end(s));
         return SA_OK;
     }
