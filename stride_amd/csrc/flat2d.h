@@ -1,6 +1,6 @@
 // Fused 2D wavefield update: OT2, any density form, sponge or CFS-PML, attenuation off / power
-// 0 / power None / power 2, in one launch per step (plus the Laplacian of the replaced level for
-// power 2) instead of the general kernels' separate passes.
+// 0 / power None / power 2, in one launch per step instead of the general kernels' separate
+// passes.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -12,6 +12,7 @@ struct Flat2DStep {
     const float *b;          // buoyancy (variable density) or null
     const float *A, *B, *G;
     const float *Cbeta, *lapother;  // power-2 attenuation (null: off); lapother = lap(other)
+    float *lapout;                  // optional: receives lap(u) (the next step's lapother)
     const float *Cpml;              // CFS-PML (null: off)
     float *u1[3], *u2[3], *u3[3];   // PML auxiliary fields per axis
     const float *pml[3][4];         // PML profiles per axis: sigma, alpha, dsigma, dalpha

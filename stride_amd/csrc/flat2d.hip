@@ -8,8 +8,9 @@
 // derivatives of u (stride/physics/boundaries/devito.py:133-171). The general path runs this as
 // up to seven launches (two staggered-derivative passes, a divergence pass, two PML passes,
 // the update); on the 2D grids (240^2 - 572^2) each of those is dominated by its fixed cost.
-// The Laplacian of `other` (power-2 attenuation) stays a launch of its own: the update
-// overwrites `other` in place, so its neighbours must be read before.
+// The Laplacian of `other` (power-2 attenuation) must be taken before the update overwrites
+// `other` in place: each launch also writes the Laplacian of its u, which is the next step's
+// other level, so a separate Laplacian launch runs only on the first step of a run.
 //
 // Work decomposition: a workgroup of 4 waves owns 64 (axis 2) x 4 (axis 1) cells, one per
 // thread. u is staged in LDS with a 9-cell halo (rows y0-9 .. y0+12, columns z0-12 .. z0+75),
@@ -123,6 +124,7 @@ __global__ __launch_bounds__(NT) void k_flat2d(Geo g, Flat2DStep x) {
             lz += g.w2[2][r] * (c0[r] + c0[-r]);
         }
         acc += x.Cbeta[p] * ((ly + lz) - x.lapother[p]);
+        if (x.lapout) x.lapout[p] = ly + lz;  // the next step's Laplacian of its other level
     }
     if (PML) {
         float s1 = 0.f;

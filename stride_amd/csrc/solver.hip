@@ -470,6 +470,11 @@ struct sa_solver {
     float *buf[2] = {};
     float *Ku = nullptr, *L2 = nullptr, *KmKu = nullptr, *lapo = nullptr;
     float *lapother = nullptr;  // Laplacian of the other level (power-2 attenuation)
+    // fused 2D update: it also writes the Laplacian of u into lapnext, which is the Laplacian of
+    // the next step's other level; lap_of is the level lapother holds the Laplacian of (null:
+    // none, e.g. at the start of a run)
+    float *lapnext = nullptr;
+    const float *lap_of = nullptr;
     float *st[3] = {};
     float *st2[3] = {};
     float *work0 = nullptr, *work1 = nullptr;
@@ -665,7 +670,7 @@ extern "C" int sa_destroy(sa_solver *s) {
     (void)hipSetDevice(s->device);
     (void)hipDeviceSynchronize();
     float *fs[] = {s->vp, s->rho, s->b, s->alpha, s->A, s->B, s->G, s->M, s->Cbeta, s->Cpml, s->tables,
-                   s->buf[0], s->buf[1], s->Ku, s->L2, s->KmKu, s->lapo, s->lapother, s->work0, s->work1, s->saved,
+                   s->buf[0], s->buf[1], s->Ku, s->L2, s->KmKu, s->lapo, s->lapother, s->lapnext, s->work0, s->work1, s->saved,
                    s->d_series, s->d_out};
     for (float *p : fs) dfree(p);
     for (float *p : s->gacc) dfree(p);
@@ -881,6 +886,7 @@ static int prepare(sa_solver *s) {
     TRY(dalloc(&s->M, s->total));
     if (c.att_power == 2) TRY(dalloc(&s->Cbeta, s->total));
     if (c.att_power == 2) TRY(dalloc(&s->lapother, s->total));
+    if (c.att_power == 2 && use_flat2d(s)) TRY(dalloc(&s->lapnext, s->total));
     if (c.boundary == 1) TRY(dalloc(&s->Cpml, s->total));
     const Geo &g = s->geo;
     k_coefficients<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(
@@ -1010,11 +1016,16 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
         x.B = s->B;
         x.G = s->G;
         if (c.att_power == 2) {
-            // `other` is overwritten in place: its Laplacian is taken by a launch of its own first
-            k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, other, nullptr, s->lapother);
-            CHECK_LAUNCH();
+            // `other` is overwritten in place, so its Laplacian must come from before: the
+            // previous step's launch wrote it (as the Laplacian of that step's u), unless this is
+            // the first step of a run
+            if (s->lap_of != other || !s->lapnext) {
+                k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, other, nullptr, s->lapother);
+                CHECK_LAUNCH();
+            }
             x.Cbeta = s->Cbeta;
             x.lapother = s->lapother;
+            x.lapout = s->lapnext;
         }
         if (c.boundary == 1) {
             x.Cpml = s->Cpml;
@@ -1033,6 +1044,12 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
         x.idt2 = 1.f / (s->dt * s->dt);
         if (flat2d_step(g, c.rho_mode, x, s->stream) != 0)
             return fail(SA_E_HIP, "fused 2D update launch: %s", hipGetErrorString(hipGetLastError()));
+        if (x.lapout) {
+            std::swap(s->lapother, s->lapnext);
+            s->lap_of = u;
+        } else {
+            s->lap_of = nullptr;
+        }
         TRY(timed_end(s));
         return SA_OK;
     }
@@ -1226,6 +1243,7 @@ static size_t box_points(const Box &b) { return (size_t)b.n[0] * b.n[1] * b.p2; 
 extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
     if (!s) return fail(SA_E_ARG, "null solver");
     TRY(prepare(s));
+    s->lap_of = nullptr;
     const sa_scheme &c = s->sch;
     TRY(coef_sparse(s, s->src, 0));
     TRY(zero_field(s, s->buf[0]));
@@ -1284,6 +1302,7 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
 
 // Recompute forward levels u_{jK} .. u_{min((j+1)K, nt-1)} into the segment buffer from checkpoint j.
 static int recompute_segment(sa_solver *s, int j) {
+    s->lap_of = nullptr;
     const int K = s->saved_f;
     const int nt = s->grid.nt;
     const size_t per = box_points(s->box);
@@ -1322,6 +1341,7 @@ extern "C" int sa_forward(sa_solver *s, const float *src, float *rec) {
 }
 
 static int zero_aux(sa_solver *s) {
+    s->lap_of = nullptr;
     TRY(zero_field(s, s->buf[0]));
     TRY(zero_field(s, s->buf[1]));
     if (s->sch.boundary == 1)
