@@ -354,12 +354,18 @@ def bench_gradient_2d(shots=8):
 
     shot(0)
     s.synchronize()
-    s.timing(True)
     t0 = time.perf_counter()
     for k in range(shots):
         shot(k)
     s.synchronize()
     wall = time.perf_counter() - t0
+    # kernel times from a separate pass: events around every launch slow these short steps
+    s.timing(True)
+    t1 = time.perf_counter()
+    for k in range(shots):
+        shot(k)
+    s.synchronize()
+    wall_timed = time.perf_counter() - t1
     ms, launches = s.timing()
     s.timing(False)
     n = int(np.prod(p.shape))
@@ -369,7 +375,8 @@ def bench_gradient_2d(shots=8):
     return dict(metric='Gpts/s fwd+adjoint 2D 512^2 SO10, CFS-PML, variable vp/rho, power-2 attenuation, '
                        'vp+rho gradients', value=round(launches * n / wall / 1e9, 3), unit='Gpts/s',
                 seconds_per_shot=round(wall / shots, 4), shots=shots, update_launches=int(launches),
-                update_kernel_us=round(ms * 1e3 / launches, 2), update_kernel_share=round(ms * 1e-3 / wall, 3),
+                update_kernel_us=round(ms * 1e3 / launches, 2),
+                update_kernel_share=round(ms * 1e-3 / wall_timed, 3),
                 dtype='f32', data='synthetic',
                 config={'workload': '2D 512x512 inner (572x572 extended), dx 0.2 mm, SO10 OT2, CFS-PML 30, '
                                     'vp 1500 + 150 N1 in a 45 mm disc, rho 1000 + 50 N2, alpha 0.5 (power 2), '

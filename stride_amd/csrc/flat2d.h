@@ -18,6 +18,12 @@ struct Flat2DStep {
     const float *pml[3][4];         // PML profiles per axis: sigma, alpha, dsigma, dalpha
     float dt;
     int backward;
+    // optional injection into the new level after the update (the deterministic scatter's sum,
+    // in its order): imap[p] = destination index or -1; contributions ioff[d] .. ioff[d + 1]
+    const int *imap, *ioff, *ipoint;
+    const float *icoef, *ivalues;
+    float isign;
+    bool isave;   // also add it, times idt2, to the p.dt2 snapshot
     float *save;  // optional fused save into `box` (save_mode 1: p.dt2, 2: raw u)
     int save_mode;
     Box box;

@@ -8,7 +8,9 @@
 // derivatives of u (stride/physics/boundaries/devito.py:133-171). The general path runs this as
 // up to seven launches (two staggered-derivative passes, a divergence pass, two PML passes,
 // the update); on the 2D grids (240^2 - 572^2) each of those is dominated by its fixed cost.
-// The Laplacian of `other` (power-2 attenuation) must be taken before the update overwrites
+// The source (forward) or residual (adjoint) injection into the new level is applied here as
+// well, by the thread that writes the destination cell. The Laplacian of `other` (power-2
+// attenuation) must be taken before the update overwrites
 // `other` in place: each launch also writes the Laplacian of its u, which is the next step's
 // other level, so a separate Laplacian launch runs only on the first step of a run.
 //
@@ -161,9 +163,18 @@ __global__ __launch_bounds__(NT) void k_flat2d(Geo g, Flat2DStep x) {
         acc -= x.Cpml[p] * s1;
     }
     const float out = x.A[p] * u + x.B[p] * o + acc;
-    x.out[p] = out;
-    if (SAVE && sa_inbox(x.box, 0, y, z))
-        x.save[sa_boxidx(x.box, 0, y, z)] = x.save_mode == 1 ? (out - 2.f * u + o) * x.idt2 : u;
+    float inj = 0.f;
+    const int d = x.imap ? x.imap[p] : -1;
+    if (d >= 0) {
+        for (int c = x.ioff[d]; c < x.ioff[d + 1]; ++c) inj += x.icoef[c] * x.ivalues[x.ipoint[c]];
+        inj *= x.isign;
+    }
+    x.out[p] = d >= 0 ? out + inj : out;
+    if (SAVE && sa_inbox(x.box, 0, y, z)) {
+        float sv = x.save_mode == 1 ? (out - 2.f * u + o) * x.idt2 : u;
+        if (d >= 0 && x.isave) sv += inj * x.idt2;
+        x.save[sa_boxidx(x.box, 0, y, z)] = sv;
+    }
 }
 
 template <int RHO, bool PML, bool BETA>

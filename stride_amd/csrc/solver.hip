@@ -442,6 +442,15 @@ struct Sparse {
     int *d_off = nullptr, *d_point = nullptr;
     float *d_cw = nullptr;    // raw weights per contribution
     float *d_coef = nullptr;  // scaled coefficients per contribution
+    int *d_map = nullptr;     // per padded cell: its destination index, -1 for none (fused 2D update)
+};
+
+// an injection the fused 2D update applies to the cells it writes (k_inject's sum, same order)
+struct InjectSpec {
+    Sparse *sp;
+    const float *values;
+    float sign;
+    bool into_save;  // also add (scaled by idt2) into the p.dt2 snapshot
 };
 
 struct sa_solver {
@@ -659,6 +668,7 @@ extern "C" int sa_create(const sa_grid *grid, sa_solver **out) {
 }
 
 static void free_sparse(Sparse &sp) {
+    dfree(sp.d_map);
     dfree(sp.d_idx); dfree(sp.d_w); dfree(sp.d_dest); dfree(sp.d_lin); dfree(sp.d_off); dfree(sp.d_point);
     dfree(sp.d_ijk);
     dfree(sp.d_cw); dfree(sp.d_coef);
@@ -981,8 +991,28 @@ static int apply_K(sa_solver *s, const float *src, const float *scale, float *ds
 }
 
 // one time step of (1): out = step(u, other); optional fused save
+__global__ void k_dest_map(int ndest, const long long *__restrict__ dest, int *map) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < ndest) map[dest[d]] = d;
+}
+
+static int dest_map(sa_solver *s, Sparse &sp) {
+    if (sp.d_map || sp.ndest == 0) return SA_OK;
+    if (hipMalloc((void **)&sp.d_map, s->total * sizeof(int)) != hipSuccess) {
+        sp.d_map = nullptr;
+        return fail(SA_E_NOMEM, "hipMalloc of the destination map failed");
+    }
+    HIP_TRY(hipMemsetAsync(sp.d_map, 0xff, s->total * sizeof(int), s->stream));
+    k_dest_map<<<(sp.ndest + 255) / 256, 256, 0, s->stream>>>(sp.ndest, sp.d_dest, sp.d_map);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
+// One time step. With `inj` given and the fused 2D update in use, the injection is applied by
+// the update itself and *injected is set; otherwise the caller injects.
 static int step_general(sa_solver *s, const float *u, const float *other, float *out, int backward, float *save,
-                        int save_mode) {
+                        int save_mode, const InjectSpec *inj = nullptr, bool *injected = nullptr) {
+    if (injected) *injected = false;
     const Geo &g = s->geo;
     const sa_scheme &c = s->sch;
     hipEvent_t e0;
@@ -1042,6 +1072,16 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
         x.save_mode = save_mode;
         x.box = s->box;
         x.idt2 = 1.f / (s->dt * s->dt);
+        if (inj && inj->sp->ndest > 0) {
+            TRY(dest_map(s, *inj->sp));
+            x.imap = inj->sp->d_map;
+            x.ioff = inj->sp->d_off;
+            x.ipoint = inj->sp->d_point;
+            x.icoef = inj->sp->d_coef;
+            x.ivalues = inj->values;
+            x.isign = inj->sign;
+            x.isave = inj->into_save;
+        }
         if (flat2d_step(g, c.rho_mode, x, s->stream) != 0)
             return fail(SA_E_HIP, "fused 2D update launch: %s", hipGetErrorString(hipGetLastError()));
         if (x.lapout) {
@@ -1050,6 +1090,7 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
         } else {
             s->lap_of = nullptr;
         }
+        if (injected && inj) *injected = true;
         TRY(timed_end(s));
         return SA_OK;
     }
@@ -1274,6 +1315,7 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
         float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
         float *save = nullptr;
         int mode = 0;
+        bool injected = false;
         if (c.save_mode != 0 && !ckpt && t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave) {
             save = s->saved + (size_t)(t / f) * per;
             mode = c.save_mode == 1 ? 1 : 2;
@@ -1284,9 +1326,11 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
             TRY(fast3d_step(s->fast, s->geo, u, o, o, save, s->box, idt2, s->stream));
             TRY(timed_end(s));
         } else {
-            TRY(step_general(s, u, o, o, 0, save, mode));
+            const InjectSpec spec{&s->src, src + (size_t)t * s->src.npoint, 1.f, mode == 1};
+            TRY(step_general(s, u, o, o, 0, save, mode, &spec, &injected));
         }
-        TRY(inject(s, s->src, src + (size_t)t * s->src.npoint, 1.f, o, mode == 1 ? save : nullptr, idt2, s->box));
+        if (!injected)
+            TRY(inject(s, s->src, src + (size_t)t * s->src.npoint, 1.f, o, mode == 1 ? save : nullptr, idt2, s->box));
         TRY(gather(s, s->rec, o, 1.f, rec + (size_t)t * s->rec.npoint));
         if (ckpt && t % f == 0) {
             // checkpoint j = t / K holds (p[jK + 1], p[jK])
@@ -1375,6 +1419,7 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
     // buf[t % 2] = p_a[t]; step t overwrites p_a[t+1] with p_a[t-1]
     for (int t = tstart; t >= tend; --t) {
         float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
+        bool injected = false;
         bool guard = t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave && (gv || gr);
         const float *ps = guard ? s->saved + (size_t)(t / f) * per : nullptr;
         if (fast) {
@@ -1383,9 +1428,10 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
             TRY(fast3d_step_adjoint(s->fast, g, u, o, o, s->saved_mode == 1 ? ps : nullptr, s->box, gv, pv, s->stream));
             TRY(timed_end(s));
         } else {
-            TRY(step_general(s, u, o, o, 1, nullptr, 0));
+            const InjectSpec spec{&s->rec, res + (size_t)t * s->rec.npoint, -1.f, false};
+            TRY(step_general(s, u, o, o, 1, nullptr, 0, &spec, &injected));
         }
-        TRY(inject(s, s->rec, res + (size_t)t * s->rec.npoint, -1.f, o, nullptr, 0.f, s->box));
+        if (!injected) TRY(inject(s, s->rec, res + (size_t)t * s->rec.npoint, -1.f, o, nullptr, 0.f, s->box));
         if (gsrc) TRY(gather(s, s->src, u, -1.f, gsrc + (size_t)t * s->src.npoint));
         if (!guard) continue;
         if (gr && s->saved_mode == 2 && s->b && use_rho3d(s) && rho3d_grad_supported(g, s->box)) {
