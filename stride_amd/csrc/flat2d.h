@@ -6,6 +6,9 @@
 
 #include "kernels.h"
 
+// tile of one workgroup: 64 cells along axis 2 x 4 along axis 1
+constexpr int kFlat2DTileZ = 64, kFlat2DTileY = 4;
+
 struct Flat2DStep {
     const float *u, *other;  // current level and the level it replaces
     float *out;              // new level (may alias other)
@@ -24,6 +27,15 @@ struct Flat2DStep {
     const float *icoef, *ivalues;
     float isign;
     bool isave;   // also add it, times idt2, to the p.dt2 snapshot
+    // optional interpolation of u at sparse points (the previous step's receivers: u is the
+    // level that step wrote): the workgroup of tile (ty, tz) computes the points listed in
+    // gpt[goff[tile] .. goff[tile + 1]), tile = ty * tiles along axis 2 + tz, as k_gather does
+    const int *goff, *gpt;
+    const long long *gidx;
+    const float *gw;
+    int gntap;
+    float gsign;
+    float *gout;
     float *save;  // optional fused save into `box` (save_mode 1: p.dt2, 2: raw u)
     int save_mode;
     Box box;

@@ -9,7 +9,9 @@
 // up to seven launches (two staggered-derivative passes, a divergence pass, two PML passes,
 // the update); on the 2D grids (240^2 - 572^2) each of those is dominated by its fixed cost.
 // The source (forward) or residual (adjoint) injection into the new level is applied here as
-// well, by the thread that writes the destination cell. The Laplacian of `other` (power-2
+// well, by the thread that writes the destination cell, and the previous step's receiver
+// interpolation (which reads this step's u) by the workgroup whose tile holds a receiver's first
+// tap. The Laplacian of `other` (power-2
 // attenuation) must be taken before the update overwrites
 // `other` in place: each launch also writes the Laplacian of its u, which is the next step's
 // other level, so a separate Laplacian launch runs only on the first step of a run.
@@ -26,7 +28,7 @@ namespace {
 
 using namespace bufio;
 
-constexpr int kTZ = 64, kNW = 4, kTY = kNW;
+constexpr int kTZ = kFlat2DTileZ, kNW = kFlat2DTileY, kTY = kNW;
 constexpr int UY = kTY + 18, UZ = 88, UC = UZ / 4;
 constexpr int BY = kTY + 10, BZ = 80, BC = BZ / 4;
 constexpr int FY = kTY + 9, FZN = 73, FZP = 76;
@@ -51,6 +53,20 @@ __global__ __launch_bounds__(NT) void k_flat2d(Geo g, Flat2DStep x) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const long long bytes = g.total * 4;
     const rsrc_t ru = make(x.u, bytes);
+    if (x.gout) {
+        // sparse points owned by this tile, read from u in memory (no launch writes u)
+        const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+        for (int e = x.goff[tile] + t; e < x.goff[tile + 1]; e += NT) {
+            const int r = x.gpt[e];
+            float acc = 0.f;
+            for (int q = 0; q < x.gntap; ++q) {
+                const long long pq = x.gidx[(long long)r * x.gntap + q];
+                if (pq < 0) continue;
+                acc += x.gw[(long long)r * x.gntap + q] * x.u[pq];
+            }
+            x.gout[r] = x.gsign * acc;
+        }
+    }
 
     // stage u (and b): rows past the padded plane read zero
     for (int e = t; e < UY * UC; e += NT) {

@@ -443,6 +443,14 @@ struct Sparse {
     float *d_cw = nullptr;    // raw weights per contribution
     float *d_coef = nullptr;  // scaled coefficients per contribution
     int *d_map = nullptr;     // per padded cell: its destination index, -1 for none (fused 2D update)
+    int *d_toff = nullptr, *d_tpt = nullptr;  // points per 2D update tile (CSR by first tap)
+};
+
+// an interpolation the fused 2D update performs from its u (k_gather's sum, same order)
+struct GatherSpec {
+    Sparse *sp;
+    float sign;
+    float *out;
 };
 
 // an injection the fused 2D update applies to the cells it writes (k_inject's sum, same order)
@@ -669,6 +677,8 @@ extern "C" int sa_create(const sa_grid *grid, sa_solver **out) {
 
 static void free_sparse(Sparse &sp) {
     dfree(sp.d_map);
+    dfree(sp.d_toff);
+    dfree(sp.d_tpt);
     dfree(sp.d_idx); dfree(sp.d_w); dfree(sp.d_dest); dfree(sp.d_lin); dfree(sp.d_off); dfree(sp.d_point);
     dfree(sp.d_ijk);
     dfree(sp.d_cw); dfree(sp.d_coef);
@@ -1008,10 +1018,39 @@ static int dest_map(sa_solver *s, Sparse &sp) {
     return SA_OK;
 }
 
+// points of `sp` listed per tile of the fused 2D update, by the tile of their first tap
+static int tile_points(sa_solver *s, Sparse &sp) {
+    if (sp.d_toff) return SA_OK;
+    const Geo &g = s->geo;
+    const int ntz = (g.n[2] + kFlat2DTileZ - 1) / kFlat2DTileZ, nty = (g.n[1] + kFlat2DTileY - 1) / kFlat2DTileY;
+    std::vector<int> tile(sp.npoint, -1), off(ntz * nty + 1, 0), pt;
+    for (int r = 0; r < sp.npoint; ++r)
+        for (int q = 0; q < sp.ntap; ++q) {
+            const long long lin = sp.idx[(size_t)r * sp.ntap + q];
+            if (lin < 0) continue;
+            const int j = (int)(lin / g.n[2]), k = (int)(lin % g.n[2]);
+            tile[r] = (j / kFlat2DTileY) * ntz + k / kFlat2DTileZ;
+            break;
+        }
+    for (int r = 0; r < sp.npoint; ++r)
+        if (tile[r] >= 0) ++off[tile[r] + 1];
+    for (size_t i = 1; i < off.size(); ++i) off[i] += off[i - 1];
+    pt.assign(std::max(off.back(), 1), 0);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int r = 0; r < sp.npoint; ++r)
+        if (tile[r] >= 0) pt[fill[tile[r]]++] = r;
+    HIP_TRY(hipMalloc((void **)&sp.d_toff, off.size() * sizeof(int)));
+    HIP_TRY(hipMalloc((void **)&sp.d_tpt, pt.size() * sizeof(int)));
+    HIP_TRY(hipMemcpy(sp.d_toff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sp.d_tpt, pt.data(), pt.size() * sizeof(int), hipMemcpyHostToDevice));
+    return SA_OK;
+}
+
 // One time step. With `inj` given and the fused 2D update in use, the injection is applied by
 // the update itself and *injected is set; otherwise the caller injects.
 static int step_general(sa_solver *s, const float *u, const float *other, float *out, int backward, float *save,
-                        int save_mode, const InjectSpec *inj = nullptr, bool *injected = nullptr) {
+                        int save_mode, const InjectSpec *inj = nullptr, bool *injected = nullptr,
+                        const GatherSpec *gat = nullptr) {
     if (injected) *injected = false;
     const Geo &g = s->geo;
     const sa_scheme &c = s->sch;
@@ -1081,6 +1120,16 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
             x.ivalues = inj->values;
             x.isign = inj->sign;
             x.isave = inj->into_save;
+        }
+        if (gat && gat->sp->npoint > 0) {
+            TRY(tile_points(s, *gat->sp));
+            x.goff = gat->sp->d_toff;
+            x.gpt = gat->sp->d_tpt;
+            x.gidx = gat->sp->d_idx;
+            x.gw = gat->sp->d_w;
+            x.gntap = gat->sp->ntap;
+            x.gsign = gat->sign;
+            x.gout = gat->out;
         }
         if (flat2d_step(g, c.rho_mode, x, s->stream) != 0)
             return fail(SA_E_HIP, "fused 2D update launch: %s", hipGetErrorString(hipGetLastError()));
@@ -1309,7 +1358,10 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
         TRY(ensure_staging(s, &s->src_keep, &s->src_keep_n, ns));
         if (ns) HIP_TRY(hipMemcpyAsync(s->src_keep, src, ns * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
     }
-    // levels: buf[t % 2] = p[t]; step t overwrites p[t-1] with p[t+1]
+    // levels: buf[t % 2] = p[t]; step t overwrites p[t-1] with p[t+1]. With the fused 2D update,
+    // the receivers of step t are interpolated by step t + 1's launch (its u is that level).
+    const bool defer = !fast && use_flat2d(s) && s->rec.npoint > 0;
+    float *pending = nullptr;  // receiver row of the previous step, not yet interpolated
     int tlast = c.tb1 - 2;
     for (int t = 1; t <= tlast; ++t) {
         float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
@@ -1327,11 +1379,15 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
             TRY(timed_end(s));
         } else {
             const InjectSpec spec{&s->src, src + (size_t)t * s->src.npoint, 1.f, mode == 1};
-            TRY(step_general(s, u, o, o, 0, save, mode, &spec, &injected));
+            const GatherSpec gspec{&s->rec, 1.f, pending};
+            TRY(step_general(s, u, o, o, 0, save, mode, &spec, &injected, pending ? &gspec : nullptr));
         }
         if (!injected)
             TRY(inject(s, s->src, src + (size_t)t * s->src.npoint, 1.f, o, mode == 1 ? save : nullptr, idt2, s->box));
-        TRY(gather(s, s->rec, o, 1.f, rec + (size_t)t * s->rec.npoint));
+        if (defer)
+            pending = rec + (size_t)t * s->rec.npoint;
+        else
+            TRY(gather(s, s->rec, o, 1.f, rec + (size_t)t * s->rec.npoint));
         if (ckpt && t % f == 0) {
             // checkpoint j = t / K holds (p[jK + 1], p[jK])
             float *slot = s->saved + (size_t)(2 * (t / f)) * per;
@@ -1340,6 +1396,7 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
             CHECK_LAUNCH();
         }
     }
+    if (pending) TRY(gather(s, s->rec, s->buf[(tlast + 1) % 2], 1.f, pending));
     s->cur = (tlast + 1) % 2;
     return SA_OK;
 }
