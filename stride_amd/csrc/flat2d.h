@@ -36,6 +36,14 @@ struct Flat2DStep {
     int gntap;
     float gsign;
     float *gout;
+    // optional gradient accumulation of the as-written adjoint on the interior (u is the
+    // adjoint level): grad_vp += pdt u, prec_vp += pdt^2 with pdt the saved p.dt2 (graw 0) or the
+    // second difference of the raw snapshots (graw 1: gps -/+ one slot, null: zero); with grho,
+    // grad_rho += f u, prec_rho += f^2, f = -sum_d D-_d(b) D-_d(p) - b lap(p) of the raw snapshot
+    const float *gps, *gpsm, *gpsp;
+    float gfdt2;
+    int graw;
+    float *ggv, *gpv, *ggr, *gpr;
     float *save;  // optional fused save into `box` (save_mode 1: p.dt2, 2: raw u)
     int save_mode;
     Box box;

@@ -186,6 +186,45 @@ __global__ __launch_bounds__(NT) void k_flat2d(Geo g, Flat2DStep x) {
         inj *= x.isign;
     }
     x.out[p] = d >= 0 ? out + inj : out;
+    if (x.gps && y >= g.nabs[1] && y < ny - g.nabs[1] && z >= g.nabs[2] && z < nz - g.nabs[2]) {
+        // gradients on the interior, from the snapshot (box layout) and the adjoint level u
+        const long long bi = sa_boxidx(x.box, 0, y, z);
+        const float *ps = x.gps;
+        if (x.ggv) {
+            float pdt;
+            if (x.graw) {
+                const float pp = x.gpsp ? x.gpsp[bi] : 0.f, pm = x.gpsm ? x.gpsm[bi] : 0.f;
+                pdt = (pp - 2.f * ps[bi] + pm) * x.gfdt2;
+            } else {
+                pdt = ps[bi];
+            }
+            x.ggv[p] += pdt * u;
+            x.gpv[p] += pdt * pdt;
+        }
+        if (RHO == 2 && x.ggr) {
+            const long long sy = x.box.p2;  // snapshot row pitch
+            const float *bc = sB + (w + 5) * BZ + lane + 8;  // own cell in the b tile
+            float acc2 = 0.f, lap = 0.f;
+#pragma unroll
+            for (int a = 1; a <= 2; ++a) {
+                const long long st = a == 1 ? sy : 1;
+                const int bs = a == 1 ? BZ : 1;
+                float db = 0.f, dp = 0.f, l = g.w2[a][0] * ps[bi];
+#pragma unroll
+                for (int r = 0; r < 5; ++r) {
+                    db += g.ws[a][r] * (bc[r * bs] - bc[-(r + 1) * bs]);
+                    dp += g.ws[a][r] * (ps[bi + r * st] - ps[bi - (r + 1) * st]);
+                }
+#pragma unroll
+                for (int r = 1; r <= 5; ++r) l += g.w2[a][r] * (ps[bi + r * st] + ps[bi - r * st]);
+                acc2 += db * dp;
+                lap += l;
+            }
+            const float f = -acc2 - bc[0] * lap;
+            x.ggr[p] += f * u;
+            x.gpr[p] += f * f;
+        }
+    }
     if (SAVE && sa_inbox(x.box, 0, y, z)) {
         float sv = x.save_mode == 1 ? (out - 2.f * u + o) * x.idt2 : u;
         if (d >= 0 && x.isave) sv += inj * x.idt2;

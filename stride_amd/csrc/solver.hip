@@ -446,6 +446,14 @@ struct Sparse {
     int *d_toff = nullptr, *d_tpt = nullptr;  // points per 2D update tile (CSR by first tap)
 };
 
+// gradient accumulation the fused 2D update performs on a guarded adjoint step
+struct GradSpec {
+    const float *ps, *psm, *psp;  // snapshot of this step; raw mode: its neighbours (null: zero)
+    float fdt2;
+    int raw;
+    float *gv, *pv, *gr, *pr;
+};
+
 // an interpolation the fused 2D update performs from its u (k_gather's sum, same order)
 struct GatherSpec {
     Sparse *sp;
@@ -1050,7 +1058,7 @@ static int tile_points(sa_solver *s, Sparse &sp) {
 // the update itself and *injected is set; otherwise the caller injects.
 static int step_general(sa_solver *s, const float *u, const float *other, float *out, int backward, float *save,
                         int save_mode, const InjectSpec *inj = nullptr, bool *injected = nullptr,
-                        const GatherSpec *gat = nullptr) {
+                        const GatherSpec *gat = nullptr, const GradSpec *grd = nullptr) {
     if (injected) *injected = false;
     const Geo &g = s->geo;
     const sa_scheme &c = s->sch;
@@ -1130,6 +1138,17 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
             x.gntap = gat->sp->ntap;
             x.gsign = gat->sign;
             x.gout = gat->out;
+        }
+        if (grd) {
+            x.gps = grd->ps;
+            x.gpsm = grd->psm;
+            x.gpsp = grd->psp;
+            x.gfdt2 = grd->fdt2;
+            x.graw = grd->raw;
+            x.ggv = grd->gv;
+            x.gpv = grd->pv;
+            x.ggr = grd->gr;
+            x.gpr = grd->pr;
         }
         if (flat2d_step(g, c.rho_mode, x, s->stream) != 0)
             return fail(SA_E_HIP, "fused 2D update launch: %s", hipGetErrorString(hipGetLastError()));
@@ -1472,11 +1491,15 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
         TRY(dalloc(&s->work0, s->total));
     }
     const bool fast = use_fast(s);
+    // the fused 2D update takes the gradients itself: vp (any density form) and rho (variable
+    // density, interior at least 5 cells from the edges, so the snapshot stencils stay inside)
+    bool grad2d = !fast && use_flat2d(s) && (s->saved_mode == 1 || s->saved_mode == 2);
+    if (gr) grad2d = grad2d && c.rho_mode == 2 && s->saved_mode == 2 && g.nabs[1] >= 5 && g.nabs[2] >= 5;
     int tstart = c.tb1 - 1 - f, tend = c.tb0 + 1;
     // buf[t % 2] = p_a[t]; step t overwrites p_a[t+1] with p_a[t-1]
     for (int t = tstart; t >= tend; --t) {
         float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
-        bool injected = false;
+        bool injected = false, grads_done = false;
         bool guard = t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave && (gv || gr);
         const float *ps = guard ? s->saved + (size_t)(t / f) * per : nullptr;
         if (fast) {
@@ -1486,11 +1509,29 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
             TRY(timed_end(s));
         } else {
             const InjectSpec spec{&s->rec, res + (size_t)t * s->rec.npoint, -1.f, false};
-            TRY(step_general(s, u, o, o, 1, nullptr, 0, &spec, &injected));
+            // the fused 2D update also accumulates this step's gradients (it reads u and the
+            // snapshots only, as the separate passes after the injection do)
+            GradSpec gspec{};
+            if (guard && grad2d) {
+                gspec.ps = ps;
+                gspec.raw = s->saved_mode == 2;
+                if (gspec.raw) {
+                    gspec.psm = t / f >= 1 ? s->saved + (size_t)(t / f - 1) * per : nullptr;
+                    gspec.psp = t / f + 1 < s->nsave ? s->saved + (size_t)(t / f + 1) * per : nullptr;
+                }
+                gspec.fdt2 = fdt2;
+                gspec.gv = gv;
+                gspec.pv = pv;
+                gspec.gr = gr;
+                gspec.pr = pr;
+            }
+            TRY(step_general(s, u, o, o, 1, nullptr, 0, &spec, &injected, nullptr,
+                             guard && grad2d ? &gspec : nullptr));
+            grads_done = guard && grad2d;
         }
         if (!injected) TRY(inject(s, s->rec, res + (size_t)t * s->rec.npoint, -1.f, o, nullptr, 0.f, s->box));
         if (gsrc) TRY(gather(s, s->src, u, -1.f, gsrc + (size_t)t * s->src.npoint));
-        if (!guard) continue;
+        if (!guard || grads_done) continue;
         if (gr && s->saved_mode == 2 && s->b && use_rho3d(s) && rho3d_grad_supported(g, s->box)) {
             // vp and rho gradients in one pass over the raw snapshots
             const float *psm = t / f >= 1 ? s->saved + (size_t)(t / f - 1) * per : nullptr;
