@@ -146,8 +146,9 @@ int sa_set_tile(sa_solver *s, int rows, int chunk, int version);
  * derivative, divergence and update passes of the operator built by
  * stride/physics/iso_acoustic/devito.py:1466-1608, 1646-1666 for those schemes. The tile is
  * 64 (axis 2) x waves*rows (axis 1) cells marched over `chunk` planes of axis 0; rows 1 or 2
- * with 4 or 8 waves, rows 1 with 16 waves. rows = 0 switches the fused update off (the separate general passes run
- * instead). Default: rows 1, chunk 60, waves 8. */
+ * with 4 or 8 waves, rows 1 with 16 waves. rows = 0 switches the fused updates off, this one
+ * and the one-launch 2D update (the separate general passes run instead). Default: rows 1,
+ * chunk 60, waves 8. */
 int sa_set_tile_density(sa_solver *s, int rows, int chunk, int waves);
 
 /* Kernel timing: when enabled, every launch of the wavefield-update kernel is bracketed by

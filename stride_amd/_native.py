@@ -380,8 +380,9 @@ class Solver:
     def set_tile(self, rows, chunk, version=2):
         self._check(self._lib.sa_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk), ctypes.c_int(version)))
 
-    def set_tile_density(self, rows, chunk=40, waves=8):
-        """Tile of the fused 3D density update; rows=0 runs the separate general passes."""
+    def set_tile_density(self, rows, chunk=60, waves=8):
+        """Tile of the fused 3D density update; rows=0 switches the fused 3D density and 2D updates
+        off (the separate general passes run)."""
         self._check(self._lib.sa_set_tile_density(self._h, ctypes.c_int(rows), ctypes.c_int(chunk),
                                                   ctypes.c_int(waves)))
 
