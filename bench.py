@@ -320,14 +320,16 @@ def bench_density(nt=2500, tile=None):
                                     '(power 0), one shot, nt %d, raw saves every 4th step, 1024 receivers' % plan.nt})
 
 
-def bench_gradient_2d(shots=8):
+def bench_gradient_2d(shots=8, workers=2):
     """The 2D gradient workload: 512^2 inner (572^2 extended), dx 0.2 mm, CFS-PML 30, variable vp /
     rho, alpha 0.5 with power 2, 128 ring receivers, nt 2500. Each shot: forward with the raw
     wavefield saved every f-th step, then the as-written adjoint with the vp and rho gradients
     (sources at ring elements 0, 16, 32, ...). The update runs as the fused 2D kernel; the
     Laplacian of the replaced level (power 2), injection and interpolation are launches of their
-    own."""
+    own. The shots run on ``workers`` solvers, each on its own stream and host thread (as that
+    many Mosaic workers sharing the GPU would); the line also gives the rate of one worker."""
     from stride_amd.configs import config2
+    from stride_amd.workers import ShotWorkers
     from stride_amd.physics.plan import build_plan
     from stride_amd._native import Solver
     cfg = config2()
@@ -336,51 +338,79 @@ def bench_gradient_2d(shots=8):
                         receiver_coordinates=cfg['receivers'], boundary_type='complex_frequency_shift_PML_2',
                         attenuation_power=2, grad_vp=True, grad_rho=True) for k in range(shots)]
     p = plans[0]
-    s = Solver(p.dim, p.shape, p.spacing, p.absorbing, p.dt, p.nt)
-    s.set_scheme(p.kernel_order, p.rho_mode, p.att_power, p.boundary, p.diff_source, p.undersampling, 2,
-                 p.tb0, p.tb1, 1, 1, 0)
-    s.set_medium(p.vp, p.rho, p.alpha)
-    s.set_pml(p.pml)
-    s.set_receivers(p.rec_idx, p.rec_w)
-    rng = np.random.default_rng(0)
-    d_src = [s._to_device(q.src_data) for q in plans]
-    d_rec = s.device_alloc(p.nt * p.nrec * 4)
-    d_res = s._to_device(rng.standard_normal((p.nt, p.nrec)).astype(np.float32) * 1e-3)
 
-    def shot(k):
-        s.set_sources(plans[k].src_idx, plans[k].src_w)
-        s.forward_dev(d_src[k], d_rec)
-        s.adjoint_dev(d_res, 3)
+    def make_solver():
+        sv = Solver(p.dim, p.shape, p.spacing, p.absorbing, p.dt, p.nt)
+        sv.set_scheme(p.kernel_order, p.rho_mode, p.att_power, p.boundary, p.diff_source, p.undersampling, 2,
+                      p.tb0, p.tb1, 1, 1, 0)
+        sv.set_medium(p.vp, p.rho, p.alpha)
+        sv.set_pml(p.pml)
+        sv.set_receivers(p.rec_idx, p.rec_w)
+        return sv
 
-    shot(0)
+    res = np.random.default_rng(0).standard_normal((p.nt, p.nrec)).astype(np.float32) * 1e-3
+    buffers = {}
+
+    def stage(sv):
+        buffers[id(sv)] = (sv._to_device(res), sv.device_alloc(p.nt * p.nrec * 4),
+                           [sv._to_device(q.src_data) for q in plans])
+
+    def release(sv):
+        d_res, d_rec, d_src = buffers.pop(id(sv))
+        for d in d_src + [d_rec, d_res]:
+            sv.device_free(d)
+
+    def shot(sv, k):
+        d_res, d_rec, d_src = buffers[id(sv)]
+        sv.set_sources(plans[k].src_idx, plans[k].src_w)
+        sv.forward_dev(d_src[k], d_rec)
+        sv.adjoint_dev(d_res, 3)
+
+    # one worker on the null stream
+    s = make_solver()
+    stage(s)
+    shot(s, 0)
     s.synchronize()
     t0 = time.perf_counter()
     for k in range(shots):
-        shot(k)
+        shot(s, k)
     s.synchronize()
-    wall = time.perf_counter() - t0
+    wall1 = time.perf_counter() - t0
     # kernel times from a separate pass: events around every launch slow these short steps
     s.timing(True)
     t1 = time.perf_counter()
     for k in range(shots):
-        shot(k)
+        shot(s, k)
     s.synchronize()
     wall_timed = time.perf_counter() - t1
     ms, launches = s.timing()
     s.timing(False)
-    n = int(np.prod(p.shape))
-    for d in d_src + [d_rec, d_res]:
-        s.device_free(d)
+    release(s)
     s.close()
+    # the same shots on `workers` solvers with their own streams and host threads
+    pool = ShotWorkers(make_solver, n=workers)
+    for sv in pool.solvers:
+        stage(sv)
+    pool.map(shot, range(len(pool)))
+    t2 = time.perf_counter()
+    pool.map(shot, range(shots))
+    wall = time.perf_counter() - t2
+    for sv in pool.solvers:
+        release(sv)
+    pool.close()
+    n = int(np.prod(p.shape))
     return dict(metric='Gpts/s fwd+adjoint 2D 512^2 SO10, CFS-PML, variable vp/rho, power-2 attenuation, '
                        'vp+rho gradients', value=round(launches * n / wall / 1e9, 3), unit='Gpts/s',
-                seconds_per_shot=round(wall / shots, 4), shots=shots, update_launches=int(launches),
+                seconds_per_shot=round(wall / shots, 4), workers=workers,
+                value_one_worker=round(launches * n / wall1 / 1e9, 3),
+                seconds_per_shot_one_worker=round(wall1 / shots, 4), shots=shots, update_launches=int(launches),
                 update_kernel_us=round(ms * 1e3 / launches, 2),
-                update_kernel_share=round(ms * 1e-3 / wall_timed, 3),
+                update_kernel_share_one_worker=round(ms * 1e-3 / wall_timed, 3),
                 dtype='f32', data='synthetic',
                 config={'workload': '2D 512x512 inner (572x572 extended), dx 0.2 mm, SO10 OT2, CFS-PML 30, '
                                     'vp 1500 + 150 N1 in a 45 mm disc, rho 1000 + 50 N2, alpha 0.5 (power 2), '
-                                    '128-element 48 mm ring, nt %d, %d shots' % (p.nt, shots)})
+                                    '128-element 48 mm ring, nt %d, %d shots, %d workers (solvers with their own '
+                                    'streams and host threads)' % (p.nt, shots, workers)})
 
 
 def main():
@@ -408,6 +438,8 @@ def main():
     ap.add_argument('--gradient-2d', action='store_true',
                     help='only run shots of the 2D gradient workload (572^2 extended, CFS-PML, variable density, '
                          'power-2 attenuation; vp and rho gradients) and exit')
+    ap.add_argument('--workers', type=int, default=2,
+                    help='with --gradient-2d: solvers running shots concurrently, each on its own stream')
     ap.add_argument('--pipelines', action='store_true',
                     help='only time the device steps either side of the solver (trace processing, misfit, '
                          'gradient post-processing, model update) at the full workload and exit')
@@ -433,7 +465,7 @@ def main():
         return
 
     if args.gradient_2d:
-        print(json.dumps(bench_gradient_2d()))
+        print(json.dumps(bench_gradient_2d(workers=args.workers)))
         return
     if args.density:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None

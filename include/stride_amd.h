@@ -151,6 +151,12 @@ int sa_set_tile(sa_solver *s, int rows, int chunk, int version);
  * chunk 60, waves 8. */
 int sa_set_tile_density(sa_solver *s, int rows, int chunk, int waves);
 
+/* A non-blocking stream of default priority on `device`, for sa_set_stream (several solvers of
+ * one GPU on their own streams overlap their work; no reference counterpart: the reference runs
+ * one Devito operator per worker process). */
+int sa_stream_create(int device, void **stream);
+int sa_stream_destroy(void *stream);
+
 /* Kernel timing: when enabled, every launch of the wavefield-update kernel is bracketed by
  * HIP events on the solver stream. sa_timing_read returns the summed kernel milliseconds
  * and launch count since the last reset. */

@@ -42,7 +42,7 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_set_medium', 'sa_set_sponge', 'sa_set_pml', 'sa_set_sources', 'sa_set_receivers',
            'sa_forward', 'sa_forward_dev', 'sa_adjoint', 'sa_adjoint_dev', 'sa_read_gradient',
            'sa_adjoint_exact', 'sa_get_wavefield', 'sa_free_saved', 'sa_set_tile', 'sa_set_tile_density',
-           'sa_timing_enable', 'sa_timing_read',
+           'sa_timing_enable', 'sa_timing_read', 'sa_stream_create', 'sa_stream_destroy',
            'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize', 'sa_batch_create',
            'sa_batch_set_taps', 'sa_batch_forward_dev', 'sa_batch_adjoint_dev', 'sa_batch_get_wavefield',
            'sa_batch_set_tile', 'sa_traces_check', 'sa_traces_mute', 'sa_traces_fir', 'sa_traces_sosfilt',
@@ -379,6 +379,26 @@ class Solver:
 
     def set_tile(self, rows, chunk, version=2):
         self._check(self._lib.sa_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk), ctypes.c_int(version)))
+
+    @staticmethod
+    def stream_create(device=0):
+        """A new non-blocking stream of default priority on ``device`` (a hipStream_t handle)."""
+        h = ctypes.c_void_p()
+        code = lib().sa_stream_create(ctypes.c_int(device), ctypes.byref(h))
+        if code != 0:
+            raise SolverError(code, lib().sa_last_error().decode())
+        return h.value
+
+    @staticmethod
+    def stream_destroy(stream):
+        code = lib().sa_stream_destroy(ctypes.c_void_p(stream))
+        if code != 0:
+            raise SolverError(code, lib().sa_last_error().decode())
+
+    def set_stream(self, stream):
+        """HIP stream (a hipStream_t handle, e.g. torch.cuda.Stream().cuda_stream) the solver issues
+        its work on; None: the null stream."""
+        self._check(self._lib.sa_set_stream(self._h, ctypes.c_void_p(stream) if stream else None))
 
     def set_tile_density(self, rows, chunk=60, waves=8):
         """Tile of the fused 3D density update; rows=0 switches the fused 3D density and 2D updates

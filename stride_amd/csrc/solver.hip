@@ -1807,6 +1807,21 @@ extern "C" int sa_memcpy(sa_solver *s, void *dst, const void *src, int64_t bytes
     return SA_OK;
 }
 
+extern "C" int sa_stream_create(int device, void **stream) {
+    if (!stream) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(device));
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
+    *stream = (void *)st;
+    return SA_OK;
+}
+
+extern "C" int sa_stream_destroy(void *stream) {
+    if (!stream) return SA_OK;
+    HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return SA_OK;
+}
+
 extern "C" int sa_synchronize(sa_solver *s) {
     if (!s) return fail(SA_E_ARG, "null solver");
     HIP_TRY(hipStreamSynchronize(s->stream));
