@@ -413,6 +413,58 @@ def bench_gradient_2d(shots=8, workers=2):
                                     'streams and host threads)' % (p.nt, shots, workers)})
 
 
+def bench_forward_2d(reps=20):
+    """The reference's own CPU-runnable case: 2D 200^2 inner (240^2 extended), dx 0.5 mm, sponge 20,
+    Gaussian inclusion, one source on the 128-element 40 mm ring, 1000 steps (OT2), traces
+    (128, 1000). The GPU run (fused 2D update, one launch per step) is timed over ``reps`` shots;
+    the oracle (C/OpenMP, test infrastructure) runs the same whole shot once on the host cores, and
+    its traces are compared."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle
+    from stride_amd.configs import config1
+    from stride_amd.physics.plan import build_plan
+    from stride_amd._native import Solver
+    cfg = config1()
+    p = build_plan(cfg['space'], cfg['time'], cfg['wavelet'][None], cfg['vp'], source_coordinates=cfg['sources'],
+                   receiver_coordinates=cfg['receivers'])
+    s = Solver(p.dim, p.shape, p.spacing, p.absorbing, p.dt, p.nt)
+    s.set_scheme(p.kernel_order, p.rho_mode, p.att_power, p.boundary, p.diff_source, p.undersampling, 0,
+                 p.tb0, p.tb1, 0, 0, 0)
+    s.set_medium(p.vp)
+    s.set_sponge(p.sponge_tables)
+    s.set_sources(p.src_idx, p.src_w)
+    s.set_receivers(p.rec_idx, p.rec_w)
+    d_src = s._to_device(p.src_data)
+    d_rec = s.device_alloc(p.nt * p.nrec * 4)
+    s.forward_dev(d_src, d_rec)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.forward_dev(d_src, d_rec)
+    s.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    rec = np.zeros((p.nt, p.nrec), np.float32)
+    s.memcpy(rec.ctypes.data, d_rec, rec.nbytes, 2)
+    s.device_free(d_src)
+    s.device_free(d_rec)
+    s.close()
+    t1 = time.perf_counter()
+    ref, _, _ = oracle.forward(p)
+    cpu = time.perf_counter() - t1
+    n = int(np.prod(p.shape))
+    steps = p.nt - 2
+    err = float(np.linalg.norm(rec.T - ref) / np.linalg.norm(ref))
+    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    return dict(metric='Gpts/s 2D 200^2 SO10 forward (the reference\'s CPU-runnable case)',
+                value=round(n * steps / wall / 1e9, 3), unit='Gpts/s', seconds_per_shot=round(wall, 5),
+                us_per_step=round(wall / steps * 1e6, 2), traces_rel_l2_vs_oracle=err,
+                cpu_baseline={'value': round(n * steps / cpu / 1e9, 4), 'unit': 'Gpts/s', 'cores': threads,
+                              'kind': 'port', 'sample': 'oracle (C/OpenMP), the whole shot, %.2f s' % cpu},
+                dtype='f32', data='synthetic',
+                config={'workload': '2D 200x200 inner (240x240 extended), dx 0.5 mm, SO10 OT2, sponge 20, '
+                                    'Gaussian inclusion, 1 source, 128 receivers, nt %d' % p.nt})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -438,6 +490,9 @@ def main():
     ap.add_argument('--gradient-2d', action='store_true',
                     help='only run shots of the 2D gradient workload (572^2 extended, CFS-PML, variable density, '
                          'power-2 attenuation; vp and rho gradients) and exit')
+    ap.add_argument('--forward-2d', action='store_true',
+                    help="only run the reference's CPU-runnable case (2D 240^2 forward, 1000 steps) on the GPU and "
+                         "the CPU oracle and exit")
     ap.add_argument('--workers', type=int, default=2,
                     help='with --gradient-2d: solvers running shots concurrently, each on its own stream')
     ap.add_argument('--pipelines', action='store_true',
@@ -464,6 +519,9 @@ def main():
         print(json.dumps({'copy_check_bytes': nbytes, 'copies': 10}))
         return
 
+    if args.forward_2d:
+        print(json.dumps(bench_forward_2d()))
+        return
     if args.gradient_2d:
         print(json.dumps(bench_gradient_2d(workers=args.workers)))
         return
