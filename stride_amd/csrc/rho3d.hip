@@ -584,7 +584,7 @@ int rho3d_grad(const Geo &g, const Box &box, const float *ps, const float *psm, 
         a.lo[d] = g.nabs[d];
         a.hi[d] = g.n[d] - g.nabs[d];
     }
-    a.chunk = 40;
+    a.chunk = 60;  // of 24 / 40 / 60 / 90 on the 360^3 grid, the fastest
     a.ntz = (a.hi[2] - a.lo[2] + kTZ - 1) / kTZ;
     a.nty = (a.hi[1] - a.lo[1] + NW - 1) / NW;
     a.ntiles = a.ntz * a.nty * ((a.hi[0] - a.lo[0] + a.chunk - 1) / a.chunk);
