@@ -20,7 +20,8 @@ class ShotWorkers:
     given its own stream. ``map(fn, items)`` calls ``fn(solver, item)`` for every item: items are
     dealt round-robin to the workers, each worker runs its items in order on its own thread and
     synchronises its stream at the end; results come back in item order, and the first exception
-    raised by a worker is raised again."""
+    raised by a worker is raised again. ``close()`` (or leaving a ``with`` block) releases the
+    solvers and streams."""
 
     def __init__(self, make_solver, n=2, device=0):
         if n < 1:
@@ -70,3 +71,10 @@ class ShotWorkers:
             Solver.stream_destroy(st)
         self.solvers = []
         self.streams = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False

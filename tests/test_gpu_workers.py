@@ -34,9 +34,8 @@ def test_two_workers_match_sequential_shots(gpu):
     seq = solver_for(plan, save_mode=2)
     want = [shot(seq, k) for k in range(len(plans))]
     seq.close()
-    pool = ShotWorkers(lambda: solver_for(plan, save_mode=2), n=2)
-    got = pool.map(shot, range(len(plans)))
-    pool.close()
+    with ShotWorkers(lambda: solver_for(plan, save_mode=2), n=2) as pool:
+        got = pool.map(shot, range(len(plans)))
     for k in range(len(plans)):
         assert np.array_equal(got[k][0], want[k][0]), k
         for key in ('grad_vp', 'prec_vp', 'grad_rho', 'prec_rho'):
