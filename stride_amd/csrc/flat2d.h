@@ -6,8 +6,8 @@
 
 #include "kernels.h"
 
-// tile of one workgroup: 64 cells along axis 2 x 4 along axis 1
-constexpr int kFlat2DTileZ = 64, kFlat2DTileY = 4;
+// tile of one workgroup: 64 cells along axis 2 x 8 along axis 1 (of 2 / 4 / 8 rows, the fastest)
+constexpr int kFlat2DTileZ = 64, kFlat2DTileY = 8;
 
 struct Flat2DStep {
     const float *u, *other;  // current level and the level it replaces

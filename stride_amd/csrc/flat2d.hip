@@ -16,8 +16,8 @@
 // `other` in place: each launch also writes the Laplacian of its u, which is the next step's
 // other level, so a separate Laplacian launch runs only on the first step of a run.
 //
-// Work decomposition: a workgroup of 4 waves owns 64 (axis 2) x 4 (axis 1) cells, one per
-// thread. u is staged in LDS with a 9-cell halo (rows y0-9 .. y0+12, columns z0-12 .. z0+75),
+// Work decomposition: a workgroup of 8 waves owns 64 (axis 2) x 8 (axis 1) cells, one per
+// thread. u is staged in LDS with a 9-cell halo (rows y0-9 .. y0+16, columns z0-12 .. z0+75),
 // b with a 5-cell halo; with density the fluxes bbar D+ u are formed once per half point into
 // LDS tiles and differenced back to the nodes.
 
