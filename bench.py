@@ -260,7 +260,32 @@ def bench_schedule(tile=None, num_shots=32):
                                     'forward only, shots paired by duration through the two-shot update'})
 
 
-def bench_density(nt=2500, tile=None):
+def cpu_baseline_density(cfg, nt=24):
+    """The oracle (C/OpenMP, test infrastructure) on the same medium: a bounded forward with raw
+    saves + as-written adjoint with vp and rho gradients, timed on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle
+    from stride_amd.physics.plan import build_plan, adjoint_source_samples
+    from stride_amd.problem import Time
+    space = cfg['space']
+    tm = Time(start=0., step=cfg['time'].step, num=nt)
+    plan = build_plan(space, tm, cfg['wavelet'][None, :nt], cfg['vp'], rho=cfg['rho'], alpha=cfg['alpha'],
+                      source_coordinates=cfg['sources'], receiver_coordinates=cfg['receivers'], kernel='OT2',
+                      save_undersampling=4, grad_vp=True, grad_rho=True)
+    res = adjoint_source_samples(plan, np.random.default_rng(0).standard_normal((plan.nrec, nt)))
+    t0 = time.perf_counter()
+    _, saved, _ = oracle.forward(plan, save=True)
+    oracle.adjoint_compat(plan, res, saved)
+    t1 = time.perf_counter()
+    n = int(np.prod(plan.shape))
+    fwd, adj = nt - 2, nt - 1 - plan.undersampling
+    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    return dict(value=round(n * (fwd + adj) / (t1 - t0) / 1e9, 4), unit='Gpts/s', cores=threads, kind='port',
+                sample='oracle (C/OpenMP) on the same 360^3 medium: %d forward steps with raw saves + %d as-written '
+                       'adjoint steps with vp and rho gradients, %.1f s' % (fwd, adj, t1 - t0))
+
+
+def bench_density(nt=2500, tile=None, with_cpu=True):
     """One 3D shot on the variable vp / rho / alpha medium (360^3 extended, attenuation power 0,
     sponge 20, OT2, 1024 receivers): forward with the raw wavefield saved every 4th step over the
     whole domain, then the as-written adjoint with the vp and rho gradients. The update runs as the
@@ -308,13 +333,14 @@ def bench_density(nt=2500, tile=None):
     for d in (d_src, d_rec, d_res):
         s.device_free(d)
     s.close()
+    cpu = cpu_baseline_density(cfg) if with_cpu else None
     return dict(metric='Gpts/s fwd+adjoint 3D 320^3 SO10, variable vp/rho/alpha, vp+rho gradients',
                 value=round(launches * n / wall / 1e9, 3), unit='Gpts/s', seconds=round(wall, 3),
                 update_launches=int(launches), update_kernel_us=round(ms * 1e3 / launches, 1),
                 update_kernel_share=round(ms * 1e-3 / wall, 3),
                 roofline={'bound': 'hbm', 'kernel': 'k_rho3d', 'achieved': round(achieved, 1),
                           'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4)},
-                dtype='f32', data='synthetic',
+                cpu_baseline=cpu, dtype='f32', data='synthetic',
                 config={'workload': '3D 320x320x320 inner (360x360x360 extended), dx 0.4 mm, SO10 OT2, sponge 20, '
                                     'variable vp / rho (Gaussian-filtered noise in the hemisphere) and alpha 0.5 '
                                     '(power 0), one shot, nt %d, raw saves every 4th step, 1024 receivers' % plan.nt})
@@ -527,7 +553,7 @@ def main():
         return
     if args.density:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
-        print(json.dumps(bench_density(args.nt, tile)))
+        print(json.dumps(bench_density(args.nt, tile, with_cpu=not args.no_cpu)))
         return
     if args.schedule:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
