@@ -346,7 +346,32 @@ def bench_density(nt=2500, tile=None, with_cpu=True):
                                     '(power 0), one shot, nt %d, raw saves every 4th step, 1024 receivers' % plan.nt})
 
 
-def bench_gradient_2d(shots=8, workers=2):
+def cpu_baseline_2d(cfg, nt=500):
+    """The oracle (C/OpenMP, test infrastructure) on the 2D gradient workload's medium and scheme: a
+    bounded forward with raw saves + as-written adjoint with vp and rho gradients."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle
+    from stride_amd.physics.plan import build_plan, adjoint_source_samples
+    from stride_amd.problem import Time
+    tm = Time(start=0., step=cfg['time'].step, num=nt)
+    plan = build_plan(cfg['space'], tm, cfg['wavelet'][None, :nt], cfg['vp'], rho=cfg['rho'], alpha=cfg['alpha'],
+                      source_coordinates=cfg['sources'][:1], receiver_coordinates=cfg['receivers'],
+                      boundary_type='complex_frequency_shift_PML_2', attenuation_power=2, grad_vp=True,
+                      grad_rho=True)
+    res = adjoint_source_samples(plan, np.random.default_rng(0).standard_normal((plan.nrec, nt)))
+    t0 = time.perf_counter()
+    _, saved, _ = oracle.forward(plan, save=True)
+    oracle.adjoint_compat(plan, res, saved)
+    t1 = time.perf_counter()
+    n = int(np.prod(plan.shape))
+    fwd, adj = nt - 2, nt - 1 - plan.undersampling
+    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    return dict(value=round(n * (fwd + adj) / (t1 - t0) / 1e9, 4), unit='Gpts/s', cores=threads, kind='port',
+                sample='oracle (C/OpenMP), one shot of %d steps: forward with raw saves + as-written adjoint '
+                       'with vp and rho gradients, %.1f s' % (nt, t1 - t0))
+
+
+def bench_gradient_2d(shots=8, workers=2, with_cpu=True):
     """The 2D gradient workload: 512^2 inner (572^2 extended), dx 0.2 mm, CFS-PML 30, variable vp /
     rho, alpha 0.5 with power 2, 128 ring receivers, nt 2500. Each shot: forward with the raw
     wavefield saved every f-th step, then the as-written adjoint with the vp and rho gradients
@@ -425,6 +450,7 @@ def bench_gradient_2d(shots=8, workers=2):
         release(sv)
     pool.close()
     n = int(np.prod(p.shape))
+    cpu = cpu_baseline_2d(cfg) if with_cpu else None
     return dict(metric='Gpts/s fwd+adjoint 2D 512^2 SO10, CFS-PML, variable vp/rho, power-2 attenuation, '
                        'vp+rho gradients', value=round(launches * n / wall / 1e9, 3), unit='Gpts/s',
                 seconds_per_shot=round(wall / shots, 4), workers=workers,
@@ -432,7 +458,7 @@ def bench_gradient_2d(shots=8, workers=2):
                 seconds_per_shot_one_worker=round(wall1 / shots, 4), shots=shots, update_launches=int(launches),
                 update_kernel_us=round(ms * 1e3 / launches, 2),
                 update_kernel_share_one_worker=round(ms * 1e-3 / wall_timed, 3),
-                dtype='f32', data='synthetic',
+                cpu_baseline=cpu, dtype='f32', data='synthetic',
                 config={'workload': '2D 512x512 inner (572x572 extended), dx 0.2 mm, SO10 OT2, CFS-PML 30, '
                                     'vp 1500 + 150 N1 in a 45 mm disc, rho 1000 + 50 N2, alpha 0.5 (power 2), '
                                     '128-element 48 mm ring, nt %d, %d shots, %d workers (solvers with their own '
@@ -549,7 +575,7 @@ def main():
         print(json.dumps(bench_forward_2d()))
         return
     if args.gradient_2d:
-        print(json.dumps(bench_gradient_2d(workers=args.workers)))
+        print(json.dumps(bench_gradient_2d(workers=args.workers, with_cpu=not args.no_cpu)))
         return
     if args.density:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
