@@ -3,7 +3,7 @@ thread, so that independent shots overlap on the device.
 
 This is what running several Mosaic workers per GPU gives Stride (each worker holds its own
 IsoAcousticDevito instance and runs whole shots, stride/__init__.py:110-141). On the 2D grids a
-time step is one short launch that leaves much of the GPU idle, and two workers give about 1.5x
+time step is one short launch that leaves much of the GPU idle, and two workers give about 1.4x
 the shot throughput of one; on the 3D grids one shot fills the GPU and a second worker gains
 nothing. The streams are the library's own (non-blocking, default priority).
 """
