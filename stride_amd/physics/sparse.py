@@ -17,7 +17,8 @@ Coordinates are shifted by 1e-3*h first (stride/physics/iso_acoustic/devito.py:3
 import numpy as np
 import scipy.special
 
-__all__ = ['linear_taps', 'hicks_coefficients', 'hicks_taps', 'point_taps', 'INTERPOLATION']
+__all__ = ['linear_taps', 'hicks_coefficients', 'smooth_source_coefficients', 'hicks_taps', 'point_taps',
+           'INTERPOLATION']
 
 INTERPOLATION = ('linear', 'hicks')
 
@@ -52,6 +53,24 @@ def linear_taps(space, coordinates):
     return _flatten(space, cells, w)
 
 
+def smooth_source_coefficients(coef):
+    """The 5-tap blend applied to source coefficients (n, dim, 8) in place
+    (stride/physics/common/devito.py:874-883), with n = 2 the column of offset -1.
+
+    The reference binds a, b, c as numpy views of columns n, n+1, n+2
+    (common/devito.py:876-878), so each assignment is seen by the lines after it: the
+    n+1 line reads the new column n, the n+2 line the new column n+1 and the n+3 line
+    the new column n+2. The columns are blended in that order, in place."""
+    n = 2
+    col = [coef[..., k] for k in range(coef.shape[-1])]  # views, as in the reference
+    col[n - 1] += col[n] * 0.01
+    col[n][...] = col[n] * 0.98 + col[n + 1] * 0.03
+    col[n + 1][...] = col[n + 1] * 0.94 + (col[n] + col[n + 2]) * 0.01
+    col[n + 2][...] = col[n + 2] * 0.98 + col[n + 1] * 0.03
+    col[n + 3] += col[n + 2] * 0.01
+    return coef
+
+
 def hicks_coefficients(space, coordinates, smooth=False):
     """(reference cells (n, dim) int32, coefficients (n, dim, 8) float64), the
     8th coefficient always zero."""
@@ -68,13 +87,7 @@ def hicks_coefficients(space, coordinates, smooth=False):
         w = np.minimum((x / width) ** 2, 1.)
         coef[:, :, half + k] = np.sinc(x) * scipy.special.iv(0, b * np.sqrt(1 - w)) / den
     if smooth:
-        n = half - 1
-        a, bb, c = coef[:, :, n].copy(), coef[:, :, n + 1].copy(), coef[:, :, n + 2].copy()
-        coef[:, :, n - 1] += a * 0.01
-        coef[:, :, n] = a * 0.98 + bb * 0.03
-        coef[:, :, n + 1] = bb * 0.94 + (a + c) * 0.01
-        coef[:, :, n + 2] = c * 0.98 + bb * 0.03
-        coef[:, :, n + 3] += c * 0.01
+        smooth_source_coefficients(coef)
     return ref, coef
 
 
