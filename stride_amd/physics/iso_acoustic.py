@@ -32,7 +32,7 @@ import logging
 import numpy as np
 
 from .._native import Solver
-from ..problem import ScalarField, Traces
+from ..problem import Grid, ScalarField, Traces
 from .boundaries import SPONGE, PML
 from .gradients import compat_finish, exact_gradients
 from .plan import build_plan, adjoint_source_samples, RHO_VARIABLE
@@ -42,17 +42,61 @@ __all__ = ['ProblemTypeBase', 'IsoAcousticHIP', 'IsoAcousticDevito']
 logger = logging.getLogger('stride_amd')
 
 
+def _next_in_mro(obj, after, attr):
+    """The first definition of ``attr`` in the classes that follow ``after`` in the method
+    resolution order of ``obj``'s class (``object`` excluded), or None."""
+    mro = type(obj).__mro__
+    for klass in mro[mro.index(after) + 1:]:
+        if klass is object:
+            continue
+        if attr in klass.__dict__:
+            return klass.__dict__[attr]
+    return None
+
+
 class ProblemTypeBase:
     """before/run/after template and per-variable gradient dispatch
-    (stride/physics/problem_type.py:58-321)."""
+    (stride/physics/problem_type.py:58-321).
+
+    The class also works as a mixin in front of Stride's own ``ProblemTypeBase``
+    (``class X(IsoAcousticHIP, stride.ProblemTypeBase)``, what a Stride maintainer writes to
+    make it a Mosaic tessera):
+
+    - ``__init__`` is cooperative: when another class follows this one in the method order
+      (Stride's ``ProblemTypeBase``, which initialises ``Gridded`` and ``Operator``,
+      problem_type.py:51-53), it is called with the same keywords, so the operator gets
+      its name, uid and grid the way Stride builds them;
+    - ``__call__`` hands over to the next ``__call__`` in the method order when there is
+      one: Stride's asynchronous ``Operator.__call__`` (stride/core.py:728-793), which
+      records the adjoint graph nodes on the outputs and calls ``forward``. Standing alone,
+      calling the operator runs ``forward`` directly;
+    - ``grid``, ``space`` and ``time`` read the grid object as ``Gridded`` does
+      (stride/problem/base.py:11-73).
+    """
 
     def __init__(self, name=None, grid=None, space=None, time=None, **kwargs):
-        self.name = name or self.__class__.__name__.lower()
-        if grid is not None:
-            space = space or grid.space
-            time = time or grid.time
-        self.space = space
-        self.time = time
+        if _next_in_mro(self, ProblemTypeBase, '__init__') is not None:
+            super().__init__(**({'name': name} if name is not None else {}), grid=grid, space=space, time=time,
+                             **kwargs)
+        if getattr(self, '_grid', None) is None:
+            if grid is not None:
+                space = space or grid.space
+                time = time or grid.time
+            self._grid = Grid(space, time)
+        if getattr(self, 'name', None) is None:
+            self.name = name or self.__class__.__name__.lower()
+
+    @property
+    def grid(self):
+        return self._grid
+
+    @property
+    def space(self):
+        return self._grid.space
+
+    @property
+    def time(self):
+        return self._grid.time
 
     def forward(self, *args, **kwargs):
         self.before_forward(*args, **kwargs)
@@ -66,7 +110,11 @@ class ProblemTypeBase:
         out = self.after_adjoint(*args, **kwargs)
         return out if out is not None else ad
 
-    __call__ = forward
+    def __call__(self, *args, **kwargs):
+        parent = _next_in_mro(self, ProblemTypeBase, '__call__')
+        if parent is not None:
+            return parent(self, *args, **kwargs)
+        return self.forward(*args, **kwargs)
 
     def _dispatch(self, prefix, variable):
         methods = [getattr(self, prefix + part, None) for part in variable.name.split('_')]
@@ -122,7 +170,7 @@ class IsoAcousticHIP(ProblemTypeBase):
     time_order = 2
 
     def __init__(self, name=None, grid=None, space=None, time=None, deviceid=0, **kwargs):
-        super().__init__(name=name, grid=grid, space=space, time=time)
+        super().__init__(name=name, grid=grid, space=space, time=time, **kwargs)
         self.kernel = 'OT4'
         self.undersampling_factor = 4
         self.boundary_type = 'sponge_boundary_2'
@@ -303,9 +351,8 @@ class IsoAcousticHIP(ProblemTypeBase):
                                        vp.data, transform=vp.transform)
         else:
             grad, prec = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
-        out = ScalarField('vp_grad', space=self.space, data=grad, shape=grad.shape, extended_shape=grad.shape)
-        out.prec = None if prec is None else ScalarField('vp_prec', space=self.space, data=prec, shape=prec.shape,
-                                                         extended_shape=prec.shape)
+        out = _gradient_field(vp, 'vp', grad, prec)
+        _dump_local(out, 'vp', kwargs)
         return out
 
     def prepare_grad_rho(self, rho, **kwargs):
@@ -325,10 +372,7 @@ class IsoAcousticHIP(ProblemTypeBase):
             grad, prec = self._inner(g['grad_rho']), self._inner(g['prec_rho'])
         else:
             grad, prec = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
-        out = ScalarField('rho_grad', space=self.space, data=grad, shape=grad.shape, extended_shape=grad.shape)
-        out.prec = None if prec is None else ScalarField('rho_prec', space=self.space, data=prec,
-                                                         shape=prec.shape, extended_shape=prec.shape)
-        return out
+        return _gradient_field(rho, 'rho', grad, prec)
 
     # the reference has no alpha gradient (problem_type.py:226-231 raises for it); the exact
     # adjoint provides one
@@ -346,9 +390,7 @@ class IsoAcousticHIP(ProblemTypeBase):
     def get_grad_alpha(self, alpha, **kwargs):
         kind, g = self._grads if self._grads is not None else (None, None)
         grad = self._inner(g['alpha']) if kind == 'exact' else np.zeros(self.space.shape, np.float32)
-        out = ScalarField('alpha_grad', space=self.space, data=grad, shape=grad.shape, extended_shape=grad.shape)
-        out.prec = None
-        return out
+        return _gradient_field(alpha, 'alpha', grad, None)
 
     def prepare_grad_wavelets(self, wavelets, **kwargs):
         return ()
@@ -374,6 +416,34 @@ class IsoAcousticHIP(ProblemTypeBase):
     def wavefield(self, which=0):
         """Last (0) or previous (1) wavefield level left in the solver, extended grid."""
         return self._solver.wavefield(which)
+
+
+def _gradient_field(variable, name, grad, prec):
+    """The gradient as the variable's own kind of field, inner-shaped, with ``.prec``:
+    ``vp.alike(name='vp_grad', data=..., shape=..., extended_shape=..., inner=None)``
+    (stride/physics/iso_acoustic/devito.py:1088-1095, 1195-1202). With Stride's ScalarField
+    that keeps its grid, transform and graph machinery."""
+    out = variable.alike(name='%s_grad' % name, data=grad, shape=grad.shape, extended_shape=grad.shape,
+                         inner=None)
+    out.prec = None if prec is None else variable.alike(name='%s_prec' % name, data=prec, shape=prec.shape,
+                                                        extended_shape=prec.shape, inner=None)
+    return out
+
+
+def _dump_local(grad, name, kwargs):
+    """dump_local_grad / dump_local_prec (devito.py:1097-1111): the shot's gradient and
+    preconditioner written to the problem's output folder, version = iteration.abs_id + 1."""
+    problem = kwargs.get('problem')
+    if problem is None:
+        return
+    iteration = kwargs.get('iteration')
+    version = iteration.abs_id + 1 if iteration is not None else 0
+    if kwargs.get('dump_local_grad', False):
+        grad.dump(path=problem.output_folder, project_name=problem.name,
+                  parameter='%s_local_grad-Shot%05d' % (name, problem.shot_id), version=version)
+    if kwargs.get('dump_local_prec', False) and grad.prec is not None:
+        grad.prec.dump(path=problem.output_folder, project_name=problem.name,
+                       parameter='%s_local_src_prec-Shot%05d' % (name, problem.shot_id), version=version)
 
 
 def _tukey_for(plan):

@@ -98,3 +98,29 @@ def test_nan_gradient_returns_none(gpu):
     pde.forward(shot.wavelets, vp, problem=problem)
     res = np.full((shot.num_points_receivers, time.num), np.nan, np.float32)
     assert pde.adjoint(res, shot.wavelets, vp, problem=problem) is None
+
+
+def test_stride_operator_contract(gpu):
+    """The INTEGRATION.md mixin driven through Stride's asynchronous Operator contract
+    (tests/stride_contract.py): traces and gradients equal those of direct calls bit for
+    bit, and the traces carry the adjoint graph node."""
+    import asyncio
+    import stride_contract as sc
+
+    class Bound(IsoAcousticHIP, sc.ProblemTypeBase):
+        pass
+
+    cfg, space, time, vp, shot, problem = _setup(nt=240)
+    shot.observed = sc.GraphTraces('observed', data=np.zeros_like(shot.observed.data))
+    direct = IsoAcousticHIP(space=space, time=time)
+    t_ref = direct.forward(shot.wavelets, vp, problem=problem)
+    res = Traces('res', data=t_ref.data * 0.1)
+    g_ref = direct.adjoint(res, shot.wavelets, vp, problem=problem)[1]
+
+    pde = Bound(space=space, time=time)
+    traces = asyncio.run(pde(shot.wavelets, vp, problem=problem))
+    assert traces.prev_op.op is pde and traces.needs_grad
+    np.testing.assert_array_equal(traces.data, t_ref.data)
+    grads = asyncio.run(pde.__call_adjoint__(res))
+    np.testing.assert_array_equal(grads[1].data, g_ref.data)
+    np.testing.assert_array_equal(grads[1].prec.data, g_ref.prec.data)
