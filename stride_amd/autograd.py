@@ -39,6 +39,8 @@ class Geometry:
         self.checkpoint_every = checkpoint_every
         self.options = options
         self._solver = None
+        # identifies the forward run whose medium and saved wavefield the solver holds
+        self.state_token = None
 
     def solver(self, plan):
         key = (plan.dim, plan.shape, plan.spacing, plan.absorbing, float(plan.dt), int(plan.nt), int(self.device))
@@ -52,6 +54,23 @@ def _np(t):
     return None if t is None else t.detach().to('cpu', torch.float32).numpy()
 
 
+def _run_forward(geometry, plan):
+    """Configure the geometry's solver for ``plan`` and run the saving forward; returns the
+    receiver samples (nt, nrec) and stamps the solver state with a fresh token."""
+    s = geometry.solver(plan)
+    save_mode = 3 if geometry.checkpoint_every is None else 4
+    s.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
+                 geometry.checkpoint_every or 1, save_mode, plan.tb0, plan.tb1, 1, 0, 0)
+    s.set_medium(plan.vp, plan.rho, plan.alpha)
+    if plan.boundary == SPONGE:
+        s.set_sponge(plan.sponge_tables)
+    s.set_sources(plan.src_idx, plan.src_w)
+    s.set_receivers(plan.rec_idx, plan.rec_w)
+    rec = s.forward(plan.src_data)
+    geometry.state_token = object()
+    return rec
+
+
 class _IsoAcoustic(torch.autograd.Function):
     @staticmethod
     def forward(ctx, vp, rho, alpha, wavelets, geometry):
@@ -60,24 +79,21 @@ class _IsoAcoustic(torch.autograd.Function):
                           receiver_coordinates=geometry.receiver_coordinates, grad_vp=True, **geometry.options)
         if plan.boundary == PML:
             raise ValueError('the exact adjoint does not support the CFS-PML boundary')
-        s = geometry.solver(plan)
-        save_mode = 3 if geometry.checkpoint_every is None else 4
-        s.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
-                     geometry.checkpoint_every or 1, save_mode, plan.tb0, plan.tb1, 1, 0, 0)
-        s.set_medium(plan.vp, plan.rho, plan.alpha)
-        if plan.boundary == SPONGE:
-            s.set_sponge(plan.sponge_tables)
-        s.set_sources(plan.src_idx, plan.src_w)
-        s.set_receivers(plan.rec_idx, plan.rec_w)
-        rec = s.forward(plan.src_data)
+        rec = _run_forward(geometry, plan)
         ctx.plan = plan
         ctx.geometry = geometry
+        ctx.token = geometry.state_token
         ctx.shapes = [None if x is None else (x.dtype, x.device) for x in (vp, rho, alpha, wavelets)]
         return torch.from_numpy(np.ascontiguousarray(rec.T)).to(vp.device)
 
     @staticmethod
     def backward(ctx, grad_traces):
         plan, geometry = ctx.plan, ctx.geometry
+        if geometry.state_token is not ctx.token:
+            # another forward ran on this geometry's solver since this one (its medium and
+            # saved wavefield are in the solver now): run this forward again first
+            _run_forward(geometry, plan)
+            ctx.token = geometry.state_token
         s = geometry.solver(plan)
         # J = <grad_traces, traces>: the adjoint source is grad_traces itself (no window)
         res = np.ascontiguousarray(_np(grad_traces).T)

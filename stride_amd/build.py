@@ -5,6 +5,7 @@ hipcc for gfx950 only and links one shared library exposing the C ABI declared
 in include/stride_amd.h.
 """
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -40,6 +41,7 @@ def build(verbose=False, objdir=None):
     os.makedirs(objdir, exist_ok=True)
     newest_header = max(os.path.getmtime(h) for h in _headers())
     objs = []
+    jobs = []
     for src in sources():
         obj = os.path.join(objdir, os.path.basename(src)[:-4] + '.o')
         fresh = keep and os.path.exists(obj) and os.path.getmtime(obj) > max(os.path.getmtime(src), newest_header)
@@ -47,8 +49,14 @@ def build(verbose=False, objdir=None):
             cmd = [hipcc, '-c', src, '-o', obj] + FLAGS
             if verbose:
                 print(' '.join(cmd), flush=True)
-            subprocess.run(cmd, check=True)
+            jobs.append(cmd)
         objs.append(obj)
+    # one compiler process per source, at most one per host core
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 16))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for done in pool.map(lambda c: subprocess.run(c), jobs):
+            if done.returncode != 0:
+                raise subprocess.CalledProcessError(done.returncode, done.args)
     tmp = LIB + '.tmp'
     cmd = [hipcc, '-shared', '-o', tmp] + objs + ['--offload-arch=%s' % ARCH, '-fPIC']
     subprocess.run(cmd, check=True)

@@ -566,6 +566,16 @@ static void dfree(void *p) {
         if (rc_ != SA_OK) return rc_; \
     } while (0)
 
+// Setters copy into (or free and reallocate) buffers that kernels already queued on the
+// solver's stream may still read: the "_dev" calls return at once, and the solver's stream
+// may be non-blocking (sa_stream_create), so the synchronous copies on the null stream would
+// not wait for it. Each setter first waits for the solver's stream.
+static int quiesce(sa_solver *s) {
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SA_OK;
+}
+
 static dim3 domain_grid(const Geo &g, dim3 blk) {
     return dim3((g.n[2] + blk.x - 1) / blk.x, (g.n[1] + blk.y - 1) / blk.y, g.n[0]);
 }
@@ -749,7 +759,7 @@ extern "C" int sa_set_scheme(sa_solver *s, const sa_scheme *sch) {
 
 extern "C" int sa_set_medium(sa_solver *s, const float *vp, const float *rho, const float *alpha) {
     if (!s || !vp) return fail(SA_E_ARG, "vp is required");
-    HIP_TRY(hipSetDevice(s->device));
+    TRY(quiesce(s));
     TRY(upload_padded(s, &s->vp, vp, true));
     if (rho) {
         std::vector<float> b(s->npts);
@@ -770,7 +780,7 @@ extern "C" int sa_set_sponge(sa_solver *s, const float *tables, int len) {
     if (!s || !tables || len < 1) return fail(SA_E_ARG, "sponge tables required");
     for (int d = 0; d < s->grid.dim; ++d)
         if (s->grid.absorbing[d] > len) return fail(SA_E_ARG, "sponge table shorter than the layer");
-    HIP_TRY(hipSetDevice(s->device));
+    TRY(quiesce(s));
     dfree(s->tables);
     s->tables = nullptr;
     // re-index tables to internal axes (2D: axis 0 unused)
@@ -787,7 +797,7 @@ extern "C" int sa_set_sponge(sa_solver *s, const float *tables, int len) {
 
 extern "C" int sa_set_pml(sa_solver *s, const float *profiles, int stride) {
     if (!s || !profiles) return fail(SA_E_ARG, "pml profiles required");
-    HIP_TRY(hipSetDevice(s->device));
+    TRY(quiesce(s));
     int off = 3 - s->grid.dim;
     for (int d = 0; d < s->grid.dim; ++d) {
         int a = d + off;
@@ -804,7 +814,7 @@ extern "C" int sa_set_pml(sa_solver *s, const float *profiles, int stride) {
 static int set_sparse(sa_solver *s, Sparse &sp, int npoint, int ntap, const int64_t *index, const float *weight) {
     if (npoint < 0 || ntap < 1 || (npoint > 0 && (!index || !weight)))
         return fail(SA_E_ARG, "sparse points need index and weight arrays");
-    HIP_TRY(hipSetDevice(s->device));
+    TRY(quiesce(s));
     free_sparse(sp);
     sp.npoint = npoint;
     sp.ntap = ntap;

@@ -6,6 +6,11 @@ IsoAcousticDevito instance and runs whole shots, stride/__init__.py:110-141). On
 time step is one short launch that leaves much of the GPU idle, and two workers give about 1.4x
 the shot throughput of one; on the 3D grids one shot fills the GPU and a second worker gains
 nothing. The streams are the library's own (non-blocking, default priority).
+
+A solver's setters (medium, boundary, taps) first wait for the work queued on that solver's
+stream. Setting new taps also frees the old tap buffers, and a device free waits for the whole
+device, so a worker that changes taps between shots briefly waits for the other workers' queued
+work too.
 """
 
 import threading
@@ -28,14 +33,20 @@ class ShotWorkers:
             raise ValueError('n must be at least 1')
         self.streams = []
         self.solvers = []
-        for _ in range(n):
-            s = make_solver()
-            if not isinstance(s, Solver):
-                raise TypeError('make_solver must return a Solver')
-            st = Solver.stream_create(device)
-            self.streams.append(st)
-            s.set_stream(st)
-            self.solvers.append(s)
+        try:
+            for _ in range(n):
+                s = make_solver()
+                if not isinstance(s, Solver):
+                    if hasattr(s, 'close'):
+                        s.close()
+                    raise TypeError('make_solver must return a Solver')
+                self.solvers.append(s)
+                st = Solver.stream_create(device)
+                self.streams.append(st)
+                s.set_stream(st)
+        except BaseException:
+            self.close()
+            raise
 
     def __len__(self):
         return len(self.solvers)
@@ -65,12 +76,13 @@ class ShotWorkers:
         return results
 
     def close(self):
-        for s in self.solvers:
-            s.close()
-        for st in self.streams:
-            Solver.stream_destroy(st)
+        solvers, streams = self.solvers, self.streams
         self.solvers = []
         self.streams = []
+        for s in solvers:
+            s.close()
+        for st in streams:
+            Solver.stream_destroy(st)
 
     def __enter__(self):
         return self

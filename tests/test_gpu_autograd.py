@@ -73,3 +73,32 @@ def test_autograd_matches_finite_differences(gpu, checkpoint):
         an = float(np.sum(grads[name].astype(np.float64) * d.numpy()))
         print('  %s: fd %.6e autograd %.6e' % (name, fd, an))
         assert abs(fd - an) <= 1e-2 * abs(fd), name
+
+
+def test_backward_after_interleaved_forward(gpu):
+    """A forward with another medium between a forward and its backward (what gradcheck does)
+    must not change that backward's gradient: the backward runs its own forward again when the
+    solver holds another run's state."""
+    import torch
+    from stride_amd.autograd import Geometry, iso_acoustic_traces
+    plan, info = small_case(dim=2, n=40, nabs=8, nt=160, kernel='OT2')
+    print('autograd interleaved forward', flush=True)
+    kw = dict(info['kw'])
+    geom = Geometry(info['space'], info['time'], kw.pop('source_coordinates'), kw.pop('receiver_coordinates'), **kw)
+    rng = np.random.default_rng(5)
+    w = torch.tensor(rng.standard_normal((plan.nrec, plan.nt)).astype(np.float32))
+
+    def grad_of(interleave):
+        vp = torch.tensor(info['vp'], dtype=torch.float32, requires_grad=True)
+        loss = torch.sum(iso_acoustic_traces(vp, None, None, torch.tensor(info['wavelet']), geom) * w)
+        if interleave:
+            with torch.no_grad():
+                other = torch.tensor(info['vp'] * 1.05, dtype=torch.float32)
+                iso_acoustic_traces(other, None, None, torch.tensor(info['wavelet']), geom)
+        loss.backward()
+        return vp.grad.numpy().copy()
+
+    g0 = grad_of(False)
+    g1 = grad_of(True)
+    assert np.abs(g0).max() > 0
+    np.testing.assert_array_equal(g1, g0)

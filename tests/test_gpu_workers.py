@@ -40,3 +40,50 @@ def test_two_workers_match_sequential_shots(gpu):
         assert np.array_equal(got[k][0], want[k][0]), k
         for key in ('grad_vp', 'prec_vp', 'grad_rho', 'prec_rho'):
             assert np.array_equal(got[k][1][key], want[k][1][key]), (k, key)
+
+
+def test_dev_shots_with_medium_changes_match_sequential(gpu):
+    """Device-pointer shots (sa_forward_dev / sa_adjoint_dev return at once) on workers whose
+    streams are non-blocking, with a different medium set before every shot: each shot's
+    traces and vp gradient equal the synchronous one-solver run bit for bit (the setters wait
+    for the work already queued on the solver's stream)."""
+    import torch
+    from stride_amd.workers import ShotWorkers
+    plan, ctx = small_case(dim=2, n=100, nabs=12, nt=260, kernel='OT2', grad_vp=True, nrec=24)
+    geom = ctx['kw']['receiver_coordinates']
+    factors = (1.0, 1.04, 0.97, 1.06)
+    plans = [build_plan(ctx['space'], ctx['time'], ctx['wavelet'], (ctx['vp'] * f).astype(np.float32),
+                        source_coordinates=geom[k:k + 1], receiver_coordinates=geom, kernel='OT2', grad_vp=True)
+             for k, f in enumerate(factors)]
+    rng = np.random.default_rng(29)
+    res = [adjoint_source_samples(p, rng.standard_normal((p.nrec, p.nt)).astype(np.float32)) for p in plans]
+    print('workers: %d device-pointer shots with medium changes' % len(plans), flush=True)
+
+    seq = solver_for(plan, save_mode=1)
+    want = []
+    for k, p in enumerate(plans):
+        seq.set_medium(p.vp)
+        seq.set_sources(p.src_idx, p.src_w)
+        rec = seq.forward(p.src_data)
+        want.append((rec, seq.adjoint(res[k], want_vp=True)['grad_vp']))
+    seq.close()
+
+    dev = torch.device('cuda', 0)
+    src_d = [torch.from_numpy(np.ascontiguousarray(p.src_data)).to(dev) for p in plans]
+    res_d = [torch.from_numpy(np.ascontiguousarray(r)).to(dev) for r in res]
+    rec_d = [torch.zeros((p.nt, p.nrec), dtype=torch.float32, device=dev) for p in plans]
+    grad_d = [torch.zeros(plan.shape, dtype=torch.float32, device=dev) for _ in plans]
+    torch.cuda.synchronize()
+
+    def shot(s, k):
+        s.set_medium(plans[k].vp)
+        s.set_sources(plans[k].src_idx, plans[k].src_w)
+        s.forward_dev(src_d[k].data_ptr(), rec_d[k].data_ptr())
+        s.adjoint_dev(res_d[k].data_ptr(), want=1)
+        s.read_gradient_dev(0, grad_d[k].data_ptr())
+
+    with ShotWorkers(lambda: solver_for(plan, save_mode=1), n=2) as pool:
+        pool.map(shot, range(len(plans)))
+    for k in range(len(plans)):
+        assert np.array_equal(rec_d[k].cpu().numpy(), want[k][0]), k
+        assert np.array_equal(grad_d[k].cpu().numpy(), want[k][1]), k

@@ -76,3 +76,31 @@ def test_rejects_bad_arguments(streams):
         workers.ShotWorkers(lambda: _Stub([]), n=0)
     with pytest.raises(TypeError):
         workers.ShotWorkers(lambda: object(), n=1)
+
+
+def test_partial_construction_failure_releases_built_workers(streams):
+    made, destroyed = streams
+    log = []
+    built = []
+
+    def make():
+        if len(built) == 2:
+            raise RuntimeError('out of device memory')
+        built.append(_Stub(log))
+        return built[-1]
+
+    with pytest.raises(RuntimeError, match='out of device memory'):
+        workers.ShotWorkers(make, n=3)
+    assert all(s.closed for s in built) and len(built) == 2
+    assert sorted(destroyed) == sorted(made) and len(made) == 2
+
+    class NotASolver:
+        closed = False
+
+        def close(self):
+            self.closed = True
+
+    stray = NotASolver()
+    with pytest.raises(TypeError):
+        workers.ShotWorkers(lambda: stray, n=2)
+    assert stray.closed
