@@ -217,6 +217,15 @@ int sa_traces_norm(sa_solver *s, float *d, int ntr, int nt, int per_trace, doubl
 /* L2DistanceLoss (stride/optimisation/loss/l2_distance.py:36-66): res = mod - obs,
  * fun = 0.5 sum res^2. */
 int sa_traces_l2(sa_solver *s, const float *mod, const float *obs, float *res, int ntr, int nt, double *fun);
+/* sa_traces_l2 with no device allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The per-row partial sums and their total live in a static
+ * device array of the library (65537 doubles, 512 KiB, one copy per GPU, held until the
+ * process ends), so ntr is at most 65536 (SA_E_ARG beyond). Calls that overlap on one GPU
+ * share that array and race; the array keeps the last call's contents. Same results as
+ * sa_traces_l2, bit for bit. */
+int sa_traces_l2_noalloc(sa_solver *s, const float *mod, const float *obs, float *res, int ntr, int nt,
+                         double *fun);
 /* out[c, r] = in[r, c] x col_scale[c] (col_scale: host, length cols, or null). Converts the
  * solver's time-major receiver traces (nt, nrec) to Traces rows (nrec, nt) (data.py:1423-1430),
  * and back for an adjoint source with the Tukey window of devito.py:863-869 as col_scale. */

@@ -46,7 +46,7 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize', 'sa_batch_create',
            'sa_batch_set_taps', 'sa_batch_forward_dev', 'sa_batch_adjoint_dev', 'sa_batch_get_wavefield',
            'sa_batch_set_tile', 'sa_traces_check', 'sa_traces_mute', 'sa_traces_fir', 'sa_traces_sosfilt',
-           'sa_traces_norm', 'sa_traces_l2', 'sa_field_mask', 'sa_field_gaussian', 'sa_field_stats', 'sa_field_axpb',
+           'sa_traces_norm', 'sa_traces_l2', 'sa_traces_l2_noalloc', 'sa_field_mask', 'sa_field_gaussian', 'sa_field_stats', 'sa_field_axpb',
            'sa_field_div_nonzero', 'sa_field_descent', 'sa_read_gradient_dev',
            'sa_traces_transpose', 'sa_field_crop', 'sa_field_embed', 'sa_field_grad_vp_finish']
 

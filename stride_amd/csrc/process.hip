@@ -9,6 +9,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
@@ -596,6 +598,61 @@ extern "C" int sa_traces_l2(sa_solver *s, const float *mod, const float *obs, fl
     Scratch sc;
     if ((rc = sc.alloc((ntr + 1) * sizeof(double)))) return rc;
     double *rows = (double *)sc.p, *total = rows + ntr;
+    if ((rc = rows_sumsq(s, mod, obs, res, ntr, nt, rows, total))) return rc;
+    hipStream_t st = sa_solver_stream(s);
+    double t = 0.0;
+    PTRY(hipMemcpyAsync(&t, total, sizeof(double), hipMemcpyDeviceToHost, st));
+    PTRY(hipStreamSynchronize(st));
+    if (fun) *fun = 0.5 * t;
+    return SA_OK;
+}
+
+// ---- misfit with no device allocation ------------------------------------------------------
+// sa_traces_l2_noalloc keeps the per-row partial sums and their total in `scratch`, a
+// file-scope device array (one copy on every GPU the library runs on, held until the process
+// ends), instead of a buffer allocated per call. Host code never passes the array's name to a
+// kernel (on the host it is only a stand-in for the symbol): the device address of each GPU's
+// copy is looked up once per device, with that device current, and kept in scratch_dev[].
+constexpr int kNoAllocRows = 1 << 16;  // at most 65536 trace rows: 512 KiB of doubles
+static __device__ __align__(256) double scratch[kNoAllocRows + 1];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double *scratch_dev[kMaxDevices] = {};
+
+// device address of this GPU's copy of `scratch` (device must be current)
+int scratch_for(int device, double **out) {
+    if (device < 0 || device >= kMaxDevices) return sa_fail(SA_E_ARG, "device ordinal out of range for scratch");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return sa_fail(SA_E_HIP, "hipGetSymbolAddress(scratch) failed");
+        scratch_dev[device] = static_cast<double *>(p);
+    }
+    *out = scratch_dev[device];
+    return SA_OK;
+}
+
+}  // namespace
+
+extern "C" int sa_traces_l2_noalloc(sa_solver *s, const float *mod, const float *obs, float *res, int ntr, int nt,
+                                    double *fun) {
+    int rc = prep(s);
+    if (rc) return rc;
+    if (!mod || !obs || !res || ntr < 1 || nt < 1) return sa_fail(SA_E_ARG, "bad misfit arguments");
+    if (ntr > kNoAllocRows) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "sa_traces_l2_noalloc: %d trace rows exceed the maximum of %d", ntr,
+                      kNoAllocRows);
+        return sa_fail(SA_E_ARG, msg);
+    }
+    double *rows = nullptr;
+    if ((rc = scratch_for(sa_solver_device(s), &rows))) return rc;
+    // every row sum and the total are written before they are read: nothing to clear
+    double *total = rows + ntr;
     if ((rc = rows_sumsq(s, mod, obs, res, ntr, nt, rows, total))) return rc;
     hipStream_t st = sa_solver_stream(s);
     double t = 0.0;
