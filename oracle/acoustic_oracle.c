@@ -47,6 +47,30 @@
 
 #define H 10 /* halo cells on each active axis (Devito uses space_order = 10) */
 
+/* Planted mistakes, compiled only into the separate mutation build (-DORACLE_MUTATIONS,
+ * _build/liboracle_mut_f64.so) that the oracle's physics checks load to show that each
+ * check fails when the term it pins is wrong. The regular builds have no such switch. */
+#ifdef ORACLE_MUTATIONS
+static int oracle_mutation = 0;
+void SFX(oracle_set_mutation)(int m) { oracle_mutation = m; }
+#define MUT(k) (oracle_mutation == (k))
+#else
+#define MUT(k) 0
+#endif
+enum {
+    MUT_ATT0_TWICE = 1,    /* attenuation power 0: 2 x the damping coefficient alpha vp */
+    MUT_ATT0_SIGN = 2,     /* attenuation power 0: damping coefficient of the wrong sign */
+    MUT_ATT2_TWICE = 3,    /* attenuation power 2: 2 x Cbeta */
+    MUT_ATT2_SIGN = 4,     /* attenuation power 2: Cbeta of the wrong sign */
+    MUT_SPONGE_TWICE = 5,  /* sponge: 2 x the vp^2 damp p_t coefficient */
+    MUT_SPONGE_SIGN = 6,   /* sponge: vp^2 damp p_t coefficient of the wrong sign */
+    MUT_PML_DSIGMA = 7,    /* CFS-PML: sigma' of the wrong sign in the forward aux updates */
+    MUT_PML_D2 = 8,        /* CFS-PML: 2 x the sigma d2p term of u2 */
+    MUT_PML_ALPHA = 9,     /* CFS-PML: alpha of the wrong sign in (alpha + sigma) */
+    MUT_PML_ADJ_FORM = 10, /* CFS-PML: the adjoint run uses the forward aux form */
+    MUT_PML_U1 = 11,       /* CFS-PML: boundary term sum u1 of the wrong sign */
+};
+
 typedef struct {
     int dim;              /* 2 or 3 */
     int n[3];             /* extended shape */
@@ -278,10 +302,18 @@ static void medium_init(const geom *g, medium *md, const float *vp, const float 
         REAL v2 = v * v;
         REAL m = c->rho_mode == 2 ? v2 * md->rho[p] : v2;
         REAL a = 0, e = 0, W = 1;
-        if (c->att_power == 0) a += dt * md->alpha[p] * v;
+        if (c->att_power == 0) {
+            REAL q = dt * md->alpha[p] * v;
+            if (MUT(MUT_ATT0_TWICE)) q *= 2;
+            if (MUT(MUT_ATT0_SIGN)) q = -q;
+            a += q;
+        }
         if (c->boundary == 0) {
             REAL D = md->damp[p];
-            a += dt * v2 * D;
+            REAL q = dt * v2 * D;
+            if (MUT(MUT_SPONGE_TWICE)) q *= 2;
+            if (MUT(MUT_SPONGE_SIGN)) q = -q;
+            a += q;
             e = dt2 * v2 * D * D;
         }
         if (c->att_power == -1) W = md->alpha[p];
@@ -291,7 +323,11 @@ static void medium_init(const geom *g, medium *md, const float *vp, const float 
         md->B[p] = -W * ((REAL)1 - a) * ia;
         md->Cq[p] = W * dt2 * ia;
         md->G[p] = md->Cq[p] * m;
-        if (c->att_power == 2) md->Cbeta[p] = md->Cq[p] * (REAL)2 * md->alpha[p] * v2 * v / dt;
+        if (c->att_power == 2) {
+            md->Cbeta[p] = md->Cq[p] * (REAL)2 * md->alpha[p] * v2 * v / dt;
+            if (MUT(MUT_ATT2_TWICE)) md->Cbeta[p] *= 2;
+            if (MUT(MUT_ATT2_SIGN)) md->Cbeta[p] = -md->Cbeta[p];
+        }
     }
 }
 
@@ -355,11 +391,14 @@ static void pml_step(const geom *g, const medium *md, pmlaux *px, const REAL *F,
             if (!(x < na || x >= g->n[a] - na)) continue;
             long p = pidx(g, i, j, k);
             REAL s = sg[x], al_ = al[x], ds = dsg[x], da = dal[x];
+            if (MUT(MUT_PML_DSIGMA) && !backward) ds = -ds;
+            if (MUT(MUT_PML_ALPHA)) al_ = -al_;
             REAL dF = d1(g, F, p, a), d2F = d2(g, F, p, a);
             REAL v3, v2, v1;
-            if (!backward) {
+            if (!backward || MUT(MUT_PML_ADJ_FORM)) {
                 v3 = u3[p] - dt * ((al_ + s) * u3[p] - s * s * (ds + da) * dF);
-                v2 = u2[p] - dt * ((al_ + s) * u2[p] - s * ((REAL)2 * ds * dF + da * dF + s * d2F) + v3);
+                v2 = u2[p] - dt * ((al_ + s) * u2[p] - s * ((REAL)2 * ds * dF + da * dF +
+                                                               (MUT(MUT_PML_D2) ? (REAL)2 : (REAL)1) * s * d2F) + v3);
                 v1 = u1[p] - dt * ((al_ + s) * u1[p] - (ds * dF + (REAL)2 * s * d2F) + v2);
             } else {
                 v3 = u3[p] - dt * ((al_ + s) * u3[p] + s * s * (ds + da) * dF);
@@ -414,6 +453,7 @@ static void step(const geom *g, const medium *md, const kops *ko, pmlaux *px, wo
             REAL s1 = 0;
             for (int a = 0; a < 3; ++a)
                 if (g->act[a]) s1 += px->u1[a][p];
+            if (MUT(MUT_PML_U1)) s1 = -s1;
             acc -= md->Cq[p] * md->vp[p] * md->vp[p] * s1;
         }
         out[p] = md->A[p] * u[p] + md->B[p] * other[p] + acc;

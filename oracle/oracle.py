@@ -5,6 +5,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and th
 imports this module.
 """
 
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -28,13 +29,37 @@ def build():
     subprocess.run(['make', '-s', '-C', _HERE], check=True)
 
 
-def _lib(precision):
-    if precision not in _LIBS:
-        path = os.path.join(_HERE, '_build', 'liboracle_%s.so' % precision)
+_PLANTED = [0]
+
+
+def _load(name):
+    if name not in _LIBS:
+        path = os.path.join(_HERE, '_build', 'liboracle_%s.so' % name)
         if not os.path.exists(path):
             build()
-        _LIBS[precision] = ctypes.CDLL(path)
-    return _LIBS[precision]
+        _LIBS[name] = ctypes.CDLL(path)
+    return _LIBS[name]
+
+
+def _lib(precision):
+    if precision == 'f64' and _PLANTED[0]:
+        return _load('mut_f64')
+    return _load(precision)
+
+
+@contextlib.contextmanager
+def planted(code):
+    """FP64 oracle calls inside the block run with one planted mistake (the MUT_* codes of
+    acoustic_oracle.c, from the separate mutation build). Only the oracle's own physics
+    checks use this, to show that each check fails when the term it pins is wrong."""
+    lib = _load('mut_f64')
+    lib.oracle_set_mutation_f64(ctypes.c_int(code))
+    _PLANTED[0] = code
+    try:
+        yield
+    finally:
+        lib.oracle_set_mutation_f64(ctypes.c_int(0))
+        _PLANTED[0] = 0
 
 
 def _ptr(a):
