@@ -191,6 +191,31 @@ int sa_batch_get_wavefield(sa_solver *s, int shot, int which, float *out);
  * chunk 24, waves 4) */
 int sa_batch_set_tile(sa_solver *s, int rows, int chunk, int waves);
 
+/* Many shots of uneven duration and receiver count, forward only. No counterpart in the
+ * reference, which runs one shot per worker call (IsoAcousticDevito.forward,
+ * stride/physics/iso_acoustic/devito.py:515-581) over the shots of an iteration
+ * (stride/__init__.py:117-146, runtime.async_for). All shots share the grid, scheme (the
+ * specialised 3D update: OT2, no density, sponge or no boundary, no attenuation, save mode 0),
+ * medium and boundary; each has its own nt (<= the grid's nt), source and receiver taps.
+ * Shots are paired longest first; one launch per time step advances every pair still running
+ * (its two shots interleaved per cell, vp read once per pair and, for pairs of one tile group,
+ * from the L2), one launch injects and one interpolates for every shot still running. Shot k
+ * runs its own steps 1 .. nt_k - 2 (run_forward with time bounds (0, nt_k)). */
+int sa_multi_create(sa_solver *s, int nshot);
+int sa_multi_set_shot(sa_solver *s, int shot, int nt, int nsrc, int src_ntap, const int64_t *src_index,
+                      const float *src_weight, int nrec, int rec_ntap, const int64_t *rec_index,
+                      const float *rec_weight);
+/* src[k]: device (nt_k, nsrc_k) windowed samples of shot k; rec[k]: device (nt_k, nrec_k) traces.
+ * graph != 0: the whole schedule is captured into one HIP graph and launched once (level clears
+ * stay outside the graph); graph == 0: the same launches are issued one by one. Returns once
+ * the work is queued on the solver's stream. */
+int sa_multi_forward_dev(sa_solver *s, const float *const *src, float *const *rec, int graph);
+/* wavefield `which` (0: last level, 1: the level before) of shot k's pair, shot k's half; the
+ * shorter shot of a pair keeps being advanced until its partner's last step. */
+int sa_multi_get_wavefield(sa_solver *s, int shot, int which, float *out);
+/* capture + instantiation time of the last graph (milliseconds) */
+int sa_multi_graph_ms(sa_solver *s, double *ms);
+
 /* ---- Steps either side of the solver, on device arrays (the solver supplies device and
  * stream). Traces are (ntr, nt) float32 rows; fields are extended-grid float32 arrays in C
  * order. Reductions are deterministic. sa_field_axpb, sa_field_div_nonzero and

@@ -46,7 +46,8 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize', 'sa_batch_create',
            'sa_batch_set_taps', 'sa_batch_forward_dev', 'sa_batch_adjoint_dev', 'sa_batch_get_wavefield',
            'sa_batch_set_tile', 'sa_traces_check', 'sa_traces_mute', 'sa_traces_fir', 'sa_traces_sosfilt',
-           'sa_traces_norm', 'sa_traces_l2', 'sa_traces_l2_noalloc', 'sa_field_mask', 'sa_field_gaussian', 'sa_field_stats', 'sa_field_axpb',
+           'sa_traces_norm', 'sa_traces_l2', 'sa_traces_l2_noalloc', 'sa_multi_create', 'sa_multi_set_shot',
+           'sa_multi_forward_dev', 'sa_multi_get_wavefield', 'sa_multi_graph_ms', 'sa_field_mask', 'sa_field_gaussian', 'sa_field_stats', 'sa_field_axpb',
            'sa_field_div_nonzero', 'sa_field_descent', 'sa_read_gradient_dev',
            'sa_traces_transpose', 'sa_field_crop', 'sa_field_embed', 'sa_field_grad_vp_finish']
 
@@ -276,6 +277,51 @@ class Solver:
         out = np.zeros(self.shape, np.float32)
         self._check(self._lib.sa_batch_get_wavefield(self._h, ctypes.c_int(shot), ctypes.c_int(which), _p(out)))
         return out
+
+    # ---- many shots of uneven duration (forward) ----
+    def multi_create(self, nshot):
+        self._check(self._lib.sa_multi_create(self._h, ctypes.c_int(nshot)))
+        self._multi = [None] * nshot
+
+    def multi_set_shot(self, shot, nt, src_idx, src_w, rec_idx, rec_w):
+        si = np.ascontiguousarray(src_idx, dtype=np.int64)
+        ri = np.ascontiguousarray(rec_idx, dtype=np.int64)
+        sw, rw = _f32(src_w), _f32(rec_w)
+        self._check(self._lib.sa_multi_set_shot(self._h, ctypes.c_int(shot), ctypes.c_int(nt),
+                                                ctypes.c_int(si.shape[0]), ctypes.c_int(si.shape[1]), _p(si), _p(sw),
+                                                ctypes.c_int(ri.shape[0]), ctypes.c_int(ri.shape[1]), _p(ri), _p(rw)))
+        self._multi[shot] = dict(nt=int(nt), nsrc=si.shape[0], nrec=ri.shape[0])
+
+    def multi_forward_dev(self, src_ptrs, rec_ptrs, graph=True):
+        n = len(self._multi)
+        sp = (ctypes.c_void_p * n)(*src_ptrs)
+        rp = (ctypes.c_void_p * n)(*rec_ptrs)
+        self._check(self._lib.sa_multi_forward_dev(self._h, sp, rp, ctypes.c_int(int(graph))))
+
+    def multi_forward(self, srcs, graph=True):
+        """Host convenience: source samples (nt_k, nsrc_k) of every shot -> traces (nt_k, nrec_k)."""
+        ds = [self._to_device(x) for x in srcs]
+        recs = [np.zeros((m['nt'], m['nrec']), np.float32) for m in self._multi]
+        dr = [self.device_alloc(max(r.nbytes, 4)) for r in recs]
+        try:
+            self.multi_forward_dev(ds, dr, graph)
+            self.synchronize()
+            for k, r in enumerate(recs):
+                self.memcpy(r.ctypes.data, dr[k], r.nbytes, 2)
+        finally:
+            for d in ds + dr:
+                self.device_free(d)
+        return recs
+
+    def multi_wavefield(self, shot, which=0):
+        out = np.zeros(self.shape, np.float32)
+        self._check(self._lib.sa_multi_get_wavefield(self._h, ctypes.c_int(shot), ctypes.c_int(which), _p(out)))
+        return out
+
+    def multi_graph_ms(self):
+        ms = ctypes.c_double()
+        self._check(self._lib.sa_multi_graph_ms(self._h, ctypes.byref(ms)))
+        return ms.value
 
     # ---- steps around the solver on device arrays (pointers) ----
     def traces_check(self, ptr, ntr, nt, zero_bad=False):

@@ -9,6 +9,10 @@
 
 #include "kernels.h"
 
+struct PairLevels {
+    float *lev0, *lev1;
+};
+
 struct FastTune {
     int rows;   // rows (axis 1) per thread; the tile is 64 (axis 2) x 4*rows (axis 1)
     int chunk;  // planes (axis 0) marched per workgroup
@@ -49,3 +53,7 @@ int fast3d_step_b2(Fast3D &f, const Geo &g, const float *u, const float *o, floa
 int fast3d_step_adjoint_b2(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, const float *ps,
                            const Box &box, float *grad, float *prec, hipStream_t stream);
 int fast3d_set_tune_b2(Fast3D &f, int rows, int chunk, int waves);
+// Several shot pairs per launch, plain step (the multi-shot schedule): pair k's two level
+// buffers (two interleaved shots per cell each) are pairs[k] (a device table); this step reads
+// u = parity ? lev1 : lev0 and writes the new level over the other one. vp is shared.
+int fast3d_step_multi(Fast3D &f, const Geo &g, const PairLevels *pairs, int npair, int parity, hipStream_t stream);

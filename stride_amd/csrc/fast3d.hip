@@ -44,6 +44,10 @@ struct FastArgs {
     float *prec;
     int ntile_y, ntile_z, chunk;
     int nblocks, per_xcd;
+    // several shot pairs per launch (the multi-shot schedule): per-pair level buffers, active
+    // pairs, which level is u (parity of the step) and the tiles per group of the mapping
+    const PairLevels *pairs;
+    int npair, parity, group;
 };
 
 // MODE 0: plain step, 1: fused save of p.dt2, 2: fused gradient accumulation
@@ -1339,15 +1343,35 @@ __device__ __forceinline__ void marchB(const Geo &g, const FastArgs &a, const Rs
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): no direct-to-LDS load outlives the workgroup
 }
 
-template <int RPT, int NW, int MODE, bool SPONGE, bool ISO>
+// MULTI: one launch advances a.npair shot pairs (the multi-shot schedule). The logical blocks
+// of all pairs are ordered tile group major, pair next, tile within the group last: the blocks
+// of all pairs for one group of a.group tiles are consecutive, so they run close together on
+// one XCD and the pairs after the first find that group's vp lines in the XCD's L2.
+template <int RPT, int NW, int MODE, bool SPONGE, bool ISO, bool MULTI>
 __global__ __launch_bounds__(NW * 64) void k_fast3d_b2(Geo g, FastArgs a) {
     constexpr int TY = NW * RPT;
     using R = Ring2W<TY, NW>;
     constexpr int KI = R::KI;
     SA_RING_SLOTS(3, R::FLOATS)
     const int b = blockIdx.x;
-    const int Lg = (b & 7) * a.per_xcd + (b >> 3);
-    if (Lg >= a.nblocks) return;
+    int Lg = (b & 7) * a.per_xcd + (b >> 3);
+    const float *pu = a.u, *po = a.o;
+    float *pout = a.out;
+    if (MULTI) {
+        if (Lg >= a.nblocks * a.npair) return;
+        const int G = a.group;
+        const int gi = Lg / (G * a.npair);
+        const int gsz = min(G, a.nblocks - gi * G);  // the last group may hold fewer tiles
+        const int r = Lg - gi * G * a.npair;
+        const int pair = r / gsz;
+        Lg = gi * G + (r - pair * gsz);
+        const PairLevels pl = a.pairs[pair];
+        pu = a.parity ? pl.lev1 : pl.lev0;
+        pout = a.parity ? pl.lev0 : pl.lev1;
+        po = pout;
+    } else if (Lg >= a.nblocks) {
+        return;
+    }
     const int tz = Lg % a.ntile_z;
     const int rest = Lg / a.ntile_z;
     const int ty = rest % a.ntile_y;
@@ -1362,10 +1386,10 @@ __global__ __launch_bounds__(NW * 64) void k_fast3d_b2(Geo g, FastArgs a) {
     const unsigned ps1 = (unsigned)g.s1;
     Rsrc rs;
     const long long bytes = g.total * 4;
-    rs.u = make_rsrc(a.u, 2 * bytes);
-    rs.o = make_rsrc(a.o, 2 * bytes);
+    rs.u = make_rsrc(pu, 2 * bytes);
+    rs.o = make_rsrc(po, 2 * bytes);
     rs.vp = make_rsrc(a.vp, bytes);
-    rs.out = make_rsrc(a.out, 2 * bytes);
+    rs.out = make_rsrc(pout, 2 * bytes);
     rs.plane_bytes = (int)(g.s0 * 4);  // of a one-shot field; pair fields use twice the offsets
     rs.box_plane_bytes = a.box.n[1] * a.box.p2 * 4;
     const long long box_bytes = (long long)a.box.n[0] * rs.box_plane_bytes;
@@ -1504,22 +1528,23 @@ int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
     return hipGetLastError() == hipSuccess ? 0 : SA_E_HIP;
 }
 
-template <int RPT, int NW, int MODE>
+template <int RPT, int NW, int MODE, bool MULTI>
 void launch_b2_rpt(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
     bool iso = true;  // the same weights on every axis (equal spacing)
     for (int k = 0; k < 6; ++k) iso = iso && g.w2[1][k] == g.w2[0][k] && g.w2[2][k] == g.w2[0][k];
     if (sponge && iso)
-        k_fast3d_b2<RPT, NW, MODE, true, true><<<grid, NW * 64, 0, st>>>(g, a);
+        k_fast3d_b2<RPT, NW, MODE, true, true, MULTI><<<grid, NW * 64, 0, st>>>(g, a);
     else if (sponge)
-        k_fast3d_b2<RPT, NW, MODE, true, false><<<grid, NW * 64, 0, st>>>(g, a);
+        k_fast3d_b2<RPT, NW, MODE, true, false, MULTI><<<grid, NW * 64, 0, st>>>(g, a);
     else if (iso)
-        k_fast3d_b2<RPT, NW, MODE, false, true><<<grid, NW * 64, 0, st>>>(g, a);
+        k_fast3d_b2<RPT, NW, MODE, false, true, MULTI><<<grid, NW * 64, 0, st>>>(g, a);
     else
-        k_fast3d_b2<RPT, NW, MODE, false, false><<<grid, NW * 64, 0, st>>>(g, a);
+        k_fast3d_b2<RPT, NW, MODE, false, false, MULTI><<<grid, NW * 64, 0, st>>>(g, a);
 }
 
+// npair > 0: the multi-shot form (a.pairs, a.parity set by the caller), MODE 0 only
 template <int MODE>
-int launch_b2(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
+int launch_b2(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st, int npair = 0) {
     if (g.total * 8 >= (1ll << 31)) return SA_E_UNSUPPORTED;  // pair buffers need 32-bit offsets
     const int rows = f.tune2.rows, waves = f.tune2.version;
     const int TY = waves * rows;
@@ -1528,17 +1553,34 @@ int launch_b2(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
     a.chunk = f.tune2.chunk;
     const int nchunk = (g.n[0] + a.chunk - 1) / a.chunk;
     a.nblocks = a.ntile_y * a.ntile_z * nchunk;
-    a.per_xcd = (a.nblocks + 7) / 8;
+    a.npair = npair;
+    a.group = a.ntile_z;  // one row of tiles along axis 2 per group
+    const long long total_blocks = (long long)a.nblocks * std::max(npair, 1);
+    if (total_blocks >= (1ll << 31) / 8) return SA_E_UNSUPPORTED;
+    a.per_xcd = (int)((total_blocks + 7) / 8);
     dim3 grid(a.per_xcd * 8);
     const bool sponge = f.boundary == 0;
-    if (waves == 8 && rows == 1)
-        launch_b2_rpt<1, 8, MODE>(sponge, grid, st, g, a);
-    else if (waves == 8 && rows == 2)
-        launch_b2_rpt<2, 8, MODE>(sponge, grid, st, g, a);
-    else if (waves == 4 && rows == 2)
-        launch_b2_rpt<2, 4, MODE>(sponge, grid, st, g, a);
-    else
+    const bool multi = npair > 0;
+    if (multi && MODE != 0) return SA_E_UNSUPPORTED;
+#define SA_B2(R, W)                                               \
+    if constexpr (MODE == 0) {                                    \
+        if (multi)                                                \
+            launch_b2_rpt<R, W, 0, true>(sponge, grid, st, g, a); \
+        else                                                      \
+            launch_b2_rpt<R, W, 0, false>(sponge, grid, st, g, a); \
+    } else {                                                      \
+        launch_b2_rpt<R, W, MODE, false>(sponge, grid, st, g, a); \
+    }
+    if (waves == 8 && rows == 1) {
+        SA_B2(1, 8)
+    } else if (waves == 8 && rows == 2) {
+        SA_B2(2, 8)
+    } else if (waves == 4 && rows == 2) {
+        SA_B2(2, 4)
+    } else {
         return SA_E_ARG;
+    }
+#undef SA_B2
     return hipGetLastError() == hipSuccess ? 0 : SA_E_HIP;
 }
 
@@ -1682,6 +1724,14 @@ int fast3d_step_adjoint_b2(Fast3D &f, const Geo &g, const float *u, const float 
         return launch_b2<2>(f, g, a, stream);
     }
     return launch_b2<0>(f, g, a, stream);
+}
+
+int fast3d_step_multi(Fast3D &f, const Geo &g, const PairLevels *pairs, int npair, int parity, hipStream_t stream) {
+    if (npair < 1 || !pairs) return SA_E_ARG;
+    FastArgs a = base_args(f, nullptr, nullptr, nullptr);
+    a.pairs = pairs;
+    a.parity = parity & 1;
+    return launch_b2<0>(f, g, a, stream, npair);
 }
 
 int fast3d_set_tune_b2(Fast3D &f, int rows, int chunk, int waves) {

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -320,6 +321,56 @@ __global__ void k_gather2(GatherSet s0, GatherSet s1, const float *__restrict__ 
     q.out[r] = sign * acc;
 }
 
+// the multi-shot schedule: one shot's injection and interpolation, its pair and the half of the
+// pair's interleaved cells it owns. Sets are ordered longest shot first, so the shots still
+// running at a step are a prefix (blockIdx.y).
+struct MInject {
+    int ndest, nsrc;
+    const long long *dest;
+    const int *off, *point;
+    const float *coef;
+    const float *values;  // (nt, nsrc) samples of this shot
+    int pair, half;
+};
+struct MGather {
+    int npt, ntap;
+    const long long *idx;
+    const float *w;
+    float *out;  // (nt, npt) traces of this shot
+    int pair, half;
+};
+
+// the level written at step t (the new one): parity = t % 2 reads lev[parity], writes lev[1 - parity]
+__device__ __forceinline__ float *written_level(const PairLevels &pl, int parity) {
+    return parity ? pl.lev0 : pl.lev1;
+}
+
+__global__ void k_inject_multi(const MInject *__restrict__ sets, const PairLevels *__restrict__ pairs, int t,
+                               int parity) {
+    const MInject q = sets[blockIdx.y];
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= q.ndest) return;
+    const float *v = q.values + (size_t)t * q.nsrc;
+    float acc = 0.f;
+    for (int c = q.off[d]; c < q.off[d + 1]; ++c) acc += q.coef[c] * v[q.point[c]];
+    written_level(pairs[q.pair], parity)[q.dest[d] * 2 + q.half] += acc;
+}
+
+__global__ void k_gather_multi(const MGather *__restrict__ sets, const PairLevels *__restrict__ pairs, int t,
+                               int parity) {
+    const MGather q = sets[blockIdx.y];
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= q.npt) return;
+    const float *field = written_level(pairs[q.pair], parity);
+    float acc = 0.f;
+    for (int k = 0; k < q.ntap; ++k) {
+        const long long p = q.idx[(long long)r * q.ntap + k];
+        if (p < 0) continue;
+        acc += q.w[(long long)r * q.ntap + k] * field[p * 2 + q.half];
+    }
+    q.out[(size_t)t * q.npt + r] = acc;
+}
+
 // per-tap values field / G (exact adjoint: lambda at each source tap); idx padded
 __global__ void k_gather_taps(int nq, const long long *__restrict__ idx, const float *__restrict__ field,
                               const float *__restrict__ G, float *__restrict__ out) {
@@ -544,7 +595,32 @@ struct sa_solver {
     size_t bsaved_bytes = 0;
     int bnsave = 0, bf = 0;
     Box bbox;
+    // many shots of uneven duration (sa_multi_*): shots paired longest first, pair j holding
+    // sorted shots 2j and 2j + 1 in two level buffers; device tables of the pairs' levels and
+    // of the shots' injections and interpolations; a stream the schedule is captured on
+    struct MShot {
+        int nt = 0;
+        bool set = false;
+        Sparse src, rec;
+    };
+    std::vector<MShot> mshot;
+    std::vector<int> morder;  // sorted position -> shot
+    std::vector<float *> mbuf;
+    PairLevels *d_pairs = nullptr;
+    MInject *d_minj = nullptr;
+    MGather *d_mgat = nullptr;
+    hipStream_t mstream = nullptr;
+    hipGraph_t mgraph = nullptr;      // the last captured schedule and its executable form, kept
+    hipGraphExec_t mexec = nullptr;   // until the next schedule or sa_destroy (after a wait)
+    float graph_ms = 0.f;  // capture + instantiation time of the last graph
 };
+
+static void release_graph(sa_solver *s) {
+    if (s->mexec) (void)hipGraphExecDestroy(s->mexec);
+    if (s->mgraph) (void)hipGraphDestroy(s->mgraph);
+    s->mexec = nullptr;
+    s->mgraph = nullptr;
+}
 
 static int dalloc(float **p, size_t n) {
     if (*p) return SA_OK;
@@ -728,6 +804,16 @@ extern "C" int sa_destroy(sa_solver *s) {
         dfree(s->bbuf[k]);
     }
     dfree(s->bsaved);
+    for (auto &m : s->mshot) {
+        free_sparse(m.src);
+        free_sparse(m.rec);
+    }
+    for (float *p : s->mbuf) dfree(p);
+    dfree(s->d_pairs);
+    dfree(s->d_minj);
+    dfree(s->d_mgat);
+    release_graph(s);
+    if (s->mstream) (void)hipStreamDestroy(s->mstream);
     fast3d_free(s->fast);
     for (auto e : s->ev) (void)hipEventDestroy(e);
     delete s;
@@ -2003,6 +2089,172 @@ extern "C" int sa_batch_get_wavefield(sa_solver *s, int shot, int which, float *
             for (int k = 0; k < g.n[2]; ++k)
                 out[((long long)i * g.n[1] + j) * g.n[2] + k] =
                     tmp[2 * (g.base + (long long)i * g.s0 + (long long)j * g.s1 + k) + shot];
+    return SA_OK;
+}
+
+// ---- many shots of uneven duration and receiver count (forward) -----------------------------
+
+extern "C" int sa_multi_create(sa_solver *s, int nshot) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    if (nshot < 1 || nshot > 4096) return fail(SA_E_ARG, "nshot must be between 1 and 4096, got %d", nshot);
+    if (s->total * 8 >= (1ll << 31)) return fail(SA_E_UNSUPPORTED, "grid too large for the shot-pair kernel");
+    TRY(quiesce(s));
+    release_graph(s);
+    for (auto &m : s->mshot) {
+        free_sparse(m.src);
+        free_sparse(m.rec);
+    }
+    for (float *p : s->mbuf) dfree(p);
+    s->mbuf.clear();
+    s->mshot.assign(nshot, sa_solver::MShot());
+    const int npair = (nshot + 1) / 2;
+    s->mbuf.assign(2 * npair, nullptr);
+    for (auto &p : s->mbuf) TRY(dalloc(&p, 2 * s->total));
+    dfree(s->d_pairs);
+    dfree(s->d_minj);
+    dfree(s->d_mgat);
+    s->d_pairs = nullptr;
+    s->d_minj = nullptr;
+    s->d_mgat = nullptr;
+    HIP_TRY(hipMalloc((void **)&s->d_pairs, npair * sizeof(PairLevels)));
+    HIP_TRY(hipMalloc((void **)&s->d_minj, nshot * sizeof(MInject)));
+    HIP_TRY(hipMalloc((void **)&s->d_mgat, nshot * sizeof(MGather)));
+    if (!s->mstream) HIP_TRY(hipStreamCreateWithFlags(&s->mstream, hipStreamNonBlocking));
+    return SA_OK;
+}
+
+extern "C" int sa_multi_set_shot(sa_solver *s, int shot, int nt, int nsrc, int src_ntap, const int64_t *src_index,
+                                 const float *src_weight, int nrec, int rec_ntap, const int64_t *rec_index,
+                                 const float *rec_weight) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    if (shot < 0 || shot >= (int)s->mshot.size()) return fail(SA_E_ARG, "shot %d outside the schedule", shot);
+    if (nt < 3 || nt > s->grid.nt) return fail(SA_E_ARG, "shot nt %d must be within 3 .. %d", nt, s->grid.nt);
+    auto &m = s->mshot[shot];
+    TRY(set_sparse(s, m.src, nsrc, src_ntap, src_index, src_weight));
+    TRY(set_sparse(s, m.rec, nrec, rec_ntap, rec_index, rec_weight));
+    m.nt = nt;
+    m.set = true;
+    return SA_OK;
+}
+
+// src[k]: device (nt_k, nsrc_k) samples of shot k; rec[k]: device (nt_k, nrec_k) traces of shot k.
+// Every shot runs its own steps 1 .. nt_k - 2 (run_forward with time bounds (0, nt_k)). With
+// graph != 0 the whole schedule is captured into one HIP graph and launched once.
+extern "C" int sa_multi_forward_dev(sa_solver *s, const float *const *src, float *const *rec, int graph) {
+    if (!s || !src || !rec) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    const int nshot = (int)s->mshot.size();
+    if (nshot == 0) return fail(SA_E_STATE, "sa_multi_create must be called first");
+    for (int k = 0; k < nshot; ++k)
+        if (!s->mshot[k].set) return fail(SA_E_STATE, "shot %d of the schedule has no taps", k);
+    TRY(prepare(s));
+    if (!use_fast(s) || s->sch.save_mode != 0)
+        return fail(SA_E_UNSUPPORTED, "the multi-shot schedule runs the specialised 3D forward (OT2, no density, "
+                                      "sponge or no boundary, no attenuation, save mode 0)");
+    TRY(quiesce(s));  // the device tables below may still be read by a previous schedule
+    release_graph(s);
+    // longest shot first: the shots (and pairs) still running at any step are a prefix
+    std::vector<int> order(nshot);
+    for (int k = 0; k < nshot; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return s->mshot[a].nt > s->mshot[b].nt; });
+    s->morder = order;
+    const int npair = (nshot + 1) / 2;
+    std::vector<PairLevels> pl(npair);
+    for (int j = 0; j < npair; ++j) pl[j] = PairLevels{s->mbuf[2 * j], s->mbuf[2 * j + 1]};
+    std::vector<MInject> inj(nshot);
+    std::vector<MGather> gat(nshot);
+    int max_dest = 0, max_rec = 0;
+    for (int i = 0; i < nshot; ++i) {
+        auto &m = s->mshot[order[i]];
+        TRY(coef_sparse(s, m.src, 0));
+        inj[i] = MInject{m.src.ndest, m.src.npoint, m.src.d_dest, m.src.d_off, m.src.d_point, m.src.d_coef,
+                         src[order[i]], i / 2, i % 2};
+        gat[i] = MGather{m.rec.npoint, m.rec.ntap, m.rec.d_idx, m.rec.d_w, rec[order[i]], i / 2, i % 2};
+        max_dest = std::max(max_dest, m.src.ndest);
+        max_rec = std::max(max_rec, m.rec.npoint);
+        if (m.rec.npoint > 0)
+            HIP_TRY(hipMemsetAsync(rec[order[i]], 0, sizeof(float) * (size_t)m.nt * m.rec.npoint, s->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));  // coefficients computed, traces cleared
+    HIP_TRY(hipMemcpy(s->d_pairs, pl.data(), pl.size() * sizeof(PairLevels), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->d_minj, inj.data(), inj.size() * sizeof(MInject), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->d_mgat, gat.data(), gat.size() * sizeof(MGather), hipMemcpyHostToDevice));
+    // clear the levels on the solver's stream, outside any graph
+    for (float *p : s->mbuf) HIP_TRY(hipMemsetAsync(p, 0, 2 * s->total * sizeof(float), s->stream));
+    const int tlast = s->mshot[order[0]].nt - 2;
+    hipStream_t st = graph ? s->mstream : s->stream;
+    if (graph) {
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = SA_OK;
+    int nact = nshot;
+    for (int t = 1; t <= tlast && rc == SA_OK; ++t) {
+        while (nact > 0 && s->mshot[order[nact - 1]].nt - 2 < t) --nact;  // shots still running
+        const int pact = (nact + 1) / 2;                                    // their pairs
+        hipEvent_t e0;
+        if (!graph && (rc = timed_begin(s, &e0)) != SA_OK) break;
+        if (fast3d_step_multi(s->fast, s->geo, s->d_pairs, pact, t % 2, st) != 0) {
+            rc = fail(SA_E_HIP, "multi-shot update launch failed");
+            break;
+        }
+        if (!graph && (rc = timed_end(s)) != SA_OK) break;
+        if (max_dest > 0)
+            k_inject_multi<<<dim3((max_dest + 127) / 128, nact), 128, 0, st>>>(s->d_minj, s->d_pairs, t, t % 2);
+        if (max_rec > 0)
+            k_gather_multi<<<dim3((max_rec + 127) / 128, nact), 128, 0, st>>>(s->d_mgat, s->d_pairs, t, t % 2);
+        if (hipGetLastError() != hipSuccess) rc = fail(SA_E_HIP, "multi-shot sparse launch failed");
+    }
+    if (graph) {
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc != SA_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess || !g) return fail(SA_E_HIP, "capture of the multi-shot schedule failed");
+        hipGraphExec_t exec = nullptr;
+        if (hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return fail(SA_E_HIP, "instantiation of the multi-shot graph failed");
+        }
+        s->graph_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        s->mgraph = g;
+        s->mexec = exec;
+        if (hipGraphLaunch(exec, s->stream) != hipSuccess) return fail(SA_E_HIP, "launch of the multi-shot graph failed");
+    }
+    return rc;
+}
+
+// wavefield `which` (0: last level, 1: the one before) of the pair holding shot `shot`, its half;
+// for the shorter shot of a pair the pair ran on to its partner's last step
+extern "C" int sa_multi_get_wavefield(sa_solver *s, int shot, int which, float *out) {
+    if (!s || !out || shot < 0 || shot >= (int)s->mshot.size() || s->morder.empty())
+        return fail(SA_E_ARG, "bad arguments");
+    HIP_TRY(hipSetDevice(s->device));
+    int pos = 0;
+    while (s->morder[pos] != shot) ++pos;
+    const int j = pos / 2, half = pos % 2;
+    const int tlast = s->mshot[s->morder[2 * j]].nt - 2;
+    const int last = (tlast + 1) % 2;
+    const Geo &g = s->geo;
+    std::vector<float> tmp(2 * (size_t)s->total);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(tmp.data(), s->mbuf[2 * j + (which == 0 ? last : 1 - last)], tmp.size() * sizeof(float),
+                      hipMemcpyDeviceToHost));
+    for (int i = 0; i < g.n[0]; ++i)
+        for (int jj = 0; jj < g.n[1]; ++jj)
+            for (int k = 0; k < g.n[2]; ++k)
+                out[((long long)i * g.n[1] + jj) * g.n[2] + k] =
+                    tmp[2 * (g.base + (long long)i * g.s0 + (long long)jj * g.s1 + k) + half];
+    return SA_OK;
+}
+
+// capture + instantiation time of the last graph-launched schedule (milliseconds)
+extern "C" int sa_multi_graph_ms(sa_solver *s, double *ms) {
+    if (!s || !ms) return fail(SA_E_ARG, "null argument");
+    *ms = s->graph_ms;
     return SA_OK;
 }
 
