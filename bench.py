@@ -198,66 +198,80 @@ def bench_pipelines(plan, reps=5):
     return out
 
 
-def bench_schedule(tile=None, num_shots=32):
-    """Batched 3D forward over uneven shots: the 360^3 grid and hemisphere medium, 32 shots whose
-    durations (1500-3000 steps) and receiver counts (256-1024 hemisphere points) come from the
-    seeded schedule. Shots are paired by duration and each pair advances together through the
-    two-shot update (interleaved wavefields, one vp read), the pair running for its longer shot's
-    steps. Counted work: each shot's own forward updates, (nt_i - 2) x grid points."""
+def bench_schedule(tile=None, num_shots=32, timed=False):
+    """Batched 3D forward over uneven shots (the multi-shot schedule): the 360^3 grid and
+    hemisphere medium, 32 shots whose durations (1500-3000 steps) and receiver counts (256-1024
+    hemisphere points) come from the seeded schedule, each with its own time axis. Shots are
+    paired longest first; every step is one update launch over all pairs still running plus one
+    injection and one interpolation launch over all shots still running, and the whole schedule
+    is one HIP graph launch. Counted work: each shot's own forward updates, (nt_i - 2) x grid
+    points. The wall time includes capturing and instantiating the graph (reported apart).
+    timed: also run the same launches one by one with HIP events around every update launch,
+    for the update kernel's time per pair and step."""
     from stride_amd.configs import config3, shot_schedule
     from stride_amd.physics.plan import build_plan
+    from stride_amd.problem import Time
     from stride_amd.utils import ellipsoidal
     from stride_amd._native import Solver
     nts, nrecs = shot_schedule(num_shots)
     nt_max = int(nts.max())
     cfg = config3(scale=1.0, nt=nt_max, nrec=1024)
-    space, time_ = cfg['space'], cfg['time']
+    space, dt = cfg['space'], cfg['time'].step
     c = tuple(l / 2 for l in space.limit)
     plans = []
     for i in range(num_shots):
         rec = ellipsoidal(int(nrecs[i]), (55e-3,) * 3, c, threshold=0.5)
-        wav = cfg['wavelet'].copy()
-        wav[int(nts[i]):] = 0  # the shot ends at its own duration
-        plans.append(build_plan(space, time_, wav[None], cfg['vp'], source_coordinates=rec[(7 * i) % len(rec)][None],
-                                receiver_coordinates=rec, kernel='OT2', save_undersampling=4))
-    order = np.argsort(nts, kind='stable')
-    pairs = [(int(order[2 * k]), int(order[2 * k + 1])) for k in range(num_shots // 2)]
+        plans.append(build_plan(space, Time(start=0., step=dt, num=int(nts[i])), cfg['wavelet'][None, :int(nts[i])],
+                                cfg['vp'], source_coordinates=rec[(7 * i) % len(rec)][None], receiver_coordinates=rec,
+                                kernel='OT2'))
     p0 = plans[0]
-    s = Solver(p0.dim, p0.shape, p0.spacing, p0.absorbing, p0.dt, p0.nt)
+    s = Solver(p0.dim, p0.shape, p0.spacing, p0.absorbing, p0.dt, nt_max)
     s.set_medium(p0.vp)
     s.set_sponge(p0.sponge_tables)
     s.set_scheme(p0.kernel_order, p0.rho_mode, p0.att_power, p0.boundary, p0.diff_source, p0.undersampling, 0,
                  0, nt_max, 0, 0, 0)
-    s.batch_create()
+    order = np.argsort(-nts, kind='stable')
     if tile is None:
         from stride_amd.autotune import autotune_batch
-        tile, _ = autotune_batch(plans[pairs[0][0]], plans[pairs[0][1]])
+        tile, _ = autotune_batch(plans[int(order[0])], plans[int(order[1])])
+    s.multi_create(num_shots)
+    for i, p in enumerate(plans):
+        s.multi_set_shot(i, p.nt, p.src_idx, p.src_w, p.rec_idx, p.rec_w)
     s.batch_set_tile(*tile)
     d_src = [s._to_device(p.src_data) for p in plans]
     d_rec = [s.device_alloc(p.nt * p.nrec * 4) for p in plans]
-    waste = 0
+    s.synchronize()
     t0 = time.perf_counter()
-    for a, b in pairs:
-        nt_pair = int(max(nts[a], nts[b]))
-        waste += 2 * nt_pair - int(nts[a]) - int(nts[b])
-        s.set_scheme(p0.kernel_order, p0.rho_mode, p0.att_power, p0.boundary, p0.diff_source, p0.undersampling, 0,
-                     0, nt_pair, 0, 0, 0)
-        for k, i in enumerate((a, b)):
-            s.batch_set_taps(k, plans[i].src_idx, plans[i].src_w, plans[i].rec_idx, plans[i].rec_w)
-        s.batch_forward_dev([d_src[a], d_src[b]], [d_rec[a], d_rec[b]])
+    s.multi_forward_dev(d_src, d_rec, graph=True)
     s.synchronize()
     wall = time.perf_counter() - t0
+    graph_ms = s.multi_graph_ms()
     n = int(np.prod(p0.shape))
-    updates = int(sum(int(t) - 2 for t in nts)) * n
+    own_steps = int(sum(int(t) - 2 for t in nts))
+    sorted_nt = nts[order]
+    pair_steps = int(sum(int(sorted_nt[2 * j]) - 2 for j in range((num_shots + 1) // 2)))
+    updates = own_steps * n
+    out = dict(metric='Gpts/s batched 3D forward, %d uneven shots' % num_shots, value=round(updates / wall / 1e9, 3),
+               unit='Gpts/s', seconds=round(wall, 3), graph_capture_ms=round(graph_ms, 1), shots=num_shots,
+               pairs=(num_shots + 1) // 2, nt_range=[int(nts.min()), int(nts.max())],
+               nrec_range=[int(nrecs.min()), int(nrecs.max())], point_updates=updates,
+               padded_steps_fraction=round((2 * pair_steps - own_steps) / float(own_steps), 4), tile=list(tile),
+               config={'workload': '3D 320x320x320 inner (360x360x360 extended), dx 0.4 mm, SO10 OT2, sponge 20, '
+                                   'forward only; shots paired longest first, one update launch per step over all '
+                                   'running pairs, one HIP graph for the whole schedule'})
+    if timed:
+        s.timing(True)
+        s.multi_forward_dev(d_src, d_rec, graph=False)
+        s.synchronize()
+        kernel_ms, launches = s.timing()
+        s.timing(False)
+        out['update_kernel_ms'] = round(kernel_ms, 1)
+        out['update_us_per_pair_step'] = round(kernel_ms * 1e3 / pair_steps, 2)
+        out['update_launches'] = launches
     for d in d_src + d_rec:
         s.device_free(d)
     s.close()
-    return dict(metric='Gpts/s batched 3D forward, %d uneven shots' % num_shots, value=round(updates / wall / 1e9, 3),
-                unit='Gpts/s', seconds=round(wall, 3), shots=num_shots, pairs=len(pairs),
-                nt_range=[int(nts.min()), int(nts.max())], nrec_range=[int(nrecs.min()), int(nrecs.max())],
-                point_updates=updates, padded_steps_fraction=round(waste / float(nts.sum()), 4), tile=list(tile),
-                config={'workload': '3D 320x320x320 inner (360x360x360 extended), dx 0.4 mm, SO10 OT2, sponge 20, '
-                                    'forward only, shots paired by duration through the two-shot update'})
+    return out
 
 
 def cpu_baseline_density(cfg, nt=24):
@@ -525,6 +539,8 @@ def main():
     ap.add_argument('--scale', type=float, default=1.0, help='grid scale (1.0 = 320^3 inner)')
     ap.add_argument('--nt', type=int, default=2500)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-schedule', action='store_true',
+                    help='leave out the secondary 32-shot batched-forward line')
     ap.add_argument('--tile', type=str, default=None,
                     help='rows,chunk,version of the 3D update tile (rows,chunk,waves with --shots 2)')
     ap.add_argument('--shots', type=int, default=2, choices=(1, 2),
@@ -583,7 +599,7 @@ def main():
         return
     if args.schedule:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
-        print(json.dumps(bench_schedule(tile)))
+        print(json.dumps(bench_schedule(tile, timed=True)))
         return
     cfg, plan = workload(scale=args.scale, nt=args.nt)
     if args.pipelines:
@@ -700,14 +716,18 @@ def main():
             'algorithmic_bytes_per_launch': int(bytes_per_launch),
         },
     }
+    for p in d_src + d_rec + d_res:
+        s.device_free(p)
+    s.close()
+    if not args.no_schedule:
+        # secondary line: the 32 uneven shots of the batched-forward workload in one graph launch
+        print('[bench] 32-shot schedule', file=sys.stderr, flush=True)
+        out['schedule'] = bench_schedule(tuple(tile) if (tile and shots == 2) else None)
     if not args.no_cpu:
         try:
             out['cpu_baseline'] = cpu_baseline(plan)
         except Exception as exc:  # the baseline is reported, never required
             out['cpu_baseline'] = {'error': repr(exc)}
-    for p in d_src + d_rec + d_res:
-        s.device_free(p)
-    s.close()
     print(json.dumps(out))
 
 

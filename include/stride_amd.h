@@ -198,8 +198,8 @@ int sa_batch_set_tile(sa_solver *s, int rows, int chunk, int waves);
  * specialised 3D update: OT2, no density, sponge or no boundary, no attenuation, save mode 0),
  * medium and boundary; each has its own nt (<= the grid's nt), source and receiver taps.
  * Shots are paired longest first; one launch per time step advances every pair still running
- * (its two shots interleaved per cell, vp read once per pair and, for pairs of one tile group,
- * from the L2), one launch injects and one interpolates for every shot still running. Shot k
+ * (its two shots interleaved per cell, vp read once per pair; the launch's tail is shared by all
+ * its pairs), one launch injects and one interpolates for every shot still running. Shot k
  * runs its own steps 1 .. nt_k - 2 (run_forward with time bounds (0, nt_k)). */
 int sa_multi_create(sa_solver *s, int nshot);
 int sa_multi_set_shot(sa_solver *s, int shot, int nt, int nsrc, int src_ntap, const int64_t *src_index,

@@ -31,6 +31,9 @@ struct Fast3D {
     float dt = 0.f;
     FastTune tune = {2, 24, 7};
     FastTune tune2 = {2, 24, 4};  // two-shot kernel: rows, chunk, waves (4 or 8)
+    // multi-shot mapping, tiles per group: 0 a row of tiles along axis 2, 1 a chunk slab, 2 a whole
+    // pair (pair-major order; the fastest: one launch's tail is shared by all its pairs)
+    int multi_group = 2;
     bool prepared = false;
 };
 
