@@ -282,6 +282,60 @@ int sa_field_grad_vp_finish(sa_solver *s, float *grad, float *prec, const float 
 /* Gradient accumulator `which` (as sa_read_gradient) copied to an unpadded device array. */
 int sa_read_gradient_dev(sa_solver *s, int which, float *dev_out);
 
+/* ---- Learned gradient preconditioner: U-Net kernels (BF16 on the matrix cores, FP32
+ * accumulation). No counterpart in the reference: the network maps the raw vp gradient of an
+ * FWI iteration (get_grad_vp, stride/physics/iso_acoustic/devito.py:1045-1111) to an update
+ * direction that is kept only if the forward solver's data misfit drops (the plain gradient
+ * step of stride/optimisation/optimisers/optimiser.py:242-330 otherwise).
+ * All buffers are device pointers; work is queued on `stream` (a hipStream_t, NULL the null
+ * stream) and the calls return at once, except sa_nn_mse with a loss pointer, which waits.
+ * Activations are NHWC bf16 (N images, H x W pixels, C channels innermost); 3x3 weights are
+ * [ky][kx][co][ci] bf16; biases and gradients fp32. H % 4 == 0, W % 64 == 0, and the channel
+ * counts of the 3x3 convolutions are multiples of 32. */
+
+/* y = act(conv3x3(x, wt) + bias), zero padding; relu 0 / 1; bias may be NULL. amask (NULL or
+ * shaped as x): x elements whose amask is <= 0 read as zero (backward data pass, with amask the
+ * output of the layer whose gradient x is). */
+int sa_nn_conv3x3(void *stream, const void *x, const void *amask, const void *wt, const float *bias, void *y,
+                  int N, int H, int W, int Cin, int Cout, int relu);
+/* wb[ky][kx][ci][co] = wt[2-ky][2-kx][co][ci]: the weights of the backward data pass */
+int sa_nn_flip_weights(void *stream, const void *wt, void *wb, int Cin, int Cout);
+/* dw[ky][kx][co][ci] (+)= sum over pixels of dy[p][co] [ymask[p][co] > 0] x[p + (ky-1, kx-1)][ci];
+ * part: nsplit * 9 * Cin * Cout floats of scratch, nsplit = ceil(N H W / px_per_split),
+ * px_per_split a multiple of 32. */
+int sa_nn_wgrad3x3(void *stream, const void *dy, const void *ymask, const void *x, float *part, float *dw,
+                   int N, int H, int W, int Cin, int Cout, int px_per_split, int accumulate);
+/* db[c] (+)= sum_p dy[p][c] [ymask[p][c] > 0]; part: ceil(npx / px_per_block) * C floats */
+int sa_nn_bias_grad(void *stream, const void *dy, const void *ymask, float *part, float *db, long long npx, int C,
+                    int px_per_block, int accumulate);
+/* 2 x 2 average pool and its backward (dx (+)= dy / 4 on each child) */
+int sa_nn_avgpool(void *stream, const void *x, void *y, int N, int H, int W, int C);
+int sa_nn_avgpool_back(void *stream, const void *dy, void *dx, int N, int H, int W, int C, int accumulate);
+/* nearest x2 upsample of x (N,H,W,C) into channels [coff, coff + C) of y (N,2H,2W,Cy), and its
+ * backward (dx = sum of the four children) */
+int sa_nn_upsample(void *stream, const void *x, void *y, int N, int H, int W, int C, int Cy, int coff);
+int sa_nn_upsample_back(void *stream, const void *dy, void *dx, int N, int H, int W, int C, int Cy, int coff);
+/* dst[p][doff + c] (+)= src[p][soff + c], c < C: concatenation along channels and its split */
+int sa_nn_copy_channels(void *stream, const void *src, int Cs, int soff, void *dst, int Cd, int doff, int C,
+                        long long npx, int accumulate);
+/* x[p][0] = bf16(scale g[p]), other channels zero: the network input from an fp32 image */
+int sa_nn_pack_input(void *stream, const float *g, float scale, void *x, long long npx, int C);
+/* out[p] = sum_c w[c] x[p][c] + b[0] (fp32): the 1 x 1 output projection, and its backward:
+ * dx = dout w masked by x > 0; dw[0..C-1] (+)= sum_p dout x, dw[C] (+)= sum_p dout;
+ * part: ceil(npx / px_per_block) * (C + 1) floats */
+int sa_nn_head(void *stream, const void *x, const float *w, const float *b, float *out, long long npx, int C);
+int sa_nn_head_back(void *stream, const float *dout, const float *w, const void *x, void *dx, float *part, float *dw,
+                    long long npx, int C, int px_per_block, int accumulate);
+/* mean-square loss 0.5 sum (out - target)^2 inv_n (to *loss, a host pointer; NULL: no wait) and
+ * d = (out - target) inv_n (may be NULL); part: ceil(n / 256) + 1 doubles of scratch */
+int sa_nn_mse(void *stream, const float *out, const float *target, float *d, long long n, float inv_n, double *part,
+              double *loss);
+/* Adam step `step` (>= 1) on fp32 masters w with moments m, v; wb (may be NULL) gets bf16(w) */
+int sa_nn_adam(void *stream, float *w, const float *g, float *m, float *v, void *wb, long long n, float lr, float b1,
+               float b2, float eps, int step);
+int sa_nn_to_bf16(void *stream, const float *a, void *b, long long n);
+int sa_nn_from_bf16(void *stream, const void *a, float *b, long long n);
+
 #ifdef __cplusplus
 }
 #endif

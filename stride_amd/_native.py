@@ -47,7 +47,10 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_batch_set_taps', 'sa_batch_forward_dev', 'sa_batch_adjoint_dev', 'sa_batch_get_wavefield',
            'sa_batch_set_tile', 'sa_traces_check', 'sa_traces_mute', 'sa_traces_fir', 'sa_traces_sosfilt',
            'sa_traces_norm', 'sa_traces_l2', 'sa_traces_l2_noalloc', 'sa_multi_create', 'sa_multi_set_shot',
-           'sa_multi_forward_dev', 'sa_multi_get_wavefield', 'sa_multi_graph_ms', 'sa_field_mask', 'sa_field_gaussian', 'sa_field_stats', 'sa_field_axpb',
+           'sa_multi_forward_dev', 'sa_multi_get_wavefield', 'sa_multi_graph_ms', 'sa_nn_conv3x3',
+           'sa_nn_flip_weights', 'sa_nn_wgrad3x3', 'sa_nn_bias_grad', 'sa_nn_avgpool', 'sa_nn_avgpool_back',
+           'sa_nn_upsample', 'sa_nn_upsample_back', 'sa_nn_copy_channels', 'sa_nn_pack_input', 'sa_nn_head',
+           'sa_nn_head_back', 'sa_nn_mse', 'sa_nn_adam', 'sa_nn_to_bf16', 'sa_nn_from_bf16', 'sa_field_mask', 'sa_field_gaussian', 'sa_field_stats', 'sa_field_axpb',
            'sa_field_div_nonzero', 'sa_field_descent', 'sa_read_gradient_dev',
            'sa_traces_transpose', 'sa_field_crop', 'sa_field_embed', 'sa_field_grad_vp_finish']
 
