@@ -274,6 +274,90 @@ def bench_schedule(tile=None, num_shots=32, timed=False):
     return out
 
 
+def unet_flops(N, H, W):
+    """Multiply-add flops (x2) of one forward pass of the preconditioner's U-Net; a training step
+    (forward, backward data, weight gradient) is three times that."""
+    from stride_amd.unet import UNet, LEVEL_CHANNELS
+    c1 = LEVEL_CHANNELS[0]
+    level = {'e1': 0, 'e2': 1, 'e3': 2, 'b': 3, 'd3': 2, 'd2': 1, 'd1': 0}
+    shapes = [('e1a', c1, c1), ('e1', c1, c1), ('e2a', 32, 64), ('e2', 64, 64), ('e3a', 64, 128), ('e3', 128, 128),
+              ('ba', 128, 256), ('b', 256, 256), ('d3a', 384, 128), ('d3', 128, 128), ('d2a', 192, 64),
+              ('d2', 64, 64), ('d1a', 96, 32), ('d1', 32, 32)]
+    total = 0
+    for name, cin, cout in shapes:
+        lv = level[name.rstrip('a')] if name.rstrip('a') in level else level[name]
+        px = N * (H >> lv) * (W >> lv)
+        total += 2 * 9 * cin * cout * px
+    return total
+
+
+def bench_learned(ntrain=400, epochs=30, neval=4, iters_per_freq=4, nsrc=16, log=None):
+    """The learned-preconditioner workload on the 2D gradient grid: solver-generated training
+    pairs, U-Net training, then inversions of unseen phantoms with frequency continuation
+    (300 - 700 kHz), learned (misfit-gated, plain gradient step as fallback) against plain
+    L-BFGS; solver calls counted as passes over all shots (a gradient is two: forward and
+    adjoint)."""
+    import torch
+    from stride_amd import learned
+    log = log or (lambda m: print('[learned] ' + m, file=sys.stderr, flush=True))
+    freqs = [300e3, 400e3, 500e3, 600e3, 700e3]
+    fwi = learned.RingFWI(nsrc=nsrc, workers=2)
+    t0 = time.perf_counter()
+    pairs = learned.make_pairs(fwi, range(ntrain), freqs, seed=0, log=log)
+    t_pairs = time.perf_counter() - t0
+    scales = learned.input_scales(pairs)
+    t0 = time.perf_counter()
+    net, losses = learned.train(pairs, scales, epochs=epochs, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    steps = epochs * (len(pairs) // 4)
+    train_flops = 3 * unet_flops(4, 512, 512) * steps
+    # inference throughput of the network alone
+    g = torch.stack([pairs[j][0] * scales[pairs[j][2]] for j in range(4)])
+    for _ in range(2):
+        net.forward(g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        net.forward(g)
+    torch.cuda.synchronize()
+    t_inf = (time.perf_counter() - t0) / 10
+    del pairs
+    results = []
+    for e in range(neval):
+        k = 100000 + e
+        true = learned.phantom(k)
+        fwi.passes = 0
+        vp_p, hp = learned.invert_plain(fwi, true, freqs, iters_per_freq)
+        log('phantom %d plain: %d passes, misfit %.4g, model error %.4f' % (k, hp[-1][0], hp[-1][1], hp[-1][2]))
+        fwi.passes = 0
+        vp_l, hl, acc = learned.invert_learned(fwi, net, scales, true, freqs, iters_per_freq)
+        log('phantom %d learned: %d passes, misfit %.4g, model error %.4f, %d of %d updates from the network'
+            % (k, hl[-1][0], hl[-1][1], hl[-1][2], acc, len(hl)))
+        last = freqs[-1]
+        plain_final = hp[-1][1]
+        reach = [h[0] for h in hl if h[3] == last and h[1] <= plain_final]
+        results.append(dict(phantom=k, plain_passes=hp[-1][0], plain_misfit=plain_final, plain_error=hp[-1][2],
+                            learned_passes=hl[-1][0], learned_misfit=hl[-1][1], learned_error=hl[-1][2],
+                            learned_passes_to_plain_misfit=reach[0] if reach else None, network_updates=acc,
+                            plain_history=[[h[0], h[1], h[2]] for h in hp],
+                            learned_history=[[h[0], h[1], h[2]] for h in hl]))
+    fwi.close()
+    ratios = [r['learned_passes_to_plain_misfit'] / r['plain_passes'] for r in results
+              if r['learned_passes_to_plain_misfit'] is not None]
+    return dict(metric='FWI misfit drop per solver call, learned preconditioner vs plain L-BFGS',
+                unit='passes over all shots', training_pairs=ntrain, epochs=epochs, sources=nsrc,
+                pair_seconds=round(t_pairs, 1), train_seconds=round(t_train, 1),
+                train_tflops=round(train_flops / t_train / 1e12, 2), final_train_loss=losses[-1],
+                first_train_loss=losses[0], inference_ms_batch4=round(t_inf * 1e3, 2),
+                inference_tflops=round(unet_flops(4, 512, 512) / t_inf / 1e12, 2),
+                eval=results, passes_ratio_to_plain_final=ratios,
+                phantoms_reaching_plain_final=len(ratios), phantoms=neval,
+                config={'workload': '2D 512x512 inner (572x572 extended), dx 0.2 mm, CFS-PML 30, vp only, '
+                                    '%d ring sources / 128 receivers, 2500 steps, 5 bands 300-700 kHz x %d '
+                                    'iterations' % (nsrc, iters_per_freq)})
+
+
 def cpu_baseline_density(cfg, nt=24):
     """The oracle (C/OpenMP, test infrastructure) on the same medium: a bounded forward with raw
     saves + as-written adjoint with vp and rho gradients, timed on this host's cores."""
@@ -563,6 +647,12 @@ def main():
                          "the CPU oracle and exit")
     ap.add_argument('--workers', type=int, default=2,
                     help='with --gradient-2d: solvers running shots concurrently, each on its own stream')
+    ap.add_argument('--learned', action='store_true',
+                    help='only run the learned-preconditioner workload (training pairs from the solver, U-Net '
+                         'training, misfit-gated inversions against plain L-BFGS) and exit')
+    ap.add_argument('--train-pairs', type=int, default=400)
+    ap.add_argument('--epochs', type=int, default=30)
+    ap.add_argument('--eval', type=int, default=4)
     ap.add_argument('--pipelines', action='store_true',
                     help='only time the device steps either side of the solver (trace processing, misfit, '
                          'gradient post-processing, model update) at the full workload and exit')
@@ -587,6 +677,9 @@ def main():
         print(json.dumps({'copy_check_bytes': nbytes, 'copies': 10}))
         return
 
+    if args.learned:
+        print(json.dumps(bench_learned(ntrain=args.train_pairs, epochs=args.epochs, neval=args.eval)))
+        return
     if args.forward_2d:
         print(json.dumps(bench_forward_2d()))
         return

@@ -263,6 +263,9 @@ int sa_field_gaussian(sa_solver *s, float *f, int dim, const int *shape, const d
 /* min, max and sum |f| (NormField, norm_field.py:24-52; apply_prec's normalisation). */
 int sa_field_stats(sa_solver *s, const float *f, int64_t n, double *vmin, double *vmax, double *abssum);
 int sa_field_axpb(sa_solver *s, float *f, int64_t n, double a, double b); /* f = a f + b */
+/* y += a x (the sum of per-shot gradients that Stride's optimisation loop forms,
+ * stride/core.py:565-633); queued on the stream */
+int sa_field_axpy(sa_solver *s, float *y, const float *x, int64_t n, double a);
 int sa_field_div_nonzero(sa_solver *s, float *g, const float *prec, int64_t n); /* g /= prec where != 0 */
 /* GradientDescent step in the variable's transformed space (optimiser.py:242-330) with Clip
  * (clip.py): transform 0 identity, 1 slowness (x -> 1/x). */

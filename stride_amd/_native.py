@@ -51,7 +51,7 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_nn_flip_weights', 'sa_nn_wgrad3x3', 'sa_nn_bias_grad', 'sa_nn_avgpool', 'sa_nn_avgpool_back',
            'sa_nn_upsample', 'sa_nn_upsample_back', 'sa_nn_copy_channels', 'sa_nn_pack_input', 'sa_nn_head',
            'sa_nn_head_back', 'sa_nn_mse', 'sa_nn_adam', 'sa_nn_to_bf16', 'sa_nn_from_bf16', 'sa_field_mask', 'sa_field_gaussian', 'sa_field_stats', 'sa_field_axpb',
-           'sa_field_div_nonzero', 'sa_field_descent', 'sa_read_gradient_dev',
+           'sa_field_div_nonzero', 'sa_field_axpy', 'sa_field_descent', 'sa_read_gradient_dev',
            'sa_traces_transpose', 'sa_field_crop', 'sa_field_embed', 'sa_field_grad_vp_finish']
 
 _lib = None
@@ -393,6 +393,10 @@ class Solver:
     def field_axpb(self, ptr, n, a, b):
         self._check(self._lib.sa_field_axpb(self._h, ctypes.c_void_p(ptr), ctypes.c_int64(n), ctypes.c_double(a),
                                             ctypes.c_double(b)))
+
+    def field_axpy(self, y_ptr, x_ptr, n, a=1.0):
+        self._check(self._lib.sa_field_axpy(self._h, ctypes.c_void_p(y_ptr), ctypes.c_void_p(x_ptr),
+                                            ctypes.c_int64(n), ctypes.c_double(a)))
 
     def field_div_nonzero(self, g_ptr, prec_ptr, n):
         self._check(self._lib.sa_field_div_nonzero(self._h, ctypes.c_void_p(g_ptr), ctypes.c_void_p(prec_ptr),

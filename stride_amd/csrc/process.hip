@@ -448,6 +448,11 @@ __global__ void k_axpb(float *__restrict__ f, long long n, float a, float b) {
     if (p < n) f[p] = a * f[p] + b;
 }
 
+__global__ void k_axpy(float *__restrict__ y, const float *__restrict__ x, long long n, float a) {
+    const long long p = (long long)blockIdx.x * kT + threadIdx.x;
+    if (p < n) y[p] += a * x[p];
+}
+
 // grad[p] *= 1 / prec[p] where prec[p] != 0
 __global__ void k_div_nonzero(float *__restrict__ g, const float *__restrict__ prec, long long n) {
     const long long p = (long long)blockIdx.x * kT + threadIdx.x;
@@ -864,6 +869,15 @@ extern "C" int sa_field_axpb(sa_solver *s, float *f, int64_t n, double a, double
     if (rc) return rc;
     if (!f || n < 0) return sa_fail(SA_E_ARG, "bad field arguments");
     if (n) k_axpb<<<blocks(n), kT, 0, sa_solver_stream(s)>>>(f, n, (float)a, (float)b);
+    PTRY(hipGetLastError());
+    return SA_OK;
+}
+
+extern "C" int sa_field_axpy(sa_solver *s, float *y, const float *x, int64_t n, double a) {
+    int rc = prep(s);
+    if (rc) return rc;
+    if (!y || !x || n < 0) return sa_fail(SA_E_ARG, "bad field arguments");
+    if (n) k_axpy<<<blocks(n), kT, 0, sa_solver_stream(s)>>>(y, x, n, (float)a);
     PTRY(hipGetLastError());
     return SA_OK;
 }
