@@ -291,7 +291,7 @@ def unet_flops(N, H, W):
     return total
 
 
-def bench_learned(ntrain=400, epochs=30, neval=4, iters_per_freq=4, nsrc=16, log=None):
+def bench_learned(ntrain=400, epochs=60, neval=4, iters_per_freq=4, nsrc=16, log=None):
     """The learned-preconditioner workload on the 2D gradient grid: solver-generated training
     pairs, U-Net training, then inversions of unseen phantoms with frequency continuation
     (300 - 700 kHz), learned (misfit-gated, plain gradient step as fallback) against plain
@@ -314,12 +314,13 @@ def bench_learned(ntrain=400, epochs=30, neval=4, iters_per_freq=4, nsrc=16, log
     train_flops = 3 * unet_flops(4, 512, 512) * steps
     # inference throughput of the network alone
     g = torch.stack([pairs[j][0] * scales[pairs[j][2]] for j in range(4)])
+    m = torch.stack([pairs[j][3] for j in range(4)])
     for _ in range(2):
-        net.forward(g)
+        net.forward(g, model=m)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(10):
-        net.forward(g)
+        net.forward(g, model=m)
     torch.cuda.synchronize()
     t_inf = (time.perf_counter() - t0) / 10
     del pairs
@@ -651,7 +652,7 @@ def main():
                     help='only run the learned-preconditioner workload (training pairs from the solver, U-Net '
                          'training, misfit-gated inversions against plain L-BFGS) and exit')
     ap.add_argument('--train-pairs', type=int, default=400)
-    ap.add_argument('--epochs', type=int, default=30)
+    ap.add_argument('--epochs', type=int, default=60)
     ap.add_argument('--eval', type=int, default=4)
     ap.add_argument('--pipelines', action='store_true',
                     help='only time the device steps either side of the solver (trace processing, misfit, '

@@ -124,6 +124,23 @@ int sa_read_gradient(sa_solver *s, int which, float *out);
 int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float *gB, float *gG, float *gm,
                      float *gb, float *lam_src);
 
+/* Device-resident forms for callers that keep the medium, residuals and gradients in HBM (the
+ * torch.autograd path; no counterpart in the reference). sa_set_medium_dev: as sa_set_medium
+ * with device pointers (unpadded extended arrays; the padding and 1/rho run on the device).
+ * sa_adjoint_exact_dev: as sa_adjoint_exact with res, the accumulators (unpadded extended grid)
+ * and lam_src device pointers (any output may be NULL); queued on the solver's stream.
+ * sa_exact_gradients_dev: the chain rule from those accumulators to dJ/dvp, dJ/drho, dJ/dalpha
+ * (alpha_scale: the medium's alpha units per caller unit, e.g. the dB-to-neper factor), by
+ * complex-step derivatives of the step coefficients in double precision on the device; with
+ * lam_src and the source samples src (nt, nsrc) the source taps' term is added to dJ/dvp and
+ * dJ/dsrc (nt, nsrc) is written to g_src. Outputs device pointers, unpadded, any may be NULL. */
+int sa_set_medium_dev(sa_solver *s, const float *vp, const float *rho, const float *alpha);
+int sa_adjoint_exact_dev(sa_solver *s, const float *res, float *gA, float *gB, float *gG, float *gm, float *gb,
+                         float *lam_src);
+int sa_exact_gradients_dev(sa_solver *s, const float *gA, const float *gB, const float *gG, const float *gm,
+                           const float *gb, const float *lam_src, const float *src, float *g_vp, float *g_rho,
+                           float *g_alpha, float *g_src, double alpha_scale);
+
 /* Copy a wavefield level to the host (extended grid). which = 0: last computed level,
  * 1: the level before it. Used for wavefield dumps (dump_forward_wavefield). */
 int sa_get_wavefield(sa_solver *s, int which, float *out);
@@ -321,8 +338,10 @@ int sa_nn_upsample_back(void *stream, const void *dy, void *dx, int N, int H, in
 /* dst[p][doff + c] (+)= src[p][soff + c], c < C: concatenation along channels and its split */
 int sa_nn_copy_channels(void *stream, const void *src, int Cs, int soff, void *dst, int Cd, int doff, int C,
                         long long npx, int accumulate);
-/* x[p][0] = bf16(scale g[p]), other channels zero: the network input from an fp32 image */
-int sa_nn_pack_input(void *stream, const float *g, float scale, void *x, long long npx, int C);
+/* x[p][0] = bf16(s0 g0[p]), x[p][1] = bf16(s1 g1[p]) (zero if g1 is NULL), other channels zero:
+ * the network input from one or two fp32 images */
+int sa_nn_pack_input(void *stream, const float *g0, float s0, const float *g1, float s1, void *x, long long npx,
+                     int C);
 /* out[p] = sum_c w[c] x[p][c] + b[0] (fp32): the 1 x 1 output projection, and its backward:
  * dx = dout w masked by x > 0; dw[0..C-1] (+)= sum_p dout x, dw[C] (+)= sum_p dout;
  * part: ceil(npx / px_per_block) * (C + 1) floats */

@@ -51,7 +51,8 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_nn_flip_weights', 'sa_nn_wgrad3x3', 'sa_nn_bias_grad', 'sa_nn_avgpool', 'sa_nn_avgpool_back',
            'sa_nn_upsample', 'sa_nn_upsample_back', 'sa_nn_copy_channels', 'sa_nn_pack_input', 'sa_nn_head',
            'sa_nn_head_back', 'sa_nn_mse', 'sa_nn_adam', 'sa_nn_to_bf16', 'sa_nn_from_bf16', 'sa_field_mask', 'sa_field_gaussian', 'sa_field_stats', 'sa_field_axpb',
-           'sa_field_div_nonzero', 'sa_field_axpy', 'sa_field_descent', 'sa_read_gradient_dev',
+           'sa_field_div_nonzero', 'sa_field_axpy', 'sa_set_medium_dev', 'sa_adjoint_exact_dev',
+           'sa_exact_gradients_dev', 'sa_field_descent', 'sa_read_gradient_dev',
            'sa_traces_transpose', 'sa_field_crop', 'sa_field_embed', 'sa_field_grad_vp_finish']
 
 _lib = None
@@ -208,6 +209,23 @@ class Solver:
                                                _p(out['gm']), _p(out['gb']), _p(lam)))
         out['lam_src'] = lam.reshape(self.nt, self.nsrc, self.src_ntap)
         return out
+
+    def set_medium_dev(self, vp_ptr, rho_ptr=None, alpha_ptr=None):
+        """Medium from device pointers (unpadded extended arrays)."""
+        vp_ = lambda x: ctypes.c_void_p(x) if x else None
+        self._check(self._lib.sa_set_medium_dev(self._h, vp_(vp_ptr), vp_(rho_ptr), vp_(alpha_ptr)))
+
+    def adjoint_exact_dev(self, res_ptr, gA, gB, gG, gm=None, gb=None, lam_ptr=None):
+        vp_ = lambda x: ctypes.c_void_p(x) if x else None
+        self._check(self._lib.sa_adjoint_exact_dev(self._h, vp_(res_ptr), vp_(gA), vp_(gB), vp_(gG), vp_(gm),
+                                                   vp_(gb), vp_(lam_ptr)))
+
+    def exact_gradients_dev(self, gA, gB, gG, gm=None, gb=None, lam_ptr=None, src_ptr=None, g_vp=None, g_rho=None,
+                            g_alpha=None, g_src=None, alpha_scale=1.0):
+        vp_ = lambda x: ctypes.c_void_p(x) if x else None
+        self._check(self._lib.sa_exact_gradients_dev(self._h, vp_(gA), vp_(gB), vp_(gG), vp_(gm), vp_(gb),
+                                                     vp_(lam_ptr), vp_(src_ptr), vp_(g_vp), vp_(g_rho),
+                                                     vp_(g_alpha), vp_(g_src), ctypes.c_double(alpha_scale)))
 
     def wavefield(self, which=0):
         out = np.zeros(self.shape, np.float32)

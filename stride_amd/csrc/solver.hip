@@ -371,6 +371,134 @@ __global__ void k_gather_multi(const MGather *__restrict__ sets, const PairLevel
     q.out[(size_t)t * q.npt + r] = acc;
 }
 
+// padded copy of an unpadded extended-grid array, edges replicated into the halo (the device
+// form of the host padding in sa_set_medium); lo: leading halo per internal axis, P: padded sizes
+__global__ void k_pad_edge(Geo g, const float *__restrict__ src, float *__restrict__ dst, int lo0, int lo1, int lo2,
+                           int P1, int P2) {
+    const int a2 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a1 = blockIdx.y * blockDim.y + threadIdx.y;
+    const int a0 = blockIdx.z;
+    if (a2 >= P2 || a1 >= P1) return;
+    const int i = min(max(a0 - lo0, 0), g.n[0] - 1);
+    const int j = min(max(a1 - lo1, 0), g.n[1] - 1);
+    const int k = min(max(a2 - lo2, 0), g.n[2] - 1);
+    dst[(long long)a0 * g.s0 + (long long)a1 * g.s1 + a2] = src[((long long)i * g.n[1] + j) * g.n[2] + k];
+}
+
+__global__ void k_reciprocal(const float *__restrict__ x, float *__restrict__ y, long long n) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) y[q] = 1.f / x[q];
+}
+
+// complex arithmetic in double precision for the complex-step derivative of the step
+// coefficients (every operation is analytic, so Im f(x + i h) / h is f'(x) to rounding)
+struct cplx {
+    double r, i;
+};
+__device__ __forceinline__ cplx operator+(cplx a, cplx b) { return {a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ cplx operator-(cplx a, cplx b) { return {a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ cplx operator*(cplx a, cplx b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ cplx operator*(double a, cplx b) { return {a * b.r, a * b.i}; }
+__device__ __forceinline__ cplx inv(cplx a) {
+    const double d = a.r * a.r + a.i * a.i;
+    return {a.r / d, -a.i / d};
+}
+
+// A, B, G, m, b of equation (1) for one point (the same formulas as k_coefficients)
+__device__ void step_coefficients(cplx v, cplx rho, cplx alpha, double prof, int boundary, int att_power,
+                                  int rho_mode, double dt, cplx out[5]) {
+    const cplx one = {1.0, 0.0}, zero = {0.0, 0.0};
+    cplx m = v * v;
+    if (rho_mode == 2) m = m * rho;
+    cplx a = zero, e = zero, W = one;
+    if (att_power == 0) a = a + dt * (alpha * v);
+    if (boundary == 0) {
+        const cplx D = (7.0 * dt * prof) * v;
+        a = a + dt * (v * v * D);
+        e = (dt * dt) * (v * v * D * D);
+    }
+    if (att_power == -1) W = alpha;
+    const cplx ia = inv(one + a);
+    out[0] = W * ((cplx{2.0, 0.0} - e) * ia);
+    out[1] = -1.0 * (W * ((one - a) * ia));
+    out[2] = (dt * dt) * (W * ia * m);
+    out[3] = m;
+    out[4] = rho_mode == 2 ? inv(rho) : zero;
+}
+
+// dJ/dvp, dJ/drho, dJ/dalpha from the sensitivities to A, B, G, m, b (unpadded extended arrays,
+// gm / gb may be null) and the medium (padded); alpha_scale converts the alpha derivative to
+// the caller's units. Any output may be null.
+__global__ void k_exact_chain(Geo g, const float *__restrict__ gA, const float *__restrict__ gB,
+                              const float *__restrict__ gG, const float *__restrict__ gm, const float *__restrict__ gb,
+                              const float *__restrict__ vp, const float *__restrict__ rho,
+                              const float *__restrict__ alpha, const float *__restrict__ tables, int tlen,
+                              int boundary, int att_power, int rho_mode, double dt, double alpha_scale,
+                              float *__restrict__ o_vp, float *__restrict__ o_rho, float *__restrict__ o_alpha) {
+    DOMAIN_INDEX
+    const long long q = ((long long)i * g.n[1] + j) * g.n[2] + k;
+    double prof = 0.0;
+    if (boundary == 0) {  // the sponge layer profile without the velocity factor (k_coefficients' rule)
+        int x[3] = {i, j, k};
+        int dist[3];
+        for (int d = 0; d < 3; ++d) dist[d] = g.act[d] ? min(x[d], g.n[d] - 1 - x[d]) : 0x7fffffff;
+        float sum = 0.f;
+        for (int d = 0; d < 3; ++d) {
+            if (!g.act[d] || g.nabs[d] == 0) continue;
+            int other = 0x7fffffff;
+            for (int e2 = 0; e2 < 3; ++e2)
+                if (e2 != d) other = min(other, dist[e2]);
+            int l0 = x[d], l1 = g.n[d] - 1 - x[d];
+            if (l0 < g.nabs[d] && other >= l0) sum += tables[d * tlen + l0];
+            if (l1 < g.nabs[d] && other >= l1) sum += tables[d * tlen + l1];
+        }
+        prof = sum;
+    }
+    const double h = 1e-30;
+    const double v0 = vp[p], r0 = rho_mode == 2 ? rho[p] : 1.0, a0 = att_power != -2 ? alpha[p] : 0.0;
+    const double w[5] = {gA[q], gB[q], gG[q], gm ? gm[q] : 0.0, gb ? gb[q] : 0.0};
+    cplx c[5];
+    for (int which = 0; which < 3; ++which) {
+        float *out = which == 0 ? o_vp : (which == 1 ? o_rho : o_alpha);
+        if (!out) continue;
+        const cplx v = {v0, which == 0 ? h : 0.0}, r = {r0, which == 1 ? h : 0.0}, al = {a0, which == 2 ? h : 0.0};
+        step_coefficients(v, r, al, prof, boundary, att_power, rho_mode, dt, c);
+        double t = 0.0;
+        for (int n = 0; n < 5; ++n) t += w[n] * (c[n].i / h);
+        out[q] = (float)(which == 2 ? t * alpha_scale : t);
+    }
+}
+
+// source taps: dJ/dvp at each tap cell (the injection coefficient w scale vp[cell]) and
+// dJ/dsrc (nt, nsrc); one thread, taps and time in order (deterministic; taps of different
+// points may share a cell). lam (nt, nsrc, ntap): the adjoint field at each tap; src (nt, nsrc).
+__global__ void k_source_grads(int nt, int nsrc, int ntap, const float *__restrict__ lam,
+                               const float *__restrict__ src, const long long *__restrict__ pidx,
+                               const long long *__restrict__ lin, const float *__restrict__ w,
+                               const float *__restrict__ vp, double scale, float *__restrict__ o_vp,
+                               float *__restrict__ o_src) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int sp = 0; sp < nsrc; ++sp)
+        for (int q = 0; q < ntap; ++q) {
+            const int c = sp * ntap + q;
+            if (pidx[c] < 0) continue;
+            double acc = 0.0;
+            for (int t = 0; t < nt; ++t) acc += (double)lam[((long long)t * nsrc + sp) * ntap + q] * src[(long long)t * nsrc + sp];
+            if (o_vp) o_vp[lin[c]] += (float)(acc * w[c] * scale);
+        }
+    if (!o_src) return;
+    for (int t = 0; t < nt; ++t)
+        for (int sp = 0; sp < nsrc; ++sp) {
+            double acc = 0.0;
+            for (int q = 0; q < ntap; ++q) {
+                const int c = sp * ntap + q;
+                if (pidx[c] < 0) continue;
+                acc += (double)lam[((long long)t * nsrc + sp) * ntap + q] * w[c] * scale * vp[pidx[c]];
+            }
+            o_src[(long long)t * nsrc + sp] = (float)acc;
+        }
+}
+
 // per-tap values field / G (exact adjoint: lambda at each source tap); idx padded
 __global__ void k_gather_taps(int nq, const long long *__restrict__ idx, const float *__restrict__ field,
                               const float *__restrict__ G, float *__restrict__ out) {
@@ -857,6 +985,33 @@ extern "C" int sa_set_medium(sa_solver *s, const float *vp, const float *rho, co
         TRY(upload_padded(s, &s->b, b.data(), true));
     }
     if (alpha) TRY(upload_padded(s, &s->alpha, alpha, true));
+    s->medium_set = true;
+    s->coef_dirty = true;
+    return SA_OK;
+}
+
+static int upload_padded_dev(sa_solver *s, float **dev, const float *src) {
+    TRY(dalloc(dev, s->total));
+    const Geo &g = s->geo;
+    int lo[3];
+    for (int a = 0; a < 3; ++a) lo[a] = g.act[a] ? (a == 2 ? SA_LEAD : SA_HALO) : 0;
+    const dim3 grid((s->P[2] + kBlk.x - 1) / kBlk.x, (s->P[1] + kBlk.y - 1) / kBlk.y, s->P[0]);
+    k_pad_edge<<<grid, kBlk, 0, s->stream>>>(g, src, *dev, lo[0], lo[1], lo[2], s->P[1], s->P[2]);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
+extern "C" int sa_set_medium_dev(sa_solver *s, const float *vp, const float *rho, const float *alpha) {
+    if (!s || !vp) return fail(SA_E_ARG, "vp is required");
+    TRY(quiesce(s));
+    TRY(upload_padded_dev(s, &s->vp, vp));
+    if (rho) {
+        TRY(upload_padded_dev(s, &s->rho, rho));
+        TRY(dalloc(&s->b, s->total));
+        k_reciprocal<<<(unsigned)((s->total + 255) / 256), 256, 0, s->stream>>>(s->rho, s->b, s->total);
+        CHECK_LAUNCH();
+    }
+    if (alpha) TRY(upload_padded_dev(s, &s->alpha, alpha));
     s->medium_set = true;
     s->coef_dirty = true;
     return SA_OK;
@@ -1708,9 +1863,10 @@ extern "C" int sa_adjoint(sa_solver *s, const float *res, float *grad_vp, float 
     return drain_timing(s);
 }
 
-extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float *gB, float *gG, float *gm,
-                                float *gb, float *lam_src) {
-    if (!s || !res) return fail(SA_E_ARG, "null argument");
+// the exact adjoint with the residual res_dev (nt, nrec) and the per-tap field lam_dev
+// (nt, nsrc * ntap) on the device; acc[5] receives the padded accumulators (gA, gB, gG, and gm /
+// gb when the scheme has them), which the caller frees
+static int adjoint_exact_core(sa_solver *s, const float *res_dev, float *lam_dev, float *(&acc)[5]) {
     HIP_TRY(hipSetDevice(s->device));
     TRY(prepare(s));
     const sa_scheme &c = s->sch;
@@ -1722,15 +1878,11 @@ extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float
         return fail(SA_E_STATE, "the exact adjoint needs a forward run with save_mode 3 or 4");
     const bool ckpt = s->saved_mode == 4;
     const int nt = s->grid.nt;
-    size_t nr = (size_t)nt * s->rec.npoint;
     size_t nq = (size_t)s->src.npoint * s->src.ntap;
-    TRY(ensure_staging(s, &s->d_series, &s->series_n, nr));
-    TRY(ensure_staging(s, &s->d_out, &s->out_n, (size_t)nt * nq));
-    if (nr) HIP_TRY(hipMemcpyAsync(s->d_series, res, nr * sizeof(float), hipMemcpyHostToDevice, s->stream));
-    if (nq) HIP_TRY(hipMemsetAsync(s->d_out, 0, (size_t)nt * nq * sizeof(float), s->stream));
+    if (nq) HIP_TRY(hipMemsetAsync(lam_dev, 0, (size_t)nt * nq * sizeof(float), s->stream));
     TRY(coef_sparse(s, s->rec, 2));
     TRY(zero_aux(s));
-    float *acc[5] = {};
+    for (auto &a : acc) a = nullptr;
     TRY(grad_outputs(s, 3, acc));
     if (c.kernel == 4) TRY(grad_outputs(s, 1, acc + 3));
     if (c.rho_mode == 2) {
@@ -1753,7 +1905,7 @@ extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float
         if (lv >= 2) {
             rc = step_general(s, m1, m2, m2, 1, nullptr, 0);  // mu_lv over mu_{lv+2}
             if (rc == SA_OK && lv - 1 >= 1 && lv - 1 <= nt - 2)
-                rc = inject(s, s->rec, s->d_series + (size_t)(lv - 1) * s->rec.npoint, 1.f, m2, nullptr, 0.f, s->box);
+                rc = inject(s, s->rec, res_dev + (size_t)(lv - 1) * s->rec.npoint, 1.f, m2, nullptr, 0.f, s->box);
         }
         int t = lv;
         if (rc != SA_OK || t < 1 || t > nt - 2) continue;
@@ -1776,7 +1928,7 @@ extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float
         }
         if (nq) {
             k_gather_taps<<<((int)nq + 127) / 128, 128, 0, s->stream>>>((int)nq, s->src.d_idx, m1, s->G,
-                                                                         s->d_out + (size_t)t * nq);
+                                                                         lam_dev + (size_t)t * nq);
         }
         k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, ut, s->work0);
         k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, utm, s->work1);
@@ -1806,19 +1958,98 @@ extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float
                                                                   acc[1], acc[2]);
         if (hipGetLastError() != hipSuccess) rc = fail(SA_E_HIP, "launch failed");
     }
+    dfree(KuT);
+    dfree(Kmu);
+    s->cur = 2 % 2;
+    return rc;
+}
+
+extern "C" int sa_adjoint_exact(sa_solver *s, const float *res, float *gA, float *gB, float *gG, float *gm,
+                                float *gb, float *lam_src) {
+    if (!s || !res) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    const int nt = s->grid.nt;
+    size_t nr = (size_t)nt * s->rec.npoint;
+    size_t nq = (size_t)s->src.npoint * s->src.ntap;
+    TRY(ensure_staging(s, &s->d_series, &s->series_n, nr));
+    TRY(ensure_staging(s, &s->d_out, &s->out_n, (size_t)nt * nq));
+    if (nr) HIP_TRY(hipMemcpyAsync(s->d_series, res, nr * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    float *acc[5] = {};
+    int rc = adjoint_exact_core(s, s->d_series, s->d_out, acc);
     float *outs[5] = {gA, gB, gG, gm, gb};
     for (int q = 0; q < 5 && rc == SA_OK; ++q)
         if (outs[q] && acc[q]) rc = download_unpadded(s, outs[q], acc[q]);
     if (rc == SA_OK && lam_src && nq) {
-        if (hipMemcpy(lam_src, s->d_out, (size_t)nt * nq * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        if (hipStreamSynchronize(s->stream) != hipSuccess ||
+            hipMemcpy(lam_src, s->d_out, (size_t)nt * nq * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
             rc = fail(SA_E_HIP, "copy of lam_src failed");
     }
     for (float *p : acc) dfree(p);
-    dfree(KuT);
-    dfree(Kmu);
-    s->cur = 2 % 2;
     if (rc == SA_OK) rc = drain_timing(s);
     return rc;
+}
+
+// device-resident form: res (nt, nrec), lam_src (nt, nsrc * ntap) and the accumulators
+// (unpadded extended grid; any may be NULL) are device pointers; queued on the solver's stream
+extern "C" int sa_adjoint_exact_dev(sa_solver *s, const float *res, float *gA, float *gB, float *gG, float *gm,
+                                    float *gb, float *lam_src) {
+    if (!s || !res) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    const int nt = s->grid.nt;
+    size_t nq = (size_t)s->src.npoint * s->src.ntap;
+    float *lam = lam_src;
+    if (!lam) {
+        TRY(ensure_staging(s, &s->d_out, &s->out_n, (size_t)nt * nq));
+        lam = s->d_out;
+    }
+    float *acc[5] = {};
+    int rc = adjoint_exact_core(s, res, lam, acc);
+    float *outs[5] = {gA, gB, gG, gm, gb};
+    for (int q = 0; q < 5 && rc == SA_OK; ++q)
+        if (outs[q] && acc[q]) {
+            k_unpad<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, acc[q], outs[q]);
+            if (hipGetLastError() != hipSuccess) rc = fail(SA_E_HIP, "launch failed");
+        }
+    for (float *p : acc) dfree(p);  // a device free waits for the queued work
+    return rc;
+}
+
+// The chain rule of the exact adjoint on the device: dJ/dvp, dJ/drho, dJ/dalpha (alpha_scale:
+// nepers per caller unit) from the accumulators of sa_adjoint_exact_dev (unpadded; gm / gb may be
+// NULL) and the solver's medium, boundary and source taps; with lam_src and src (device,
+// (nt, nsrc * ntap) and (nt, nsrc)) the source taps' term is added to dJ/dvp and dJ/dsrc is
+// written to g_src (nt, nsrc). Outputs unpadded extended grid, device; any may be NULL.
+extern "C" int sa_exact_gradients_dev(sa_solver *s, const float *gA, const float *gB, const float *gG,
+                                      const float *gm, const float *gb, const float *lam_src, const float *src,
+                                      float *g_vp, float *g_rho, float *g_alpha, float *g_src, double alpha_scale) {
+    if (!s || !gA || !gB || !gG) return fail(SA_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(s->device));
+    TRY(prepare(s));
+    const sa_scheme &c = s->sch;
+    if (g_rho && c.rho_mode != 2) return fail(SA_E_ARG, "a density gradient needs variable density");
+    if (g_alpha && c.att_power == -2) return fail(SA_E_ARG, "an alpha gradient needs attenuation");
+    k_exact_chain<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(
+        s->geo, gA, gB, gG, gm, gb, s->vp, s->rho, s->alpha, s->tables, s->tlen, c.boundary, c.att_power,
+        c.rho_mode, (double)s->grid.dt, alpha_scale, g_vp, g_rho, g_alpha);
+    CHECK_LAUNCH();
+    if (lam_src && src && s->src.npoint > 0) {
+        const double scale = c.diff_source ? 2.0 * s->grid.dt * s->grid.dt / s->hmax : 2.0 * s->grid.dt / s->hmax;
+        // unpadded linear index of each tap cell (the outputs' layout)
+        std::vector<long long> lin(s->src.idx.begin(), s->src.idx.end());
+        long long *d_lin = nullptr;
+        HIP_TRY(hipMalloc((void **)&d_lin, std::max<size_t>(lin.size(), 1) * sizeof(long long)));
+        int rc = SA_OK;
+        if (hipMemcpy(d_lin, lin.data(), lin.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(SA_E_HIP, "copy of the tap indices failed");
+        if (rc == SA_OK) {
+            k_source_grads<<<1, 64, 0, s->stream>>>(s->grid.nt, s->src.npoint, s->src.ntap, lam_src, src, s->src.d_idx,
+                                                     d_lin, s->src.d_w, s->vp, scale, g_vp, g_src);
+            if (hipGetLastError() != hipSuccess) rc = fail(SA_E_HIP, "launch failed");
+        }
+        dfree(d_lin);  // waits for the launch
+        return rc;
+    }
+    return SA_OK;
 }
 
 extern "C" int sa_get_wavefield(sa_solver *s, int which, float *out) {

@@ -349,15 +349,19 @@ __global__ void k_copy_channels(const bf16 *__restrict__ src, int Cs, int soff, 
 }
 
 // ---- network input and output --------------------------------------------------------------
-// x[p][0] = bf16(scale g[p]), channels 1 .. C-1 zero
-__global__ void k_pack_input(const float *__restrict__ g, float scale, bf16 *__restrict__ x, long long npx, int C) {
+// x[p][0] = bf16(s0 g0[p]), x[p][1] = bf16(s1 g1[p]) (zero without g1), channels 2 .. C-1 zero
+__global__ void k_pack_input(const float *__restrict__ g0, float s0, const float *__restrict__ g1, float s1,
+                             bf16 *__restrict__ x, long long npx, int C) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int C8 = C / 8;
     if (idx >= npx * C8) return;
     const int c = (int)(idx % C8) * 8;
     const long long p = idx / C8;
     bf16x8 v = zero8();
-    if (c == 0) v[0] = (bf16)(scale * g[p]);
+    if (c == 0) {
+        v[0] = (bf16)(s0 * g0[p]);
+        if (g1) v[1] = (bf16)(s1 * g1[p]);
+    }
     *(bf16x8 *)(x + (size_t)p * C + c) = v;
 }
 
@@ -582,9 +586,10 @@ extern "C" int sa_nn_copy_channels(void *stream, const void *src, int Cs, int so
     return SA_OK;
 }
 
-extern "C" int sa_nn_pack_input(void *stream, const float *g, float scale, void *x, long long npx, int C) {
-    if (C % 8) return sa_fail(SA_E_ARG, "U-Net input: C % 8 == 0");
-    k_pack_input<<<nblk(npx * (C / 8)), kT, 0, as_stream(stream)>>>(g, scale, (bf16 *)x, npx, C);
+extern "C" int sa_nn_pack_input(void *stream, const float *g0, float s0, const float *g1, float s1, void *x,
+                                long long npx, int C) {
+    if (C % 8 || !g0 || !x) return sa_fail(SA_E_ARG, "U-Net input: C % 8 == 0 and g0, x given");
+    k_pack_input<<<nblk(npx * (C / 8)), kT, 0, as_stream(stream)>>>(g0, s0, g1, s1, (bf16 *)x, npx, C);
     NTRY(hipGetLastError());
     return SA_OK;
 }

@@ -202,7 +202,7 @@ def partial_model(true, rng, smooth_cells=10.0):
 
 def make_pairs(fwi, ks, freqs, seed=0, log=None):
     """Solver-generated pairs for phantoms ks: (gradient (device), true perturbation / 150 m/s
-    (device), frequency)."""
+    (device), frequency, the model the gradient was taken at, (m - 1500) / 150 m/s (device))."""
     torch = _torch()
     rng = np.random.default_rng(seed)
     pairs = []
@@ -214,7 +214,8 @@ def make_pairs(fwi, ks, freqs, seed=0, log=None):
         fwi.observe(true, freq)
         _, g = fwi.gradient(base, freq)
         target = torch.from_numpy((true - base) / TARGET_SCALE).to(fwi.dev)
-        pairs.append((g, target, freq))
+        model = torch.from_numpy((base - 1500.) / TARGET_SCALE).to(fwi.dev)
+        pairs.append((g, target, freq, model))
         if log and (i + 1) % 25 == 0:
             log('pairs %d / %d (%.1f s)' % (i + 1, len(ks), time.perf_counter() - t0))
     return pairs
@@ -225,12 +226,14 @@ def input_scales(pairs):
     normalisation, so the network sees the gradient's amplitude)."""
     out = {}
     for f in sorted(set(p[2] for p in pairs)):
-        m = [float(g.abs().max()) for g, _, ff in pairs if ff == f]
+        m = [float(p[0].abs().max()) for p in pairs if p[2] == f]
         out[f] = 1.0 / max(float(np.median(m)), 1e-30)
     return out
 
 
-def train(pairs, scales, epochs=30, batch=4, lr=1e-3, seed=0, device=0, log=None):
+def train(pairs, scales, epochs=60, batch=4, lr=1e-3, seed=0, device=0, log=None):
+    """Adam with a cosine-decayed rate on the mean-square error of the predicted perturbation; the
+    network sees the scaled gradient and the model it was taken at."""
     torch = _torch()
     net = UNet(device=device, seed=seed)
     rng = np.random.default_rng(seed)
@@ -244,8 +247,9 @@ def train(pairs, scales, epochs=30, batch=4, lr=1e-3, seed=0, device=0, log=None
             idx = order[i:i + batch]
             g = torch.stack([pairs[j][0] * scales[pairs[j][2]] for j in idx])
             y = torch.stack([pairs[j][1] for j in idx])
+            m = torch.stack([pairs[j][3] for j in idx])
             rate = lr * 0.5 * (1 + np.cos(np.pi * ep / epochs))   # cosine decay
-            ep_loss.append(net.train_step(g, y, scale=1.0, lr=rate))
+            ep_loss.append(net.train_step(g, y, scale=1.0, lr=rate, model=m))
         losses.append(float(np.mean(ep_loss)))
         if log:
             log('epoch %d loss %.4g (%.1f s)' % (ep, losses[-1], time.perf_counter() - t0))
@@ -290,10 +294,12 @@ def _gradient_step(vp, g, step):
     return np.clip(1 / (1 / vp.astype(np.float64) - step * g), *VP_BOUNDS).astype(np.float32)
 
 
-def invert_learned(fwi, net, scales, true, freqs, iters_per_freq=4, first_step=20.0, max_backtracks=3):
-    """Each iteration: the gradient at the current model, the network's update proposed and kept
-    only if the misfit drops; otherwise the plain gradient step in slowness (vp changed by at most
-    about first_step m/s, halved until the misfit drops). Returns (vp, history of (solver passes,
+def invert_learned(fwi, net, scales, true, freqs, iters_per_freq=4, first_step=20.0, max_backtracks=3,
+                   proposal_scales=(1.0, 0.5)):
+    """Each iteration: the gradient at the current model and the network's update d from it and
+    the model; m + s d is kept for the first s of proposal_scales whose misfit is lower than the
+    current one; if none is, the plain gradient step in slowness (vp changed by at most about
+    first_step m/s, halved until the misfit drops). Returns (vp, history of (solver passes,
     misfit, model error, band), number of network updates kept)."""
     torch = _torch()
     vp = np.full(fwi.space.shape, 1500., np.float32)
@@ -305,13 +311,18 @@ def invert_learned(fwi, net, scales, true, freqs, iters_per_freq=4, first_step=2
         for _ in range(iters_per_freq):
             f, g = fwi.gradient(vp, freq)
             scale = scales[min(scales, key=lambda k: abs(k - freq))]
-            d = net.forward((g * scale)[None], scale=1.0)[0] * TARGET_SCALE
-            prop = np.clip(vp + d.cpu().numpy(), *VP_BOUNDS).astype(np.float32)
-            fp = fwi.misfit(prop, freq)
-            if fp < f:
-                vp, f = prop, fp
-                accepted += 1
-            else:
+            model = torch.from_numpy((vp - 1500.) / TARGET_SCALE).to(g.device)
+            d = (net.forward((g * scale)[None], scale=1.0, model=model[None])[0] * TARGET_SCALE).cpu().numpy()
+            kept = False
+            for ps in proposal_scales:
+                prop = np.clip(vp + ps * d, *VP_BOUNDS).astype(np.float32)
+                fp = fwi.misfit(prop, freq)
+                if fp < f:
+                    vp, f = prop, fp
+                    accepted += 1
+                    kept = True
+                    break
+            if not kept:
                 gn = g.double().cpu().numpy()
                 step = first_step / 1500. ** 2 / max(np.abs(gn).max(), 1e-30)
                 for _ in range(max_backtracks):

@@ -6,8 +6,9 @@ x2 upsampling on the way up, the encoder's output of the same level concatenated
 upsampled channels, two 3x3 convolutions with ReLU per level, and a 1 x 1 projection to one
 output channel. Weights and activations are bf16 and every product accumulates in fp32 on the
 matrix cores (``sa_nn_conv3x3`` / ``sa_nn_wgrad3x3``, v_mfma_f32_16x16x32_bf16); fp32 master
-weights, Adam moments and the loss stay fp32. The one input channel is padded to 32 zero
-channels so the first convolution runs on the same kernel.
+weights, Adam moments and the loss stay fp32. The input has two channels (the gradient and,
+optionally, the current model), padded to 32 with zero channels so the first convolution runs
+on the same kernel.
 
 The network has no counterpart in the reference. It maps the raw vp gradient of an FWI
 iteration (``get_grad_vp``, stride/physics/iso_acoustic/devito.py:1045-1111) to a model update;
@@ -73,10 +74,10 @@ class UNet:
                       ('d2a', c2 + c3, c2), ('d2', c2, c2), ('d1a', c1 + c2, c1), ('d1', c1, c1)]
         self.params = {}
         for name, cin, cout in self.convs:
-            fan_in = 9 * (1 if name == 'e1a' else cin)          # the padded input has one live channel
+            fan_in = 9 * (2 if name == 'e1a' else cin)          # the padded input has two live channels
             w = rng.standard_normal((3, 3, cout, cin)) * np.sqrt(2.0 / fan_in)
             if name == 'e1a':
-                w[:, :, :, 1:] = 0
+                w[:, :, :, 2:] = 0
             self.params[name + '.w'] = _Param(w, self.device, True)
             self.params[name + '.b'] = _Param(np.zeros(cout), self.device, False)
         self.params['head.w'] = _Param(rng.standard_normal(c1) * np.sqrt(1.0 / c1) * 0.1, self.device, False)
@@ -114,17 +115,19 @@ class UNet:
         return y
 
     # ---- forward ------------------------------------------------------------------------------
-    def forward(self, g, scale=1.0):
-        """g: (N, H, W) fp32 device tensor -> (N, H, W) fp32 output."""
+    def forward(self, g, scale=1.0, model=None, model_scale=1.0):
+        """g (and the optional second input channel `model`): (N, H, W) fp32 device tensors ->
+        (N, H, W) fp32 output."""
         torch = _torch()
         N, H, W = g.shape
         if H % 32 or W % 512:
             raise ValueError('U-Net input must be (N, H, W) with H % 32 == 0 and W % 512 == 0')
         c1, c2, c3, c4 = LEVEL_CHANNELS
         g = g.contiguous().to(torch.float32)
+        m = None if model is None else model.contiguous().to(torch.float32)
         x0 = self._empty(N, H, W, c1)
-        _check(self.lib.sa_nn_pack_input(self._stream(), _p(g), ctypes.c_float(scale), _p(x0),
-                                         ctypes.c_longlong(N * H * W), c1))
+        _check(self.lib.sa_nn_pack_input(self._stream(), _p(g), ctypes.c_float(scale), _p(m),
+                                         ctypes.c_float(model_scale), _p(x0), ctypes.c_longlong(N * H * W), c1))
         a = dict(x0=x0)
         a['e1a'] = self._conv(x0, 'e1a', N, H, W)
         a['e1'] = self._conv(a['e1a'], 'e1', N, H, W)
@@ -254,8 +257,8 @@ class UNet:
         # the padded input channels of the first layer stay zero
         # (their gradient is zero: the padded input channels are zero)
 
-    def train_step(self, g, target, scale=1.0, lr=1e-3):
-        out = self.forward(g, scale)
+    def train_step(self, g, target, scale=1.0, lr=1e-3, model=None, model_scale=1.0):
+        out = self.forward(g, scale, model, model_scale)
         loss, d = self.mse(out, target)
         self.backward(d)
         self.adam_step(lr)

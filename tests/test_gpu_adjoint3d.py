@@ -1,0 +1,100 @@
+"""The exact adjoint on the 3D workload grid (360^3 extended) with the variable vp / rho /
+alpha medium of the 3D gradient workload (attenuation power 0), on a short run:
+
+- checkpointed recomputation (save mode 4, K = 8) gives the stored-levels result (save mode 3);
+- the vp, rho and alpha gradients match the oracle's exact adjoint (1e-4);
+- the alpha and vp gradients match central finite differences of J = <r, F(m)> run on the GPU.
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import solver_for, rel_l2
+from stride_amd.physics.gradients import exact_gradients
+from stride_amd.physics.plan import build_plan, adjoint_source_samples
+
+pytestmark = pytest.mark.gpu
+
+NT = 40
+
+
+def _medium():
+    from stride_amd.configs import config4
+    from stride_amd.problem import Time
+    from stride_amd.utils import ellipsoidal, tone_burst
+    cfg = config4(scale=1.0, nt=200)
+    space, dt = cfg['space'], cfg['time'].step
+    cen = np.array([l / 2 for l in space.limit])
+    src = cen + 0.8 * (cfg['sources'][0] - cen)
+    rec = ellipsoidal(64, (3e-3,) * 3, tuple(src))
+    wav = tone_burst(400e3, 3, 200, dt)[:NT][None]
+    kw = dict(source_coordinates=src[None], receiver_coordinates=rec, kernel='OT2', grad_vp=True)
+    return cfg, Time(start=0., step=dt, num=NT), wav, kw, src
+
+
+def _plan(cfg, time, wav, kw, vp=None, alpha=None):
+    return build_plan(cfg['space'], time, wav, cfg['vp'] if vp is None else vp, rho=cfg['rho'],
+                      alpha=cfg['alpha'] if alpha is None else alpha, **kw)
+
+
+def _exact(plan, res, save_mode, K=1):
+    s = solver_for(plan, save_mode=save_mode)
+    s.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source, K, save_mode,
+                 plan.tb0, plan.tb1, 1, 0, 0)
+    rec = s.forward(plan.src_data)
+    acc = s.adjoint_exact(res, want_b=True)
+    s.close()
+    return rec, acc
+
+
+def test_checkpointed_exact_adjoint_on_workload_grid(gpu):
+    cfg, time, wav, kw, _ = _medium()
+    plan = _plan(cfg, time, wav, kw)
+    print('3D exact adjoint %s nt %d: stored vs checkpointed vs oracle' % (plan.shape, NT), flush=True)
+    rng = np.random.default_rng(4)
+    res = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, NT)).astype(np.float32))
+    rec3, a3 = _exact(plan, res, 3)
+    rec4, a4 = _exact(plan, res, 4, K=8)
+    assert np.array_equal(rec3, rec4)
+    for key in ('gA', 'gB', 'gG', 'gb'):
+        assert np.abs(a3[key]).max() > 0
+        assert rel_l2(a4[key], a3[key]) < 1e-6, key
+    assert rel_l2(a4['lam_src'], a3['lam_src']) < 1e-6
+    g_got = exact_gradients(plan, a3, a3['lam_src'])
+    states = oracle.forward_states(plan)
+    ref = oracle.adjoint_exact(plan, res, states)
+    del states
+    g_ref = exact_gradients(plan, ref, ref['lam_src'])
+    for key in ('vp', 'rho', 'alpha'):
+        err = rel_l2(g_got[key], g_ref[key])
+        print('  d/d%s rel L2 %.3e' % (key, err), flush=True)
+        assert err < 1e-4, key
+
+
+def test_alpha_and_vp_gradients_finite_difference_on_workload_grid(gpu):
+    cfg, time, wav, kw, src = _medium()
+    plan = _plan(cfg, time, wav, kw)
+    rng = np.random.default_rng(6)
+    res = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, NT)).astype(np.float32))
+    _, acc = _exact(plan, res, 4, K=8)
+    g = exact_gradients(plan, acc, acc['lam_src'])
+    # a perturbation around the source, where the 40-step wavefield lives
+    space = cfg['space']
+    c = [int(round(x / h)) + e for x, h, e in zip(src, space.spacing, space.extra)]
+    box = tuple(slice(ci - 8, ci + 9) for ci in c)
+    d = np.zeros(plan.shape, np.float32)
+    d[box] = rng.standard_normal(d[box].shape).astype(np.float32)
+
+    def J(**medium):
+        p = _plan(cfg, time, wav, kw, **medium)
+        s = solver_for(p)
+        out = float(np.sum(s.forward(p.src_data).astype(np.float64) * res))
+        s.close()
+        return out
+
+    for name, eps, base in (('alpha', 0.05, cfg['alpha']), ('vp', 4.0, cfg['vp'])):
+        fd = (J(**{name: base + eps * d}) - J(**{name: base - eps * d})) / (2 * eps)
+        an = float(np.sum(g[name].astype(np.float64) * d))
+        print('  %s: FD %.6e exact adjoint %.6e' % (name, fd, an), flush=True)
+        assert abs(fd) > 0 and abs(fd - an) <= 2e-3 * abs(fd), name

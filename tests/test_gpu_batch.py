@@ -120,3 +120,28 @@ def test_two_shots_anisotropic_spacing(gpu):
     for key in ('grad_vp', 'prec_vp'):
         want = single[0][1][key].astype(np.float64) + single[1][1][key]
         assert rel_l2(g[key], want) < 1e-5, key
+
+
+def test_two_shots_without_boundary(gpu):
+    """No absorbing boundary (the zero halo): the two-shot update's plain branch on every tile,
+    traces and summed gradient against one-shot runs and the oracle."""
+    plan0, plan1 = _two_shots(33, 6)
+    for p in (plan0, plan1):
+        p.boundary = -1
+    print('batch without boundary', flush=True)
+    rng = np.random.default_rng(13)
+    res = [adjoint_source_samples(p, rng.standard_normal((p.nrec, p.nt))) for p in (plan0, plan1)]
+    single = []
+    for p, r in zip((plan0, plan1), res):
+        s = solver_for(p, save_mode=1)
+        rec = s.forward(p.src_data)
+        single.append((rec, s.adjoint(r, want_vp=True)))
+        s.close()
+    recs, g = _batched(plan0, plan1, (2, 12, 4), res)
+    for k, p in enumerate((plan0, plan1)):
+        assert rel_l2(recs[k], single[k][0]) < 1e-6, k
+        ref, _, _ = oracle.forward(p, save=False)
+        assert rel_l2(recs[k].T, ref) < 1e-5, k
+    for key in ('grad_vp', 'prec_vp'):
+        want = single[0][1][key].astype(np.float64) + single[1][1][key]
+        assert rel_l2(g[key], want) < 1e-5, key

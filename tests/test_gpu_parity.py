@@ -73,6 +73,11 @@ def test_forward_traces(gpu, case):
 
 COMPAT_CASES = [
     (2, dict(kernel='OT2')),
+    # time_bounds (tb0, tb1) inside the run: saves on tb0 <= t <= tb1, the adjoint from tb1-1-f
+    # down to tb0+1, the residual windowed over [tb0, tb1) (devito.py:577-581, 865-866, 916-920)
+    (2, dict(kernel='OT2', time_bounds=(30, 180))),
+    (2, dict(kernel='OT2', rho='var', grad_rho=True, time_bounds=(25, 200))),
+    (3, dict(kernel='OT2', time_bounds=(20, 100))),
     (2, dict(kernel='OT4', interpolation_type='hicks')),
     (2, dict(kernel='OT2', rho='var', grad_rho=True)),
     (2, dict(kernel='OT2', rho='var', alpha='p0', boundary_type='complex_frequency_shift_PML_2')),

@@ -172,7 +172,7 @@ def test_pool_upsample_copy_head_loss(gpu):
     assert rel(d.double().cpu(), r / n) < 1e-6
 
 
-def _ref_network(net, g, scale):
+def _ref_network(net, g, scale, model=None, model_scale=1.0):
     """The same network in float64 on the CPU from the network's parameters (bf16-rounded
     weights, activations rounded to bf16 where the kernels round)."""
     torch = _t()
@@ -191,6 +191,8 @@ def _ref_network(net, g, scale):
     N, H, W = g.shape
     x0 = torch.zeros((N, H, W, 32), dtype=torch.float64)
     x0[..., 0] = bf(scale * g)
+    if model is not None:
+        x0[..., 1] = bf(model_scale * model)
     e1 = conv(conv(x0, 'e1a'), 'e1')
     e2 = conv(conv(pool(e1), 'e2a'), 'e2')
     e3 = conv(conv(pool(e2), 'e3a'), 'e3')
@@ -211,17 +213,18 @@ def test_unet_forward_backward_against_reference(gpu):
     g = torch.tensor(rng.standard_normal((N, H, W)), dtype=torch.float32)
     target = torch.tensor(rng.standard_normal((N, H, W)), dtype=torch.float32) * 0.1
     print('unet forward/backward %s' % ((N, H, W),), flush=True)
-    out = net.forward(g.cuda(), scale=0.5)
+    model = torch.tensor(rng.standard_normal((N, H, W)), dtype=torch.float32)
+    out = net.forward(g.cuda(), scale=0.5, model=model.cuda(), model_scale=0.25)
     loss, d = net.mse(out, target.cuda())
     net.backward(d)
-    ref_out, P = _ref_network(net, g.double(), 0.5)
+    ref_out, P = _ref_network(net, g.double(), 0.5, model.double(), 0.25)
     ref_loss = 0.5 * ((ref_out - target.double()) ** 2).mean()
     ref_loss.backward()
     assert rel(out.cpu(), ref_out.detach()) < 2e-2
     assert loss == pytest.approx(float(ref_loss), rel=2e-2)
     for name, prm in net.params.items():
         if name == 'e1a.w':
-            got, want = prm.g.cpu().numpy()[..., 0], P[name].grad.numpy()[..., 0]   # the live input channel
+            got, want = prm.g.cpu().numpy()[..., :2], P[name].grad.numpy()[..., :2]  # the live input channels
         else:
             got, want = prm.g.cpu().numpy(), P[name].grad.numpy()
         assert rel(got, want) < 5e-2, name

@@ -44,7 +44,7 @@ class _FakeNet:
     def __init__(self, proposals):
         self.proposals = list(proposals)
 
-    def forward(self, g, scale=1.0):
+    def forward(self, g, scale=1.0, model=None, model_scale=1.0):
         import torch
         return torch.tensor(self.proposals.pop(0)[None] / learned.TARGET_SCALE, dtype=torch.float32)
 
@@ -56,7 +56,7 @@ def test_gated_updates():
     fwi = _FakeFWI(true)
     net = _FakeNet([good, bad, np.full((4, 4), 3., np.float32)])
     vp, hist, accepted = learned.invert_learned(fwi, net, {500e3: 1.0}, true, [500e3], iters_per_freq=3,
-                                                first_step=5.0)
+                                                first_step=5.0, proposal_scales=(1.0,))
     assert accepted == 2
     fs = [h[1] for h in hist]
     assert all(b < a for a, b in zip(fs, fs[1:]))
@@ -92,6 +92,6 @@ def test_phantom_and_partial_model():
 
 def test_input_scales():
     import torch
-    pairs = [(torch.full((2, 2), v), None, f) for v, f in ((2.0, 1.0), (4.0, 1.0), (8.0, 2.0))]
+    pairs = [(torch.full((2, 2), v), None, f, None) for v, f in ((2.0, 1.0), (4.0, 1.0), (8.0, 2.0))]
     s = learned.input_scales(pairs)
     assert s[1.0] == pytest.approx(1 / 3.0) and s[2.0] == pytest.approx(1 / 8.0)
