@@ -56,7 +56,9 @@ def test_checkpointed_exact_adjoint_on_workload_grid(gpu):
     res = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, NT)).astype(np.float32))
     rec3, a3 = _exact(plan, res, 3)
     rec4, a4 = _exact(plan, res, 4, K=8)
-    assert np.array_equal(rec3, rec4)
+    # the two save modes may run different update kernels (the fused density update when no
+    # level is stored every step): equal to rounding
+    assert rel_l2(rec4, rec3) < 1e-6
     for key in ('gA', 'gB', 'gG', 'gb'):
         assert np.abs(a3[key]).max() > 0
         assert rel_l2(a4[key], a3[key]) < 1e-6, key
