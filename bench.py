@@ -122,19 +122,45 @@ def cpu_baseline(plan_full, seconds_budget=20.0):
     adj = nt - 1 - plan.undersampling
     gpts = n * (fwd + adj) / (t2 - t0) / 1e9
     threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
-    return dict(value=round(gpts, 4), unit='Gpts/s', cores=threads, kind='port',
+    model, nproc = host_cpu()
+    return dict(value=round(gpts, 4), unit='Gpts/s', cores=threads, kind='port', cpu_model=model, nproc=nproc,
                 sample='oracle (C/OpenMP) on the same 360^3 grid: %d forward steps with p.dt2 save + %d '
                        'as-written adjoint steps with vp gradient, %.1f s' % (fwd, adj, t2 - t0))
 
 
-def traffic_from_profile(shots=1):
+def traffic_from_profile(shots=1, tile=None):
+    """Calibrated HBM bytes per launch of the update kernel from the committed counter run, or
+    None when that run measured another tile than the one timed (the figure would be stale)."""
     path = os.path.join(ROOT, 'profiles', 'fast3d_traffic.json' if shots == 1 else 'fast3d_b2_traffic.json')
     if not os.path.exists(path):
         return None
     try:
-        return json.load(open(path)).get('bytes_per_launch')
+        rec = json.load(open(path))
     except Exception:
         return None
+    if tile is None:
+        return None
+    by_tile = rec.get('by_tile', {})
+    key = ','.join(str(int(v)) for v in tile)
+    if key in by_tile:
+        return by_tile[key].get('bytes_per_launch')
+    if rec.get('tile') is None or list(rec['tile']) != list(tile):
+        return None
+    return rec.get('bytes_per_launch')
+
+
+def host_cpu():
+    """The host's CPU model and logical core count (what the CPU baseline ran on)."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
 
 
 def bench_pipelines(plan, reps=5):
@@ -802,7 +828,8 @@ def main():
             'peak': HBM_PEAK_GBS,
             'unit': 'GB/s',
             'frac': round(achieved / HBM_PEAK_GBS, 4),
-            'traffic': traffic_from_profile(shots),
+            'traffic': traffic_from_profile(shots, tile),
+
             'kernel': 'k_fast3d_b2 (two-shot wavefield update)' if shots == 2 else 'k_fast3d (wavefield update)',
             'tile': list(tile) if tile else None,
             'avg_launch_us': round(avg_launch_s * 1e6, 2),

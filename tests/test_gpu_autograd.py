@@ -102,3 +102,69 @@ def test_backward_after_interleaved_forward(gpu):
     g1 = grad_of(True)
     assert np.abs(g0).max() > 0
     np.testing.assert_array_equal(g1, g0)
+
+
+def _autograd_grads(info, geom, device, obs):
+    import torch
+    from stride_amd.autograd import iso_acoustic_traces
+    dev = torch.device(device)
+    vp = torch.tensor(info['vp'], dtype=torch.float32, device=dev, requires_grad=True)
+    rho = torch.tensor(info['rho'], dtype=torch.float32, device=dev, requires_grad=True)
+    alpha = torch.tensor(info['alpha'], dtype=torch.float32, device=dev, requires_grad=True)
+    wav = torch.tensor(info['wavelet'], dtype=torch.float32, device=dev, requires_grad=True)
+    tr = iso_acoustic_traces(vp, rho, alpha, wav, geom)
+    loss = 0.5 * torch.sum((tr.double() - obs.to(dev).double()) ** 2)
+    loss.backward()
+    return tr.detach().cpu().numpy(), {k: v.grad.cpu().numpy() for k, v in
+                                       (('vp', vp), ('rho', rho), ('alpha', alpha), ('wav', wav))}
+
+
+@pytest.mark.parametrize('checkpoint', [None, 9])
+def test_device_path_matches_host_path(gpu, checkpoint):
+    """GPU tensors: medium set from device pointers, device accumulators, the chain rule on the
+    device; the same traces and gradients as the host-tensor path."""
+    import torch
+    from stride_amd.autograd import Geometry
+    plan, info = small_case(dim=2, n=48, nabs=10, nt=220, kernel='OT2', rho='var', alpha='p0')
+    print('autograd device path checkpoint=%s' % checkpoint, flush=True)
+    kw = dict(info['kw'])
+    src, rec = kw.pop('source_coordinates'), kw.pop('receiver_coordinates')
+    obs = torch.tensor(np.random.default_rng(12).standard_normal((plan.nrec, plan.nt)).astype(np.float32))
+    tr_h, g_h = _autograd_grads(info, Geometry(info['space'], info['time'], src, rec, checkpoint_every=checkpoint,
+                                               **kw), 'cpu', obs)
+    geom = Geometry(info['space'], info['time'], src, rec, checkpoint_every=checkpoint, **kw)
+    tr_d, g_d = _autograd_grads(info, geom, 'cuda', obs)
+    assert rel_l2(tr_d, tr_h) < 1e-6
+    for k in g_h:
+        assert np.abs(g_h[k]).max() > 0
+        assert rel_l2(g_d[k], g_h[k]) < 1e-5, k
+    # the cached plan is reused on the next call with the same wavelets and kind of medium
+    plan0 = geom._dev_plan
+    _autograd_grads(info, geom, 'cuda', obs)
+    assert geom._dev_plan is plan0
+
+
+def test_device_path_workload_grid(gpu):
+    """The device path on the 360^3 grid with the variable vp / rho / alpha medium (40 steps,
+    checkpoints every 8): gradients equal to the host path's."""
+    import torch
+    from stride_amd.autograd import Geometry
+    from stride_amd.configs import config4
+    from stride_amd.problem import Time
+    from stride_amd.utils import ellipsoidal, tone_burst
+    cfg = config4(scale=1.0, nt=200)
+    space, dt = cfg['space'], cfg['time'].step
+    cen = np.array([l / 2 for l in space.limit])
+    src = cen + 0.8 * (cfg['sources'][0] - cen)
+    rec = ellipsoidal(64, (3e-3,) * 3, tuple(src))
+    time = Time(start=0., step=dt, num=40)
+    info = dict(vp=cfg['vp'], rho=cfg['rho'], alpha=cfg['alpha'], wavelet=tone_burst(400e3, 3, 200, dt)[:40][None])
+    obs = torch.tensor(np.random.default_rng(2).standard_normal((64, 40)).astype(np.float32))
+    print('autograd device path on %s' % (space.extended_shape,), flush=True)
+    kw = dict(kernel='OT2')
+    tr_d, g_d = _autograd_grads(info, Geometry(space, time, src[None], rec, checkpoint_every=8, **kw), 'cuda', obs)
+    tr_h, g_h = _autograd_grads(info, Geometry(space, time, src[None], rec, checkpoint_every=8, **kw), 'cpu', obs)
+    assert rel_l2(tr_d, tr_h) < 1e-6
+    for k in g_h:
+        assert np.abs(g_h[k]).max() > 0
+        assert rel_l2(g_d[k], g_h[k]) < 1e-5, k
