@@ -317,9 +317,10 @@ def unet_flops(N, H, W):
     return total
 
 
-def bench_learned(ntrain=400, epochs=60, neval=4, iters_per_freq=4, nsrc=16, log=None):
+def bench_learned(ntrain=30, epochs=40, neval=4, iters_per_freq=4, nsrc=16, log=None):
     """The learned-preconditioner workload on the 2D gradient grid: solver-generated training
-    pairs, U-Net training, then inversions of unseen phantoms with frequency continuation
+    pairs (every gradient of plain inversions of ntrain training phantoms), U-Net training, then
+    inversions of unseen phantoms with frequency continuation
     (300 - 700 kHz), learned (misfit-gated, plain gradient step as fallback) against plain
     L-BFGS; solver calls counted as passes over all shots (a gradient is two: forward and
     adjoint)."""
@@ -329,7 +330,7 @@ def bench_learned(ntrain=400, epochs=60, neval=4, iters_per_freq=4, nsrc=16, log
     freqs = [300e3, 400e3, 500e3, 600e3, 700e3]
     fwi = learned.RingFWI(nsrc=nsrc, workers=2)
     t0 = time.perf_counter()
-    pairs = learned.make_pairs(fwi, range(ntrain), freqs, seed=0, log=log)
+    pairs = learned.make_trajectory_pairs(fwi, range(ntrain), freqs, iters_per_freq, log=log)
     t_pairs = time.perf_counter() - t0
     scales = learned.input_scales(pairs)
     t0 = time.perf_counter()
@@ -337,6 +338,7 @@ def bench_learned(ntrain=400, epochs=60, neval=4, iters_per_freq=4, nsrc=16, log
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     steps = epochs * (len(pairs) // 4)
+    npairs = len(pairs)
     train_flops = 3 * unet_flops(4, 512, 512) * steps
     # inference throughput of the network alone
     g = torch.stack([pairs[j][0] * scales[pairs[j][2]] for j in range(4)])
@@ -373,7 +375,8 @@ def bench_learned(ntrain=400, epochs=60, neval=4, iters_per_freq=4, nsrc=16, log
     ratios = [r['learned_passes_to_plain_misfit'] / r['plain_passes'] for r in results
               if r['learned_passes_to_plain_misfit'] is not None]
     return dict(metric='FWI misfit drop per solver call, learned preconditioner vs plain L-BFGS',
-                unit='passes over all shots', training_pairs=ntrain, epochs=epochs, sources=nsrc,
+                unit='passes over all shots', training_phantoms=ntrain, training_pairs=npairs, epochs=epochs,
+                sources=nsrc,
                 pair_seconds=round(t_pairs, 1), train_seconds=round(t_train, 1),
                 train_tflops=round(train_flops / t_train / 1e12, 2), final_train_loss=losses[-1],
                 first_train_loss=losses[0], inference_ms_batch4=round(t_inf * 1e3, 2),
@@ -677,8 +680,8 @@ def main():
     ap.add_argument('--learned', action='store_true',
                     help='only run the learned-preconditioner workload (training pairs from the solver, U-Net '
                          'training, misfit-gated inversions against plain L-BFGS) and exit')
-    ap.add_argument('--train-pairs', type=int, default=400)
-    ap.add_argument('--epochs', type=int, default=60)
+    ap.add_argument('--train-phantoms', type=int, default=30)
+    ap.add_argument('--epochs', type=int, default=40)
     ap.add_argument('--eval', type=int, default=4)
     ap.add_argument('--pipelines', action='store_true',
                     help='only time the device steps either side of the solver (trace processing, misfit, '
@@ -705,7 +708,7 @@ def main():
         return
 
     if args.learned:
-        print(json.dumps(bench_learned(ntrain=args.train_pairs, epochs=args.epochs, neval=args.eval)))
+        print(json.dumps(bench_learned(ntrain=args.train_phantoms, epochs=args.epochs, neval=args.eval)))
         return
     if args.forward_2d:
         print(json.dumps(bench_forward_2d()))

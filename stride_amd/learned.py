@@ -33,7 +33,8 @@ from .unet import UNet
 from .utils import tone_burst, elliptical
 from .workers import ShotWorkers
 
-__all__ = ['RingFWI', 'phantom', 'make_pairs', 'train', 'invert_plain', 'invert_learned']
+__all__ = ['RingFWI', 'phantom', 'make_pairs', 'make_trajectory_pairs', 'train', 'invert_plain',
+           'invert_learned']
 
 PML = 'complex_frequency_shift_PML_2'
 VP_BOUNDS = (1300., 1800.)
@@ -262,12 +263,33 @@ def _model_error(vp, true):
     return float(np.linalg.norm(vp - true) / np.linalg.norm(true - 1500.))
 
 
-def invert_plain(fwi, true, freqs, iters_per_freq=4, first_step=20.0):
+def make_trajectory_pairs(fwi, ks, freqs, iters_per_freq=4, log=None):
+    """Solver-generated pairs along plain inversions of the training phantoms ks: every gradient
+    the plain L-BFGS evaluates (line searches included), with the true perturbation from the model
+    it was taken at: (gradient (device), (m_true - m) / 150 m/s, frequency, (m - 1500) / 150 m/s).
+    These are the models an inversion meets, partly recovered by the inversion itself."""
+    torch = _torch()
+    pairs = []
+    t0 = time.perf_counter()
+    for i, k in enumerate(ks):
+        true = phantom(k, n=fwi.space.shape[0], dx=fwi.space.spacing[0])
+
+        def record(g, vp, freq):
+            pairs.append((g, torch.from_numpy((true - vp) / TARGET_SCALE).to(fwi.dev), freq,
+                          torch.from_numpy((vp - 1500.) / TARGET_SCALE).to(fwi.dev)))
+
+        invert_plain(fwi, true, freqs, iters_per_freq, record=record)
+        if log:
+            log('trajectory %d / %d: %d pairs (%.1f s)' % (i + 1, len(ks), len(pairs), time.perf_counter() - t0))
+    return pairs
+
+
+def invert_plain(fwi, true, freqs, iters_per_freq=4, first_step=20.0, record=None):
     """L-BFGS per frequency band (restarted at each band) from water, in slowness, the space the
     gradient of get_grad_vp belongs to with vp's default transform (the reference steps in the
     transformed variable, stride/optimisation/optimisers/optimiser.py:242-330). The first step
     changes vp by at most about first_step m/s. Returns (vp, history of (solver passes, misfit at
-    the band, model error, band))."""
+    the band, model error, band)); record(gradient, vp, band), when given, sees every gradient."""
     from .optim import LBFGS
     shape = fwi.space.shape
     x = np.full(int(np.prod(shape)), 1 / 1500., np.float64)
@@ -279,7 +301,10 @@ def invert_plain(fwi, true, freqs, iters_per_freq=4, first_step=20.0):
         opt = LBFGS(memory=5, first_step=first_step / 1500. ** 2, bounds=bounds)
 
         def fg(xs):
-            f, g = fwi.gradient((1 / xs).reshape(shape).astype(np.float32), freq)
+            vp = (1 / xs).reshape(shape).astype(np.float32)
+            f, g = fwi.gradient(vp, freq)
+            if record is not None:
+                record(g, vp, freq)
             return f, g.double().cpu().numpy().reshape(-1)
 
         def cb(xs, f):
