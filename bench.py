@@ -655,6 +655,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-schedule', action='store_true',
                     help='leave out the secondary 32-shot batched-forward line')
+    ap.add_argument('--no-density', action='store_true',
+                    help='leave out the secondary 3D variable-density gradient shot')
     ap.add_argument('--tile', type=str, default=None,
                     help='rows,chunk,version of the 3D update tile (rows,chunk,waves with --shots 2)')
     ap.add_argument('--shots', type=int, default=2, choices=(1, 2),
@@ -847,6 +849,10 @@ def main():
         # secondary line: the 32 uneven shots of the batched-forward workload in one graph launch
         print('[bench] 32-shot schedule', file=sys.stderr, flush=True)
         out['schedule'] = bench_schedule(tuple(tile) if (tile and shots == 2) else None)
+    if not args.no_density:
+        # secondary line: one shot of the 3D gradient workload's variable vp / rho / alpha medium
+        print('[bench] 3D density shot', file=sys.stderr, flush=True)
+        out['density'] = bench_density(args.nt, None, with_cpu=False)
     if not args.no_cpu:
         try:
             out['cpu_baseline'] = cpu_baseline(plan)
