@@ -61,9 +61,14 @@ typedef struct {
     int diff_source;     /* source injected as is (0) or already differentiated (1): changes the dt power */
     int undersampling;   /* forward wavefield save every f steps */
     int save_mode;       /* 0: no save, 1: p.dt2 (vp gradient), 2: raw p (rho / wavelet gradients),
-                            3: raw p every step (exact adjoint) */
+                            3: raw p every step (exact adjoint), 4: exact adjoint by checkpointed
+                            recomputation (undersampling = levels per segment), 5: the raw p snapshots
+                            of mode 2 by checkpointed recomputation (`checkpoint` snapshots per
+                            segment): the forward keeps one state per segment, the adjoint recomputes
+                            each segment's snapshots from it; the gradients equal mode 2's */
     int tb0, tb1;        /* time_bounds */
     int grad_vp, grad_rho, grad_src;
+    int checkpoint;      /* save_mode 5: snapshots per recomputation segment (>= 1) */
 } sa_scheme;
 
 const char *sa_last_error(void);
@@ -147,6 +152,25 @@ int sa_get_wavefield(sa_solver *s, int which, float *out);
 
 /* deallocate_wavefield (devito.py:191-196): release the saved forward wavefield. */
 int sa_free_saved(sa_solver *s);
+
+/* cache_forward (devito.py:632-668 after the forward run, :820-842 before the adjoint): the
+ * stored snapshots (save_mode 1 or 2) to host memory and back. Host layout: (nsave, region),
+ * region = the stored box without margin[a] cells at each end of axis a (grid->dim values;
+ * NULL keeps the whole box; the reference keeps `inner_wavefield`, margin = space.extra).
+ * sa_saved_count gives the float count of that layout; sa_saved_to_host with release = 1 frees
+ * the device copy (the adjoint then refuses to run until sa_saved_from_host);
+ * sa_saved_from_host re-allocates it, zero outside the region. */
+int sa_saved_count(sa_solver *s, const int *margin, long long *n);
+int sa_saved_to_host(sa_solver *s, float *host, long long n, const int *margin, int release);
+int sa_saved_from_host(sa_solver *s, const float *host, long long n, const int *margin);
+
+/* dump_forward_wavefield / dump_adjoint_wavefield (devito.py:421-432, 765-778): keep the raw level
+ * of every factor-th step (0: off) over the whole extended grid, (tb1 - tb0 + f) / f + 1 slots
+ * at slot t / f, as undersampled_time_function (common/devito.py:552-578) sizes them. which = 0
+ * forward, 1 adjoint; sa_dump_to_host copies (slots, extended shape) after the run. */
+int sa_set_dump(sa_solver *s, int forward_factor, int adjoint_factor);
+int sa_dump_count(sa_solver *s, int which, long long *n);
+int sa_dump_to_host(sa_solver *s, int which, float *host, long long n);
 
 /* Tile of the specialised 3D update (3D, OT2, no density, sponge or no boundary, no
  * attenuation): 64 x waves*rows cells per workgroup, marching `chunk` planes of the slowest

@@ -35,7 +35,8 @@ class _Scheme(ctypes.Structure):
     _fields_ = [('kernel', ctypes.c_int), ('rho_mode', ctypes.c_int), ('att_power', ctypes.c_int),
                 ('boundary', ctypes.c_int), ('diff_source', ctypes.c_int), ('undersampling', ctypes.c_int),
                 ('save_mode', ctypes.c_int), ('tb0', ctypes.c_int), ('tb1', ctypes.c_int),
-                ('grad_vp', ctypes.c_int), ('grad_rho', ctypes.c_int), ('grad_src', ctypes.c_int)]
+                ('grad_vp', ctypes.c_int), ('grad_rho', ctypes.c_int), ('grad_src', ctypes.c_int),
+                ('checkpoint', ctypes.c_int)]
 
 
 EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_stream', 'sa_set_scheme',
@@ -52,7 +53,8 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_nn_upsample', 'sa_nn_upsample_back', 'sa_nn_copy_channels', 'sa_nn_pack_input', 'sa_nn_head',
            'sa_nn_head_back', 'sa_nn_mse', 'sa_nn_adam', 'sa_nn_to_bf16', 'sa_nn_from_bf16', 'sa_field_mask', 'sa_field_gaussian', 'sa_field_stats', 'sa_field_axpb',
            'sa_field_div_nonzero', 'sa_field_axpy', 'sa_set_medium_dev', 'sa_adjoint_exact_dev',
-           'sa_exact_gradients_dev', 'sa_field_descent', 'sa_read_gradient_dev',
+           'sa_exact_gradients_dev', 'sa_field_descent', 'sa_read_gradient_dev', 'sa_saved_count',
+           'sa_saved_to_host', 'sa_saved_from_host', 'sa_set_dump', 'sa_dump_count', 'sa_dump_to_host',
            'sa_traces_transpose', 'sa_field_crop', 'sa_field_embed', 'sa_field_grad_vp_finish']
 
 _lib = None
@@ -131,9 +133,9 @@ class Solver:
         return self._h
 
     def set_scheme(self, kernel, rho_mode, att_power, boundary, diff_source, undersampling, save_mode,
-                   tb0, tb1, grad_vp=0, grad_rho=0, grad_src=0):
+                   tb0, tb1, grad_vp=0, grad_rho=0, grad_src=0, checkpoint=0):
         s = _Scheme(kernel, rho_mode, att_power, boundary, int(diff_source), undersampling, save_mode,
-                    tb0, tb1, int(grad_vp), int(grad_rho), int(grad_src))
+                    tb0, tb1, int(grad_vp), int(grad_rho), int(grad_src), int(checkpoint))
         self._check(self._lib.sa_set_scheme(self._h, ctypes.byref(s)))
 
     def set_medium(self, vp, rho=None, alpha=None):
@@ -234,6 +236,42 @@ class Solver:
 
     def free_saved(self):
         self._check(self._lib.sa_free_saved(self._h))
+
+    # ---- stored snapshots in host memory (cache_forward) and wavefield dumps ----
+    def _margin(self, margin):
+        if margin is None:
+            return None
+        m = (ctypes.c_int * 3)(*[int(x) for x in margin] + [0] * (3 - len(margin)))
+        return m
+
+    def saved_count(self, margin=None):
+        n = ctypes.c_longlong(0)
+        self._check(self._lib.sa_saved_count(self._h, self._margin(margin), ctypes.byref(n)))
+        return n.value
+
+    def saved_to_host(self, margin=None, release=True):
+        """The stored snapshots as a flat float32 array (nsave x region), the device copy released
+        (release=True). margin: cells left out at each end of every axis (None: the whole box)."""
+        out = np.empty(self.saved_count(margin), np.float32)
+        self._check(self._lib.sa_saved_to_host(self._h, _p(out), ctypes.c_longlong(out.size), self._margin(margin),
+                                               ctypes.c_int(int(release))))
+        return out
+
+    def saved_from_host(self, data, margin=None):
+        data = np.ascontiguousarray(data, np.float32).ravel()
+        self._check(self._lib.sa_saved_from_host(self._h, _p(data), ctypes.c_longlong(data.size),
+                                                 self._margin(margin)))
+
+    def set_dump(self, forward_factor=0, adjoint_factor=0):
+        self._check(self._lib.sa_set_dump(self._h, ctypes.c_int(int(forward_factor)), ctypes.c_int(int(adjoint_factor))))
+
+    def dump_to_host(self, which=0):
+        """(slots, *extended shape) of the forward (0) or adjoint (1) wavefield dump."""
+        n = ctypes.c_longlong(0)
+        self._check(self._lib.sa_dump_count(self._h, ctypes.c_int(which), ctypes.byref(n)))
+        out = np.empty(n.value, np.float32)
+        self._check(self._lib.sa_dump_to_host(self._h, ctypes.c_int(which), _p(out), ctypes.c_longlong(out.size)))
+        return out.reshape((-1,) + tuple(self.shape))
 
     # ---- two shots per pass (interleaved wavefields, shared vp, summed vp gradient) ----
     def batch_create(self):

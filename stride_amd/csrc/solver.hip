@@ -534,6 +534,20 @@ __global__ void k_pack(Geo g, Box box, const float *__restrict__ src, float *__r
     if (sa_inbox(box, i, j, k)) snap[sa_boxidx(box, i, j, k)] = src[p];
 }
 
+// snapshot (box layout) <-> contiguous array over the region r (grid coordinates inside the box)
+__global__ void k_region(Box box, Box r, float *snap, float *flat, int to_flat) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int i = blockIdx.z;
+    if (k >= r.n[2] || j >= r.n[1] || i >= r.n[0]) return;
+    long long q = ((long long)i * r.n[1] + j) * r.n[2] + k;
+    long long b = sa_boxidx(box, r.lo[0] + i, r.lo[1] + j, r.lo[2] + k);
+    if (to_flat)
+        flat[q] = snap[b];
+    else
+        snap[b] = flat[q];
+}
+
 // compat vp gradient on the interior (prepare_grad_vp, devito.py:985-1024)
 __global__ void k_grad_vp(Geo g, Box box, const float *__restrict__ ps, const float *__restrict__ psm,
                           const float *__restrict__ psp, float fdt2, const float *__restrict__ pa, float *grad,
@@ -699,6 +713,21 @@ struct sa_solver {
     float *fb[2] = {};
     float *src_keep = nullptr;
     size_t src_keep_n = 0;
+    // checkpointed raw saves (save_mode 5): `saved` holds the states (p[T_j], p[T_j - 1]),
+    // T_j = (jK - 1) f, of segments j >= 1; `segment` is a ring of K + 2 snapshots (snapshot i in
+    // slot i % (K + 2)) holding snapshots jK - 1 .. (j + 1)K of segment `ck_seg`; snapshots outside
+    // [ck_first, ck_last] read as zero, as the slots mode 2 never writes
+    int ck_K = 0, ck_J = 0, ck_first = 0, ck_last = -1, ck_seg = -1;
+    // snapshots moved to host memory (cache_forward): the layout above is kept, `saved` is null
+    bool saved_on_host = false;
+    float *xfer = nullptr;  // one region of one snapshot, staged for host copies
+    size_t xfer_n = 0;
+    // wavefield dumps (dump_forward_wavefield / dump_adjoint_wavefield): raw level every
+    // dump_f[w] steps over the whole domain, w = 0 forward, 1 adjoint
+    int dump_f[2] = {};
+    int dump_n[2] = {};
+    float *dump[2] = {};
+    size_t dump_bytes[2] = {};
     // host staging
     float *d_series = nullptr;
     size_t series_n = 0;
@@ -920,6 +949,9 @@ extern "C" int sa_destroy(sa_solver *s) {
     dfree(s->fb[0]);
     dfree(s->fb[1]);
     dfree(s->src_keep);
+    dfree(s->xfer);
+    dfree(s->dump[0]);
+    dfree(s->dump[1]);
     for (int a = 0; a < 3; ++a) {
         dfree(s->u1[a]); dfree(s->u2[a]); dfree(s->u3[a]); dfree(s->st[a]); dfree(s->st2[a]);
         for (int r = 0; r < 4; ++r) dfree(s->pml[a][r]);
@@ -964,7 +996,14 @@ extern "C" int sa_set_scheme(sa_solver *s, const sa_scheme *sch) {
     if (sch->undersampling < 1) return fail(SA_E_ARG, "undersampling must be >= 1");
     if (sch->tb0 < 0 || sch->tb1 > s->grid.nt || sch->tb1 <= sch->tb0 + 2)
         return fail(SA_E_ARG, "time bounds (%d, %d) invalid for nt %d", sch->tb0, sch->tb1, s->grid.nt);
-    if (sch->save_mode < 0 || sch->save_mode > 4) return fail(SA_E_ARG, "save_mode %d", sch->save_mode);
+    if (sch->save_mode < 0 || sch->save_mode > 5) return fail(SA_E_ARG, "save_mode %d", sch->save_mode);
+    if (sch->save_mode == 5) {
+        if (sch->checkpoint < 1) return fail(SA_E_ARG, "save_mode 5 needs checkpoint >= 1 snapshots per segment");
+        // the recomputation restarts from the two time levels alone
+        if (sch->boundary == 1 || sch->att_power == 2)
+            return fail(SA_E_UNSUPPORTED, "checkpointed raw saves (save_mode 5) support the sponge or no boundary "
+                                          "and attenuation none/0/None");
+    }
     if (!s->have_scheme || memcmp(&s->sch, sch, sizeof(*sch)) != 0) s->coef_dirty = true;
     s->sch = *sch;
     s->have_scheme = true;
@@ -1547,7 +1586,17 @@ static int alloc_saved(sa_solver *s) {
     int nsave = c.tb1 / f + 1;
     if (c.save_mode == 4) nsave = 2 * ((c.tb1 - 2) / f + 1);  // checkpoint pairs
     size_t per = (size_t)b.n[0] * b.n[1] * b.p2;
-    size_t bytes = per * nsave * sizeof(float);
+    size_t slots = nsave;
+    if (c.save_mode == 5) {
+        // snapshots first..last as mode 2 writes them; segments of K snapshots; states of segments >= 1
+        s->ck_first = (std::max(1, c.tb0) + f - 1) / f;
+        s->ck_last = std::min((c.tb1 - 2) / f, nsave - 1);
+        s->ck_K = std::min(c.checkpoint, std::max(1, s->ck_last + 1));  // one segment at most
+        s->ck_J = s->ck_last >= 0 ? s->ck_last / s->ck_K + 1 : 1;
+        s->ck_seg = -1;
+        slots = (size_t)std::max(1, 2 * (s->ck_J - 1));
+    }
+    size_t bytes = per * slots * sizeof(float);
     if (!s->saved || s->saved_bytes < bytes) {
         dfree(s->saved);
         s->saved = nullptr;
@@ -1562,7 +1611,9 @@ static int alloc_saved(sa_solver *s) {
     // buffer). Every-f saves (modes 1, 2) write every box point of the slots t / f for
     // t % f == 0, max(1, tb0) <= t <= tb1 - 2, so only the slots outside that run are cleared;
     // the other modes clear the whole buffer.
-    if (c.save_mode == 1 || c.save_mode == 2) {
+    if (c.save_mode == 5) {
+        // every state slot the adjoint reads is written by the forward run
+    } else if (c.save_mode == 1 || c.save_mode == 2) {
         const int first = (std::max(1, c.tb0) + f - 1) / f;
         const int last = std::min((c.tb1 - 2) / f, nsave - 1);
         if (first > last) {
@@ -1579,9 +1630,10 @@ static int alloc_saved(sa_solver *s) {
     s->box = b;
     s->nsave = nsave;
     s->saved_mode = c.save_mode;
+    s->saved_on_host = false;
     s->saved_f = f;
-    if (c.save_mode == 4) {
-        size_t sb = per * (size_t)(f + 1) * sizeof(float);
+    if (c.save_mode == 4 || c.save_mode == 5) {
+        size_t sb = per * (size_t)(c.save_mode == 4 ? f + 1 : s->ck_K + 2) * sizeof(float);
         if (!s->segment || s->segment_bytes < sb) {
             dfree(s->segment);
             s->segment = nullptr;
@@ -1599,6 +1651,30 @@ static int alloc_saved(sa_solver *s) {
 
 // floats per snapshot slot (rows padded to the pitch)
 static size_t box_points(const Box &b) { return (size_t)b.n[0] * b.n[1] * b.p2; }
+
+static int dump_alloc(sa_solver *s, int w);
+static int dump_level(sa_solver *s, int w, int t, const float *u);
+
+// One forward step t: p[t+1] into o from u = p[t] and o = p[t-1], then the source injection.
+// The forward run and the checkpointed recomputation issue the same launches through it, so the
+// recomputed levels equal the forward run's bit for bit.
+static int forward_step(sa_solver *s, int t, float *u, float *o, const float *src, float *save, int mode,
+                        const GatherSpec *gat) {
+    const float idt2 = 1.f / (s->dt * s->dt);
+    bool injected = false;
+    if (use_fast(s)) {
+        hipEvent_t e0;
+        TRY(timed_begin(s, &e0));
+        TRY(fast3d_step(s->fast, s->geo, u, o, o, save, s->box, idt2, s->stream));
+        TRY(timed_end(s));
+    } else {
+        const InjectSpec spec{&s->src, src + (size_t)t * s->src.npoint, 1.f, mode == 1};
+        TRY(step_general(s, u, o, o, 0, save, mode, &spec, &injected, gat));
+    }
+    if (!injected)
+        TRY(inject(s, s->src, src + (size_t)t * s->src.npoint, 1.f, o, mode == 1 ? save : nullptr, idt2, s->box));
+    return SA_OK;
+}
 
 extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
     if (!s) return fail(SA_E_ARG, "null solver");
@@ -1618,12 +1694,12 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
     if (s->lapo) TRY(zero_field(s, s->lapo));
     if (s->rec.npoint > 0) HIP_TRY(hipMemsetAsync(rec, 0, sizeof(float) * (size_t)s->grid.nt * s->rec.npoint, s->stream));
     if (c.save_mode != 0) TRY(alloc_saved(s));
+    TRY(dump_alloc(s, 0));
     const int f = c.save_mode == 3 ? 1 : c.undersampling;
     const size_t per = box_points(s->box);
-    const float idt2 = 1.f / (s->dt * s->dt);
     const bool fast = use_fast(s);
-    const bool ckpt = c.save_mode == 4;
-    if (ckpt) {
+    const bool ckpt = c.save_mode == 4, raw_ckpt = c.save_mode == 5;
+    if (ckpt || raw_ckpt) {
         size_t ns = (size_t)s->grid.nt * s->src.npoint;
         TRY(ensure_staging(s, &s->src_keep, &s->src_keep_n, ns));
         if (ns) HIP_TRY(hipMemcpyAsync(s->src_keep, src, ns * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
@@ -1637,23 +1713,13 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
         float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
         float *save = nullptr;
         int mode = 0;
-        bool injected = false;
-        if (c.save_mode != 0 && !ckpt && t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave) {
+        if (c.save_mode != 0 && !ckpt && !raw_ckpt && t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave) {
             save = s->saved + (size_t)(t / f) * per;
             mode = c.save_mode == 1 ? 1 : 2;
         }
-        if (fast) {
-            hipEvent_t e0;
-            TRY(timed_begin(s, &e0));
-            TRY(fast3d_step(s->fast, s->geo, u, o, o, save, s->box, idt2, s->stream));
-            TRY(timed_end(s));
-        } else {
-            const InjectSpec spec{&s->src, src + (size_t)t * s->src.npoint, 1.f, mode == 1};
-            const GatherSpec gspec{&s->rec, 1.f, pending};
-            TRY(step_general(s, u, o, o, 0, save, mode, &spec, &injected, pending ? &gspec : nullptr));
-        }
-        if (!injected)
-            TRY(inject(s, s->src, src + (size_t)t * s->src.npoint, 1.f, o, mode == 1 ? save : nullptr, idt2, s->box));
+        TRY(dump_level(s, 0, t, u));
+        const GatherSpec gspec{&s->rec, 1.f, pending};
+        TRY(forward_step(s, t, u, o, src, save, mode, pending ? &gspec : nullptr));
         if (defer)
             pending = rec + (size_t)t * s->rec.npoint;
         else
@@ -1664,6 +1730,16 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
             k_pack<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, s->box, o, slot);
             k_pack<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, s->box, u, slot + per);
             CHECK_LAUNCH();
+        }
+        if (raw_ckpt && (t + 1) % f == 0 && t + 1 >= 2 && ((t + 1) / f + 1) % s->ck_K == 0) {
+            // the state of segment j, (p[T_j], p[T_j - 1]) with T_j = (jK - 1) f = t + 1
+            const int j = ((t + 1) / f + 1) / s->ck_K;
+            if (j >= 1 && j < s->ck_J) {
+                float *slot = s->saved + (size_t)(2 * (j - 1)) * per;
+                k_pack<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, s->box, o, slot);
+                k_pack<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, s->box, u, slot + per);
+                CHECK_LAUNCH();
+            }
         }
     }
     if (pending) TRY(gather(s, s->rec, s->buf[(tlast + 1) % 2], 1.f, pending));
@@ -1694,6 +1770,48 @@ static int recompute_segment(sa_solver *s, int j) {
         k_pack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, o, s->segment + (size_t)(t + 1 - t0) * per);
         CHECK_LAUNCH();
     }
+    return SA_OK;
+}
+
+// Checkpointed raw saves: recompute snapshots jK - 1 .. (j + 1)K of segment j (p[i f] for
+// snapshot i) into the ring, from the state (p[T_j], p[T_j - 1]), T_j = (jK - 1) f, or from the
+// zero state before step 1. Snapshots outside [ck_first, ck_last] are zero.
+static int recompute_raw_segment(sa_solver *s, int j) {
+    s->lap_of = nullptr;
+    const int f = s->saved_f, K = s->ck_K, R = K + 2;
+    const size_t per = box_points(s->box);
+    const Geo &g = s->geo;
+    auto slot = [&](int i) { return s->segment + (size_t)(i % R) * per; };
+    const int ilo = std::max(j * K - 1, 0), ihi = (j + 1) * K;
+    for (int i = ilo; i <= ihi; ++i)
+        if (i < s->ck_first || i > s->ck_last) HIP_TRY(hipMemsetAsync(slot(i), 0, per * sizeof(float), s->stream));
+    const int vlo = std::max(ilo, s->ck_first), vhi = std::min(ihi, s->ck_last);
+    s->ck_seg = j;
+    if (vlo > vhi) return SA_OK;
+    const int T = (j * K - 1) * f;
+    int t0 = 1;
+    if (j >= 1 && T >= 2) {
+        const float *state = s->saved + (size_t)(2 * (j - 1)) * per;  // (p[T], p[T - 1])
+        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, state, s->fb[T % 2]);
+        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, state + per, s->fb[(T - 1) % 2]);
+        CHECK_LAUNCH();
+        t0 = T;
+    } else {
+        TRY(zero_field(s, s->fb[0]));
+        TRY(zero_field(s, s->fb[1]));
+    }
+    const int tend = vhi * f;
+    for (int t = t0; t < tend; ++t) {
+        float *u = s->fb[t % 2], *o = s->fb[(t + 1) % 2];
+        if (t % f == 0 && t / f >= vlo) {
+            k_pack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, u, slot(t / f));
+            CHECK_LAUNCH();
+        }
+        TRY(forward_step(s, t, u, o, s->src_keep, nullptr, 0, nullptr));
+    }
+    k_pack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, s->fb[tend % 2], slot(vhi));
+    CHECK_LAUNCH();
+    s->lap_of = nullptr;
     return SA_OK;
 }
 
@@ -1730,11 +1848,15 @@ static int zero_aux(sa_solver *s) {
 static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, float *gr, float *pr, float *gsrc) {
     const sa_scheme &c = s->sch;
     const Geo &g = s->geo;
-    if (!s->saved || s->saved_mode == 0 || s->saved_mode == 3)
-        return fail(SA_E_STATE, "the adjoint needs a forward run that saved the wavefield (save_mode 1 or 2)");
-    if (gr && s->saved_mode != 2) return fail(SA_E_STATE, "the density gradient needs the raw wavefield (save_mode 2)");
+    if (s->saved_on_host) return fail(SA_E_STATE, "the saved wavefield is in host memory: sa_saved_from_host first");
+    if (!s->saved || s->saved_mode == 0 || s->saved_mode == 3 || s->saved_mode == 4)
+        return fail(SA_E_STATE, "the adjoint needs a forward run that saved the wavefield (save_mode 1, 2 or 5)");
+    const bool raw = s->saved_mode == 2 || s->saved_mode == 5, raw_ckpt = s->saved_mode == 5;
+    if (gr && !raw) return fail(SA_E_STATE, "the density gradient needs the raw wavefield (save_mode 2 or 5)");
     TRY(coef_sparse(s, s->rec, 1));
+    if (raw_ckpt) s->ck_seg = -1;
     TRY(zero_aux(s));
+    TRY(dump_alloc(s, 1));
     const int f = s->saved_f;
     const size_t per = box_points(s->box);
     const float fdt2 = (float)(1.0 / ((double)f * s->grid.dt * (double)f * s->grid.dt));
@@ -1744,15 +1866,21 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
     const bool fast = use_fast(s);
     // the fused 2D update takes the gradients itself: vp (any density form) and rho (variable
     // density, interior at least 5 cells from the edges, so the snapshot stencils stay inside)
-    bool grad2d = !fast && use_flat2d(s) && (s->saved_mode == 1 || s->saved_mode == 2);
-    if (gr) grad2d = grad2d && c.rho_mode == 2 && s->saved_mode == 2 && g.nabs[1] >= 5 && g.nabs[2] >= 5;
+    bool grad2d = !fast && use_flat2d(s) && (s->saved_mode == 1 || raw);
+    if (gr) grad2d = grad2d && c.rho_mode == 2 && raw && g.nabs[1] >= 5 && g.nabs[2] >= 5;
+    // snapshot i: slot i of `saved`, or slot i % (K + 2) of the recomputed ring
+    auto snap = [&](int i) -> const float * {
+        return raw_ckpt ? s->segment + (size_t)(i % (s->ck_K + 2)) * per : s->saved + (size_t)i * per;
+    };
     int tstart = c.tb1 - 1 - f, tend = c.tb0 + 1;
     // buf[t % 2] = p_a[t]; step t overwrites p_a[t+1] with p_a[t-1]
     for (int t = tstart; t >= tend; --t) {
         float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
         bool injected = false, grads_done = false;
         bool guard = t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave && (gv || gr);
-        const float *ps = guard ? s->saved + (size_t)(t / f) * per : nullptr;
+        TRY(dump_level(s, 1, t, u));
+        if (guard && raw_ckpt && (t / f) / s->ck_K != s->ck_seg) TRY(recompute_raw_segment(s, (t / f) / s->ck_K));
+        const float *ps = guard ? snap(t / f) : nullptr;
         if (fast) {
             hipEvent_t e0;
             TRY(timed_begin(s, &e0));
@@ -1765,10 +1893,10 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
             GradSpec gspec{};
             if (guard && grad2d) {
                 gspec.ps = ps;
-                gspec.raw = s->saved_mode == 2;
+                gspec.raw = raw;
                 if (gspec.raw) {
-                    gspec.psm = t / f >= 1 ? s->saved + (size_t)(t / f - 1) * per : nullptr;
-                    gspec.psp = t / f + 1 < s->nsave ? s->saved + (size_t)(t / f + 1) * per : nullptr;
+                    gspec.psm = t / f >= 1 ? snap(t / f - 1) : nullptr;
+                    gspec.psp = t / f + 1 < s->nsave ? snap(t / f + 1) : nullptr;
                 }
                 gspec.fdt2 = fdt2;
                 gspec.gv = gv;
@@ -1783,19 +1911,19 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
         if (!injected) TRY(inject(s, s->rec, res + (size_t)t * s->rec.npoint, -1.f, o, nullptr, 0.f, s->box));
         if (gsrc) TRY(gather(s, s->src, u, -1.f, gsrc + (size_t)t * s->src.npoint));
         if (!guard || grads_done) continue;
-        if (gr && s->saved_mode == 2 && s->b && use_rho3d(s) && rho3d_grad_supported(g, s->box)) {
+        if (gr && raw && s->b && use_rho3d(s) && rho3d_grad_supported(g, s->box)) {
             // vp and rho gradients in one pass over the raw snapshots
-            const float *psm = t / f >= 1 ? s->saved + (size_t)(t / f - 1) * per : nullptr;
-            const float *psp = t / f + 1 < s->nsave ? s->saved + (size_t)(t / f + 1) * per : nullptr;
+            const float *psm = t / f >= 1 ? snap(t / f - 1) : nullptr;
+            const float *psp = t / f + 1 < s->nsave ? snap(t / f + 1) : nullptr;
             if (rho3d_grad(g, s->box, ps, psm, psp, fdt2, s->b, u, gv, pv, gr, pr, s->stream) != 0)
                 return fail(SA_E_HIP, "density gradient launch: %s", hipGetErrorString(hipGetLastError()));
             continue;
         }
         if (gv && !(fast && s->saved_mode == 1)) {
             const float *psm = nullptr, *psp = nullptr;
-            if (s->saved_mode == 2) {
-                psm = t / f >= 1 ? s->saved + (size_t)(t / f - 1) * per : nullptr;
-                psp = t / f + 1 < s->nsave ? s->saved + (size_t)(t / f + 1) * per : nullptr;
+            if (raw) {
+                psm = t / f >= 1 ? snap(t / f - 1) : nullptr;
+                psp = t / f + 1 < s->nsave ? snap(t / f + 1) : nullptr;
                 if (!psm || !psp) {
                     // out-of-range neighbours read as zero: stage a zero snapshot
                     TRY(dalloc(&s->work1, std::max<long long>(s->total, (long long)per)));
@@ -2067,7 +2195,168 @@ extern "C" int sa_free_saved(sa_solver *s) {
     s->saved = nullptr;
     s->saved_bytes = 0;
     s->saved_mode = 0;
+    s->saved_on_host = false;
+    dfree(s->segment);
+    s->segment = nullptr;
+    s->segment_bytes = 0;
     return SA_OK;
+}
+
+// the whole domain in snapshot layout
+static Box whole_box(const Geo &g) {
+    Box b;
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = 0;
+        b.n[a] = g.n[a];
+    }
+    b.off2 = 0;
+    b.p2 = (b.n[2] + 31) / 32 * 32;
+    return b;
+}
+
+// the part of the box at least margin[a] cells from the domain edges on each axis (margin null:
+// the whole box); margins follow the caller's axis order (dim of them), not the padded one
+static Box region_of(const sa_solver *s, const Box &box, const int *margin) {
+    Box r = box;
+    if (margin)
+        for (int a = 0; a < 3; ++a) {
+            const int src = a - (3 - s->grid.dim);  // 2D: axis 0 is the unit axis
+            const int m = src >= 0 ? margin[src] : 0;
+            int lo = std::max(box.lo[a], m);
+            int hi = std::min(box.lo[a] + box.n[a], s->geo.n[a] - m);
+            r.lo[a] = lo;
+            r.n[a] = std::max(0, hi - lo);
+        }
+    return r;
+}
+
+// copy nsnap snapshots of a store between device (box layout) and host (region, contiguous)
+static int store_copy(sa_solver *s, float *store, int nsnap, const Box &box, const Box &r, float *host, bool to_host) {
+    const size_t m = (size_t)r.n[0] * r.n[1] * r.n[2];
+    const size_t per = box_points(box);
+    if (m == 0 || nsnap == 0) return SA_OK;
+    TRY(ensure_staging(s, &s->xfer, &s->xfer_n, m));
+    const dim3 grid((r.n[2] + kBlk.x - 1) / kBlk.x, (r.n[1] + kBlk.y - 1) / kBlk.y, r.n[0]);
+    for (int i = 0; i < nsnap; ++i) {
+        float *slot = store + (size_t)i * per;
+        if (to_host) {
+            k_region<<<grid, kBlk, 0, s->stream>>>(box, r, slot, s->xfer, 1);
+            CHECK_LAUNCH();
+            HIP_TRY(hipMemcpyAsync(host + (size_t)i * m, s->xfer, m * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+        } else {
+            HIP_TRY(hipMemcpyAsync(s->xfer, host + (size_t)i * m, m * sizeof(float), hipMemcpyHostToDevice, s->stream));
+            k_region<<<grid, kBlk, 0, s->stream>>>(box, r, slot, s->xfer, 0);
+            CHECK_LAUNCH();
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SA_OK;
+}
+
+static bool stored_snapshots(const sa_solver *s) { return s->saved_mode == 1 || s->saved_mode == 2; }
+
+extern "C" int sa_saved_count(sa_solver *s, const int *margin, long long *n) {
+    if (!s || !n) return fail(SA_E_ARG, "null argument");
+    if (!stored_snapshots(s) || (!s->saved && !s->saved_on_host))
+        return fail(SA_E_STATE, "no stored snapshots (a forward run with save_mode 1 or 2)");
+    for (int a = 0; margin && a < s->grid.dim; ++a)
+        if (margin[a] < 0) return fail(SA_E_ARG, "margins must be >= 0");
+    const Box r = region_of(s, s->box, margin);
+    *n = (long long)s->nsave * r.n[0] * r.n[1] * r.n[2];
+    return SA_OK;
+}
+
+extern "C" int sa_saved_to_host(sa_solver *s, float *host, long long n, const int *margin, int release) {
+    long long want = 0;
+    TRY(sa_saved_count(s, margin, &want));
+    if (!host || n != want) return fail(SA_E_ARG, "host buffer of %lld floats, the snapshots hold %lld", n, want);
+    if (!s->saved) return fail(SA_E_STATE, "the snapshots are already in host memory");
+    HIP_TRY(hipSetDevice(s->device));
+    TRY(store_copy(s, s->saved, s->nsave, s->box, region_of(s, s->box, margin), host, true));
+    if (release) {
+        dfree(s->saved);
+        s->saved = nullptr;
+        s->saved_bytes = 0;
+        s->saved_on_host = true;
+    }
+    return SA_OK;
+}
+
+extern "C" int sa_saved_from_host(sa_solver *s, const float *host, long long n, const int *margin) {
+    long long want = 0;
+    TRY(sa_saved_count(s, margin, &want));
+    if (!host || n != want) return fail(SA_E_ARG, "host buffer of %lld floats, the snapshots hold %lld", n, want);
+    HIP_TRY(hipSetDevice(s->device));
+    const size_t bytes = box_points(s->box) * (size_t)s->nsave * sizeof(float);
+    if (!s->saved) {
+        if (hipMalloc((void **)&s->saved, bytes) != hipSuccess) {
+            s->saved = nullptr;
+            return fail(SA_E_NOMEM, "cannot allocate %.2f GB for the saved wavefield", bytes / 1e9);
+        }
+        s->saved_bytes = bytes;
+    }
+    // what the host copy does not cover reads as zero
+    HIP_TRY(hipMemsetAsync(s->saved, 0, bytes, s->stream));
+    TRY(store_copy(s, s->saved, s->nsave, s->box, region_of(s, s->box, margin), const_cast<float *>(host), false));
+    s->saved_on_host = false;
+    return SA_OK;
+}
+
+extern "C" int sa_set_dump(sa_solver *s, int forward_factor, int adjoint_factor) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    if (forward_factor < 0 || adjoint_factor < 0) return fail(SA_E_ARG, "dump factors must be >= 0");
+    s->dump_f[0] = forward_factor;
+    s->dump_f[1] = adjoint_factor;
+    return SA_OK;
+}
+
+// (tb1 - tb0 + f) / f + 1 slots, slot t / f (common/devito.py:552-578)
+static int dump_alloc(sa_solver *s, int w) {
+    const int f = s->dump_f[w];
+    s->dump_n[w] = 0;
+    if (f <= 0) return SA_OK;
+    const int n = (s->sch.tb1 - s->sch.tb0 + f) / f + 1;
+    const size_t bytes = box_points(whole_box(s->geo)) * (size_t)n * sizeof(float);
+    if (!s->dump[w] || s->dump_bytes[w] < bytes) {
+        dfree(s->dump[w]);
+        s->dump[w] = nullptr;
+        s->dump_bytes[w] = 0;
+        if (hipMalloc((void **)&s->dump[w], bytes) != hipSuccess) {
+            s->dump[w] = nullptr;
+            return fail(SA_E_NOMEM, "cannot allocate %.2f GB for the wavefield dump", bytes / 1e9);
+        }
+        s->dump_bytes[w] = bytes;
+    }
+    HIP_TRY(hipMemsetAsync(s->dump[w], 0, bytes, s->stream));
+    s->dump_n[w] = n;
+    return SA_OK;
+}
+
+// level `u` of time t into the dump of pass w when t is a dump step
+static int dump_level(sa_solver *s, int w, int t, const float *u) {
+    const int f = s->dump_f[w];
+    if (f <= 0 || !s->dump[w] || t % f != 0 || t < s->sch.tb0 || t > s->sch.tb1 || t / f >= s->dump_n[w])
+        return SA_OK;
+    const Box b = whole_box(s->geo);
+    k_pack<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, b, u, s->dump[w] + (size_t)(t / f) * box_points(b));
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
+extern "C" int sa_dump_count(sa_solver *s, int which, long long *n) {
+    if (!s || !n || which < 0 || which > 1) return fail(SA_E_ARG, "bad argument");
+    *n = (long long)s->dump_n[which] * s->geo.n[0] * s->geo.n[1] * s->geo.n[2];
+    return SA_OK;
+}
+
+extern "C" int sa_dump_to_host(sa_solver *s, int which, float *host, long long n) {
+    long long want = 0;
+    TRY(sa_dump_count(s, which, &want));
+    if (want == 0) return fail(SA_E_STATE, "no %s wavefield dump: sa_set_dump before the run", which ? "adjoint" : "forward");
+    if (!host || n != want) return fail(SA_E_ARG, "host buffer of %lld floats, the dump holds %lld", n, want);
+    HIP_TRY(hipSetDevice(s->device));
+    const Box b = whole_box(s->geo);
+    return store_copy(s, s->dump[which], s->dump_n[which], b, b, host, true);
 }
 
 extern "C" int sa_set_tile(sa_solver *s, int rows, int chunk, int version) {

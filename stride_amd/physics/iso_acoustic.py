@@ -159,6 +159,30 @@ class ProblemTypeBase:
 
 # grids from this size up are autotuned by default (the tuning runs take about a second)
 _AUTOTUNE_MIN_POINTS = 1 << 24
+# raw snapshots above this many bytes per shot are recomputed from checkpoints instead of stored
+SAVE_BUDGET_BYTES = 64 << 30
+
+
+def raw_checkpoint(plan, checkpoint=None, budget=SAVE_BUDGET_BYTES):
+    """Snapshots per recomputation segment for the raw saves of the rho / alpha / wavelet
+    gradients (0: store them all, as devito.py:366-437 does). `checkpoint`: an explicit segment
+    length, or None for automatic: segments of K = ceil(sqrt(2 N)) snapshots (N snapshots), which
+    keeps 2 N / K + K + 2 snapshot-sized buffers, once the N stored snapshots would take more
+    than `budget` bytes. The recomputation needs the two time levels alone to restart, so the
+    CFS-PML and power-2 attenuation schemes always store."""
+    if plan.boundary == PML or plan.att_power == 2:
+        return 0
+    nsnap = plan.tb1 // plan.undersampling + 1
+    if checkpoint is not None:
+        return max(0, int(checkpoint))
+    if nsnap * int(np.prod(plan.shape)) * 4 <= budget:
+        return 0
+    return int(np.ceil(np.sqrt(2 * nsnap)))
+
+
+def _dump_factor(option, plan):
+    # an int samples every `option` steps; True the save undersampling (devito.py:421-424)
+    return option if type(option) is int else plan.undersampling
 
 
 def _needs_grad(*variables):
@@ -183,6 +207,8 @@ class IsoAcousticHIP(ProblemTypeBase):
         self._grads = None
         self._save_mode = 0
         self._bandwidth = (0., 0., 0.)
+        self._cache = None  # cache_forward: the snapshots (array) or the file holding them
+        self._cache_folder = None
 
     # operator lifetime -------------------------------------------------------------------
 
@@ -194,8 +220,21 @@ class IsoAcousticHIP(ProblemTypeBase):
     def deallocate_wavefield(self, platform='gpu', deallocate=False, **kwargs):
         if self._solver is not None:
             self._solver.free_saved()
+        self._cache = None
         if deallocate:
             self.clear_operators()
+
+    @property
+    def wavefield(self):
+        """The stored forward snapshots, (nsave, stored box) (devito.py:198-212); None when the
+        last forward run kept none or recomputes them from checkpoints."""
+        if self._solver is None or self._save_mode not in (1, 2) or self._cache is not None:
+            return None
+        data = self._solver.saved_to_host(release=False)
+        plan = self._plan
+        box = plan.shape if self._save_mode == 2 else tuple(n - 2 * a for n, a in zip(plan.shape, plan.absorbing))
+        return ScalarField('p', data=data.reshape((-1,) + tuple(box)), shape=(data.size // int(np.prod(box)),) + box,
+                           extended_shape=(data.size // int(np.prod(box)),) + box)
 
     @property
     def solver(self):
@@ -259,12 +298,23 @@ class IsoAcousticHIP(ProblemTypeBase):
             save_mode = 2
         else:
             save_mode = 1
+        checkpoint = 0
+        if save_mode == 2:
+            checkpoint = raw_checkpoint(plan, kwargs.get('checkpoint_snapshots'),
+                                        kwargs.get('save_budget', SAVE_BUDGET_BYTES))
+            if checkpoint:
+                save_mode = 5
+                logger.debug('(ShotID %d) raw snapshots recomputed in segments of %d', shot.id, checkpoint)
         self._save_mode = save_mode
+        self._cache = None
 
         solver = self._get_solver(plan, device)
         solver.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
                           plan.undersampling, save_mode, plan.tb0, plan.tb1, plan.grad_vp, plan.grad_rho,
-                          plan.grad_src)
+                          plan.grad_src, checkpoint=checkpoint)
+        dump = kwargs.get('dump_forward_wavefield', False)
+        dump_id = kwargs.get('dump_wavefield_id', shot.id)
+        solver.set_dump(_dump_factor(dump, plan) if dump and dump_id == shot.id else 0, 0)
         solver.set_medium(plan.vp, plan.rho, plan.alpha)
         if plan.boundary == SPONGE:
             solver.set_sponge(plan.sponge_tables)
@@ -290,8 +340,15 @@ class IsoAcousticHIP(ProblemTypeBase):
         self._traces = self._solver.forward(self._plan.src_data)
 
     def after_forward(self, wavelets, vp, rho=None, alpha=None, **kwargs):
-        shot = kwargs['problem'].shot
+        problem = kwargs['problem']
+        shot = problem.shot
         plan = self._plan
+        dump = kwargs.get('dump_forward_wavefield', False)
+        if dump and kwargs.get('dump_wavefield_id', shot.id) == shot.id and self._traces is not None:
+            _dump_wavefield(self._solver.dump_to_host(0), 'forward_wavefield-Shot%05d' % shot.id, self.space,
+                            problem, kwargs)
+        if kwargs.get('cache_forward', False) and self._save_mode in (1, 2) and self._traces is not None:
+            self._cache_snapshots(problem, kwargs.get('cache_location'))
         data = self._traces.T if self._traces is not None else np.zeros((plan.nrec, plan.nt), np.float32)
         traces = shot.observed.alike(name='modelled', data=np.ascontiguousarray(data, dtype=np.float32))
         if kwargs.get('deallocate', False) and self._save_mode == 0:
@@ -306,6 +363,13 @@ class IsoAcousticHIP(ProblemTypeBase):
         data = adjoint_source.data if hasattr(adjoint_source, 'data') else adjoint_source
         self._res = adjoint_source_samples(self._plan, data)
         self._res_norm = np.linalg.norm(data)
+        if self._cache is not None:
+            self._restore_snapshots()
+        problem = kwargs.get('problem')
+        dump = kwargs.get('dump_adjoint_wavefield', False)
+        shot_id = problem.shot_id if problem is not None else None
+        self._solver.set_dump(0, _dump_factor(dump, self._plan) if dump and
+                              kwargs.get('dump_wavefield_id', shot_id) == shot_id else 0)
         self.init_grad(wavelets, vp, rho, alpha, **kwargs)
 
     def run_adjoint(self, adjoint_source, wavelets, vp, rho=None, alpha=None, **kwargs):
@@ -324,7 +388,40 @@ class IsoAcousticHIP(ProblemTypeBase):
                                        want_src=bool(getattr(wavelets, 'needs_grad', False)))
             self._grads = ('compat', out)
 
+    # cache_forward (devito.py:632-668, 820-842): the interior of the snapshots in host memory, or
+    # in a .npy file under cache_location, between the forward and the adjoint run
+    def _cache_snapshots(self, problem, location):
+        extra = tuple(self.space.extra)
+        data = self._solver.saved_to_host(margin=extra, release=True)
+        if location is None:
+            self._cache = data
+            return
+        import glob
+        import os
+        import tempfile
+        if self._cache_folder is None:
+            prev = glob.glob(os.path.join(location, 'stride-*'))
+            self._cache_folder = prev[0] if prev else tempfile.mkdtemp(prefix='stride-', dir=location)
+        filename = os.path.join(self._cache_folder, '%s-%s-%05d.npy' % (problem.name, 'P', problem.shot_id))
+        np.save(filename, data)
+        self._cache = filename
+
+    def _restore_snapshots(self):
+        import os
+        data = self._cache
+        if isinstance(data, str):
+            filename, data = data, np.load(data)
+            os.remove(filename)
+        self._solver.saved_from_host(data, margin=tuple(self.space.extra))
+        self._cache = None
+
     def after_adjoint(self, adjoint_source, wavelets, vp, rho=None, alpha=None, **kwargs):
+        problem = kwargs.get('problem')
+        dump = kwargs.get('dump_adjoint_wavefield', False)
+        if dump and problem is not None and kwargs.get('dump_wavefield_id', problem.shot_id) == problem.shot_id \
+                and self._res_norm >= 1e-31:
+            _dump_wavefield(self._solver.dump_to_host(1), 'adjoint_wavefield-Shot%05d' % problem.shot_id, self.space,
+                            problem, kwargs)
         grads = self.get_grad(wavelets, vp, rho, alpha, **kwargs)
         self.deallocate_wavefield(deallocate=kwargs.get('deallocate', False))
         return grads
@@ -444,6 +541,15 @@ def _dump_local(grad, name, kwargs):
     if kwargs.get('dump_local_prec', False) and grad.prec is not None:
         grad.prec.dump(path=problem.output_folder, project_name=problem.name,
                        parameter='%s_local_src_prec-Shot%05d' % (name, problem.shot_id), version=version)
+
+
+def _dump_wavefield(data, name, space, problem, kwargs):
+    """The dumped levels (slots, extended shape) written as the reference's StructuredData dump
+    (devito.py:617-623, 955-963), version = iteration.abs_id + 1."""
+    iteration = kwargs.get('iteration')
+    version = iteration.abs_id + 1 if iteration is not None else 0
+    field = ScalarField(name, space=space, data=data, shape=data.shape, extended_shape=data.shape)
+    return field.dump(path=problem.output_folder, project_name=problem.name, version=version)
 
 
 def _tukey_for(plan):

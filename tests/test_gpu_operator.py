@@ -124,3 +124,59 @@ def test_stride_operator_contract(gpu):
     grads = asyncio.run(pde.__call_adjoint__(res))
     np.testing.assert_array_equal(grads[1].data, g_ref.data)
     np.testing.assert_array_equal(grads[1].prec.data, g_ref.prec.data)
+
+
+def test_storage_options_keep_gradients(gpu, tmp_path):
+    """Raw-snapshot storage through the operator (rho and vp gradients, sponge, attenuation 0):
+    recomputed from checkpoints (checkpoint_snapshots, and automatically above save_budget),
+    cached in host memory and on disk (cache_forward / cache_location, devito.py:632-668,
+    820-842) and with the forward and adjoint wavefields dumped (dump_forward_wavefield /
+    dump_adjoint_wavefield). Recomputation and the host round trip of the whole box leave the
+    gradients bit for bit; cache_forward keeps the interior only, as the reference does."""
+    from stride_amd.problem import ScalarField as SF
+    cfg = config2(scale=0.125, nt=200)
+    space, time = cfg['space'], cfg['time']
+    shot = Shot(0, cfg['sources'][:1], cfg['receivers'], wavelets=cfg['wavelet'][None].astype(np.float32))
+    problem = SubProblem(shot, space, time, name='storage', output_folder=str(tmp_path))
+
+    factor = [None]
+
+    def run(**kw):
+        vp = ScalarField('vp', space=space, data=cfg['vp'], needs_grad=True)
+        rho = ScalarField('rho', space=space, data=cfg['rho'], needs_grad=True)
+        alpha = ScalarField('alpha', space=space, data=cfg['alpha'])
+        pde = IsoAcousticHIP(space=space, time=time)
+        opts = dict(problem=problem, attenuation_power=0, boundary_type='sponge_boundary_2', **kw)
+        traces = pde.forward(shot.wavelets, vp, rho, alpha, **opts)
+        mode = pde._save_mode
+        out = pde.adjoint(Traces('res', data=traces.data * 0.1), shot.wavelets, vp, rho, alpha, **opts)
+        factor[0] = pde.undersampling_factor
+        pde.clear_operators()
+        return mode, out[1].data.copy(), out[2].data.copy()
+
+    mode, gv, gr = run()
+    assert mode == 2 and np.abs(gv).max() > 0 and np.abs(gr).max() > 0
+    for kw in (dict(checkpoint_snapshots=3), dict(save_budget=1 << 20)):
+        mode, v, r = run(**kw)
+        print('  %s: save mode %d' % (kw, mode), flush=True)
+        assert mode == 5
+        assert np.array_equal(v, gv) and np.array_equal(r, gr)
+    for kw in (dict(cache_forward=True), dict(cache_forward=True, cache_location=str(tmp_path))):
+        mode, v, r = run(**kw)
+        ev, er = np.linalg.norm(v - gv) / np.linalg.norm(gv), np.linalg.norm(r - gr) / np.linalg.norm(gr)
+        print('  %s: rel L2 vs stored: vp %.3e, rho %.3e' % (sorted(kw), ev, er), flush=True)
+        # the vp gradient reads the snapshots point by point: unchanged on the interior; the rho
+        # gradient's derivatives see the zeroed layer next to the interior's edge
+        assert ev <= 1e-6 and np.isfinite(r).all()
+    assert not list(tmp_path.glob('stride-*/*.npy'))
+    run(dump_forward_wavefield=True, dump_adjoint_wavefield=2)
+    fwd = SF('f').load(path=str(tmp_path), project_name='storage', parameter='forward_wavefield-Shot00000',
+                       version=0)
+    adj = SF('a').load(path=str(tmp_path), project_name='storage', parameter='adjoint_wavefield-Shot00000',
+                       version=0)
+    ext = space.extended_shape
+    assert fwd.extended_data.shape[1:] == ext and adj.extended_data.shape[1:] == ext
+    assert np.abs(fwd.extended_data).max() > 0 and np.abs(adj.extended_data).max() > 0
+    f = factor[0]
+    assert fwd.extended_data.shape[0] == (time.num + f) // f + 1
+    assert adj.extended_data.shape[0] == (time.num + 2) // 2 + 1

@@ -278,3 +278,21 @@ def test_autotune_scheme_predicates():
     assert not uses_density_update(pml) and not uses_fast_update(pml, 0)
     flat, _ = small_case(dim=2, n=20, nabs=4, nt=160, kernel='OT2', rho='var')
     assert not uses_density_update(flat) and not uses_fast_update(flat, 0)
+
+
+def test_raw_checkpoint_choice():
+    """Segment length of the recomputed raw snapshots: stored below the budget, K = ceil(sqrt(2N))
+    above it, an explicit value as given, and never for the schemes whose state has more than
+    the two time levels (CFS-PML, power-2 attenuation)."""
+    from types import SimpleNamespace
+    from stride_amd.physics.iso_acoustic import raw_checkpoint, _dump_factor
+    from stride_amd.physics.boundaries import SPONGE, PML
+    plan = SimpleNamespace(boundary=SPONGE, att_power=0, tb1=2500, undersampling=4, shape=(360, 360, 360))
+    n = 2500 // 4 + 1
+    assert raw_checkpoint(plan) == int(np.ceil(np.sqrt(2 * n))) == 36  # 117 GB of snapshots
+    assert raw_checkpoint(plan, budget=1 << 40) == 0
+    assert raw_checkpoint(plan, checkpoint=5) == 5 and raw_checkpoint(plan, checkpoint=0) == 0
+    assert raw_checkpoint(SimpleNamespace(**{**vars(plan), 'boundary': PML})) == 0
+    assert raw_checkpoint(SimpleNamespace(**{**vars(plan), 'att_power': 2}), checkpoint=4) == 0
+    # dump factors: an int as given, True the save undersampling (type(x) is int, devito.py:421-424)
+    assert _dump_factor(3, plan) == 3 and _dump_factor(True, plan) == 4
