@@ -320,15 +320,15 @@ def unet_flops(N, H, W):
 def bench_learned(ntrain=30, epochs=40, neval=4, iters_per_freq=4, nsrc=16, log=None):
     """The learned-preconditioner workload on the 2D gradient grid: solver-generated training
     pairs (every gradient of plain inversions of ntrain training phantoms), U-Net training, then
-    inversions of unseen phantoms with frequency continuation
-    (300 - 700 kHz), learned (misfit-gated, plain gradient step as fallback) against plain
-    L-BFGS; solver calls counted as passes over all shots (a gradient is two: forward and
-    adjoint)."""
+    inversions of unseen phantoms with frequency continuation (300 - 700 kHz): L-BFGS whose first
+    step in each band is the network's update when the misfit drops enough (the scaled gradient
+    step otherwise) against plain L-BFGS; solver calls counted as passes over all shots (a
+    gradient is two: forward and adjoint; a rejected line-search trial one, both methods)."""
     import torch
     from stride_amd import learned
     log = log or (lambda m: print('[learned] ' + m, file=sys.stderr, flush=True))
     freqs = [300e3, 400e3, 500e3, 600e3, 700e3]
-    fwi = learned.RingFWI(nsrc=nsrc, workers=2)
+    fwi = learned.RingFWI(nsrc=nsrc, workers=nsrc)
     t0 = time.perf_counter()
     pairs = learned.make_trajectory_pairs(fwi, range(ntrain), freqs, iters_per_freq, log=log)
     t_pairs = time.perf_counter() - t0
@@ -360,15 +360,17 @@ def bench_learned(ntrain=30, epochs=40, neval=4, iters_per_freq=4, nsrc=16, log=
         vp_p, hp = learned.invert_plain(fwi, true, freqs, iters_per_freq)
         log('phantom %d plain: %d passes, misfit %.4g, model error %.4f' % (k, hp[-1][0], hp[-1][1], hp[-1][2]))
         fwi.passes = 0
-        vp_l, hl, acc = learned.invert_learned(fwi, net, scales, true, freqs, iters_per_freq)
-        log('phantom %d learned: %d passes, misfit %.4g, model error %.4f, %d of %d updates from the network'
-            % (k, hl[-1][0], hl[-1][1], hl[-1][2], acc, len(hl)))
+        vp_l, hl, acc = learned.invert_learned_lbfgs(fwi, net, scales, true, freqs, iters_per_freq)
+        log('phantom %d learned: %d passes, misfit %.4g, model error %.4f, %d of %d band-first steps from the '
+            'network' % (k, hl[-1][0], hl[-1][1], hl[-1][2], acc, len(freqs)))
         last = freqs[-1]
         plain_final = hp[-1][1]
         reach = [h[0] for h in hl if h[3] == last and h[1] <= plain_final]
+        reach_err = [h[0] for h in hl if h[2] <= hp[-1][2]]
         results.append(dict(phantom=k, plain_passes=hp[-1][0], plain_misfit=plain_final, plain_error=hp[-1][2],
                             learned_passes=hl[-1][0], learned_misfit=hl[-1][1], learned_error=hl[-1][2],
-                            learned_passes_to_plain_misfit=reach[0] if reach else None, network_updates=acc,
+                            learned_passes_to_plain_misfit=reach[0] if reach else None,
+                            learned_passes_to_plain_error=reach_err[0] if reach_err else None, network_updates=acc,
                             plain_history=[[h[0], h[1], h[2]] for h in hp],
                             learned_history=[[h[0], h[1], h[2]] for h in hl]))
     fwi.close()

@@ -785,12 +785,22 @@ static int dalloc(float **p, size_t n) {
         *p = nullptr;
         return fail(SA_E_NOMEM, "hipMalloc of %zu floats failed", n);
     }
-    if (hipMemset(*p, 0, n * sizeof(float)) != hipSuccess) return fail(SA_E_HIP, "hipMemset failed");
+    // hipMemset on device memory may return before the fill has run, and the solver's kernels run
+    // on non-blocking streams that do not wait for the null stream: wait for it here
+    if (hipMemset(*p, 0, n * sizeof(float)) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess)
+        return fail(SA_E_HIP, "hipMemset failed");
     return SA_OK;
 }
 
 static void dfree(void *p) {
     if (p) (void)hipFree(p);
+}
+
+// Host-to-device upload on the null stream, waited for: the solver's kernels run on streams that
+// may be non-blocking (sa_stream_create), which do not wait for null-stream work.
+static hipError_t h2d(void *dst, const void *src, size_t bytes) {
+    hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    return e != hipSuccess ? e : hipStreamSynchronize(nullptr);
 }
 
 #define TRY(x)                  \
@@ -851,7 +861,7 @@ static int upload_padded(sa_solver *s, float **dev, const float *host, bool edge
     TRY(dalloc(dev, s->total));
     std::vector<float> tmp;
     pad_host(s, tmp, host, edge);
-    HIP_TRY(hipMemcpy(*dev, tmp.data(), s->total * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(h2d(*dev, tmp.data(), s->total * sizeof(float)));
     return SA_OK;
 }
 
@@ -1069,7 +1079,7 @@ extern "C" int sa_set_sponge(sa_solver *s, const float *tables, int len) {
     for (int d = 0; d < s->grid.dim; ++d)
         for (int l = 0; l < len; ++l) t[(size_t)(d + off) * len + l] = tables[(size_t)d * len + l];
     TRY(dalloc(&s->tables, t.size()));
-    HIP_TRY(hipMemcpy(s->tables, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(h2d(s->tables, t.data(), t.size() * sizeof(float)));
     s->tlen = len;
     s->coef_dirty = true;
     return SA_OK;
@@ -1084,8 +1094,7 @@ extern "C" int sa_set_pml(sa_solver *s, const float *profiles, int stride) {
         if (stride < s->geo.n[a]) return fail(SA_E_ARG, "pml stride too short");
         for (int r = 0; r < 4; ++r) {
             TRY(dalloc(&s->pml[a][r], s->geo.n[a]));
-            HIP_TRY(hipMemcpy(s->pml[a][r], profiles + ((size_t)d * 4 + r) * stride, s->geo.n[a] * sizeof(float),
-                              hipMemcpyHostToDevice));
+            HIP_TRY(h2d(s->pml[a][r], profiles + ((size_t)d * 4 + r) * stride, s->geo.n[a] * sizeof(float)));
         }
     }
     return SA_OK;
@@ -1115,8 +1124,8 @@ static int set_sparse(sa_solver *s, Sparse &sp, int npoint, int ntap, const int6
     for (size_t q = 0; q < nq; ++q) pidx[q] = padded(sp.idx[q]);
     HIP_TRY(hipMalloc((void **)&sp.d_idx, nq * sizeof(long long)));
     HIP_TRY(hipMalloc((void **)&sp.d_w, nq * sizeof(float)));
-    HIP_TRY(hipMemcpy(sp.d_idx, pidx.data(), nq * sizeof(long long), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(sp.d_w, sp.w.data(), nq * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(h2d(sp.d_idx, pidx.data(), nq * sizeof(long long)));
+    HIP_TRY(h2d(sp.d_w, sp.w.data(), nq * sizeof(float)));
     // destination-sorted CSR for the deterministic scatter (stable order of contributions)
     std::vector<size_t> ord;
     for (size_t q = 0; q < nq; ++q)
@@ -1150,12 +1159,12 @@ static int set_sparse(sa_solver *s, Sparse &sp, int npoint, int ntap, const int6
     HIP_TRY(hipMalloc((void **)&sp.d_cw, cw.size() * sizeof(float)));
     HIP_TRY(hipMalloc((void **)&sp.d_coef, cw.size() * sizeof(float)));
     HIP_TRY(hipMalloc((void **)&sp.d_ijk, ijk.size() * sizeof(int)));
-    HIP_TRY(hipMemcpy(sp.d_ijk, ijk.data(), ijk.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(sp.d_dest, dest.data(), dest.size() * sizeof(long long), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(sp.d_lin, lin.data(), lin.size() * sizeof(long long), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(sp.d_off, offv.data(), offv.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(sp.d_point, point.data(), point.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(sp.d_cw, cw.data(), cw.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_TRY(h2d(sp.d_ijk, ijk.data(), ijk.size() * sizeof(int)));
+    HIP_TRY(h2d(sp.d_dest, dest.data(), dest.size() * sizeof(long long)));
+    HIP_TRY(h2d(sp.d_lin, lin.data(), lin.size() * sizeof(long long)));
+    HIP_TRY(h2d(sp.d_off, offv.data(), offv.size() * sizeof(int)));
+    HIP_TRY(h2d(sp.d_point, point.data(), point.size() * sizeof(int)));
+    HIP_TRY(h2d(sp.d_cw, cw.data(), cw.size() * sizeof(float)));
     return SA_OK;
 }
 
@@ -1339,8 +1348,8 @@ static int tile_points(sa_solver *s, Sparse &sp) {
         if (tile[r] >= 0) pt[fill[tile[r]]++] = r;
     HIP_TRY(hipMalloc((void **)&sp.d_toff, off.size() * sizeof(int)));
     HIP_TRY(hipMalloc((void **)&sp.d_tpt, pt.size() * sizeof(int)));
-    HIP_TRY(hipMemcpy(sp.d_toff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(sp.d_tpt, pt.data(), pt.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(h2d(sp.d_toff, off.data(), off.size() * sizeof(int)));
+    HIP_TRY(h2d(sp.d_tpt, pt.data(), pt.size() * sizeof(int)));
     return SA_OK;
 }
 
@@ -2167,7 +2176,7 @@ extern "C" int sa_exact_gradients_dev(sa_solver *s, const float *gA, const float
         long long *d_lin = nullptr;
         HIP_TRY(hipMalloc((void **)&d_lin, std::max<size_t>(lin.size(), 1) * sizeof(long long)));
         int rc = SA_OK;
-        if (hipMemcpy(d_lin, lin.data(), lin.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess)
+        if (h2d(d_lin, lin.data(), lin.size() * sizeof(long long)) != hipSuccess)
             rc = fail(SA_E_HIP, "copy of the tap indices failed");
         if (rc == SA_OK) {
             k_source_grads<<<1, 64, 0, s->stream>>>(s->grid.nt, s->src.npoint, s->src.ntap, lam_src, src, s->src.d_idx,
@@ -2696,9 +2705,9 @@ extern "C" int sa_multi_forward_dev(sa_solver *s, const float *const *src, float
             HIP_TRY(hipMemsetAsync(rec[order[i]], 0, sizeof(float) * (size_t)m.nt * m.rec.npoint, s->stream));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));  // coefficients computed, traces cleared
-    HIP_TRY(hipMemcpy(s->d_pairs, pl.data(), pl.size() * sizeof(PairLevels), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->d_minj, inj.data(), inj.size() * sizeof(MInject), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->d_mgat, gat.data(), gat.size() * sizeof(MGather), hipMemcpyHostToDevice));
+    HIP_TRY(h2d(s->d_pairs, pl.data(), pl.size() * sizeof(PairLevels)));
+    HIP_TRY(h2d(s->d_minj, inj.data(), inj.size() * sizeof(MInject)));
+    HIP_TRY(h2d(s->d_mgat, gat.data(), gat.size() * sizeof(MGather)));
     // clear the levels on the solver's stream, outside any graph
     for (float *p : s->mbuf) HIP_TRY(hipMemsetAsync(p, 0, 2 * s->total * sizeof(float), s->stream));
     const int tlast = s->mshot[order[0]].nt - 2;

@@ -7,8 +7,13 @@ on pairs (gradient at a model m_b, true perturbation m_true - m_b) generated by 
 random smooth phantoms (1400 - 1650 m/s, sigma 2 - 5 mm, inside a 45 mm disc), where m_b is the
 water background plus a random fraction of the smoothed perturbation (a partly recovered
 model). During an inversion the network's update is kept only if the forward solver's misfit
-drops; otherwise the plain gradient step is taken (with backtracking). The plain inversion it is
-compared with is L-BFGS on the same misfit and gradient.
+drops; otherwise the plain step is taken. Two forms: ``invert_learned`` proposes at every
+iteration (plain gradient step with backtracking as the fallback); ``invert_learned_lbfgs`` runs
+the plain inversion's L-BFGS and replaces only the first step of each frequency band by the
+network's update when that passes the line search's sufficient-decrease test. The plain
+inversion it is compared with is L-BFGS on the same misfit and gradient. With one worker per
+shot, a trial point's adjoint runs only when its forward misfit passes the test, so a rejected
+trial costs one pass over the shots, in the plain and the learned inversions alike.
 
 No counterpart in the reference: Stride's inversion loop (stride/__init__.py:172-494,
 stride/optimisation/optimisation_loop.py) steps with the gradient of ``get_grad_vp``
@@ -34,7 +39,7 @@ from .utils import tone_burst, elliptical
 from .workers import ShotWorkers
 
 __all__ = ['RingFWI', 'phantom', 'make_pairs', 'make_trajectory_pairs', 'train', 'invert_plain',
-           'invert_learned']
+           'invert_learned', 'invert_learned_lbfgs']
 
 PML = 'complex_frequency_shift_PML_2'
 VP_BOUNDS = (1300., 1800.)
@@ -148,9 +153,12 @@ class RingFWI:
         self.passes += 1
         return f
 
-    def gradient(self, vp_inner, freq):
+    def gradient(self, vp_inner, freq, below=None):
         """(misfit, vp gradient on the inner grid as a device tensor): the shots' vp gradients
-        summed, with get_grad_vp's slowness rescale (x 1 / vp)."""
+        summed, with get_grad_vp's slowness rescale (x 1 / vp). With `below`, when every shot has a
+        worker of its own (workers >= nsrc, so each keeps its forward wavefield): the forward
+        runs over all shots first and, when the misfit is not below `below`, (misfit, None) is
+        returned after that one pass, without the adjoint."""
         torch = _torch()
         plan = self._plan(vp_inner, freq, grad=True)
         src = self._configure(plan, 1)
@@ -160,19 +168,35 @@ class RingFWI:
         torch.cuda.synchronize(self.dev)
         wid = {id(s): i for i, s in enumerate(self.pool.solvers)}
 
-        def shot(s, k):
+        def forward(s, k):
             b = self.buf[wid[id(s)]]
             s.set_sources(plan.src_idx[k:k + 1], plan.src_w[k:k + 1])
             s.forward_dev(src[k].data_ptr(), b['rec'].data_ptr())
-            f = s.traces_l2(b['rec'].data_ptr(), obs[k].data_ptr(), b['res'].data_ptr(), self.nt, self.nrec)
+            return s.traces_l2(b['rec'].data_ptr(), obs[k].data_ptr(), b['res'].data_ptr(), self.nt, self.nrec)
+
+        def adjoint(s, k):
+            b = self.buf[wid[id(s)]]
             # the residual is zero at t = 0 and nt - 1 (no receiver samples there), where the
             # adjoint source window of devito.py:863-869 is zero: the window changes nothing
             s.adjoint_dev(b['res'].data_ptr(), want=1)
             s.read_gradient_dev(0, b['g'].data_ptr())
             s.field_axpy(b['total'].data_ptr(), b['g'].data_ptr(), self.ext_points, 1.0)
-            return f
 
-        f = float(sum(self.pool.map(shot, range(self.nsrc))))
+        if below is not None and len(self.pool.solvers) >= self.nsrc:
+            f = float(sum(self.pool.map(forward, range(self.nsrc))))
+            self.passes += 1
+            if not f < below:
+                return f, None
+            self.pool.map(adjoint, range(self.nsrc))
+            self.passes += 1
+        else:
+            def shot(s, k):
+                f = forward(s, k)
+                adjoint(s, k)
+                return f
+
+            f = float(sum(self.pool.map(shot, range(self.nsrc))))
+            self.passes += 2
         s0 = self.pool.solvers[0]
         total = self.buf[0]['total']
         for b in self.buf[1:]:
@@ -184,7 +208,6 @@ class RingFWI:
         vp = torch.from_numpy(np.ascontiguousarray(vp_inner, dtype=np.float32)).to(self.dev)
         s0.field_grad_vp_finish(g.data_ptr(), None, vp.data_ptr(), g.numel(), slowness=True)
         s0.synchronize()
-        self.passes += 2
         return f, g
 
     def close(self):
@@ -284,6 +307,20 @@ def make_trajectory_pairs(fwi, ks, freqs, iters_per_freq=4, log=None):
     return pairs
 
 
+def _fun_and_grad(fwi, shape, freq, record=None):
+    """L-BFGS objective in slowness: (misfit, gradient) at x, or (misfit, None) when the misfit
+    exceeds the trial's threshold (the adjoint skipped)."""
+    def fg(xs, threshold=None):
+        vp = (1 / xs).reshape(shape).astype(np.float32)
+        f, g = fwi.gradient(vp, freq, below=threshold)
+        if g is None:
+            return f, None
+        if record is not None:
+            record(g, vp, freq)
+        return f, g.double().cpu().numpy().reshape(-1)
+    return fg
+
+
 def invert_plain(fwi, true, freqs, iters_per_freq=4, first_step=20.0, record=None):
     """L-BFGS per frequency band (restarted at each band) from water, in slowness, the space the
     gradient of get_grad_vp belongs to with vp's default transform (the reference steps in the
@@ -300,18 +337,46 @@ def invert_plain(fwi, true, freqs, iters_per_freq=4, first_step=20.0, record=Non
         fwi.observe(true, freq)
         opt = LBFGS(memory=5, first_step=first_step / 1500. ** 2, bounds=bounds)
 
-        def fg(xs):
-            vp = (1 / xs).reshape(shape).astype(np.float32)
-            f, g = fwi.gradient(vp, freq)
-            if record is not None:
-                record(g, vp, freq)
-            return f, g.double().cpu().numpy().reshape(-1)
+        def cb(xs, f):
+            hist.append((fwi.passes - base, f, _model_error((1 / xs).reshape(shape), true), freq))
+
+        x, _ = opt.minimize(x, _fun_and_grad(fwi, shape, freq, record), iters_per_freq, callback=cb, gated=True)
+    return (1 / x).reshape(shape).astype(np.float32), hist
+
+
+def invert_learned_lbfgs(fwi, net, scales, true, freqs, iters_per_freq=4, first_step=20.0):
+    """L-BFGS per band exactly as invert_plain, except that the first step of every band is the
+    network's update from that band's first gradient and the current model: it is kept when the
+    misfit drops enough (the line search's sufficient-decrease test at full length), otherwise
+    the scaled gradient step of invert_plain is taken. Returns (vp, history, proposals kept)."""
+    from .optim import LBFGS
+    torch = _torch()
+    shape = fwi.space.shape
+    x = np.full(int(np.prod(shape)), 1 / 1500., np.float64)
+    hist = []
+    kept = 0
+    base = fwi.passes
+    bounds = (1 / VP_BOUNDS[1], 1 / VP_BOUNDS[0])
+    for freq in freqs:
+        fwi.observe(true, freq)
+        opt = LBFGS(memory=5, first_step=first_step / 1500. ** 2, bounds=bounds)
+        scale = scales[min(scales, key=lambda k: abs(k - freq))]
+
+        def proposal(xs, g):
+            vp = (1 / xs).reshape(shape)
+            gt = torch.from_numpy(g.reshape(shape).astype(np.float32)).to(fwi.dev)
+            model = torch.from_numpy(((vp - 1500.) / TARGET_SCALE).astype(np.float32)).to(fwi.dev)
+            d = (net.forward((gt * scale)[None], scale=1.0, model=model[None])[0] * TARGET_SCALE).cpu().numpy()
+            new = np.clip(vp + d, *VP_BOUNDS)
+            return (xs - 1 / new.reshape(-1)).astype(np.float64)  # the step is x -> 1 / new
 
         def cb(xs, f):
             hist.append((fwi.passes - base, f, _model_error((1 / xs).reshape(shape), true), freq))
 
-        x, _ = opt.minimize(x, fg, iters_per_freq, callback=cb)
-    return (1 / x).reshape(shape).astype(np.float32), hist
+        x, _ = opt.minimize(x, _fun_and_grad(fwi, shape, freq), iters_per_freq, callback=cb,
+                            first_direction=proposal, gated=True)
+        kept += opt.proposals_kept
+    return (1 / x).reshape(shape).astype(np.float32), hist, kept
 
 
 def _gradient_step(vp, g, step):

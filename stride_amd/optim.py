@@ -112,17 +112,23 @@ class LBFGS:
         self.bounds = bounds
         self.calls = 0
         self.history = []
+        self.proposals_kept = 0  # first steps from first_direction that were kept
 
     def _clip(self, x):
         return x if self.bounds is None else np.clip(x, *self.bounds)
 
-    def minimize(self, x0, fun_and_grad, iterations, callback=None):
+    def minimize(self, x0, fun_and_grad, iterations, callback=None, first_direction=None, gated=False):
+        """``first_direction(x, g)``, when given, proposes the first step of the run (the step
+        is -r for its return value r, None for the scaled gradient); it is tried at full length
+        once and, when it fails the sufficient-decrease test, replaced by the scaled gradient.
+        ``gated``: ``fun_and_grad(x, threshold)`` may return ``(f, None)`` without the gradient
+        when ``f`` exceeds ``threshold`` (a rejected trial then costs the forward pass only)."""
         x = np.asarray(x0, dtype=np.float64).copy()
-        f, g = fun_and_grad(x)
+        f, g = fun_and_grad(x, None) if gated else fun_and_grad(x)
         self.calls += 1
         self.history.append((self.calls, f))
         S, Y = [], []
-        for _ in range(iterations):
+        for it in range(iterations):
             q = g.copy()
             alphas = []
             for s, y in reversed(list(zip(S, Y))):
@@ -144,16 +150,28 @@ class LBFGS:
                 S, Y = [], []
                 d = -g * (self.first_step / max(np.max(np.abs(g)), 1e-30))
                 slope = np.dot(g, d)
-            t = 1.0
+            trials = []
+            if it == 0 and first_direction is not None:
+                rp = first_direction(x, g)
+                if rp is not None and np.dot(g, -rp) < 0:
+                    trials.append((-np.asarray(rp, np.float64), 1))  # tried once at full length
+            trials.append((d, self.max_backtracks))
             accepted = False
-            for _ in range(self.max_backtracks):
-                xn = self._clip(x + t * d)
-                fn, gn = fun_and_grad(xn)
-                self.calls += 1
-                if fn <= f + self.c1 * t * slope:
-                    accepted = True
+            for k, (d, tries) in enumerate(trials):
+                slope = np.dot(g, d)
+                t = 1.0
+                for _ in range(tries):
+                    xn = self._clip(x + t * d)
+                    threshold = f + self.c1 * t * slope
+                    fn, gn = fun_and_grad(xn, threshold) if gated else fun_and_grad(xn)
+                    self.calls += 1
+                    if gn is not None and fn <= threshold:
+                        accepted = True
+                        break
+                    t *= 0.5
+                if accepted:
+                    self.proposals_kept += int(k + 1 < len(trials))
                     break
-                t *= 0.5
             if not accepted:
                 break
             s_vec, y_vec = xn - x, gn - g

@@ -87,3 +87,38 @@ def test_dev_shots_with_medium_changes_match_sequential(gpu):
     for k in range(len(plans)):
         assert np.array_equal(rec_d[k].cpu().numpy(), want[k][0]), k
         assert np.array_equal(grad_d[k].cpu().numpy(), want[k][1]), k
+
+
+def test_sixteen_fresh_workers_match_sequential(gpu):
+    """Sixteen fresh solvers, one shot each, all allocating their buffers at once on their own
+    non-blocking streams: every shot equals the one-solver run bit for bit. (The buffers a solver
+    allocates are cleared on the null stream, which non-blocking streams do not wait for; the
+    clear is waited for before the buffer is used, or a late clear zeroes a wavefield already
+    being computed.) Each worker also checks its own misfit against the oracle's traces."""
+    import oracle
+    from stride_amd.workers import ShotWorkers
+    plan, ctx = small_case(dim=2, n=100, nabs=12, nt=260, kernel='OT2', grad_vp=True, nrec=32,
+                           boundary_type='complex_frequency_shift_PML_2')
+    geom = ctx['kw']['receiver_coordinates']
+    plans = [build_plan(ctx['space'], ctx['time'], ctx['wavelet'], ctx['vp'], source_coordinates=geom[k:k + 1],
+                        receiver_coordinates=geom, kernel='OT2', boundary_type='complex_frequency_shift_PML_2',
+                        grad_vp=True) for k in range(0, 32, 2)]
+    rng = np.random.default_rng(31)
+    res = [adjoint_source_samples(p, rng.standard_normal((p.nrec, p.nt)).astype(np.float32)) for p in plans]
+    print('workers: 16 fresh solvers, one shot each', flush=True)
+
+    def shot(s, k):
+        s.set_sources(plans[k].src_idx, plans[k].src_w)
+        rec = s.forward(plans[k].src_data)
+        return rec, s.adjoint(res[k], want_vp=True)['grad_vp']
+
+    seq = solver_for(plan, save_mode=1)
+    want = [shot(seq, k) for k in range(len(plans))]
+    seq.close()
+    with ShotWorkers(lambda: solver_for(plan, save_mode=1), n=16) as pool:
+        got = pool.map(shot, range(len(plans)))
+    for k in range(len(plans)):
+        assert np.array_equal(got[k][0], want[k][0]), k
+        assert np.array_equal(got[k][1], want[k][1]), k
+    ref, _, _ = oracle.forward(plans[3])
+    assert np.linalg.norm(got[3][0].T - ref) / np.linalg.norm(ref) < 1e-5

@@ -21,6 +21,7 @@ class _FakeFWI:
         self.k = k
         self.space = _Space()
         self.passes = 0
+        self.dev = 'cpu'
 
     def observe(self, true, freq):
         pass
@@ -33,10 +34,13 @@ class _FakeFWI:
         self.passes += 1
         return self._f(vp)[0]
 
-    def gradient(self, vp, freq):
+    def gradient(self, vp, freq, below=None):
         import torch
-        self.passes += 2
         f, g = self._f(vp)
+        if below is not None and not f < below:
+            self.passes += 1
+            return f, None
+        self.passes += 2
         return f, torch.tensor(g, dtype=torch.float32)
 
 
@@ -95,3 +99,65 @@ def test_input_scales():
     pairs = [(torch.full((2, 2), v), None, f, None) for v, f in ((2.0, 1.0), (4.0, 1.0), (8.0, 2.0))]
     s = learned.input_scales(pairs)
     assert s[1.0] == pytest.approx(1 / 3.0) and s[2.0] == pytest.approx(1 / 8.0)
+
+
+def test_lbfgs_gated_and_first_direction():
+    """L-BFGS with gated trials (a rejected trial returns no gradient) and a proposed first step:
+    a good proposal is kept and lands at once, a bad one costs one trial and the scaled gradient
+    step follows; both converge on a quadratic like the plain run."""
+    from stride_amd.optim import LBFGS
+    A = np.diag(np.linspace(1.0, 5.0, 6))
+    xs = np.linspace(-1, 1, 6)
+    calls = {'full': 0, 'gated': 0}
+
+    def fg(x, threshold=None):
+        f = 0.5 * float((x - xs) @ A @ (x - xs))
+        if threshold is not None and not f <= threshold:
+            calls['gated'] += 1
+            return f, None
+        calls['full'] += 1
+        return f, A @ (x - xs)
+
+    x0 = np.zeros(6)
+    plain = LBFGS(memory=5, first_step=0.1)
+    xp, fp = plain.minimize(x0, fg, 12, gated=True)
+    good = LBFGS(memory=5, first_step=0.1)
+    xg, fgood = good.minimize(x0, fg, 1, first_direction=lambda x, g: x - xs, gated=True)
+    assert good.proposals_kept == 1 and fgood < 1e-20
+    calls['gated'] = 0
+    bad = LBFGS(memory=5, first_step=0.1)
+    xb, fb = bad.minimize(x0, fg, 12, first_direction=lambda x, g: -10 * (x - xs) + 0 * g, gated=True)
+    assert bad.proposals_kept == 0 and fb < 1e-6 and fp < 1e-6
+    # the bad proposal is not a descent direction: it is never evaluated
+    bad2 = LBFGS(memory=5, first_step=0.1)
+    calls['gated'] = 0
+    bad2.minimize(x0, fg, 1, first_direction=lambda x, g: 3 * (x - xs) + 20 * g, gated=True)
+    assert bad2.proposals_kept == 0 and calls['gated'] >= 1
+
+
+def test_learned_lbfgs_first_step():
+    """invert_learned_lbfgs: the network's update replaces the first step of a band when it lowers
+    the misfit enough; the band otherwise runs as invert_plain, pass for pass."""
+    true = np.full((4, 4), 1550., np.float32)
+    good = np.full((4, 4), 49., np.float32)
+    bad = np.full((4, 4), -60., np.float32)
+    fp = _FakeFWI(true)
+    _, hp = learned.invert_plain(fp, true, [500e3], iters_per_freq=3, first_step=5.0)
+    fgood = _FakeFWI(true)
+    _, hg, kept = learned.invert_learned_lbfgs(fgood, _FakeNet([good]), {500e3: 1.0}, true, [500e3],
+                                               iters_per_freq=3, first_step=5.0)
+    assert kept == 1 and hg[0][1] < hp[0][1] and hg[0][0] == hp[0][0] == 4
+    # an overshoot (toward the truth, too far): tried, rejected at one forward pass, then the
+    # plain run's steps
+    over = np.full((4, 4), 200., np.float32)
+    fo = _FakeFWI(true)
+    _, ho, kept = learned.invert_learned_lbfgs(fo, _FakeNet([over]), {500e3: 1.0}, true, [500e3],
+                                               iters_per_freq=3, first_step=5.0)
+    assert kept == 0
+    assert [h[0] for h in ho] == [h[0] + 1 for h in hp]
+    assert np.allclose([h[1] for h in ho], [h[1] for h in hp])
+    # away from the truth: not a descent direction, never evaluated
+    fbad = _FakeFWI(true)
+    _, hb, kept = learned.invert_learned_lbfgs(fbad, _FakeNet([bad]), {500e3: 1.0}, true, [500e3],
+                                               iters_per_freq=3, first_step=5.0)
+    assert kept == 0 and [h[0] for h in hb] == [h[0] for h in hp]
