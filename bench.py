@@ -415,12 +415,14 @@ def cpu_baseline_density(cfg, nt=24):
                        'adjoint steps with vp and rho gradients, %.1f s' % (fwd, adj, t1 - t0))
 
 
-def bench_density(nt=2500, tile=None, with_cpu=True):
+def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0):
     """One 3D shot on the variable vp / rho / alpha medium (360^3 extended, attenuation power 0,
     sponge 20, OT2, 1024 receivers): forward with the raw wavefield saved every 4th step over the
     whole domain, then the as-written adjoint with the vp and rho gradients. The update runs as the
     fused density kernel; `roofline` covers it (algorithmic bytes: read u, the other level, b, A, B,
-    G, write the new level = 28 B per point update, + 4 B per point on save steps)."""
+    G, write the new level = 28 B per point update, + 4 B per point on save steps). checkpoint=K:
+    the raw snapshots recomputed from one state per K snapshots instead (save mode 5); the rate
+    then counts the shot's own updates only, and the recomputation's launches are extra work."""
     from stride_amd.configs import config4
     from stride_amd.physics.plan import build_plan
     from stride_amd._native import Solver
@@ -431,7 +433,7 @@ def bench_density(nt=2500, tile=None, with_cpu=True):
                       save_undersampling=4, grad_vp=True, grad_rho=True)
     s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, plan.nt)
     s.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
-                 plan.undersampling, 2, plan.tb0, plan.tb1, 1, 1, 0)
+                 plan.undersampling, 5 if checkpoint else 2, plan.tb0, plan.tb1, 1, 1, 0, checkpoint=checkpoint)
     s.set_medium(plan.vp, plan.rho, plan.alpha)
     s.set_sponge(plan.sponge_tables)
     s.set_sources(plan.src_idx, plan.src_w)
@@ -457,15 +459,27 @@ def bench_density(nt=2500, tile=None, with_cpu=True):
     ms, launches = s.timing()
     s.timing(False)
     n = int(np.prod(plan.shape))
-    nsave = len(range(0, plan.nt - 1, plan.undersampling))
-    alg = launches * 28.0 * n + nsave * 4.0 * n
+    f = plan.undersampling
+    nsave = len(range(0, plan.nt - 1, f))
+    alg = launches * 28.0 * n + (0 if checkpoint else nsave * 4.0 * n)
     achieved = alg / (ms * 1e-3) / 1e9
+    own = (plan.tb1 - 2) + (plan.tb1 - 1 - f - plan.tb0)  # the shot's forward and adjoint steps
+    per_gb = plan.shape[0] * plan.shape[1] * (-(-plan.shape[2] // 32) * 32) * 4 / 1e9  # one snapshot
+    nsnap = plan.tb1 // f + 1
+    if checkpoint:
+        last = (plan.tb1 - 2) // f
+        k = min(checkpoint, last + 1)
+        stored = 2 * (last // k) + k + 2  # segment states + the recomputation ring
+    else:
+        stored = nsnap
     for d in (d_src, d_rec, d_res):
         s.device_free(d)
     s.close()
     cpu = cpu_baseline_density(cfg) if with_cpu else None
     return dict(metric='Gpts/s fwd+adjoint 3D 320^3 SO10, variable vp/rho/alpha, vp+rho gradients',
-                value=round(launches * n / wall / 1e9, 3), unit='Gpts/s', seconds=round(wall, 3),
+                value=round(own * n / wall / 1e9, 3), unit='Gpts/s', seconds=round(wall, 3),
+                snapshots='recomputed, segments of %d' % checkpoint if checkpoint else 'stored',
+                snapshot_store_gb=round(stored * per_gb, 1),
                 update_launches=int(launches), update_kernel_us=round(ms * 1e3 / launches, 1),
                 update_kernel_share=round(ms * 1e-3 / wall, 3),
                 roofline={'bound': 'hbm', 'kernel': 'k_rho3d', 'achieved': round(achieved, 1),
@@ -670,6 +684,8 @@ def main():
     ap.add_argument('--schedule', action='store_true',
                     help='only run the batched 3D forward over 32 shots of uneven duration and receiver count '
                          '(seeded schedule) and exit')
+    ap.add_argument('--checkpoint', type=int, default=0,
+                    help='with --density: recompute the raw snapshots from one state per this many snapshots')
     ap.add_argument('--density', action='store_true',
                     help='only run one 3D shot on the variable vp / rho / alpha medium (fused density update, '
                          'vp and rho gradients) and exit')
@@ -722,7 +738,7 @@ def main():
         return
     if args.density:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
-        print(json.dumps(bench_density(args.nt, tile, with_cpu=not args.no_cpu)))
+        print(json.dumps(bench_density(args.nt, tile, with_cpu=not args.no_cpu, checkpoint=args.checkpoint)))
         return
     if args.schedule:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
@@ -855,6 +871,12 @@ def main():
         # secondary line: one shot of the 3D gradient workload's variable vp / rho / alpha medium
         print('[bench] 3D density shot', file=sys.stderr, flush=True)
         out['density'] = bench_density(args.nt, None, with_cpu=False)
+        # the same shot with the raw snapshots recomputed from checkpoints (K = ceil(sqrt(2N)))
+        print('[bench] 3D density shot, checkpointed snapshots', file=sys.stderr, flush=True)
+        k = int(np.ceil(np.sqrt(2 * (args.nt // 4 + 1))))
+        cp = bench_density(args.nt, None, with_cpu=False, checkpoint=k)
+        out['density']['checkpointed'] = {key: cp[key] for key in ('value', 'seconds', 'snapshots', 'snapshot_store_gb',
+                                                                   'update_launches', 'update_kernel_us')}
     if not args.no_cpu:
         try:
             out['cpu_baseline'] = cpu_baseline(plan)
