@@ -9,6 +9,12 @@ Each sparse point becomes a fixed-size list of (grid cell, weight) taps:
   b = 4.14, window extended to 3/0.99, tensor product of 7 taps per axis; sources
   get the 5-tap smoothing blend (stride/physics/common/devito.py:843-885).
   Support offsets are -3..+3 around ``floor`` of the grid coordinate.
+- ``sinc``: the reference hands it to Devito (``SparseTimeFunction(interpolation='sinc',
+  r=7)``, stride/physics/common/devito.py:634-638). Devito's ``SincInterpolator`` (Devito >= 4.8;
+  the reference pins ``devito>=4.6``) is absent here, so its published method is restated:
+  Kaiser-windowed sinc of Hicks (2002), 2r taps per axis at ``floor(x) - r + 1 .. floor(x) + r``,
+  weight ``sinc(d) I0(b sqrt(1 - (d / r)^2)) / I0(b)`` for the offset d of the tap from the
+  point, b from Hicks's Table 1 as Devito tabulates it (r = 7: 7.51). No smoothing.
 
 Taps that fall outside the extended grid are dropped (weight 0, index -1).
 Coordinates are shifted by 1e-3*h first (stride/physics/iso_acoustic/devito.py:307-309).
@@ -17,10 +23,14 @@ Coordinates are shifted by 1e-3*h first (stride/physics/iso_acoustic/devito.py:3
 import numpy as np
 import scipy.special
 
-__all__ = ['linear_taps', 'hicks_coefficients', 'smooth_source_coefficients', 'hicks_taps', 'point_taps',
-           'INTERPOLATION']
+__all__ = ['linear_taps', 'hicks_coefficients', 'smooth_source_coefficients', 'hicks_taps', 'sinc_coefficients',
+           'sinc_taps', 'point_taps', 'INTERPOLATION']
 
-INTERPOLATION = ('linear', 'hicks')
+INTERPOLATION = ('linear', 'hicks', 'sinc')
+
+# Kaiser window parameter per half-width r (Hicks 2002, Table 1, as Devito's SincInterpolator
+# tabulates it)
+SINC_B = {2: 2.94, 3: 4.53, 4: 4.14, 5: 5.26, 6: 6.40, 7: 7.51, 8: 8.56, 9: 9.56, 10: 10.64}
 
 
 def _grid_coordinates(space, coordinates):
@@ -103,6 +113,34 @@ def hicks_taps(space, coordinates, smooth=False):
     return _flatten(space, cells, w)
 
 
+def sinc_coefficients(space, coordinates, r=7):
+    """(first tap cells (n, dim) int64 = floor(x) - r + 1, coefficients (n, dim, 2r) float64)."""
+    if r not in SINC_B:
+        raise ValueError('sinc interpolation radius %d not in %s' % (r, sorted(SINC_B)))
+    g = _grid_coordinates(space, coordinates)
+    ref = np.floor(g)
+    frac = g - ref
+    b = SINC_B[r]
+    coef = np.zeros((g.shape[0], space.dim, 2 * r))
+    for i in range(2 * r):
+        d = i - r + 1 - frac  # tap minus point, in cells: within [-r, r]
+        coef[:, :, i] = np.sinc(d) * scipy.special.i0(b * np.sqrt(np.maximum(1 - (d / r) ** 2, 0.))) \
+            / scipy.special.i0(b)
+    return ref.astype(np.int64) - r + 1, coef
+
+
+def sinc_taps(space, coordinates, r=7):
+    first, coef = sinc_coefficients(space, coordinates, r)
+    dim = space.dim
+    k = np.arange(2 * r)
+    grids = np.array(np.meshgrid(*[k] * dim, indexing='ij')).reshape(dim, -1).T  # ((2r)^d, dim)
+    cells = first[:, None, :] + grids[None, :, :]
+    w = np.ones((first.shape[0], grids.shape[0]))
+    for d in range(dim):
+        w = w * coef[:, d, grids[:, d]]
+    return _flatten(space, cells, w)
+
+
 def point_taps(space, coordinates, interpolation_type='linear', smooth=False):
     """Shift by 1e-3*h and return (index (n, taps) int64, weight (n, taps) float32)."""
     coords = np.asarray(coordinates, dtype=np.float32) + 1e-3 * np.array(space.spacing).reshape(1, -1)
@@ -110,4 +148,6 @@ def point_taps(space, coordinates, interpolation_type='linear', smooth=False):
         return linear_taps(space, coords)
     if interpolation_type == 'hicks':
         return hicks_taps(space, coords, smooth=smooth)
+    if interpolation_type == 'sinc':
+        return sinc_taps(space, coords)
     raise ValueError('Only "linear" and "hicks" interpolations are allowed.')

@@ -20,6 +20,7 @@ FORWARD_CASES = [
     (2, dict(kernel='OT2')),
     (2, dict(kernel='OT4')),
     (2, dict(kernel='OT2', interpolation_type='hicks')),
+    (2, dict(kernel='OT2', interpolation_type='sinc')),
     (2, dict(kernel='OT2', rho='const')),
     (2, dict(kernel='OT4', rho='var')),
     (2, dict(kernel='OT2', rho='var', alpha='p0')),
