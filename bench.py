@@ -317,7 +317,8 @@ def unet_flops(N, H, W):
     return total
 
 
-def bench_learned(ntrain=30, epochs=40, neval=4, iters_per_freq=4, nsrc=16, log=None):
+def bench_learned(ntrain=30, epochs=40, neval=4, iters_per_freq=4, nsrc=16, log=None, save_net=None, load_net=None,
+                  eval_start=0):
     """The learned-preconditioner workload on the 2D gradient grid: solver-generated training
     pairs (every gradient of plain inversions of ntrain training phantoms), U-Net training, then
     inversions of unseen phantoms with frequency continuation (300 - 700 kHz): L-BFGS whose first
@@ -329,31 +330,42 @@ def bench_learned(ntrain=30, epochs=40, neval=4, iters_per_freq=4, nsrc=16, log=
     log = log or (lambda m: print('[learned] ' + m, file=sys.stderr, flush=True))
     freqs = [300e3, 400e3, 500e3, 600e3, 700e3]
     fwi = learned.RingFWI(nsrc=nsrc, workers=nsrc)
-    t0 = time.perf_counter()
-    pairs = learned.make_trajectory_pairs(fwi, range(ntrain), freqs, iters_per_freq, log=log)
-    t_pairs = time.perf_counter() - t0
-    scales = learned.input_scales(pairs)
-    t0 = time.perf_counter()
-    net, losses = learned.train(pairs, scales, epochs=epochs, log=log)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    steps = epochs * (len(pairs) // 4)
-    npairs = len(pairs)
-    train_flops = 3 * unet_flops(4, 512, 512) * steps
-    # inference throughput of the network alone
-    g = torch.stack([pairs[j][0] * scales[pairs[j][2]] for j in range(4)])
-    m = torch.stack([pairs[j][3] for j in range(4)])
-    for _ in range(2):
-        net.forward(g, model=m)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(10):
-        net.forward(g, model=m)
-    torch.cuda.synchronize()
-    t_inf = (time.perf_counter() - t0) / 10
-    del pairs
+    info = {}
+    if load_net:
+        net, scales = learned.load_network(load_net)
+        info.update(network=os.path.basename(load_net))
+    else:
+        t0 = time.perf_counter()
+        pairs = learned.make_trajectory_pairs(fwi, range(ntrain), freqs, iters_per_freq, log=log)
+        t_pairs = time.perf_counter() - t0
+        scales = learned.input_scales(pairs)
+        t0 = time.perf_counter()
+        net, losses = learned.train(pairs, scales, epochs=epochs, log=log)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        steps = epochs * (len(pairs) // 4)
+        train_flops = 3 * unet_flops(4, 512, 512) * steps
+        # inference throughput of the network alone
+        g = torch.stack([pairs[j][0] * scales[pairs[j][2]] for j in range(4)])
+        m = torch.stack([pairs[j][3] for j in range(4)])
+        for _ in range(2):
+            net.forward(g, model=m)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            net.forward(g, model=m)
+        torch.cuda.synchronize()
+        t_inf = (time.perf_counter() - t0) / 10
+        info.update(training_phantoms=ntrain, training_pairs=len(pairs), epochs=epochs,
+                    pair_seconds=round(t_pairs, 1), train_seconds=round(t_train, 1),
+                    train_tflops=round(train_flops / t_train / 1e12, 2), final_train_loss=losses[-1],
+                    first_train_loss=losses[0], inference_ms_batch4=round(t_inf * 1e3, 2),
+                    inference_tflops=round(unet_flops(4, 512, 512) / t_inf / 1e12, 2))
+        del pairs
+        if save_net:
+            learned.save_network(net, scales, save_net)
     results = []
-    for e in range(neval):
+    for e in range(eval_start, eval_start + neval):
         k = 100000 + e
         true = learned.phantom(k)
         fwi.passes = 0
@@ -376,14 +388,11 @@ def bench_learned(ntrain=30, epochs=40, neval=4, iters_per_freq=4, nsrc=16, log=
     fwi.close()
     ratios = [r['learned_passes_to_plain_misfit'] / r['plain_passes'] for r in results
               if r['learned_passes_to_plain_misfit'] is not None]
+    err_ratios = [r['learned_passes_to_plain_error'] / r['plain_passes'] for r in results
+                  if r['learned_passes_to_plain_error'] is not None]
     return dict(metric='FWI misfit drop per solver call, learned preconditioner vs plain L-BFGS',
-                unit='passes over all shots', training_phantoms=ntrain, training_pairs=npairs, epochs=epochs,
-                sources=nsrc,
-                pair_seconds=round(t_pairs, 1), train_seconds=round(t_train, 1),
-                train_tflops=round(train_flops / t_train / 1e12, 2), final_train_loss=losses[-1],
-                first_train_loss=losses[0], inference_ms_batch4=round(t_inf * 1e3, 2),
-                inference_tflops=round(unet_flops(4, 512, 512) / t_inf / 1e12, 2),
-                eval=results, passes_ratio_to_plain_final=ratios,
+                unit='passes over all shots', sources=nsrc, **info,
+                eval=results, passes_ratio_to_plain_final=ratios, passes_ratio_to_plain_error=err_ratios,
                 phantoms_reaching_plain_final=len(ratios), phantoms=neval,
                 config={'workload': '2D 512x512 inner (572x572 extended), dx 0.2 mm, CFS-PML 30, vp only, '
                                     '%d ring sources / 128 receivers, 2500 steps, 5 bands 300-700 kHz x %d '
@@ -703,6 +712,10 @@ def main():
     ap.add_argument('--train-phantoms', type=int, default=30)
     ap.add_argument('--epochs', type=int, default=40)
     ap.add_argument('--eval', type=int, default=4)
+    ap.add_argument('--eval-start', type=int, default=0, help='with --learned: first unseen phantom (100000 + this)')
+    ap.add_argument('--save-net', type=str, default=None, help='with --learned: write the trained network here')
+    ap.add_argument('--load-net', type=str, default=None,
+                    help='with --learned: evaluate this saved network instead of generating pairs and training')
     ap.add_argument('--pipelines', action='store_true',
                     help='only time the device steps either side of the solver (trace processing, misfit, '
                          'gradient post-processing, model update) at the full workload and exit')
@@ -728,7 +741,8 @@ def main():
         return
 
     if args.learned:
-        print(json.dumps(bench_learned(ntrain=args.train_phantoms, epochs=args.epochs, neval=args.eval)))
+        print(json.dumps(bench_learned(ntrain=args.train_phantoms, epochs=args.epochs, neval=args.eval,
+                                   save_net=args.save_net, load_net=args.load_net, eval_start=args.eval_start)))
         return
     if args.forward_2d:
         print(json.dumps(bench_forward_2d()))

@@ -39,7 +39,7 @@ from .utils import tone_burst, elliptical
 from .workers import ShotWorkers
 
 __all__ = ['RingFWI', 'phantom', 'make_pairs', 'make_trajectory_pairs', 'train', 'invert_plain',
-           'invert_learned', 'invert_learned_lbfgs']
+           'invert_learned', 'invert_learned_lbfgs', 'save_network', 'load_network']
 
 PML = 'complex_frequency_shift_PML_2'
 VP_BOUNDS = (1300., 1800.)
@@ -278,6 +278,23 @@ def train(pairs, scales, epochs=60, batch=4, lr=1e-3, seed=0, device=0, log=None
         if log:
             log('epoch %d loss %.4g (%.1f s)' % (ep, losses[-1], time.perf_counter() - t0))
     return net, losses
+
+
+def save_network(net, scales, path):
+    """The trained network's fp32 parameters and the input scales, as an .npz of plain arrays."""
+    arrays = {'w:' + k: v for k, v in net.state().items()}
+    freqs = sorted(scales)
+    np.savez(path, scale_freqs=np.array(freqs, np.float64), scale_values=np.array([scales[f] for f in freqs]),
+             **arrays)
+
+
+def load_network(path, device=0):
+    """(network, input scales) from save_network's file."""
+    with np.load(path, allow_pickle=False) as z:
+        net = UNet(device=device)
+        net.load_state({k[2:]: z[k] for k in z.files if k.startswith('w:')})
+        scales = dict(zip(z['scale_freqs'].tolist(), z['scale_values'].tolist()))
+    return net, scales
 
 
 # ---- inversions ------------------------------------------------------------------------------------

@@ -241,3 +241,22 @@ def test_unet_training_lowers_loss(gpu):
     losses = [net.train_step(g, target, scale=1.0, lr=3e-3) for _ in range(30)]
     print('unet training losses %.4g -> %.4g' % (losses[0], losses[-1]), flush=True)
     assert losses[-1] < 0.3 * losses[0]
+
+
+def test_saved_network_reloads(gpu, tmp_path):
+    """save_network / load_network: the reloaded network (and its input scales) gives the same
+    output bit for bit."""
+    torch = _t()
+    from stride_amd.unet import UNet
+    from stride_amd import learned
+    net = UNet(device=0, seed=3)
+    rng = np.random.default_rng(4)
+    g = torch.tensor(rng.standard_normal((2, 32, 64)), dtype=torch.float32, device='cuda')
+    net.train_step(g, 0.3 * g, scale=1.0, lr=1e-3)
+    path = str(tmp_path / 'net.npz')
+    learned.save_network(net, {300e3: 0.5, 700e3: 2.0}, path)
+    net2, scales = learned.load_network(path)
+    assert scales == {300e3: 0.5, 700e3: 2.0}
+    a, b = net.forward(g), net2.forward(g)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
