@@ -371,29 +371,37 @@ def bench_learned(ntrain=30, epochs=40, neval=4, iters_per_freq=4, nsrc=16, log=
         fwi.passes = 0
         vp_p, hp = learned.invert_plain(fwi, true, freqs, iters_per_freq)
         log('phantom %d plain: %d passes, misfit %.4g, model error %.4f' % (k, hp[-1][0], hp[-1][1], hp[-1][2]))
-        fwi.passes = 0
-        vp_l, hl, acc = learned.invert_learned_lbfgs(fwi, net, scales, true, freqs, iters_per_freq)
-        log('phantom %d learned: %d passes, misfit %.4g, model error %.4f, %d of %d band-first steps from the '
-            'network' % (k, hl[-1][0], hl[-1][1], hl[-1][2], acc, len(freqs)))
         last = freqs[-1]
         plain_final = hp[-1][1]
-        reach = [h[0] for h in hl if h[3] == last and h[1] <= plain_final]
-        reach_err = [h[0] for h in hl if h[2] <= hp[-1][2]]
-        results.append(dict(phantom=k, plain_passes=hp[-1][0], plain_misfit=plain_final, plain_error=hp[-1][2],
-                            learned_passes=hl[-1][0], learned_misfit=hl[-1][1], learned_error=hl[-1][2],
-                            learned_passes_to_plain_misfit=reach[0] if reach else None,
-                            learned_passes_to_plain_error=reach_err[0] if reach_err else None, network_updates=acc,
-                            plain_history=[[h[0], h[1], h[2]] for h in hp],
-                            learned_history=[[h[0], h[1], h[2]] for h in hl]))
+        row = dict(phantom=k, plain_passes=hp[-1][0], plain_misfit=plain_final, plain_error=hp[-1][2],
+                   plain_history=[[h[0], h[1], h[2]] for h in hp])
+        # the learned inversion on the same schedule, and on a schedule of half the iterations per band
+        for tag, ipf in (('learned', iters_per_freq), ('learned_half', max(1, iters_per_freq // 2))):
+            fwi.passes = 0
+            vp_l, hl, acc = learned.invert_learned_lbfgs(fwi, net, scales, true, freqs, ipf)
+            log('phantom %d %s (%d per band): %d passes, misfit %.4g, model error %.4f, %d of %d band-first steps '
+                'from the network' % (k, tag, ipf, hl[-1][0], hl[-1][1], hl[-1][2], acc, len(freqs)))
+            reach = [h[0] for h in hl if h[3] == last and h[1] <= plain_final]
+            reach_err = [h[0] for h in hl if h[2] <= hp[-1][2]]
+            row.update({tag + '_passes': hl[-1][0], tag + '_misfit': hl[-1][1], tag + '_error': hl[-1][2],
+                        tag + '_passes_to_plain_misfit': reach[0] if reach else None,
+                        tag + '_passes_to_plain_error': reach_err[0] if reach_err else None,
+                        tag + '_network_updates': acc, tag + '_history': [[h[0], h[1], h[2]] for h in hl]})
+        results.append(row)
     fwi.close()
-    ratios = [r['learned_passes_to_plain_misfit'] / r['plain_passes'] for r in results
-              if r['learned_passes_to_plain_misfit'] is not None]
-    err_ratios = [r['learned_passes_to_plain_error'] / r['plain_passes'] for r in results
-                  if r['learned_passes_to_plain_error'] is not None]
+
+    def ratios(key):
+        return [r[key] / r['plain_passes'] for r in results if r[key] is not None]
+
+    summary = {}
+    for tag in ('learned', 'learned_half'):
+        summary[tag] = dict(passes_ratio_to_plain_final_misfit=ratios(tag + '_passes_to_plain_misfit'),
+                            passes_ratio_to_plain_final_error=ratios(tag + '_passes_to_plain_error'),
+                            final_misfit_below_plain=sum(r[tag + '_misfit'] < r['plain_misfit'] for r in results),
+                            final_error_below_plain=sum(r[tag + '_error'] < r['plain_error'] for r in results))
     return dict(metric='FWI misfit drop per solver call, learned preconditioner vs plain L-BFGS',
                 unit='passes over all shots', sources=nsrc, **info,
-                eval=results, passes_ratio_to_plain_final=ratios, passes_ratio_to_plain_error=err_ratios,
-                phantoms_reaching_plain_final=len(ratios), phantoms=neval,
+                eval=results, summary=summary, phantoms=neval, first_phantom=100000 + eval_start,
                 config={'workload': '2D 512x512 inner (572x572 extended), dx 0.2 mm, CFS-PML 30, vp only, '
                                     '%d ring sources / 128 receivers, 2500 steps, 5 bands 300-700 kHz x %d '
                                     'iterations' % (nsrc, iters_per_freq)})
