@@ -70,7 +70,7 @@ def test_reference_errors(gpu):
     with pytest.raises(ValueError):
         pde.forward(shot.wavelets, vp, problem=problem, boundary_type='higdon')
     with pytest.raises(ValueError):
-        pde.forward(shot.wavelets, vp, problem=problem, interpolation_type='sinc')
+        pde.forward(shot.wavelets, vp, problem=problem, interpolation_type='cubic')
     alpha = ScalarField('alpha', space=space, data=np.zeros(space.extended_shape, np.float32), needs_grad=True)
     with pytest.raises(ValueError):
         pde.forward(shot.wavelets, vp, None, alpha, problem=problem, attenuation_power=1)

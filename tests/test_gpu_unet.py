@@ -251,7 +251,7 @@ def test_saved_network_reloads(gpu, tmp_path):
     from stride_amd import learned
     net = UNet(device=0, seed=3)
     rng = np.random.default_rng(4)
-    g = torch.tensor(rng.standard_normal((2, 32, 64)), dtype=torch.float32, device='cuda')
+    g = torch.tensor(rng.standard_normal((2, 32, 512)), dtype=torch.float32, device='cuda')
     net.train_step(g, 0.3 * g, scale=1.0, lr=1e-3)
     path = str(tmp_path / 'net.npz')
     learned.save_network(net, {300e3: 0.5, 700e3: 2.0}, path)
