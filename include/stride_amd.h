@@ -174,12 +174,10 @@ int sa_dump_to_host(sa_solver *s, int which, float *host, long long n);
 
 /* Tile of the specialised 3D update (3D, OT2, no density, sponge or no boundary, no
  * attenuation): 64 x waves*rows cells per workgroup, marching `chunk` planes of the slowest
- * axis. version 1 (shifted register queue; rows 2, 4, 8), 2 (buffer loads, rotated queue,
- * next plane prefetched, double-buffered LDS; rows 2, 4, 8), 3 (as 2, two planes ahead;
- * rows 2, 4, 8), 5 (LDS ring of planes filled by direct-to-LDS loads two planes ahead,
- * 4 waves; rows 2, 4), 6 (as 5 with 8 waves; rows 1, 2), 7 (as 5, u read once: axis-0
- * terms pushed into partial sums; rows 2) or 8 (as 7 with 8 waves; rows 1, 2). Default:
- * version 7, rows 2, chunk 24. Used by the autotuner (stride_amd.autotune). */
+ * axis. version 7 (LDS ring of planes filled by direct-to-LDS loads two planes ahead, u read
+ * once: axis-0 terms pushed into partial sums; 4 waves, rows 2) or 8 (as 7 with 8 waves; rows
+ * 1, 2). Default: version 7, rows 2, chunk 24. Used by the autotuner (stride_amd.autotune).
+ * Grids whose fields reach 2 GiB run the general kernels. */
 int sa_set_tile(sa_solver *s, int rows, int chunk, int version);
 
 /* Tile of the fused 3D update for the div(b grad) forms (constant or variable density; OT2,

@@ -486,7 +486,7 @@ class Solver:
     def read_gradient_dev(self, which, dev_ptr):
         self._check(self._lib.sa_read_gradient_dev(self._h, ctypes.c_int(which), ctypes.c_void_p(dev_ptr)))
 
-    def set_tile(self, rows, chunk, version=2):
+    def set_tile(self, rows, chunk, version=7):
         self._check(self._lib.sa_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk), ctypes.c_int(version)))
 
     @staticmethod

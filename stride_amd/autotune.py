@@ -1,10 +1,9 @@
 """Tile autotuner for the specialised 3D update.
 
 For a grid and scheme, times short forward runs of the update kernel (HIP events
-around every launch) for each candidate — kernel version (1-3: register-staged
-planes, 4 waves; 5/6: LDS plane ring filled by direct-to-LDS loads, 4/8 waves;
-7/8: as 5/6 with u read once, axis-0 terms pushed into partial sums),
-rows per lane (tile 64 x waves*rows cells) and march chunk lengths that split the
+around every launch) for each candidate — kernel version (7: 4 waves, 8: 8 waves; an
+LDS plane ring filled by direct-to-LDS loads, u read once, axis-0 terms pushed into
+partial sums), rows per lane (tile 64 x waves*rows cells) and march chunk lengths that split the
 slowest axis into 3..16 pieces — and returns the fastest. Results are cached per
 (shape, absorbing, boundary) for the life of the process. ``autotune_batch`` does the same for
 the two-shot update and ``autotune_density`` for the fused update of the density schemes
@@ -22,14 +21,13 @@ __all__ = ['candidates', 'autotune', 'candidates_batch', 'autotune_batch', 'cand
 _CACHE = {}
 
 
-def candidates(shape, versions=(3, 5, 6, 7, 8)):
+def candidates(shape, versions=(7, 8)):
     nx = shape[0]
     chunks = sorted({max(1, -(-nx // k)) for k in (3, 4, 5, 6, 7, 8, 10, 12, 16)})
-    # versions 5-8 march whole 12-plane periods: chunk lengths that are multiples of 12
+    # the march runs whole 12-plane periods: chunk lengths that are multiples of 12
     chunks12 = sorted({12 * max(1, -(-c // 12)) for c in chunks})
-    rows_of = {1: (2, 4, 8), 2: (2, 4), 3: (2, 4), 5: (2, 4), 6: (1, 2), 7: (2,), 8: (1, 2)}
-    return [(rows, chunk, version) for version in versions for rows in rows_of[version]
-            for chunk in (chunks12 if version >= 5 else chunks)]
+    rows_of = {7: (2,), 8: (1, 2)}
+    return [(rows, chunk, version) for version in versions for rows in rows_of[version] for chunk in chunks12]
 
 
 def autotune(plan, steps=24, device=0, verbose=False, cands=None):

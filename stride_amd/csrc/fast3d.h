@@ -14,16 +14,12 @@ struct PairLevels {
 };
 
 struct FastTune {
-    int rows;   // rows (axis 1) per thread; the tile is 64 (axis 2) x 4*rows (axis 1)
-    int chunk;  // planes (axis 0) marched per workgroup
-    int version;  // 1: shifted register queue, two barriers; 2/3: rotated queue, register prefetch
-                  // 1 or 2 planes ahead; 5/6: LDS plane ring filled by direct-to-LDS loads, 4 or 8 waves
+    int rows;     // rows (axis 1) per thread; the tile is 64 (axis 2) x waves*rows (axis 1)
+    int chunk;    // planes (axis 0) marched per workgroup
+    int version;  // 7: 4 waves (rows 2), 8: 8 waves (rows 1 or 2)
 };
 
 struct Fast3D {
-    float *tables = nullptr;  // sponge corner table in use (device)
-    float *corner = nullptr;  // owned corner table, (nabs0+1) x (nabs1+1) x (nabs2+1)
-    int tlen = 0;
     const float *axis = nullptr;  // per-axis layer tables [3][alen] (device, owned by the solver)
     int alen = 0;
     int boundary = -1;
@@ -38,6 +34,8 @@ struct Fast3D {
 };
 
 bool fast3d_enabled();
+// true when the grid's fields fit the kernels' 32-bit buffer offsets (else: general kernels)
+bool fast3d_supported(const Geo &g);
 int fast3d_prepare(Fast3D &f, const Geo &g, const float *vp, const float *tables, int tlen, int boundary, float dt,
                    hipStream_t stream);
 // forward step: out = step(u, o) (out may alias o); optional fused save of p.dt2 into `save` (box layout)

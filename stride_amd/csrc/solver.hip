@@ -1191,7 +1191,7 @@ static bool use_flat2d(const sa_solver *s) {
 static bool use_fast(const sa_solver *s) {
     const sa_scheme &c = s->sch;
     return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 && c.att_power == -2 && c.boundary != 1 &&
-           (c.save_mode == 0 || c.save_mode == 1) && fast3d_enabled();
+           (c.save_mode == 0 || c.save_mode == 1) && fast3d_enabled() && fast3d_supported(s->geo);
 }
 
 // build per-point coefficients and boundary state for the current scheme + medium
@@ -2372,8 +2372,7 @@ extern "C" int sa_set_tile(sa_solver *s, int rows, int chunk, int version) {
     if (!s) return fail(SA_E_ARG, "null solver");
     s->fast.prepared = true;  // an explicit tile overrides the default
     if (fast3d_set_tune(s->fast, rows, chunk, version) != 0)
-        return fail(SA_E_ARG, "tile: chunk >= 1 and version 1-3 with rows 2/4/8, 5 with rows 2/4, 6 or 8 with rows "
-                                 "1/2, 7 with rows 2");
+        return fail(SA_E_ARG, "tile: chunk >= 1 and version 7 with rows 2 or version 8 with rows 1/2");
     return SA_OK;
 }
 
