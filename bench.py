@@ -507,6 +507,70 @@ def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0):
                                     '(power 0), one shot, nt %d, raw saves every 4th step, 1024 receivers' % plan.nt})
 
 
+def bench_power2(nt=500, general=False):
+    """One 3D shot with power-2 attenuation (alpha 0.5 dB/cm/MHz, the 2D gradient workload's
+    value) on the 360^3 grid of the headline workload: forward with p.dt2 saved every 4th step,
+    then the as-written adjoint with the vp gradient. The update runs as the specialised one-shot
+    kernel with the attenuation term fused (algorithmic bytes: read u, the other level, vp, alpha
+    and the other level's Laplacian, write the new level and the Laplacian of u = 28 B per point
+    update; + 4 B per interior point on save steps, + 20 B on gradient steps); general=True runs
+    the separate general passes instead, for comparison."""
+    from stride_amd.configs import config3
+    from stride_amd.physics.plan import build_plan
+    from stride_amd.autotune import tune_solver
+    if general:
+        os.environ['STRIDE_AMD_GENERAL_ONLY'] = '1'
+    try:
+        cfg = config3(scale=1.0, nt=nt, nrec=1024)
+        alpha = np.full(cfg['space'].extended_shape, 0.5, np.float32)
+        plan = build_plan(cfg['space'], cfg['time'], cfg['wavelet'][None], cfg['vp'], alpha=alpha,
+                          attenuation_power=2, source_coordinates=cfg['sources'],
+                          receiver_coordinates=cfg['receivers'], kernel='OT2', save_undersampling=4, grad_vp=True)
+        s = make_solver(plan)
+        s.set_medium(plan.vp, None, plan.alpha)
+        if not general:
+            tune_solver(s, plan, 1)
+        d_src = s._to_device(plan.src_data)
+        d_rec = s.device_alloc(plan.nt * plan.nrec * 4)
+        d_res = s._to_device(np.random.default_rng(0).standard_normal((plan.nt, plan.nrec)).astype(np.float32) * 1e-3)
+
+        def shot():
+            s.forward_dev(d_src, d_rec)
+            s.adjoint_dev(d_res, 1)
+            s.synchronize()
+
+        shot()
+        s.timing(True)
+        t0 = time.perf_counter()
+        shot()
+        wall = time.perf_counter() - t0
+        ms, launches = s.timing()
+        s.timing(False)
+        for d in (d_src, d_rec, d_res):
+            s.device_free(d)
+        s.close()
+    finally:
+        if general:
+            os.environ.pop('STRIDE_AMD_GENERAL_ONLY', None)
+    n = int(np.prod(plan.shape))
+    interior = int(np.prod([a - 2 * b for a, b in zip(plan.shape, plan.absorbing)]))
+    f = plan.undersampling
+    fwd, adj = plan.tb1 - 2, plan.tb1 - 1 - f - plan.tb0
+    saves = sum(1 for t in range(1, plan.tb1 - 1) if t % f == 0)
+    grads = sum(1 for t in range(plan.tb1 - 1 - f, plan.tb0, -1) if t % f == 0)
+    out = dict(metric='Gpts/s fwd+adjoint 3D 320^3 SO10, power-2 attenuation, vp gradient',
+               value=round((fwd + adj) * n / wall / 1e9, 3), unit='Gpts/s', seconds=round(wall, 3), nt=plan.nt,
+               path='general passes' if general else 'specialised update (attenuation fused)',
+               kernel_launches=int(launches), kernel_ms=round(ms, 2))
+    if not general:
+        alg = 28.0 * n * (fwd + adj) + 4.0 * interior * saves + 20.0 * interior * grads
+        achieved = alg / (ms * 1e-3) / 1e9
+        out['roofline'] = {'bound': 'hbm', 'kernel': 'k_fast3d_v7 (power 2)', 'achieved': round(achieved, 1),
+                           'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
+                           'avg_launch_us': round(ms * 1e3 / launches, 1)}
+    return out
+
+
 def cpu_baseline_2d(cfg, nt=500):
     """The oracle (C/OpenMP, test infrastructure) on the 2D gradient workload's medium and scheme: a
     bounded forward with raw saves + as-written adjoint with vp and rho gradients."""
@@ -703,6 +767,9 @@ def main():
                          '(seeded schedule) and exit')
     ap.add_argument('--checkpoint', type=int, default=0,
                     help='with --density: recompute the raw snapshots from one state per this many snapshots')
+    ap.add_argument('--power2', action='store_true',
+                    help='one 3D shot with power-2 attenuation on the fused update and on the general passes '
+                         '(use with --nt) and exit')
     ap.add_argument('--density', action='store_true',
                     help='only run one 3D shot on the variable vp / rho / alpha medium (fused density update, '
                          'vp and rho gradients) and exit')
@@ -757,6 +824,9 @@ def main():
         return
     if args.gradient_2d:
         print(json.dumps(bench_gradient_2d(workers=args.workers, with_cpu=not args.no_cpu)))
+        return
+    if args.power2:
+        print(json.dumps({'fused': bench_power2(args.nt), 'general': bench_power2(args.nt, general=True)}))
         return
     if args.density:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None

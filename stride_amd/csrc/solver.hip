@@ -1188,11 +1188,16 @@ static bool use_flat2d(const sa_solver *s) {
     return s->grid.dim == 2 && s->sch.kernel == 2 && s->rho3d.enabled && flat2d_supported(s->geo);
 }
 
+// the specialised 3D update: no density term, no attenuation or power 2 (one shot per pass)
 static bool use_fast(const sa_solver *s) {
     const sa_scheme &c = s->sch;
-    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 && c.att_power == -2 && c.boundary != 1 &&
-           (c.save_mode == 0 || c.save_mode == 1) && fast3d_enabled() && fast3d_supported(s->geo);
+    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 && (c.att_power == -2 || c.att_power == 2) &&
+           c.boundary != 1 && (c.save_mode == 0 || c.save_mode == 1) && fast3d_enabled() &&
+           fast3d_supported(s->geo);
 }
+
+// the two-shot and multi-shot forms of the specialised update: no attenuation
+static bool use_fast_pairs(const sa_solver *s) { return use_fast(s) && s->sch.att_power == -2; }
 
 // build per-point coefficients and boundary state for the current scheme + medium
 static int prepare(sa_solver *s) {
@@ -1213,7 +1218,7 @@ static int prepare(sa_solver *s) {
     TRY(dalloc(&s->M, s->total));
     if (c.att_power == 2) TRY(dalloc(&s->Cbeta, s->total));
     if (c.att_power == 2) TRY(dalloc(&s->lapother, s->total));
-    if (c.att_power == 2 && use_flat2d(s)) TRY(dalloc(&s->lapnext, s->total));
+    if (c.att_power == 2 && (use_flat2d(s) || use_fast(s))) TRY(dalloc(&s->lapnext, s->total));
     if (c.boundary == 1) TRY(dalloc(&s->Cpml, s->total));
     const Geo &g = s->geo;
     k_coefficients<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(
@@ -1664,6 +1669,22 @@ static size_t box_points(const Box &b) { return (size_t)b.n[0] * b.n[1] * b.p2; 
 static int dump_alloc(sa_solver *s, int w);
 static int dump_level(sa_solver *s, int w, int t, const float *u);
 
+// Power-2 attenuation on the specialised 3D update: the update reads the Laplacian of the level it
+// overwrites (kept from the previous step, or taken here at the first step of a run) and writes
+// the Laplacian of u, which becomes the next step's.
+static int lap_other(sa_solver *s, const float *other) {
+    if (s->lap_of != other) {
+        k_laplace<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, other, nullptr, s->lapother);
+        CHECK_LAUNCH();
+    }
+    return SA_OK;
+}
+
+static void lap_advance(sa_solver *s, const float *u) {
+    std::swap(s->lapother, s->lapnext);
+    s->lap_of = u;
+}
+
 // One forward step t: p[t+1] into o from u = p[t] and o = p[t-1], then the source injection.
 // The forward run and the checkpointed recomputation issue the same launches through it, so the
 // recomputed levels equal the forward run's bit for bit.
@@ -1672,10 +1693,18 @@ static int forward_step(sa_solver *s, int t, float *u, float *o, const float *sr
     const float idt2 = 1.f / (s->dt * s->dt);
     bool injected = false;
     if (use_fast(s)) {
+        FastAtt2 at{}, *pat = nullptr;
+        if (s->sch.att_power == 2) {
+            TRY(lap_other(s, o));
+            at = {s->alpha, s->lapother, s->lapnext};
+            pat = &at;
+        }
         hipEvent_t e0;
         TRY(timed_begin(s, &e0));
-        TRY(fast3d_step(s->fast, s->geo, u, o, o, save, s->box, idt2, s->stream));
+        if (fast3d_step(s->fast, s->geo, u, o, o, save, s->box, idt2, s->stream, pat) != 0)
+            return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
         TRY(timed_end(s));
+        if (pat) lap_advance(s, u);
     } else {
         const InjectSpec spec{&s->src, src + (size_t)t * s->src.npoint, 1.f, mode == 1};
         TRY(step_general(s, u, o, o, 0, save, mode, &spec, &injected, gat));
@@ -1891,10 +1920,19 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
         if (guard && raw_ckpt && (t / f) / s->ck_K != s->ck_seg) TRY(recompute_raw_segment(s, (t / f) / s->ck_K));
         const float *ps = guard ? snap(t / f) : nullptr;
         if (fast) {
+            FastAtt2 at{}, *pat = nullptr;
+            if (c.att_power == 2) {
+                TRY(lap_other(s, o));
+                at = {s->alpha, s->lapother, s->lapnext};
+                pat = &at;
+            }
             hipEvent_t e0;
             TRY(timed_begin(s, &e0));
-            TRY(fast3d_step_adjoint(s->fast, g, u, o, o, s->saved_mode == 1 ? ps : nullptr, s->box, gv, pv, s->stream));
+            if (fast3d_step_adjoint(s->fast, g, u, o, o, s->saved_mode == 1 ? ps : nullptr, s->box, gv, pv, s->stream,
+                                    pat) != 0)
+                return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
             TRY(timed_end(s));
+            if (pat) lap_advance(s, u);
         } else {
             const InjectSpec spec{&s->rec, res + (size_t)t * s->rec.npoint, -1.f, false};
             // the fused 2D update also accumulates this step's gradients (it reads u and the
@@ -2461,7 +2499,7 @@ extern "C" int sa_synchronize(sa_solver *s) {
 
 static bool batch_ok(const sa_solver *s) {
     const sa_scheme &c = s->sch;
-    return use_fast(s) && (c.save_mode == 0 || c.save_mode == 1) && s->nshot == 2;
+    return use_fast_pairs(s) && (c.save_mode == 0 || c.save_mode == 1) && s->nshot == 2;
 }
 
 extern "C" int sa_batch_create(sa_solver *s, int nshot) {
@@ -2676,7 +2714,7 @@ extern "C" int sa_multi_forward_dev(sa_solver *s, const float *const *src, float
     for (int k = 0; k < nshot; ++k)
         if (!s->mshot[k].set) return fail(SA_E_STATE, "shot %d of the schedule has no taps", k);
     TRY(prepare(s));
-    if (!use_fast(s) || s->sch.save_mode != 0)
+    if (!use_fast_pairs(s) || s->sch.save_mode != 0)
         return fail(SA_E_UNSUPPORTED, "the multi-shot schedule runs the specialised 3D forward (OT2, no density, "
                                       "sponge or no boundary, no attenuation, save mode 0)");
     TRY(quiesce(s));  // the device tables below may still be read by a previous schedule

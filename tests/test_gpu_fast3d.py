@@ -94,6 +94,28 @@ def test_fast_anisotropic_spacing(gpu, tile):
     assert rel_l2(g_f['grad_vp'], gref['grad_vp']) < 1e-4
 
 
+@pytest.mark.parametrize('n,nabs,tile', [(37, 5, (2, 13, 7)), (41, 7, (1, 24, 8)), (30, 6, (2, 12, 8))])
+def test_fast_power2_attenuation(gpu, n, nabs, tile):
+    """Power-2 attenuation on the specialised update (the Cbeta (lap u - lap other) term, the
+    Laplacian of u handed to the next step) against the general passes and the oracle: traces and
+    the as-written vp gradient."""
+    plan, _ = small_case(dim=3, n=n, nabs=nabs, nt=90, kernel='OT2', grad_vp=True, nrec=20, alpha='p2',
+                         attenuation_power=2)
+    assert plan.att_power == 2
+    print('fast3d power-2 n=%d nabs=%d tile=%s' % (n, nabs, tile), flush=True)
+    rng = np.random.default_rng(6)
+    res_s = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, plan.nt)))
+    rec_f, g_f = _run(plan, res_s, general=False, tile=tile)
+    rec_g, g_g = _run(plan, res_s, general=True)
+    ref, saved, _ = oracle.forward(plan, save=True)
+    gref = oracle.adjoint_compat(plan, res_s, saved)
+    assert rel_l2(rec_f, rec_g) < 1e-6
+    assert rel_l2(rec_f.T, ref) < 1e-5
+    for key in ('grad_vp', 'prec_vp'):
+        assert rel_l2(g_f[key], g_g[key]) < 1e-5, key
+        assert rel_l2(g_f[key], gref[key]) < 1e-4, key
+
+
 def test_tile_versions(gpu):
     """Only versions 7 (rows 2) and 8 (rows 1, 2) exist; other tiles are refused."""
     plan, _ = small_case(dim=3, n=30, nabs=6, nt=90, kernel="OT2", grad_vp=True, nrec=4)
