@@ -145,10 +145,10 @@ def autotune_density(plan, steps=24, device=0, verbose=False, cands=None):
 
 
 def uses_fast_update(plan, save_mode):
-    """The scheme runs on the specialised 3D update (3D, OT2, no density, no attenuation or power 2,
-    no PML); the tile is tuned on the plain update."""
-    return (plan.dim == 3 and plan.kernel_order == 2 and plan.rho_mode == 0 and plan.att_power in (-2, 2)
-            and plan.boundary != 1 and save_mode in (0, 1))
+    """The scheme runs on the specialised 3D update (3D, OT2, no density, no PML; any attenuation
+    form); the tile is tuned on the plain update."""
+    return (plan.dim == 3 and plan.kernel_order == 2 and plan.rho_mode == 0 and plan.boundary != 1
+            and save_mode in (0, 1))
 
 
 def uses_density_update(plan):

@@ -1,8 +1,9 @@
 // Specialised 3D wavefield update for the headline schedule: OT2, no density
-// term, sponge or no boundary, no attenuation (one shot per pass: also power-2
-// attenuation). One pass per time step reads u (with halo), the other level, vp,
-// and writes the new level in place of the other level: 16 algorithmic bytes per
-// grid-point update (power 2: + alpha, lap other, lap u = 28 B). The every-f-th-step
+// term, sponge or no boundary, no attenuation (one shot per pass: also attenuation
+// power 0, 2 or None). One pass per time step reads u (with halo), the other level,
+// vp, and writes the new level in place of the other level: 16 algorithmic bytes per
+// grid-point update (+ 4 B alpha with attenuation; power 2 also lap other and lap u:
+// 28 B). The every-f-th-step
 // save of p.dt2 (forward) and the vp gradient / preconditioner accumulation
 // (adjoint) are fused into the same pass.
 #pragma once
@@ -39,21 +40,23 @@ bool fast3d_enabled();
 bool fast3d_supported(const Geo &g);
 int fast3d_prepare(Fast3D &f, const Geo &g, const float *vp, const float *tables, int tlen, int boundary, float dt,
                    hipStream_t stream);
-// Power-2 attenuation operands of the one-shot update: alpha (Np), the Laplacian of the level
-// `o` (read; out may overwrite o, so it was kept from the previous step) and the Laplacian of u
-// (written: the next step's). The update adds Cbeta (lap u - lap o), Cbeta = dt^2 ia 2 alpha
-// vp^3 / dt, as the general update does.
-struct FastAtt2 {
+// Attenuation operands of the one-shot update: the power (0, 2, or -1 for None; -2 none) and
+// alpha (Np); for power 2 also the Laplacian of the level `o` (read; out may overwrite o, so it
+// was kept from the previous step) and the Laplacian of u (written: the next step's). The
+// coefficients are the general update's: power 0 adds dt alpha vp to the damping, power None
+// scales the step by alpha, power 2 adds Cbeta (lap u - lap o), Cbeta = dt^2 ia 2 alpha vp^3 / dt.
+struct FastAtt {
+    int power;
     const float *alpha, *lapo;
     float *lapn;
 };
 // forward step: out = step(u, o) (out may alias o); optional fused save of p.dt2 into `save` (box layout)
 int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,
-                float idt2, hipStream_t stream, const FastAtt2 *att2 = nullptr);
+                float idt2, hipStream_t stream, const FastAtt *att = nullptr);
 // adjoint step with optional fused gradient: grad += ps * u, prec += ps^2 on the interior (box layout ps)
 int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, const float *ps,
                         const Box &box, float *grad, float *prec, hipStream_t stream,
-                        const FastAtt2 *att2 = nullptr);
+                        const FastAtt *att = nullptr);
 void fast3d_free(Fast3D &f);
 // tile shape used by later launches (autotuner)
 int fast3d_set_tune(Fast3D &f, int rows, int chunk, int version);

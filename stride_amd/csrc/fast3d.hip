@@ -47,8 +47,10 @@ struct FastArgs {
     // pairs, which level is u (parity of the step) and the tiles per group of the mapping
     const PairLevels *pairs;
     int npair, parity, group;
-    // power-2 attenuation (one-shot kernel): alpha in Np, the Laplacian of the other level
-    // (read) and of u (written: the next step's other level)
+    // attenuation (one-shot kernel): form (0 none, 1 power 0, 2 power 2, 3 power None), alpha
+    // in Np, and for power 2 the Laplacian of the other level (read) and of u (written: the next
+    // step's other level)
+    int att;
     const float *alpha, *lapo;
     float *lapn;
 };
@@ -118,16 +120,17 @@ __device__ __forceinline__ void dma16(rsrc_t r, float *lds, unsigned voff, int s
     __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void *)lds, 16, voff, soff, 0, 0);
 }
 
-// ATT2: power-2 attenuation, alpha and the other level's Laplacian are own-cell operands too
-template <int RPT, int MODE, bool ATT2>
+// ATT: attenuation form, 0 none, 1 power 0, 2 power 2, 3 power None. alpha is an own-cell
+// operand with attenuation, and with power 2 also the other level's Laplacian.
+template <int RPT, int MODE, int ATT>
 struct Stage5 {
     float o[RPT], v[RPT];
     float ps[MODE == 2 ? RPT : 1], gr[MODE == 2 ? RPT : 1], pr[MODE == 2 ? RPT : 1];
-    float al[ATT2 ? RPT : 1], lo[ATT2 ? RPT : 1];
+    float al[ATT ? RPT : 1], lo[ATT == 2 ? RPT : 1];
 };
 
-template <int RPT, int MODE, bool ATT2>
-__device__ __forceinline__ void load_stage5(Stage5<RPT, MODE, ATT2> &st, const Rsrc &rs, const unsigned (&row)[RPT],
+template <int RPT, int MODE, int ATT>
+__device__ __forceinline__ void load_stage5(Stage5<RPT, MODE, ATT> &st, const Rsrc &rs, const unsigned (&row)[RPT],
                                             const unsigned (&grow)[RPT], const unsigned (&brow)[RPT], int x,
                                             const FastArgs &a) {
     const int sp = (x + SA_HALO) * rs.plane_bytes;
@@ -135,10 +138,8 @@ __device__ __forceinline__ void load_stage5(Stage5<RPT, MODE, ATT2> &st, const R
     for (int r = 0; r < RPT; ++r) {
         st.o[r] = bload_nt(rs.o, row[r], sp);
         st.v[r] = bload_nt(rs.vp, row[r], sp);
-        if (ATT2) {
-            st.al[r] = bload_nt(rs.alpha, row[r], sp);
-            st.lo[r] = bload_nt(rs.lapo, row[r], sp);
-        }
+        if (ATT) st.al[r] = bload_nt(rs.alpha, row[r], sp);
+        if (ATT == 2) st.lo[r] = bload_nt(rs.lapo, row[r], sp);
     }
     if (MODE == 2) {
         // box planes outside [lo, lo + n) read as zero (offset past the end)
@@ -176,10 +177,10 @@ __device__ __forceinline__ void load_stage5(Stage5<RPT, MODE, ATT2> &st, const R
 // p - 5, now complete, is updated and stored. The first five planes of a chunk get their
 // contributions from own-column loads before the march, the last five are finished after it
 // with own-column loads of the five planes past the chunk.
-template <int RPT, int MODE, bool SPONGE, bool ATT2>
+template <int RPT, int MODE, bool SPONGE, int ATT>
 __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const Rsrc &rs, const int x, const int x0,
                                         const int x1, const float (&lap)[RPT], const float (&c)[RPT],
-                                        const Stage5<RPT, MODE, ATT2> &sx, const unsigned (&orow)[RPT],
+                                        const Stage5<RPT, MODE, ATT> &sx, const unsigned (&orow)[RPT],
                                         const unsigned (&grow)[RPT], const unsigned (&brow)[RPT],
                                         const int (&d1)[RPT], const float (&t1v)[RPT], const int d2, const float t2v) {
     const int nx = g.n[0];
@@ -202,24 +203,31 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
         const float v = sx.v[r];
         const float v2 = v * v;
         float A = 2.f, B = -1.f, G = a.dt2 * v2, cq = a.dt2;
-        if (SPONGE) {
-            // branch-free layer rule (axis d contributes its table value at distance d_d from
-            // the edge when no other axis is closer to its own edge); with sm = 0 every
-            // expression reduces exactly to the interior values
-            float sm = d0 <= min(d1[r], d2) ? t0v : 0.f;
-            sm += d1[r] <= min(d0, d2) ? t1v[r] : 0.f;
-            sm += d2 <= min(d0, d1[r]) ? t2v : 0.f;
-            const float D = (7.f * (sm * v)) * a.dt;
-            const float aa = a.dt * v2 * D;
-            const float e = a.dt * a.dt * v2 * D * D;
+        if (SPONGE || ATT == 1 || ATT == 3) {
+            // the coefficients of the general update (k_coefficients): power 0 adds dt alpha vp
+            // to the damping, power None scales the step by W = alpha
+            float aa = 0.f, e = 0.f;
+            if (ATT == 1) aa += a.dt * sx.al[r] * v;
+            if (SPONGE) {
+                // branch-free layer rule (axis d contributes its table value at distance d_d from
+                // the edge when no other axis is closer to its own edge); with sm = 0 every
+                // expression reduces exactly to the interior values
+                float sm = d0 <= min(d1[r], d2) ? t0v : 0.f;
+                sm += d1[r] <= min(d0, d2) ? t1v[r] : 0.f;
+                sm += d2 <= min(d0, d1[r]) ? t2v : 0.f;
+                const float D = (7.f * (sm * v)) * a.dt;
+                aa += a.dt * v2 * D;
+                e = a.dt * a.dt * v2 * D * D;
+            }
+            const float W = ATT == 3 ? sx.al[r] : 1.f;
             const float ia = __builtin_amdgcn_rcpf(1.f + aa);  // v_rcp_f32 (1 ulp), no divide sequence
-            A = (2.f - e) * ia;
-            B = -(1.f - aa) * ia;
-            cq = a.dt2 * ia;
+            A = W * (2.f - e) * ia;
+            B = -W * (1.f - aa) * ia;
+            cq = W * a.dt2 * ia;
             G = cq * v2;
         }
         float acc = G * lap[r];
-        if (ATT2) {
+        if (ATT == 2) {
             // power-2 attenuation (the general update's Cbeta term): Cbeta (lap u - lap other),
             // Cbeta = Cq 2 alpha vp^3 / dt; the Laplacian of u is the next step's lap other
             const float cb = cq * 2.f * sx.al[r] * v2 * v / a.dt;
@@ -241,10 +249,10 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
 // One step of the march: plane p (LDS slot J % 3, J = p - x0 mod 12). Partial-sum slot
 // (J + k) % 12 holds plane p + k (k = -5 .. 5); centre slot J % 6 holds u of plane p; operand
 // stage (J + 1) % 3 holds plane p - 5.
-template <int RPT, int NW, int MODE, bool SPONGE, bool ATT2, int NS, int J>
+template <int RPT, int NW, int MODE, bool SPONGE, int ATT, int NS, int J>
 __device__ __forceinline__ void march7_step(const Geo &g, const FastArgs &a, const Rsrc &rs, float *const (&sl)[6],
                                             float (&acc)[RPT][12],
-                                            float (&cq)[RPT][6], Stage5<RPT, MODE, ATT2> (&st)[3], const int p,
+                                            float (&cq)[RPT][6], Stage5<RPT, MODE, ATT> (&st)[3], const int p,
                                             const int x0, const int x1, const int lane, const int w,
                                             const unsigned (&row)[RPT], const unsigned (&orow)[RPT],
                                             const unsigned (&grow)[RPT], const unsigned (&brow)[RPT],
@@ -298,15 +306,15 @@ __device__ __forceinline__ void march7_step(const Geo &g, const FastArgs &a, con
         cq[r][J % 6] = c;
     }
     // 4. update and store plane p - 5; operands of plane p - 2 into its stage
-    Stage5<RPT, MODE, ATT2> &sx = st[(J + 1) % 3];
-    finish7<RPT, MODE, SPONGE, ATT2>(g, a, rs, p - 5, x0, x1, lap, c5, sx, orow, grow, brow, d1, t1v, d2, t2v);
-    load_stage5<RPT, MODE, ATT2>(st[(J + 1) % 3], rs, row, grow, brow, min(max(p - 2, 0), xlast), a);
+    Stage5<RPT, MODE, ATT> &sx = st[(J + 1) % 3];
+    finish7<RPT, MODE, SPONGE, ATT>(g, a, rs, p - 5, x0, x1, lap, c5, sx, orow, grow, brow, d1, t1v, d2, t2v);
+    load_stage5<RPT, MODE, ATT>(st[(J + 1) % 3], rs, row, grow, brow, min(max(p - 2, 0), xlast), a);
 }
 
 // Epilogue step: plane p past the march (own column from memory), finishing plane p - 5.
-template <int RPT, int NW, int MODE, bool SPONGE, bool ATT2, int J>
+template <int RPT, int NW, int MODE, bool SPONGE, int ATT, int J>
 __device__ __forceinline__ void tail7_step(const Geo &g, const FastArgs &a, const Rsrc &rs, float (&acc)[RPT][12],
-                                           float (&cq)[RPT][6], Stage5<RPT, MODE, ATT2> (&st)[3],
+                                           float (&cq)[RPT][6], Stage5<RPT, MODE, ATT> (&st)[3],
                                            const float (&up)[RPT],
                                            const int p, const int x0, const int x1, const unsigned (&row)[RPT],
                                            const unsigned (&orow)[RPT], const unsigned (&grow)[RPT],
@@ -322,13 +330,13 @@ __device__ __forceinline__ void tail7_step(const Geo &g, const FastArgs &a, cons
         lap[r] = acc[r][(J + 7) % 12];
         c5[r] = cq[r][(J + 1) % 6];
     }
-    finish7<RPT, MODE, SPONGE, ATT2>(g, a, rs, p - 5, x0, x1, lap, c5, st[(J + 1) % 3], orow, grow, brow, d1, t1v,
+    finish7<RPT, MODE, SPONGE, ATT>(g, a, rs, p - 5, x0, x1, lap, c5, st[(J + 1) % 3], orow, grow, brow, d1, t1v,
                                      d2, t2v);
-    if (J < 2) load_stage5<RPT, MODE, ATT2>(st[(J + 1) % 3], rs, row, grow, brow, min(p - 2, xlast), a);
+    if (J < 2) load_stage5<RPT, MODE, ATT>(st[(J + 1) % 3], rs, row, grow, brow, min(p - 2, xlast), a);
 }
 
 // The march of one chunk; SPONGE: the layer coefficients are evaluated on every cell.
-template <int RPT, int NW, int MODE, bool SPONGE, bool ATT2, int NS>
+template <int RPT, int NW, int MODE, bool SPONGE, int ATT, int NS>
 __device__ __forceinline__ void march7(const Geo &g, const FastArgs &a, const Rsrc &rs, float *const (&sl)[6],
                                        const int x0, const int x1,
                                        const int lane, const int w, const unsigned (&row)[RPT],
@@ -346,7 +354,7 @@ __device__ __forceinline__ void march7(const Geo &g, const FastArgs &a, const Rs
         for (int k = 0; k < KI; ++k)
             dma16(rs.u, sl[p] + (w + NW * k) * 256, dv[k], (min(x0 + p, xlast) + SA_HALO) * rs.plane_bytes);
     float acc[RPT][12], cq[RPT][6];
-    Stage5<RPT, MODE, ATT2> st[3] = {};  // filled inside the march before their first stored use
+    Stage5<RPT, MODE, ATT> st[3] = {};  // filled inside the march before their first stored use
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
 #pragma unroll
@@ -366,7 +374,7 @@ __device__ __forceinline__ void march7(const Geo &g, const FastArgs &a, const Rs
     for (int it = 0; it < nper; ++it) {
         const int xb = x0 + 12 * it;
 #define SA_STEP7(J)                                                                                              \
-    march7_step<RPT, NW, MODE, SPONGE, ATT2, NS, J>(g, a, rs, sl, acc, cq, st, xb + J, x0, x1, lane, w, row, orow, \
+    march7_step<RPT, NW, MODE, SPONGE, ATT, NS, J>(g, a, rs, sl, acc, cq, st, xb + J, x0, x1, lane, w, row, orow, \
                                           grow, brow, dv, d1, t1v, d2, t2v);
         SA_STEP7(0) SA_STEP7(1) SA_STEP7(2) SA_STEP7(3) SA_STEP7(4) SA_STEP7(5)
         SA_STEP7(6) SA_STEP7(7) SA_STEP7(8) SA_STEP7(9) SA_STEP7(10) SA_STEP7(11)
@@ -379,7 +387,7 @@ __device__ __forceinline__ void march7(const Geo &g, const FastArgs &a, const Rs
 #pragma unroll
         for (int r = 0; r < RPT; ++r) up[j][r] = bload(rs.u, row[r], (min(pend + j, xlast) + SA_HALO) * rs.plane_bytes);
 #define SA_TAIL7(J)                                                                                                 \
-    tail7_step<RPT, NW, MODE, SPONGE, ATT2, J>(g, a, rs, acc, cq, st, up[J], pend + J, x0, x1, row, orow, grow, brow, \
+    tail7_step<RPT, NW, MODE, SPONGE, ATT, J>(g, a, rs, acc, cq, st, up[J], pend + J, x0, x1, row, orow, grow, brow, \
                                                d1, t1v, d2, t2v);
     SA_TAIL7(0) SA_TAIL7(1) SA_TAIL7(2) SA_TAIL7(3) SA_TAIL7(4)
 #undef SA_TAIL7
@@ -390,7 +398,7 @@ __device__ __forceinline__ void march7(const Geo &g, const FastArgs &a, const Rs
 // register budget with one row per lane: 80 VGPRs (6 waves per SIMD);
 // without it the scheduler may trade occupancy for latency hiding that this
 // bandwidth-bound loop does not need
-template <int RPT, int NW, int MODE, bool SPONGE, bool ATT2, int NS>
+template <int RPT, int NW, int MODE, bool SPONGE, int ATT, int NS>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(RPT == 1 ? 6 : 1))) void k_fast3d_v7(
     Geo g, FastArgs a) {
     constexpr int TY = NW * RPT;
@@ -428,8 +436,8 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(RPT == 
         rs.grad = make_rsrc(a.grad, bytes);
         rs.prec = make_rsrc(a.prec, bytes);
     }
-    if (ATT2) {
-        rs.alpha = make_rsrc(a.alpha, bytes);
+    if (ATT) rs.alpha = make_rsrc(a.alpha, bytes);
+    if (ATT == 2) {
         rs.lapo = make_rsrc(a.lapo, bytes);
         rs.lapn = make_rsrc(a.lapn, bytes);
     }
@@ -470,10 +478,10 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(RPT == 
     const bool interior = y0 >= g.nabs[1] && y0 + TY <= ny - g.nabs[1] && z0 >= g.nabs[2] &&
                           z0 + kTZ <= nz - g.nabs[2] && x0 >= g.nabs[0] && x1 <= g.n[0] - g.nabs[0];
     if (SPONGE && !interior)
-        march7<RPT, NW, MODE, true, ATT2, NS>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2,
+        march7<RPT, NW, MODE, true, ATT, NS>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2,
                                               t2v);
     else
-        march7<RPT, NW, MODE, false, ATT2, NS>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2,
+        march7<RPT, NW, MODE, false, ATT, NS>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2,
                                                t2v);
 }
 
@@ -855,17 +863,22 @@ __global__ __launch_bounds__(NW * 64) void k_fast3d_b2(Geo g, FastArgs a) {
         marchB<RPT, NW, MODE, false, ISO>(g, a, rs, sl, x0, x1, lane, w, row, orow, grow, brow, dv, d1, t1v, d2, t2v);
 }
 
+template <int RPT, int NW, int MODE, int ATT, int NS>
+void launch_v7_att(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
+    if (sponge)
+        k_fast3d_v7<RPT, NW, MODE, true, ATT, NS><<<grid, NW * 64, 0, st>>>(g, a);
+    else
+        k_fast3d_v7<RPT, NW, MODE, false, ATT, NS><<<grid, NW * 64, 0, st>>>(g, a);
+}
+
 template <int RPT, int NW, int MODE, int NS>
 void launch_v7(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
-    const bool att2 = a.alpha != nullptr;
-    if (sponge && att2)
-        k_fast3d_v7<RPT, NW, MODE, true, true, NS><<<grid, NW * 64, 0, st>>>(g, a);
-    else if (sponge)
-        k_fast3d_v7<RPT, NW, MODE, true, false, NS><<<grid, NW * 64, 0, st>>>(g, a);
-    else if (att2)
-        k_fast3d_v7<RPT, NW, MODE, false, true, NS><<<grid, NW * 64, 0, st>>>(g, a);
-    else
-        k_fast3d_v7<RPT, NW, MODE, false, false, NS><<<grid, NW * 64, 0, st>>>(g, a);
+    switch (a.att) {
+        case 1: launch_v7_att<RPT, NW, MODE, 1, NS>(sponge, grid, st, g, a); break;
+        case 2: launch_v7_att<RPT, NW, MODE, 2, NS>(sponge, grid, st, g, a); break;
+        case 3: launch_v7_att<RPT, NW, MODE, 3, NS>(sponge, grid, st, g, a); break;
+        default: launch_v7_att<RPT, NW, MODE, 0, NS>(sponge, grid, st, g, a); break;
+    }
 }
 
 template <int MODE>
@@ -1019,13 +1032,15 @@ int fast3d_prepare(Fast3D &f, const Geo &g, const float *vp, const float *tables
 }
 
 int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,
-                float idt2, hipStream_t stream, const FastAtt2 *att2) {
+                float idt2, hipStream_t stream, const FastAtt *att) {
     FastArgs a = base_args(f, u, o, out);
-    if (att2) {
-        if (!att2->alpha || !att2->lapo || !att2->lapn) return SA_E_ARG;
-        a.alpha = att2->alpha;
-        a.lapo = att2->lapo;
-        a.lapn = att2->lapn;
+    if (att) {
+        if (att->power == 2 && (!att->alpha || !att->lapo || !att->lapn)) return SA_E_ARG;
+        if (att->power != -2 && !att->alpha) return SA_E_ARG;
+        a.att = att->power == 0 ? 1 : att->power == 2 ? 2 : att->power == -1 ? 3 : 0;
+        a.alpha = att->alpha;
+        a.lapo = att->lapo;
+        a.lapn = att->lapn;
     }
     if (save) {
         a.save = save;
@@ -1037,13 +1052,15 @@ int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *
 }
 
 int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, const float *ps,
-                        const Box &box, float *grad, float *prec, hipStream_t stream, const FastAtt2 *att2) {
+                        const Box &box, float *grad, float *prec, hipStream_t stream, const FastAtt *att) {
     FastArgs a = base_args(f, u, o, out);
-    if (att2) {
-        if (!att2->alpha || !att2->lapo || !att2->lapn) return SA_E_ARG;
-        a.alpha = att2->alpha;
-        a.lapo = att2->lapo;
-        a.lapn = att2->lapn;
+    if (att) {
+        if (att->power == 2 && (!att->alpha || !att->lapo || !att->lapn)) return SA_E_ARG;
+        if (att->power != -2 && !att->alpha) return SA_E_ARG;
+        a.att = att->power == 0 ? 1 : att->power == 2 ? 2 : att->power == -1 ? 3 : 0;
+        a.alpha = att->alpha;
+        a.lapo = att->lapo;
+        a.lapn = att->lapn;
     }
     if (ps && grad && prec) {
         a.ps = ps;

@@ -1188,10 +1188,10 @@ static bool use_flat2d(const sa_solver *s) {
     return s->grid.dim == 2 && s->sch.kernel == 2 && s->rho3d.enabled && flat2d_supported(s->geo);
 }
 
-// the specialised 3D update: no density term, no attenuation or power 2 (one shot per pass)
+// the specialised 3D update: no density term; attenuation of any form with one shot per pass
 static bool use_fast(const sa_solver *s) {
     const sa_scheme &c = s->sch;
-    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 && (c.att_power == -2 || c.att_power == 2) &&
+    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 &&
            c.boundary != 1 && (c.save_mode == 0 || c.save_mode == 1) && fast3d_enabled() &&
            fast3d_supported(s->geo);
 }
@@ -1693,10 +1693,10 @@ static int forward_step(sa_solver *s, int t, float *u, float *o, const float *sr
     const float idt2 = 1.f / (s->dt * s->dt);
     bool injected = false;
     if (use_fast(s)) {
-        FastAtt2 at{}, *pat = nullptr;
-        if (s->sch.att_power == 2) {
-            TRY(lap_other(s, o));
-            at = {s->alpha, s->lapother, s->lapnext};
+        FastAtt at{}, *pat = nullptr;
+        if (s->sch.att_power != -2) {
+            if (s->sch.att_power == 2) TRY(lap_other(s, o));
+            at = {s->sch.att_power, s->alpha, s->lapother, s->lapnext};
             pat = &at;
         }
         hipEvent_t e0;
@@ -1704,7 +1704,7 @@ static int forward_step(sa_solver *s, int t, float *u, float *o, const float *sr
         if (fast3d_step(s->fast, s->geo, u, o, o, save, s->box, idt2, s->stream, pat) != 0)
             return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
         TRY(timed_end(s));
-        if (pat) lap_advance(s, u);
+        if (s->sch.att_power == 2) lap_advance(s, u);
     } else {
         const InjectSpec spec{&s->src, src + (size_t)t * s->src.npoint, 1.f, mode == 1};
         TRY(step_general(s, u, o, o, 0, save, mode, &spec, &injected, gat));
@@ -1920,10 +1920,10 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
         if (guard && raw_ckpt && (t / f) / s->ck_K != s->ck_seg) TRY(recompute_raw_segment(s, (t / f) / s->ck_K));
         const float *ps = guard ? snap(t / f) : nullptr;
         if (fast) {
-            FastAtt2 at{}, *pat = nullptr;
-            if (c.att_power == 2) {
-                TRY(lap_other(s, o));
-                at = {s->alpha, s->lapother, s->lapnext};
+            FastAtt at{}, *pat = nullptr;
+            if (c.att_power != -2) {
+                if (c.att_power == 2) TRY(lap_other(s, o));
+                at = {c.att_power, s->alpha, s->lapother, s->lapnext};
                 pat = &at;
             }
             hipEvent_t e0;
@@ -1932,7 +1932,7 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
                                     pat) != 0)
                 return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
             TRY(timed_end(s));
-            if (pat) lap_advance(s, u);
+            if (c.att_power == 2) lap_advance(s, u);
         } else {
             const InjectSpec spec{&s->rec, res + (size_t)t * s->rec.npoint, -1.f, false};
             // the fused 2D update also accumulates this step's gradients (it reads u and the

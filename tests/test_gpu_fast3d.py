@@ -94,15 +94,24 @@ def test_fast_anisotropic_spacing(gpu, tile):
     assert rel_l2(g_f['grad_vp'], gref['grad_vp']) < 1e-4
 
 
-@pytest.mark.parametrize('n,nabs,tile', [(37, 5, (2, 13, 7)), (41, 7, (1, 24, 8)), (30, 6, (2, 12, 8))])
-def test_fast_power2_attenuation(gpu, n, nabs, tile):
-    """Power-2 attenuation on the specialised update (the Cbeta (lap u - lap other) term, the
-    Laplacian of u handed to the next step) against the general passes and the oracle: traces and
-    the as-written vp gradient."""
-    plan, _ = small_case(dim=3, n=n, nabs=nabs, nt=90, kernel='OT2', grad_vp=True, nrec=20, alpha='p2',
-                         attenuation_power=2)
-    assert plan.att_power == 2
-    print('fast3d power-2 n=%d nabs=%d tile=%s' % (n, nabs, tile), flush=True)
+@pytest.mark.parametrize('n,nabs,tile,power', [
+    (37, 5, (2, 13, 7), 2), (41, 7, (1, 24, 8), 2), (30, 6, (2, 12, 8), 2),
+    (37, 5, (2, 13, 7), 0), (41, 7, (1, 24, 8), 0), (30, 6, (2, 24, 7), None),
+])
+def test_fast_attenuation(gpu, n, nabs, tile, power):
+    """Attenuation on the specialised update against the general passes and the oracle (traces and
+    the as-written vp gradient): power 0 (dt alpha vp in the damping), power None (the step scaled
+    by alpha) and power 2 (the Cbeta (lap u - lap other) term, the Laplacian of u handed to the
+    next step)."""
+    if power is None:
+        plan, _ = small_case(dim=3, n=n, nabs=nabs, nt=90, kernel='OT2', grad_vp=True, nrec=20,
+                             alpha='none-power')
+        assert plan.att_power == -1
+    else:
+        plan, _ = small_case(dim=3, n=n, nabs=nabs, nt=90, kernel='OT2', grad_vp=True, nrec=20, alpha='p',
+                             attenuation_power=power)
+        assert plan.att_power == power
+    print('fast3d attenuation %s n=%d nabs=%d tile=%s' % (power, n, nabs, tile), flush=True)
     rng = np.random.default_rng(6)
     res_s = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, plan.nt)))
     rec_f, g_f = _run(plan, res_s, general=False, tile=tile)

@@ -36,6 +36,8 @@ FORWARD_CASES = [
     (3, dict(kernel='OT4', rho='var', interpolation_type='hicks')),
     (3, dict(kernel='OT2', alpha='p0', boundary_type='complex_frequency_shift_PML_2')),
     (3, dict(kernel='OT2', alpha='p2', attenuation_power=2)),
+    (3, dict(kernel='OT2', alpha='p0')),
+    (3, dict(kernel='OT2', alpha='none-power')),
 ]
 
 
