@@ -571,6 +571,55 @@ def bench_power2(nt=500, general=False):
     return out
 
 
+def bench_autograd(nt=2500, checkpoint=0):
+    """One shot of the 3D gradient workload through the torch.autograd.Function (exact adjoint):
+    variable vp / rho / alpha (attenuation power 0) on the 360^3 grid, device tensors, the forward
+    keeping one pair of levels every K steps (K = ceil(sqrt(2 nt)) unless given) and the backward
+    recomputing each K-step segment; dJ/dvp, dJ/drho, dJ/dalpha of J = 1/2 |traces - obs|^2. The
+    rate counts the shot's forward and adjoint updates (the recomputation is extra work)."""
+    import torch
+    from stride_amd.autograd import Geometry, iso_acoustic_traces
+    from stride_amd.configs import config4
+    cfg = config4(scale=1.0, nt=nt, nrec=1024)
+    space, time_ = cfg['space'], cfg['time']
+    K = checkpoint or int(np.ceil(np.sqrt(2 * nt)))
+    geom = Geometry(space, time_, cfg['sources'], cfg['receivers'], checkpoint_every=K, kernel='OT2')
+    dev = torch.device('cuda')
+    vp = torch.tensor(np.asarray(cfg['vp'], np.float32), device=dev, requires_grad=True)
+    rho = torch.tensor(np.asarray(cfg['rho'], np.float32), device=dev, requires_grad=True)
+    alpha = torch.tensor(np.asarray(cfg['alpha'], np.float32), device=dev, requires_grad=True)
+    wav = torch.tensor(cfg['wavelet'][None].astype(np.float32), device=dev)
+    obs = torch.zeros((cfg['receivers'].shape[0], nt), device=dev)
+
+    def shot():
+        for t in (vp, rho, alpha):
+            t.grad = None
+        tr = iso_acoustic_traces(vp, rho, alpha, wav, geom)
+        loss = 0.5 * torch.sum((tr - obs) ** 2)
+        loss.backward()
+        torch.cuda.synchronize()
+        return float(loss.detach())
+
+    t0 = time.perf_counter()
+    shot()  # plan, allocations
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = shot()
+    wall = time.perf_counter() - t0
+    n = int(np.prod(space.extended_shape))
+    ok = all(torch.isfinite(t.grad).all().item() and t.grad.abs().max().item() > 0 for t in (vp, rho, alpha))
+    per_gb = n * 4 / 1e9
+    return dict(metric='Gpts/s fwd+adjoint 3D 320^3 SO10, exact adjoint through torch.autograd, '
+                       'vp/rho/alpha gradients',
+                value=round(2 * (nt - 1) * n / wall / 1e9, 3), unit='Gpts/s', seconds=round(wall, 3),
+                first_call_seconds=round(first, 3), checkpoint_every=K,
+                wavefield_store_gb=round((2 * (nt // K + 1) + K + 1) * per_gb, 1), loss=loss,
+                gradients_finite_nonzero=bool(ok), dtype='f32', data='synthetic',
+                config={'workload': '3D 320x320x320 inner (360x360x360 extended), dx 0.4 mm, SO10 OT2, sponge 20, '
+                                    'variable vp / rho / alpha (power 0), one shot, nt %d, 1024 receivers, '
+                                    'checkpoints every %d steps' % (nt, K)})
+
+
 def cpu_baseline_2d(cfg, nt=500):
     """The oracle (C/OpenMP, test infrastructure) on the 2D gradient workload's medium and scheme: a
     bounded forward with raw saves + as-written adjoint with vp and rho gradients."""
@@ -767,6 +816,9 @@ def main():
                          '(seeded schedule) and exit')
     ap.add_argument('--checkpoint', type=int, default=0,
                     help='with --density: recompute the raw snapshots from one state per this many snapshots')
+    ap.add_argument('--autograd', action='store_true',
+                    help='one 3D gradient-workload shot through the autograd exact adjoint with checkpoints '
+                         '(--checkpoint K, default ceil(sqrt(2 nt))) and exit')
     ap.add_argument('--power2', action='store_true',
                     help='one 3D shot with power-2 attenuation on the fused update and on the general passes '
                          '(use with --nt) and exit')
@@ -824,6 +876,9 @@ def main():
         return
     if args.gradient_2d:
         print(json.dumps(bench_gradient_2d(workers=args.workers, with_cpu=not args.no_cpu)))
+        return
+    if args.autograd:
+        print(json.dumps(bench_autograd(args.nt, args.checkpoint)))
         return
     if args.power2:
         print(json.dumps({'fused': bench_power2(args.nt), 'general': bench_power2(args.nt, general=True)}))
