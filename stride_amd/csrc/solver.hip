@@ -713,6 +713,8 @@ struct sa_solver {
     float *fb[2] = {};
     float *src_keep = nullptr;
     size_t src_keep_n = 0;
+    float *medium_stage = nullptr;  // unpadded medium array on its way to the padded layout
+    size_t medium_stage_n = 0;
     // checkpointed raw saves (save_mode 5): `saved` holds the states (p[T_j], p[T_j - 1]),
     // T_j = (jK - 1) f, of segments j >= 1; `segment` is a ring of K + 2 snapshots (snapshot i in
     // slot i % (K + 2)) holding snapshots jK - 1 .. (j + 1)K of segment `ck_seg`; snapshots outside
@@ -835,36 +837,7 @@ static int zero_field2(sa_solver *s, float *p) {  // two interleaved shots
     return SA_OK;
 }
 
-// padded copy-in from an unpadded extended-grid host array; edge: replicate into halo
-static void pad_host(const sa_solver *s, std::vector<float> &dst, const float *src, bool edge) {
-    const Geo &g = s->geo;
-    dst.assign(s->total, 0.f);
-    int lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-        int halo_lo = g.act[a] ? (a == 2 ? SA_LEAD : SA_HALO) : 0;
-        int halo_hi = g.act[a] ? (s->P[a] - halo_lo - g.n[a]) : 0;
-        lo[a] = edge ? -halo_lo : 0;
-        hi[a] = edge ? g.n[a] + halo_hi : g.n[a];
-    }
-    for (int i = lo[0]; i < hi[0]; ++i)
-        for (int j = lo[1]; j < hi[1]; ++j)
-            for (int k = lo[2]; k < hi[2]; ++k) {
-                int ii = std::min(std::max(i, 0), g.n[0] - 1);
-                int jj = std::min(std::max(j, 0), g.n[1] - 1);
-                int kk = std::min(std::max(k, 0), g.n[2] - 1);
-                long long p = g.base + (long long)i * g.s0 + (long long)j * g.s1 + k;
-                dst[p] = src[((long long)ii * g.n[1] + jj) * g.n[2] + kk];
-            }
-}
-
-static int upload_padded(sa_solver *s, float **dev, const float *host, bool edge) {
-    TRY(dalloc(dev, s->total));
-    std::vector<float> tmp;
-    pad_host(s, tmp, host, edge);
-    HIP_TRY(h2d(*dev, tmp.data(), s->total * sizeof(float)));
-    return SA_OK;
-}
-
+// unpadded extended-grid host copy of a padded device array
 static int download_unpadded(sa_solver *s, float *host, const float *dev) {
     const Geo &g = s->geo;
     std::vector<float> tmp(s->total);
@@ -959,6 +932,7 @@ extern "C" int sa_destroy(sa_solver *s) {
     dfree(s->fb[0]);
     dfree(s->fb[1]);
     dfree(s->src_keep);
+    dfree(s->medium_stage);
     dfree(s->xfer);
     dfree(s->dump[0]);
     dfree(s->dump[1]);
@@ -1020,20 +994,36 @@ extern "C" int sa_set_scheme(sa_solver *s, const sa_scheme *sch) {
     return SA_OK;
 }
 
+static int upload_padded_dev(sa_solver *s, float **dev, const float *src);
+static int ensure_staging(sa_solver *s, float **p, size_t *have, size_t n);
+
+// A host medium array goes to the device unpadded (one copy of the extended grid) and is padded
+// there (edge halo, k_pad_edge); the staging buffer is reused by the next array once the padding
+// launch has run.
+static int upload_medium(sa_solver *s, float **dev, const float *host) {
+    TRY(ensure_staging(s, &s->medium_stage, &s->medium_stage_n, (size_t)s->npts));
+    HIP_TRY(h2d(s->medium_stage, host, (size_t)s->npts * sizeof(float)));
+    TRY(upload_padded_dev(s, dev, s->medium_stage));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SA_OK;
+}
+
 extern "C" int sa_set_medium(sa_solver *s, const float *vp, const float *rho, const float *alpha) {
     if (!s || !vp) return fail(SA_E_ARG, "vp is required");
     TRY(quiesce(s));
-    TRY(upload_padded(s, &s->vp, vp, true));
-    if (rho) {
-        std::vector<float> b(s->npts);
-        for (long long q = 0; q < s->npts; ++q) {
+    if (rho)
+        for (long long q = 0; q < s->npts; ++q)
             if (!(rho[q] > 0)) return fail(SA_E_ARG, "rho must be positive");
-            b[q] = 1.f / rho[q];
-        }
-        TRY(upload_padded(s, &s->rho, rho, true));
-        TRY(upload_padded(s, &s->b, b.data(), true));
+    TRY(upload_medium(s, &s->vp, vp));
+    if (rho) {
+        TRY(upload_medium(s, &s->rho, rho));
+        // buoyancy 1 / rho over the whole padded array (the edge halo of rho is positive)
+        TRY(dalloc(&s->b, s->total));
+        k_reciprocal<<<(unsigned)((s->total + 255) / 256), 256, 0, s->stream>>>(s->rho, s->b, s->total);
+        CHECK_LAUNCH();
     }
-    if (alpha) TRY(upload_padded(s, &s->alpha, alpha, true));
+    if (alpha) TRY(upload_medium(s, &s->alpha, alpha));
+    HIP_TRY(hipStreamSynchronize(s->stream));
     s->medium_set = true;
     s->coef_dirty = true;
     return SA_OK;
