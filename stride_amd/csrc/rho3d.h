@@ -22,6 +22,7 @@ struct Rho3DStep {
     int save_mode;
     Box box;
     float idt2;
+    int konly = 0;  // out = K u only (A, B, G, other and save unused)
 };
 
 // true when the layout fits the kernel's 32-bit buffer offsets

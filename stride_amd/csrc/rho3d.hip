@@ -41,6 +41,7 @@ struct RhoArgs {
     int save_mode;
     float idt2;
     Box box;
+    int konly;  // out = K u (no update, no save)
 };
 
 template <int RPT, int NW, bool VARB>
@@ -92,8 +93,9 @@ __device__ __forceinline__ void load_tile(State<RPT, NW, VARB> &st, const Res &r
     }
 }
 
-template <int S, int RPT, int NW, bool VARB>
+template <int S, int RPT, int NW, bool VARB, bool KONLY>
 __device__ __forceinline__ void load_own(State<RPT, NW, VARB> &st, const Res &rs, unsigned poff) {
+    if (KONLY) return;  // out = K u: no own operands
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         st.ov[S][r] = load_nt(rs.o, st.own[r] + poff);
@@ -103,7 +105,7 @@ __device__ __forceinline__ void load_own(State<RPT, NW, VARB> &st, const Res &rs
     }
 }
 
-template <int RPT, int NW, bool VARB, bool SAVE, int J>
+template <int RPT, int NW, bool VARB, bool SAVE, bool KONLY, int J>
 __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res &rs, State<RPT, NW, VARB> &st,
                                       float *sU, float *sB, float *sFy, float *sFz, const int m, const int x0,
                                       const int x1, const int y0, const int z0, const int lane, const int w) {
@@ -123,7 +125,7 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
     load_tile<S>(st, rs, (unsigned)((m + 2) * a.plane_bytes));
     const int p = m - 9;
     const bool emit = p >= x0 && p < x1;
-    load_own<1 - S>(st, rs, (unsigned)((p + 1) * a.plane_bytes));
+    load_own<1 - S, RPT, NW, VARB, KONLY>(st, rs, (unsigned)((p + 1) * a.plane_bytes));
     lds_barrier();  // plane m's tiles are in LDS
     const bool inplane = m >= x0 && m < x1;
     if (inplane) {
@@ -198,7 +200,8 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const float u = st.uq[r][(J + 1) % 10];
-            const float out = st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][r] + st.Gv[S][r] * st.nd[r][(J + 1) % 10];
+            const float out = KONLY ? st.nd[r][(J + 1) % 10]
+                                    : st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][r] + st.Gv[S][r] * st.nd[r][(J + 1) % 10];
             store_nt(out, rs.out, emit ? st.own[r] + po : kOOB);
             if (SAVE && emit) {
                 const int y = y0 + w * RPT + r, z = z0 + lane;
@@ -211,7 +214,7 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
     for (int r = 0; r < RPT; ++r) st.nd[r][(J + 1) % 10] = 0.f;
 }
 
-template <int RPT, int NW, bool VARB, bool SAVE>
+template <int RPT, int NW, bool VARB, bool SAVE, bool KONLY>
 __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     using T = Tile<RPT, NW, VARB>;
     __shared__ float4 sU4[T::UY * T::UZ / 4 + 1];  // + a dump slot for threads without a chunk
@@ -277,7 +280,8 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     const int nper = (x1 - x0 + 18 + 9) / 10;
     for (int q = 0; q < nper; ++q) {
         const int m = mbeg + 10 * q;
-#define SA_RHO_PLANE(J) plane<RPT, NW, VARB, SAVE, J>(g, a, rs, st, sU, sB, sFy, sFz, m + J, x0, x1, y0, z0, lane, w)
+#define SA_RHO_PLANE(J) \
+    plane<RPT, NW, VARB, SAVE, KONLY, J>(g, a, rs, st, sU, sB, sFy, sFz, m + J, x0, x1, y0, z0, lane, w)
         SA_RHO_PLANE(0);
         SA_RHO_PLANE(1);
         SA_RHO_PLANE(2);
@@ -299,10 +303,12 @@ int launch(const Geo &g, RhoArgs a, bool varb, bool save, hipStream_t stream) {
     a.ntiles = a.ntz * a.nty * ((g.n[0] + a.chunk - 1) / a.chunk);
     a.per_xcd = (a.ntiles + 7) / 8;
     const dim3 grid(8 * a.per_xcd), block(NW * 64);
-    if (varb && save) k_rho3d<RPT, NW, true, true><<<grid, block, 0, stream>>>(g, a);
-    else if (varb) k_rho3d<RPT, NW, true, false><<<grid, block, 0, stream>>>(g, a);
-    else if (save) k_rho3d<RPT, NW, false, true><<<grid, block, 0, stream>>>(g, a);
-    else k_rho3d<RPT, NW, false, false><<<grid, block, 0, stream>>>(g, a);
+    if (a.konly && varb) k_rho3d<RPT, NW, true, false, true><<<grid, block, 0, stream>>>(g, a);
+    else if (a.konly) k_rho3d<RPT, NW, false, false, true><<<grid, block, 0, stream>>>(g, a);
+    else if (varb && save) k_rho3d<RPT, NW, true, true, false><<<grid, block, 0, stream>>>(g, a);
+    else if (varb) k_rho3d<RPT, NW, true, false, false><<<grid, block, 0, stream>>>(g, a);
+    else if (save) k_rho3d<RPT, NW, false, true, false><<<grid, block, 0, stream>>>(g, a);
+    else k_rho3d<RPT, NW, false, false, false><<<grid, block, 0, stream>>>(g, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -544,6 +550,8 @@ int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_
     a.save_mode = st.save_mode;
     a.idt2 = st.idt2;
     a.box = st.box;
+    a.konly = st.konly;
+    if (st.konly && st.save) return -1;
     const bool varb = st.b != nullptr, save = st.save != nullptr;
     if (t.waves == 4 && t.rows == 1) return launch<1, 4>(g, a, varb, save, stream);
     if (t.waves == 4 && t.rows == 2) return launch<2, 4>(g, a, varb, save, stream);

@@ -579,16 +579,6 @@ __global__ void k_grad_rho(Geo g, const float *__restrict__ work, const float *_
 }
 
 // exact adjoint accumulation for equation t: lambda = mu_{t+1} / G
-__global__ void k_exact_acc(Geo g, const float *__restrict__ mu, const float *__restrict__ G,
-                            const float *__restrict__ u, const float *__restrict__ um, const float *__restrict__ Ku,
-                            float *gA, float *gB, float *gG) {
-    DOMAIN_INDEX
-    float lam = mu[p] / G[p];
-    gA[p] += lam * u[p];
-    gB[p] += lam * um[p];
-    gG[p] += lam * Ku[p];
-}
-
 __global__ void k_exact_acc_m(Geo g, const float *__restrict__ Kmu, const float *__restrict__ Ku, float c4,
                               float *gm) {
     DOMAIN_INDEX
@@ -600,18 +590,29 @@ __global__ void k_axpy_domain(Geo g, float alpha, const float *__restrict__ x, f
     y[p] += alpha * x[p];
 }
 
-// d/db through the half-point buoyancy: -1/2 sum_a (D+ mu D+ u)(p) + (D+ mu D+ u)(p - s_a)
-__global__ void k_exact_acc_b(Geo g, const float *su0, const float *su1, const float *su2, const float *sm0,
-                              const float *sm1, const float *sm2, float *gb) {
+// The exact accumulations of one adjoint step in one pass: gA += lam u, gB += lam um,
+// gG += lam Ku (lam = mu / G) and, with gb, d/db through the half-point buoyancy,
+// gb -= 1/2 sum_a (D+ mu D+ u)(p) + (D+ mu D+ u)(p - s_a), the staggered first derivatives formed
+// here from u and mu (sa_dplus, the staggered derivative of k_stagger).
+__global__ void k_exact_acc_fused(Geo g, const float *__restrict__ mu, const float *__restrict__ G,
+                                  const float *__restrict__ u, const float *__restrict__ um,
+                                  const float *__restrict__ Ku, float *gA, float *gB, float *gG, float *gb) {
     DOMAIN_INDEX
-    const float *su[3] = {su0, su1, su2}, *sm[3] = {sm0, sm1, sm2};
-    float acc = 0.f;
-    for (int a = 0; a < 3; ++a) {
-        if (!g.act[a]) continue;
-        long long s = sa_stride(g, a);
-        acc += sm[a][p] * su[a][p] + sm[a][p - s] * su[a][p - s];
+    float lam = mu[p] / G[p];
+    gA[p] += lam * u[p];
+    gB[p] += lam * um[p];
+    gG[p] += lam * Ku[p];
+    if (gb) {
+        float acc = 0.f;
+        for (int a = 0; a < 3; ++a) {
+            if (!g.act[a]) continue;
+            const long long s = sa_stride(g, a);
+            const float up = sa_dplus(g, u, p, a), ul = sa_dplus(g, u, p - s, a);
+            const float mp = sa_dplus(g, mu, p, a), ml = sa_dplus(g, mu, p - s, a);
+            acc += mp * up + ml * ul;
+        }
+        gb[p] += -0.5f * acc;
     }
-    gb[p] += -0.5f * acc;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -694,7 +695,6 @@ struct sa_solver {
     float *lapnext = nullptr;
     const float *lap_of = nullptr;
     float *st[3] = {};
-    float *st2[3] = {};
     float *work0 = nullptr, *work1 = nullptr;
     int cur = 0;  // index of the last computed level in buf
     // sparse
@@ -937,7 +937,7 @@ extern "C" int sa_destroy(sa_solver *s) {
     dfree(s->dump[0]);
     dfree(s->dump[1]);
     for (int a = 0; a < 3; ++a) {
-        dfree(s->u1[a]); dfree(s->u2[a]); dfree(s->u3[a]); dfree(s->st[a]); dfree(s->st2[a]);
+        dfree(s->u1[a]); dfree(s->u2[a]); dfree(s->u3[a]); dfree(s->st[a]);
         for (int r = 0; r < 4; ++r) dfree(s->pml[a][r]);
     }
     free_sparse(s->src);
@@ -1289,6 +1289,23 @@ static int apply_K(sa_solver *s, const float *src, const float *scale, float *ds
         return SA_OK;
     }
     const float *b = s->sch.rho_mode == 2 ? s->b : nullptr;
+    if (!scale && s->grid.dim == 3 && s->rho3d.enabled && rho3d_supported(g)) {
+        // K u of the density forms in one pass (the fused density update's K-only form)
+        Rho3DStep st;
+        st.u = src;
+        st.other = nullptr;
+        st.out = dst;
+        st.b = b;
+        st.A = st.B = st.G = nullptr;
+        st.save = nullptr;
+        st.save_mode = 0;
+        st.box = s->box;
+        st.idt2 = 0.f;
+        st.konly = 1;
+        if (rho3d_step(s->rho3d, g, st, s->stream) != 0)
+            return fail(SA_E_HIP, "K-only density launch: %s", hipGetErrorString(hipGetLastError()));
+        return SA_OK;
+    }
     for (int a = 0; a < 3; ++a) {
         if (!g.act[a]) continue;
         int ext[3] = {g.n[0], g.n[1], g.n[2]};
@@ -2050,11 +2067,7 @@ static int adjoint_exact_core(sa_solver *s, const float *res_dev, float *lam_dev
     for (auto &a : acc) a = nullptr;
     TRY(grad_outputs(s, 3, acc));
     if (c.kernel == 4) TRY(grad_outputs(s, 1, acc + 3));
-    if (c.rho_mode == 2) {
-        TRY(grad_outputs(s, 1, acc + 4));
-        for (int a = 0; a < 3; ++a)
-            if (g.act[a]) TRY(dalloc(&s->st2[a], s->total));
-    }
+    if (c.rho_mode == 2) TRY(grad_outputs(s, 1, acc + 4));
     TRY(dalloc(&s->work0, s->total));
     TRY(dalloc(&s->work1, s->total));
     float *KuT = nullptr, *Kmu = nullptr;
@@ -2106,21 +2119,10 @@ static int adjoint_exact_core(sa_solver *s, const float *res_dev, float *lam_dev
             if ((rc = apply_K(s, KuT, s->M, Kmu)) != SA_OK) break;
             k_axpy_domain<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, c4, Kmu, KuT);
         }
-        if (c.rho_mode == 2) {
-            // staggered derivatives of u_t (work0) and mu_{t+1} (m1) without buoyancy
-            for (int a = 0; a < 3; ++a) {
-                if (!g.act[a]) continue;
-                int ext[3] = {g.n[0], g.n[1], g.n[2]};
-                ext[a] += 10;
-                dim3 grid((ext[2] + kBlk.x - 1) / kBlk.x, (ext[1] + kBlk.y - 1) / kBlk.y, ext[0]);
-                k_stagger<<<grid, kBlk, 0, s->stream>>>(g, a, s->work0, nullptr, nullptr, s->st[a]);
-                k_stagger<<<grid, kBlk, 0, s->stream>>>(g, a, m1, nullptr, nullptr, s->st2[a]);
-            }
-            k_exact_acc_b<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->st[0], s->st[1], s->st[2], s->st2[0],
-                                                                        s->st2[1], s->st2[2], acc[4]);
-        }
-        k_exact_acc<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, m1, s->G, s->work0, s->work1, KuT, acc[0],
-                                                                  acc[1], acc[2]);
+        // gA, gB, gG and (variable density) gb, with the staggered derivatives of u_t (work0) and
+        // mu_{t+1} (m1) formed in the same pass
+        k_exact_acc_fused<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(
+            g, m1, s->G, s->work0, s->work1, KuT, acc[0], acc[1], acc[2], c.rho_mode == 2 ? acc[4] : nullptr);
         if (hipGetLastError() != hipSuccess) rc = fail(SA_E_HIP, "launch failed");
     }
     dfree(KuT);
