@@ -37,3 +37,9 @@ bool rho3d_valid_tune(int rows, int chunk, int waves);
 bool rho3d_grad_supported(const Geo &g, const Box &box);
 int rho3d_grad(const Geo &g, const Box &box, const float *ps, const float *psm, const float *psp, float fdt2,
                const float *b, const float *pa, float *gv, float *pv, float *gr, float *pr, hipStream_t stream);
+
+// Exact-adjoint accumulations of one step over the whole domain (3D): lam = mu / G, gA += lam u,
+// gB += lam um, gG += lam Ku and, with gb, gb -= 1/2 sum_a (D+ mu D+ u)(p) + (D+ mu D+ u)(p - s_a)
+// (all arrays padded layout, zero halo).
+int rho3d_exact_acc(const Geo &g, const float *u, const float *mu, const float *G, const float *um, const float *Ku,
+                    float *gA, float *gB, float *gG, float *gb, hipStream_t stream);

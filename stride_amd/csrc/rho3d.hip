@@ -520,6 +520,196 @@ __global__ __launch_bounds__(NW * 64) void k_grad_density(Geo g, GradArgs a) {
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Exact-adjoint accumulations of one step (every node of the domain), in one march:
+//   lam = mu / G,  gA += lam u,  gB += lam um,  gG += lam Ku,
+//   gb -= 1/2 sum_a (D+ mu D+ u)(p) + (D+ mu D+ u)(p - s_a)   (d/db through the half-point buoyancy)
+// with u = u_t and mu = mu_{t+1} (padded layout, zero halo). The plane tiles of u and mu with a
+// 5-cell halo sit in LDS for the axis-1/2 staggered derivatives; register rings of the own column
+// of u and mu (planes p-5 .. p+5) give the axis-0 ones and the in-plane products of each node
+// wait in a ring until the node's axis-0 term is complete (plane p + 5).
+
+struct XAccArgs {
+    const float *u, *mu, *G, *um, *Ku;
+    float *gA, *gB, *gG, *gb;
+    long long bytes;
+    int plane_bytes;
+    int chunk, ntz, nty, ntiles, per_xcd;
+};
+
+template <int NW>
+struct XState {
+    using T = GTile<NW>;
+    float ur[12], mr[12], ir[12];  // own u, own mu, in-plane products of plane p (slot (p - mbeg) % 12)
+    float4 su[2][T::NL], sm[2][T::NL];
+    unsigned off[T::NL];
+    int lds[T::NL];
+    unsigned own;
+    float G[2], um[2], Ku[2], gA[2], gB[2], gG[2], gb[2];
+};
+
+struct XRes {
+    rsrc_t u, mu, G, um, Ku, gA, gB, gG, gb;
+};
+
+template <int S, int NW>
+__device__ __forceinline__ void xload_tile(XState<NW> &st, const XRes &rs, const XAccArgs &a, int m) {
+    using T = GTile<NW>;
+    const unsigned pp = (unsigned)(m * a.plane_bytes);
+#pragma unroll
+    for (int i = 0; i < T::NL; ++i) {
+        st.su[S][i] = load4(rs.u, st.off[i] + pp);
+        st.sm[S][i] = load4(rs.mu, st.off[i] + pp);
+    }
+}
+
+template <int S, int NW>
+__device__ __forceinline__ void xload_own(XState<NW> &st, const XRes &rs, const XAccArgs &a, int p) {
+    const unsigned po = st.own + (unsigned)(p * a.plane_bytes);
+    st.G[S] = load_nt(rs.G, po);
+    st.um[S] = load_nt(rs.um, po);
+    st.Ku[S] = load_nt(rs.Ku, po);
+    st.gA[S] = load_nt(rs.gA, po);
+    st.gB[S] = load_nt(rs.gB, po);
+    st.gG[S] = load_nt(rs.gG, po);
+    if (a.gb) st.gb[S] = load_nt(rs.gb, po);
+}
+
+template <int NW, int J>
+__device__ __forceinline__ void xplane(const Geo &g, const XAccArgs &a, const XRes &rs, XState<NW> &st, float *sU,
+                                       float *sM, const int m, const int x0, const int x1, const int lane,
+                                       const int w) {
+    using T = GTile<NW>;
+    constexpr int S = J % 2;
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < T::NL; ++i) {
+        *reinterpret_cast<float4 *>(sU + st.lds[i]) = st.su[S][i];
+        *reinterpret_cast<float4 *>(sM + st.lds[i]) = st.sm[S][i];
+    }
+    xload_tile<S>(st, rs, a, m + 2);
+    const int p = m - 5;
+    xload_own<1 - S>(st, rs, a, p + 1);
+    lds_barrier();
+    const int c = (w + 5) * T::LZ + lane + 8;  // own cell in the tiles
+    st.ur[J] = sU[c];
+    st.mr[J] = sM[c];
+    if (a.gb && m >= x0 && m < x1) {
+        // axis 1 (rows) then axis 2 (columns): products at the half points +1/2 and -1/2
+        float prod = 0.f;
+        {
+            float up = 0.f, ul = 0.f, mp = 0.f, ml = 0.f;
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                up += g.ws[1][r] * (sU[c + (r + 1) * T::LZ] - sU[c - r * T::LZ]);
+                ul += g.ws[1][r] * (sU[c + r * T::LZ] - sU[c - (r + 1) * T::LZ]);
+                mp += g.ws[1][r] * (sM[c + (r + 1) * T::LZ] - sM[c - r * T::LZ]);
+                ml += g.ws[1][r] * (sM[c + r * T::LZ] - sM[c - (r + 1) * T::LZ]);
+            }
+            prod += mp * up + ml * ul;
+        }
+        {
+            float up = 0.f, ul = 0.f, mp = 0.f, ml = 0.f;
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                up += g.ws[2][r] * (sU[c + r + 1] - sU[c - r]);
+                ul += g.ws[2][r] * (sU[c + r] - sU[c - (r + 1)]);
+                mp += g.ws[2][r] * (sM[c + r + 1] - sM[c - r]);
+                ml += g.ws[2][r] * (sM[c + r] - sM[c - (r + 1)]);
+            }
+            prod += mp * up + ml * ul;
+        }
+        st.ir[J] = prod;
+    }
+    // node plane p = m - 5 (ring slot (J + 7) % 12): axis 0 from the rings of planes p-5 .. p+5
+    {
+        constexpr int C = (J + 7) % 12;
+        const bool emit = p >= x0 && p < x1;
+        const unsigned po = emit ? st.own + (unsigned)(p * a.plane_bytes) : kOOB;
+        const float lam = st.mr[C] / st.G[S];
+        store_nt(st.gA[S] + lam * st.ur[C], rs.gA, po);
+        store_nt(st.gB[S] + lam * st.um[S], rs.gB, po);
+        store_nt(st.gG[S] + lam * st.Ku[S], rs.gG, po);
+        if (a.gb) {
+            float up = 0.f, ul = 0.f, mp = 0.f, ml = 0.f;
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                up += g.ws[0][r] * (st.ur[(J + 8 + r) % 12] - st.ur[(J + 7 - r) % 12]);
+                ul += g.ws[0][r] * (st.ur[(J + 7 + r) % 12] - st.ur[(J + 6 - r) % 12]);
+                mp += g.ws[0][r] * (st.mr[(J + 8 + r) % 12] - st.mr[(J + 7 - r) % 12]);
+                ml += g.ws[0][r] * (st.mr[(J + 7 + r) % 12] - st.mr[(J + 6 - r) % 12]);
+            }
+            const float acc = (mp * up + ml * ul) + st.ir[C];
+            store_nt(st.gb[S] + -0.5f * acc, rs.gb, po);
+        }
+    }
+}
+
+template <int NW>
+__global__ __launch_bounds__(NW * 64) void k_exact_acc_density(Geo g, XAccArgs a) {
+    using T = GTile<NW>;
+    __shared__ float4 sU4[T::LY * T::LZ / 4 + 1];  // + a dump slot for threads without a chunk
+    __shared__ float4 sM4[T::LY * T::LZ / 4 + 1];
+    float *sU = reinterpret_cast<float *>(sU4), *sM = reinterpret_cast<float *>(sM4);
+    const int L = (blockIdx.x & 7) * a.per_xcd + (blockIdx.x >> 3);
+    if (L >= a.ntiles) return;
+    const int tz = L % a.ntz, rest = L / a.ntz;
+    const int ty = rest % a.nty, ch = rest / a.nty;
+    const int z0 = tz * kTZ, y0 = ty * T::TY, x0 = ch * a.chunk;
+    const int x1 = min(x0 + a.chunk, g.n[0]);
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    XRes rs;
+    rs.u = make(a.u, a.bytes);
+    rs.mu = make(a.mu, a.bytes);
+    rs.G = make(a.G, a.bytes);
+    rs.um = make(a.um, a.bytes);
+    rs.Ku = make(a.Ku, a.bytes);
+    rs.gA = make(a.gA, a.bytes);
+    rs.gB = make(a.gB, a.bytes);
+    rs.gG = make(a.gG, a.bytes);
+    rs.gb = make(a.gb ? a.gb : a.gA, a.gb ? a.bytes : 0);  // null: every read is zero
+    XState<NW> st;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) st.ur[k] = st.mr[k] = st.ir[k] = 0.f;
+    {
+        const int y = y0 + w, z = z0 + lane;
+        st.own = (y < g.n[1] && z < g.n[2]) ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
+    }
+    // tile rows y0-5 .. y0+TY+4, columns z0-8 .. z0+71; rows past the domain read zero (their
+    // cells are halo or beyond it)
+#pragma unroll
+    for (int i = 0; i < T::NL; ++i) {
+        const int e = t + T::NT * i;
+        const int row = e / T::LC, c = e - row * T::LC;
+        const int y = y0 - 5 + row, z = z0 - 8 + 4 * c;
+        const bool ok = e < T::LY * T::LC && y < g.n[1];
+        st.lds[i] = e < T::LY * T::LC ? row * T::LZ + 4 * c : T::LY * T::LZ;
+        st.off[i] = ok ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
+    }
+    const int mbeg = x0 - 5;
+    xload_tile<0>(st, rs, a, mbeg);
+    xload_tile<1>(st, rs, a, mbeg + 1);
+    xload_own<0>(st, rs, a, mbeg - 5);
+    const int nper = (x1 - x0 + 10 + 11) / 12;
+    for (int q = 0; q < nper; ++q) {
+        const int m = mbeg + 12 * q;
+#define SA_XACC_PLANE(J) xplane<NW, J>(g, a, rs, st, sU, sM, m + J, x0, x1, lane, w)
+        SA_XACC_PLANE(0);
+        SA_XACC_PLANE(1);
+        SA_XACC_PLANE(2);
+        SA_XACC_PLANE(3);
+        SA_XACC_PLANE(4);
+        SA_XACC_PLANE(5);
+        SA_XACC_PLANE(6);
+        SA_XACC_PLANE(7);
+        SA_XACC_PLANE(8);
+        SA_XACC_PLANE(9);
+        SA_XACC_PLANE(10);
+        SA_XACC_PLANE(11);
+#undef SA_XACC_PLANE
+    }
+}
+
 }  // namespace
 
 bool rho3d_supported(const Geo &g) {
@@ -598,5 +788,30 @@ int rho3d_grad(const Geo &g, const Box &box, const float *ps, const float *psm, 
     a.ntiles = a.ntz * a.nty * ((a.hi[0] - a.lo[0] + a.chunk - 1) / a.chunk);
     a.per_xcd = (a.ntiles + 7) / 8;
     k_grad_density<NW><<<dim3(8 * a.per_xcd), dim3(NW * 64), 0, stream>>>(g, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int rho3d_exact_acc(const Geo &g, const float *u, const float *mu, const float *G, const float *um, const float *Ku,
+                    float *gA, float *gB, float *gG, float *gb, hipStream_t stream) {
+    if (!rho3d_supported(g)) return -1;
+    constexpr int NW = 8;
+    XAccArgs a;
+    a.u = u;
+    a.mu = mu;
+    a.G = G;
+    a.um = um;
+    a.Ku = Ku;
+    a.gA = gA;
+    a.gB = gB;
+    a.gG = gG;
+    a.gb = gb;
+    a.bytes = g.total * 4;
+    a.plane_bytes = (int)(g.s0 * 4);
+    a.chunk = 60;
+    a.ntz = (g.n[2] + kTZ - 1) / kTZ;
+    a.nty = (g.n[1] + NW - 1) / NW;
+    a.ntiles = a.ntz * a.nty * ((g.n[0] + a.chunk - 1) / a.chunk);
+    a.per_xcd = (a.ntiles + 7) / 8;
+    k_exact_acc_density<NW><<<dim3(8 * a.per_xcd), dim3(NW * 64), 0, stream>>>(g, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

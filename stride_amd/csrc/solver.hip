@@ -2120,9 +2120,15 @@ static int adjoint_exact_core(sa_solver *s, const float *res_dev, float *lam_dev
             k_axpy_domain<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, c4, Kmu, KuT);
         }
         // gA, gB, gG and (variable density) gb, with the staggered derivatives of u_t (work0) and
-        // mu_{t+1} (m1) formed in the same pass
-        k_exact_acc_fused<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(
-            g, m1, s->G, s->work0, s->work1, KuT, acc[0], acc[1], acc[2], c.rho_mode == 2 ? acc[4] : nullptr);
+        // mu_{t+1} (m1) formed in the same pass (3D: a march with LDS tiles)
+        float *gbp = c.rho_mode == 2 ? acc[4] : nullptr;
+        if (s->grid.dim == 3 && rho3d_supported(g)) {
+            if (rho3d_exact_acc(g, s->work0, m1, s->G, s->work1, KuT, acc[0], acc[1], acc[2], gbp, s->stream) != 0)
+                rc = fail(SA_E_HIP, "exact accumulation launch: %s", hipGetErrorString(hipGetLastError()));
+        } else {
+            k_exact_acc_fused<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, m1, s->G, s->work0, s->work1, KuT,
+                                                                              acc[0], acc[1], acc[2], gbp);
+        }
         if (hipGetLastError() != hipSuccess) rc = fail(SA_E_HIP, "launch failed");
     }
     dfree(KuT);
