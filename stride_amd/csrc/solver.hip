@@ -1654,7 +1654,10 @@ static int alloc_saved(sa_solver *s) {
     s->saved_on_host = false;
     s->saved_f = f;
     if (c.save_mode == 4 || c.save_mode == 5) {
-        size_t sb = per * (size_t)(c.save_mode == 4 ? f + 1 : s->ck_K + 2) * sizeof(float);
+        // save mode 4: the K + 1 recomputed levels of a segment in the padded layout (the adjoint
+        // reads them with their halo); mode 5: K + 2 raw snapshots in the box layout
+        size_t sb = c.save_mode == 4 ? s->total * (size_t)(f + 1) * sizeof(float)
+                                     : per * (size_t)(s->ck_K + 2) * sizeof(float);
         if (!s->segment || s->segment_bytes < sb) {
             dfree(s->segment);
             s->segment = nullptr;
@@ -1664,6 +1667,8 @@ static int alloc_saved(sa_solver *s) {
             }
             s->segment_bytes = sb;
         }
+        // the recomputed levels of save mode 4 are read with their halo, which no step writes
+        if (c.save_mode == 4) HIP_TRY(hipMemsetAsync(s->segment, 0, sb, s->stream));
         TRY(dalloc(&s->fb[0], s->total));
         TRY(dalloc(&s->fb[1], s->total));
     }
@@ -1792,7 +1797,14 @@ extern "C" int sa_forward_dev(sa_solver *s, const float *src, float *rec) {
     return SA_OK;
 }
 
+// Slot of level t in the recomputed segment j (save mode 4): padded fields, zero halo.
+static float *segment_level(sa_solver *s, int j, int t) {
+    return s->segment + (size_t)(t - j * s->saved_f) * s->total;
+}
+
 // Recompute forward levels u_{jK} .. u_{min((j+1)K, nt-1)} into the segment buffer from checkpoint j.
+// Every step writes its new level into its own slot (the levels stay in the padded layout, so the
+// adjoint reads them without unpacking).
 static int recompute_segment(sa_solver *s, int j) {
     s->lap_of = nullptr;
     const int K = s->saved_f;
@@ -1802,18 +1814,14 @@ static int recompute_segment(sa_solver *s, int j) {
     const float *pair = s->saved + (size_t)(2 * j) * per;  // (p[jK+1], p[jK])
     int t0 = j * K;
     int tend = std::min((j + 1) * K, nt - 1);
-    HIP_TRY(hipMemcpyAsync(s->segment, pair + per, per * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+    k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, pair + per, segment_level(s, j, t0));
     if (tend >= t0 + 1)
-        HIP_TRY(hipMemcpyAsync(s->segment + per, pair, per * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
-    k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, pair, s->fb[(t0 + 1) % 2]);
-    k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, pair + per, s->fb[t0 % 2]);
+        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, pair, segment_level(s, j, t0 + 1));
     CHECK_LAUNCH();
     for (int t = t0 + 1; t < tend; ++t) {
-        float *u = s->fb[t % 2], *o = s->fb[(t + 1) % 2];
-        TRY(step_general(s, u, o, o, 0, nullptr, 0));
-        TRY(inject(s, s->src, s->src_keep + (size_t)t * s->src.npoint, 1.f, o, nullptr, 0.f, s->box));
-        k_pack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, o, s->segment + (size_t)(t + 1 - t0) * per);
-        CHECK_LAUNCH();
+        float *out = segment_level(s, j, t + 1);
+        TRY(step_general(s, segment_level(s, j, t), segment_level(s, j, t - 1), out, 0, nullptr, 0));
+        TRY(inject(s, s->src, s->src_keep + (size_t)t * s->src.npoint, 1.f, out, nullptr, 0.f, s->box));
     }
     return SA_OK;
 }
@@ -2098,8 +2106,8 @@ static int adjoint_exact_core(sa_solver *s, const float *res_dev, float *lam_dev
                 if (rc != SA_OK) break;
                 seg = j;
             }
-            ut = s->segment + (size_t)(t - j * K) * per;
-            utm = s->segment + (size_t)(t - 1 - j * K) * per;
+            ut = segment_level(s, j, t);
+            utm = segment_level(s, j, t - 1);
         } else {
             ut = s->saved + (size_t)t * per;
             utm = s->saved + (size_t)(t - 1) * per;
@@ -2108,10 +2116,16 @@ static int adjoint_exact_core(sa_solver *s, const float *res_dev, float *lam_dev
             k_gather_taps<<<((int)nq + 127) / 128, 128, 0, s->stream>>>((int)nq, s->src.d_idx, m1, s->G,
                                                                          lam_dev + (size_t)t * nq);
         }
-        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, ut, s->work0);
-        k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, utm, s->work1);
+        // u_t and u_{t-1} in the padded layout: the recomputed slots as they are, stored levels unpacked
+        const float *u0 = ut, *u1 = utm;
+        if (!ckpt) {
+            k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, ut, s->work0);
+            k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, utm, s->work1);
+            u0 = s->work0;
+            u1 = s->work1;
+        }
         if ((rc = (hipGetLastError() == hipSuccess ? SA_OK : fail(SA_E_HIP, "launch failed"))) != SA_OK) break;
-        if ((rc = apply_K(s, s->work0, nullptr, KuT)) != SA_OK) break;
+        if ((rc = apply_K(s, u0, nullptr, KuT)) != SA_OK) break;
         if (c.kernel == 4) {
             // gm += dt^2/12 (K mu)(K u); Ku += dt^2/12 K(m Ku)
             if ((rc = apply_K(s, m1, nullptr, Kmu)) != SA_OK) break;
@@ -2119,15 +2133,15 @@ static int adjoint_exact_core(sa_solver *s, const float *res_dev, float *lam_dev
             if ((rc = apply_K(s, KuT, s->M, Kmu)) != SA_OK) break;
             k_axpy_domain<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, c4, Kmu, KuT);
         }
-        // gA, gB, gG and (variable density) gb, with the staggered derivatives of u_t (work0) and
+        // gA, gB, gG and (variable density) gb, with the staggered derivatives of u_t and
         // mu_{t+1} (m1) formed in the same pass (3D: a march with LDS tiles)
         float *gbp = c.rho_mode == 2 ? acc[4] : nullptr;
         if (s->grid.dim == 3 && rho3d_supported(g)) {
-            if (rho3d_exact_acc(g, s->work0, m1, s->G, s->work1, KuT, acc[0], acc[1], acc[2], gbp, s->stream) != 0)
+            if (rho3d_exact_acc(g, u0, m1, s->G, u1, KuT, acc[0], acc[1], acc[2], gbp, s->stream) != 0)
                 rc = fail(SA_E_HIP, "exact accumulation launch: %s", hipGetErrorString(hipGetLastError()));
         } else {
-            k_exact_acc_fused<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, m1, s->G, s->work0, s->work1, KuT,
-                                                                              acc[0], acc[1], acc[2], gbp);
+            k_exact_acc_fused<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, m1, s->G, u0, u1, KuT, acc[0],
+                                                                              acc[1], acc[2], gbp);
         }
         if (hipGetLastError() != hipSuccess) rc = fail(SA_E_HIP, "launch failed");
     }
