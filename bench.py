@@ -432,12 +432,14 @@ def cpu_baseline_density(cfg, nt=24):
                        'adjoint steps with vp and rho gradients, %.1f s' % (fwd, adj, t1 - t0))
 
 
-def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0):
+def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0, att_power=0):
     """One 3D shot on the variable vp / rho / alpha medium (360^3 extended, attenuation power 0,
     sponge 20, OT2, 1024 receivers): forward with the raw wavefield saved every 4th step over the
     whole domain, then the as-written adjoint with the vp and rho gradients. The update runs as the
     fused density kernel; `roofline` covers it (algorithmic bytes: read u, the other level, b, A, B,
-    G, write the new level = 28 B per point update, + 4 B per point on save steps). checkpoint=K:
+    G, write the new level = 28 B per point update, + 4 B per point on save steps; att_power=2 adds
+    Cbeta and the Laplacians of both levels: 40 B, and the step's time then includes the launch that
+    forms the Laplacian of u). checkpoint=K:
     the raw snapshots recomputed from one state per K snapshots instead (save mode 5); the rate
     then counts the shot's own updates only, and the recomputation's launches are extra work."""
     from stride_amd.configs import config4
@@ -446,8 +448,9 @@ def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0):
     cfg = config4(scale=1.0, nt=nt, nrec=1024)
     space, time_ = cfg['space'], cfg['time']
     plan = build_plan(space, time_, cfg['wavelet'][None], cfg['vp'], rho=cfg['rho'], alpha=cfg['alpha'],
-                      source_coordinates=cfg['sources'], receiver_coordinates=cfg['receivers'], kernel='OT2',
-                      save_undersampling=4, grad_vp=True, grad_rho=True)
+                      attenuation_power=att_power, source_coordinates=cfg['sources'],
+                      receiver_coordinates=cfg['receivers'], kernel='OT2', save_undersampling=4, grad_vp=True,
+                      grad_rho=True)
     s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, plan.nt)
     s.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
                  plan.undersampling, 5 if checkpoint else 2, plan.tb0, plan.tb1, 1, 1, 0, checkpoint=checkpoint)
@@ -478,7 +481,9 @@ def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0):
     n = int(np.prod(plan.shape))
     f = plan.undersampling
     nsave = len(range(0, plan.nt - 1, f))
-    alg = launches * 28.0 * n + (0 if checkpoint else nsave * 4.0 * n)
+    # power 2: + Cbeta and the two Laplacians read (the Laplacian of u is its own launch)
+    per_update = 40.0 if att_power == 2 else 28.0
+    alg = launches * per_update * n + (0 if checkpoint else nsave * 4.0 * n)
     achieved = alg / (ms * 1e-3) / 1e9
     own = (plan.tb1 - 2) + (plan.tb1 - 1 - f - plan.tb0)  # the shot's forward and adjoint steps
     per_gb = plan.shape[0] * plan.shape[1] * (-(-plan.shape[2] // 32) * 32) * 4 / 1e9  # one snapshot
@@ -504,7 +509,8 @@ def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0):
                 cpu_baseline=cpu, dtype='f32', data='synthetic',
                 config={'workload': '3D 320x320x320 inner (360x360x360 extended), dx 0.4 mm, SO10 OT2, sponge 20, '
                                     'variable vp / rho (Gaussian-filtered noise in the hemisphere) and alpha 0.5 '
-                                    '(power 0), one shot, nt %d, raw saves every 4th step, 1024 receivers' % plan.nt})
+                                    '(power %d), one shot, nt %d, raw saves every 4th step, 1024 receivers'
+                                    % (att_power, plan.nt)})
 
 
 def bench_power2(nt=500, general=False):
@@ -821,6 +827,8 @@ def main():
     ap.add_argument('--autograd', action='store_true',
                     help='one 3D gradient-workload shot through the autograd exact adjoint with checkpoints '
                          '(--checkpoint K, default ceil(sqrt(2 nt))) and exit')
+    ap.add_argument('--att-power', type=int, default=0, choices=(0, 2),
+                    help='with --density: attenuation power of the medium (default 0)')
     ap.add_argument('--power2', action='store_true',
                     help='one 3D shot with power-2 attenuation on the fused update and on the general passes '
                          '(use with --nt) and exit')
@@ -887,7 +895,8 @@ def main():
         return
     if args.density:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
-        print(json.dumps(bench_density(args.nt, tile, with_cpu=not args.no_cpu, checkpoint=args.checkpoint)))
+        print(json.dumps(bench_density(args.nt, tile, with_cpu=not args.no_cpu, checkpoint=args.checkpoint,
+                                       att_power=args.att_power)))
         return
     if args.schedule:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None

@@ -153,9 +153,8 @@ def uses_fast_update(plan, save_mode):
 
 def uses_density_update(plan):
     """The scheme runs on the fused 3D density update (3D, OT2, constant or variable density, no
-    PML, no power-2 attenuation)."""
-    return (plan.dim == 3 and plan.kernel_order == 2 and plan.rho_mode != 0 and plan.att_power != 2
-            and plan.boundary != 1)
+    PML; any attenuation form)."""
+    return plan.dim == 3 and plan.kernel_order == 2 and plan.rho_mode != 0 and plan.boundary != 1
 
 
 def tune_solver(solver, plan, save_mode, device=0):

@@ -1,5 +1,5 @@
 // Fused 3D wavefield update for the div(b grad) forms of the operator (constant or variable
-// density), OT2, sponge or no boundary, attenuation power 0 / None / off: one pass per time
+// density), OT2, sponge or no boundary, attenuation power 0 / 2 / None / off: one pass per time
 // step in place of three staggered-derivative passes, a divergence pass and the update pass.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -23,6 +23,9 @@ struct Rho3DStep {
     Box box;
     float idt2;
     int konly = 0;  // out = K u only (A, B, G, other and save unused)
+    // power-2 attenuation: out += Cbeta (lap u - lap other) with the centred SO10 Laplacians of
+    // the other level and of u read from lapother and lapu
+    const float *Cbeta = nullptr, *lapother = nullptr, *lapu = nullptr;
 };
 
 // true when the layout fits the kernel's 32-bit buffer offsets

@@ -42,6 +42,9 @@ struct RhoArgs {
     float idt2;
     Box box;
     int konly;  // out = K u (no update, no save)
+    // power-2 attenuation (Cbeta non-null): Cbeta and the centred Laplacians of the other level
+    // and of u
+    const float *Cbeta, *lapo, *lapu;
 };
 
 template <int RPT, int NW, bool VARB>
@@ -65,6 +68,9 @@ struct State {
     float4 su[2][T::NLU];  // u tiles of the next two planes, staged in registers (slot: plane % 2)
     float4 sb[2][T::NLB];
     float ov[2][RPT], Av[2][RPT], Bv[2][RPT], Gv[2][RPT];  // own operands of the next node plane
+    // power-2 attenuation: own Cbeta and the centred Laplacians of the other level and of u of the
+    // next node plane
+    float Cb[2][RPT], Lo[2][RPT], Lu[2][RPT];
     unsigned uoff[T::NLU], boff[T::NLB];  // in-plane byte offsets of the tile chunks (kOOB: none)
     int ulds[T::NLU], blds[T::NLB];       // their LDS float offsets (the dump slot: none)
     unsigned own[RPT];                    // own cells' in-plane byte offsets (kOOB outside the domain)
@@ -79,7 +85,7 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 struct Res {
-    rsrc_t u, o, b, A, B, G, out;
+    rsrc_t u, o, b, A, B, G, out, Cb, Lo, Lu;
 };
 
 template <int S, int RPT, int NW, bool VARB>
@@ -93,19 +99,24 @@ __device__ __forceinline__ void load_tile(State<RPT, NW, VARB> &st, const Res &r
     }
 }
 
-template <int S, int RPT, int NW, bool VARB, bool KONLY>
+template <int S, int RPT, int NW, bool VARB, int FORM>
 __device__ __forceinline__ void load_own(State<RPT, NW, VARB> &st, const Res &rs, unsigned poff) {
-    if (KONLY) return;  // out = K u: no own operands
+    if (FORM == 1) return;  // out = K u: no own operands
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         st.ov[S][r] = load_nt(rs.o, st.own[r] + poff);
         st.Av[S][r] = load_nt(rs.A, st.own[r] + poff);
         st.Bv[S][r] = load_nt(rs.B, st.own[r] + poff);
         st.Gv[S][r] = load_nt(rs.G, st.own[r] + poff);
+        if (FORM == 2) {
+            st.Cb[S][r] = load_nt(rs.Cb, st.own[r] + poff);
+            st.Lo[S][r] = load_nt(rs.Lo, st.own[r] + poff);
+            st.Lu[S][r] = load_nt(rs.Lu, st.own[r] + poff);
+        }
     }
 }
 
-template <int RPT, int NW, bool VARB, bool SAVE, bool KONLY, int J>
+template <int RPT, int NW, bool VARB, bool SAVE, int FORM, int J>
 __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res &rs, State<RPT, NW, VARB> &st,
                                       float *sU, float *sB, float *sFy, float *sFz, const int m, const int x0,
                                       const int x1, const int y0, const int z0, const int lane, const int w) {
@@ -125,7 +136,7 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
     load_tile<S>(st, rs, (unsigned)((m + 2) * a.plane_bytes));
     const int p = m - 9;
     const bool emit = p >= x0 && p < x1;
-    load_own<1 - S, RPT, NW, VARB, KONLY>(st, rs, (unsigned)((p + 1) * a.plane_bytes));
+    load_own<1 - S, RPT, NW, VARB, FORM>(st, rs, (unsigned)((p + 1) * a.plane_bytes));
     lds_barrier();  // plane m's tiles are in LDS
     const bool inplane = m >= x0 && m < x1;
     if (inplane) {
@@ -200,8 +211,9 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const float u = st.uq[r][(J + 1) % 10];
-            const float out = KONLY ? st.nd[r][(J + 1) % 10]
-                                    : st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][r] + st.Gv[S][r] * st.nd[r][(J + 1) % 10];
+            float acc = st.Gv[S][r] * st.nd[r][(J + 1) % 10];
+            if (FORM == 2) acc += st.Cb[S][r] * (st.Lu[S][r] - st.Lo[S][r]);  // power 2: Cbeta (lap u - lap other)
+            const float out = FORM == 1 ? st.nd[r][(J + 1) % 10] : st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][r] + acc;
             store_nt(out, rs.out, emit ? st.own[r] + po : kOOB);
             if (SAVE && emit) {
                 const int y = y0 + w * RPT + r, z = z0 + lane;
@@ -214,7 +226,7 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
     for (int r = 0; r < RPT; ++r) st.nd[r][(J + 1) % 10] = 0.f;
 }
 
-template <int RPT, int NW, bool VARB, bool SAVE, bool KONLY>
+template <int RPT, int NW, bool VARB, bool SAVE, int FORM>
 __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     using T = Tile<RPT, NW, VARB>;
     __shared__ float4 sU4[T::UY * T::UZ / 4 + 1];  // + a dump slot for threads without a chunk
@@ -241,6 +253,11 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     rs.B = make(a.B, a.bytes);
     rs.G = make(a.G, a.bytes);
     rs.out = make(a.out, a.bytes);
+    if (FORM == 2) {
+        rs.Cb = make(a.Cbeta, a.bytes);
+        rs.Lo = make(a.lapo, a.bytes);
+        rs.Lu = make(a.lapu, a.bytes);
+    }
 
     State<RPT, NW, VARB> st;
 #pragma unroll
@@ -281,7 +298,7 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     for (int q = 0; q < nper; ++q) {
         const int m = mbeg + 10 * q;
 #define SA_RHO_PLANE(J) \
-    plane<RPT, NW, VARB, SAVE, KONLY, J>(g, a, rs, st, sU, sB, sFy, sFz, m + J, x0, x1, y0, z0, lane, w)
+    plane<RPT, NW, VARB, SAVE, FORM, J>(g, a, rs, st, sU, sB, sFy, sFz, m + J, x0, x1, y0, z0, lane, w)
         SA_RHO_PLANE(0);
         SA_RHO_PLANE(1);
         SA_RHO_PLANE(2);
@@ -303,12 +320,16 @@ int launch(const Geo &g, RhoArgs a, bool varb, bool save, hipStream_t stream) {
     a.ntiles = a.ntz * a.nty * ((g.n[0] + a.chunk - 1) / a.chunk);
     a.per_xcd = (a.ntiles + 7) / 8;
     const dim3 grid(8 * a.per_xcd), block(NW * 64);
-    if (a.konly && varb) k_rho3d<RPT, NW, true, false, true><<<grid, block, 0, stream>>>(g, a);
-    else if (a.konly) k_rho3d<RPT, NW, false, false, true><<<grid, block, 0, stream>>>(g, a);
-    else if (varb && save) k_rho3d<RPT, NW, true, true, false><<<grid, block, 0, stream>>>(g, a);
-    else if (varb) k_rho3d<RPT, NW, true, false, false><<<grid, block, 0, stream>>>(g, a);
-    else if (save) k_rho3d<RPT, NW, false, true, false><<<grid, block, 0, stream>>>(g, a);
-    else k_rho3d<RPT, NW, false, false, false><<<grid, block, 0, stream>>>(g, a);
+    if (a.konly && varb) k_rho3d<RPT, NW, true, false, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.konly) k_rho3d<RPT, NW, false, false, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cbeta && varb && save) k_rho3d<RPT, NW, true, true, 2><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cbeta && varb) k_rho3d<RPT, NW, true, false, 2><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cbeta && save) k_rho3d<RPT, NW, false, true, 2><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cbeta) k_rho3d<RPT, NW, false, false, 2><<<grid, block, 0, stream>>>(g, a);
+    else if (varb && save) k_rho3d<RPT, NW, true, true, 0><<<grid, block, 0, stream>>>(g, a);
+    else if (varb) k_rho3d<RPT, NW, true, false, 0><<<grid, block, 0, stream>>>(g, a);
+    else if (save) k_rho3d<RPT, NW, false, true, 0><<<grid, block, 0, stream>>>(g, a);
+    else k_rho3d<RPT, NW, false, false, 0><<<grid, block, 0, stream>>>(g, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -741,7 +762,11 @@ int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_
     a.idt2 = st.idt2;
     a.box = st.box;
     a.konly = st.konly;
-    if (st.konly && st.save) return -1;
+    a.Cbeta = st.Cbeta;
+    a.lapo = st.lapother;
+    a.lapu = st.lapu;
+    if (st.konly && (st.save || st.Cbeta)) return -1;
+    if (st.Cbeta && (!st.lapother || !st.lapu)) return -1;
     const bool varb = st.b != nullptr, save = st.save != nullptr;
     if (t.waves == 4 && t.rows == 1) return launch<1, 4>(g, a, varb, save, stream);
     if (t.waves == 4 && t.rows == 2) return launch<2, 4>(g, a, varb, save, stream);

@@ -1168,10 +1168,13 @@ extern "C" int sa_set_receivers(sa_solver *s, int npoint, int ntap, const int64_
     return set_sparse(s, s->rec, npoint, ntap, index, weight);
 }
 
+static int lap_other(sa_solver *s, const float *other);
+static void lap_advance(sa_solver *s, const float *u);
+
 static bool use_rho3d(const sa_solver *s) {
     const sa_scheme &c = s->sch;
-    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode != 0 && c.att_power != 2 && c.boundary != 1 &&
-           s->rho3d.enabled && rho3d_supported(s->geo);
+    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode != 0 && c.boundary != 1 && s->rho3d.enabled &&
+           rho3d_supported(s->geo);
 }
 
 static bool use_flat2d(const sa_solver *s) {
@@ -1208,7 +1211,9 @@ static int prepare(sa_solver *s) {
     TRY(dalloc(&s->M, s->total));
     if (c.att_power == 2) TRY(dalloc(&s->Cbeta, s->total));
     if (c.att_power == 2) TRY(dalloc(&s->lapother, s->total));
-    if (c.att_power == 2 && (use_flat2d(s) || use_fast(s))) TRY(dalloc(&s->lapnext, s->total));
+    // the Laplacian of u for the next step (the fused updates); allocated for every power-2 scheme,
+    // since which update runs can change after this (sa_set_tile_density) without a new prepare
+    if (c.att_power == 2) TRY(dalloc(&s->lapnext, s->total));
     if (c.boundary == 1) TRY(dalloc(&s->Cpml, s->total));
     const Geo &g = s->geo;
     k_coefficients<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(
@@ -1388,8 +1393,21 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
         st.save_mode = save_mode;
         st.box = s->box;
         st.idt2 = 1.f / (s->dt * s->dt);
+        if (c.att_power == 2) {
+            // the Laplacians of the two levels: the other level's kept from the previous step (or
+            // taken here at the first step of a run), u's by a launch of its own (the next step's)
+            if (!s->lapother || !s->lapnext || !s->Cbeta)
+                return fail(SA_E_STATE, "power-2 attenuation buffers missing (prepare the scheme first)");
+            TRY(lap_other(s, other));
+            k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, u, nullptr, s->lapnext);
+            CHECK_LAUNCH();
+            st.Cbeta = s->Cbeta;
+            st.lapother = s->lapother;
+            st.lapu = s->lapnext;
+        }
         if (rho3d_step(s->rho3d, g, st, s->stream) != 0)
             return fail(SA_E_HIP, "fused density update launch: %s", hipGetErrorString(hipGetLastError()));
+        if (c.att_power == 2) lap_advance(s, u);
         TRY(timed_end(s));
         return SA_OK;
     }
@@ -1685,6 +1703,7 @@ static int dump_level(sa_solver *s, int w, int t, const float *u);
 // overwrites (kept from the previous step, or taken here at the first step of a run) and writes
 // the Laplacian of u, which becomes the next step's.
 static int lap_other(sa_solver *s, const float *other) {
+    if (!s->lapother) return fail(SA_E_STATE, "power-2 attenuation buffers missing");
     if (s->lap_of != other) {
         k_laplace<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, other, nullptr, s->lapother);
         CHECK_LAUNCH();

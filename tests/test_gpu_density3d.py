@@ -1,5 +1,5 @@
 """GPU parity of the fused 3D density update (the div(b grad) forms: constant and variable
-density, sponge boundary, attenuation off / power 0 / power None) against the oracle, and
+density, sponge boundary, attenuation off / power 0 / power 2 / power None) against the oracle, and
 against the separate general passes it replaces, over several tile shapes.
 
 Grids: 44^3 extended (one tile across axis 2) and 86^3 extended (two tiles across axis 2,
@@ -21,6 +21,8 @@ FORWARD = [
     dict(rho='const'),
     dict(rho='var'),
     dict(rho='var', alpha='none-power'),
+    dict(rho='var', alpha='p2', attenuation_power=2),
+    dict(rho='const', alpha='p2', attenuation_power=2),
 ]
 
 
@@ -67,8 +69,9 @@ def test_fused_density_large_grid_and_tiles(gpu):
     s.close()
 
 
-@pytest.mark.parametrize('opts', [dict(rho='var', alpha='p0', grad_rho=True), dict(rho='const')],
-                         ids=['var-grad-rho', 'const-grad-vp'])
+@pytest.mark.parametrize('opts', [dict(rho='var', alpha='p0', grad_rho=True), dict(rho='const'),
+                                  dict(rho='var', alpha='p2', attenuation_power=2, grad_rho=True)],
+                         ids=['var-grad-rho', 'const-grad-vp', 'var-power2-grad-rho'])
 def test_fused_density_compat_adjoint(gpu, opts):
     """Adjoint with the fused update both ways: raw saves + vp / rho gradients (variable
     density), p.dt2 saves + vp gradient (constant density)."""
@@ -162,3 +165,24 @@ def test_operator_autotunes_density_update(gpu):
     assert len(table) > 1
     ref, _, _ = oracle.forward(plan)
     assert rel_l2(traces.data, ref) <= 1e-5
+
+
+def test_fused_density_power2_tiles_against_general(gpu):
+    """Power-2 attenuation on the fused density update (the centred Laplacian of u carried to the
+    next step): 86^3 extended, several tiles, against the separate general passes and the
+    oracle."""
+    plan, _ = _case(n=70, nabs=8, nt=100, rho='var', alpha='p2', attenuation_power=2)
+    assert plan.att_power == 2
+    print('density3d power-2 tiles', plan.shape, plan.nt, flush=True)
+    ref, _, _ = oracle.forward(plan)
+    s = solver_for(plan)
+    s.set_tile_density(0)
+    general = s.forward(plan.src_data).T
+    assert rel_l2(general, ref) <= 1e-5
+    for rows, chunk, waves in ((1, 40, 8), (2, 13, 4), (1, 23, 16)):
+        print('  tile', rows, chunk, waves, flush=True)
+        s.set_tile_density(rows, chunk, waves)
+        got = s.forward(plan.src_data).T
+        assert rel_l2(got, general) <= 1e-5, (rows, chunk, waves)
+        assert rel_l2(got, ref) <= 1e-5, (rows, chunk, waves)
+    s.close()

@@ -319,8 +319,7 @@ def test_anastasio_fixture():
 def test_autotune_scheme_predicates():
     """The update a scheme runs on (what the autotuner tunes): the specialised 3D update for
     OT2 without density / PML (any attenuation form) and saves of p.dt2 at most, the fused
-    density update for the 3D density forms without PML or power-2 attenuation, the general
-    kernels otherwise."""
+    density update for the 3D density forms without PML, the general kernels otherwise."""
     from helpers import small_case
     from stride_amd.autotune import uses_fast_update, uses_density_update
     plain, _ = small_case(dim=3, n=20, nabs=4, nt=160, kernel='OT2')
@@ -331,6 +330,8 @@ def test_autotune_scheme_predicates():
     assert uses_fast_update(p2, 1) and not uses_density_update(p2)
     p0, _ = small_case(dim=3, n=20, nabs=4, nt=160, kernel='OT2', alpha='p0')
     assert uses_fast_update(p0, 1) and not uses_fast_update(p0, 2) and not uses_density_update(p0)
+    d2, _ = small_case(dim=3, n=20, nabs=4, nt=160, kernel='OT2', rho='var', alpha='p2', attenuation_power=2)
+    assert uses_density_update(d2) and not uses_fast_update(d2, 0)
     pml, _ = small_case(dim=3, n=20, nabs=4, nt=160, kernel='OT2', rho='var',
                         boundary_type="complex_frequency_shift_PML_2")
     assert not uses_density_update(pml) and not uses_fast_update(pml, 0)
