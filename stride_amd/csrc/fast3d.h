@@ -57,6 +57,8 @@ int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *
 int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, const float *ps,
                         const Box &box, float *grad, float *prec, hipStream_t stream,
                         const FastAtt *att = nullptr);
+// out = the centred SO10 Laplacian of u over the domain (the update's own stencil, one pass)
+int fast3d_laplacian(Fast3D &f, const Geo &g, const float *u, float *out, hipStream_t stream);
 void fast3d_free(Fast3D &f);
 // tile shape used by later launches (autotuner)
 int fast3d_set_tune(Fast3D &f, int rows, int chunk, int version);

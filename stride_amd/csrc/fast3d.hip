@@ -134,6 +134,7 @@ __device__ __forceinline__ void load_stage5(Stage5<RPT, MODE, ATT> &st, const Rs
                                             const unsigned (&grow)[RPT], const unsigned (&brow)[RPT], int x,
                                             const FastArgs &a) {
     const int sp = (x + SA_HALO) * rs.plane_bytes;
+    if (MODE == 3) return;  // the Laplacian alone: no own operands
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         st.o[r] = bload_nt(rs.o, row[r], sp);
@@ -189,6 +190,11 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
     // computed but not stored
     const bool xin = (x >= x0) & (x < x1);
     // bitwise, not short-circuit: a branch here splits the plane's schedule
+    if (MODE == 3) {  // the centred Laplacian of u
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) bstore_nt(lap[r], rs.out, xin ? orow[r] : kOOB, sxo);
+        return;
+    }
     const bool xin_box = MODE != 0 && (xin & (x >= a.box.lo[0]) & (x < a.box.lo[0] + a.box.n[0]));
     const int sbx = MODE == 1 && xin_box ? (x - a.box.lo[0]) * rs.box_plane_bytes : 0;
     int d0 = 0;
@@ -1070,6 +1076,16 @@ int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o,
         return launch<2>(f, g, a, stream);
     }
     return launch<0>(f, g, a, stream);
+}
+
+int fast3d_laplacian(Fast3D &f, const Geo &g, const float *u, float *out, hipStream_t stream) {
+    if (!u || !out) return SA_E_ARG;
+    FastArgs a = base_args(f, u, nullptr, out);
+    const int boundary = f.boundary;
+    f.boundary = -1;  // no layer coefficients: the Laplacian alone
+    const int rc = launch<3>(f, g, a, stream);
+    f.boundary = boundary;
+    return rc;
 }
 
 int fast3d_step_b2(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,

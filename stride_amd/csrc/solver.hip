@@ -1171,6 +1171,20 @@ extern "C" int sa_set_receivers(sa_solver *s, int npoint, int ntap, const int64_
 static int lap_other(sa_solver *s, const float *other);
 static void lap_advance(sa_solver *s, const float *u);
 
+// out = the centred Laplacian of u over the domain: in 3D the specialised update's Laplacian-only
+// pass, else the general kernel
+static int laplacian(sa_solver *s, const float *u, float *out) {
+    const Geo &g = s->geo;
+    if (s->grid.dim == 3 && fast3d_enabled() && fast3d_supported(g)) {
+        if (fast3d_laplacian(s->fast, g, u, out, s->stream) != 0)
+            return fail(SA_E_HIP, "Laplacian launch: %s", hipGetErrorString(hipGetLastError()));
+        return SA_OK;
+    }
+    k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, u, nullptr, out);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
 static bool use_rho3d(const sa_solver *s) {
     const sa_scheme &c = s->sch;
     return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode != 0 && c.boundary != 1 && s->rho3d.enabled &&
@@ -1289,6 +1303,7 @@ static int drain_timing(sa_solver *s) {
 static int apply_K(sa_solver *s, const float *src, const float *scale, float *dst) {
     const Geo &g = s->geo;
     if (s->sch.rho_mode == 0) {
+        if (!scale) return laplacian(s, src, dst);
         k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, src, scale, dst);
         CHECK_LAUNCH();
         return SA_OK;
@@ -1399,8 +1414,7 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
             if (!s->lapother || !s->lapnext || !s->Cbeta)
                 return fail(SA_E_STATE, "power-2 attenuation buffers missing (prepare the scheme first)");
             TRY(lap_other(s, other));
-            k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, u, nullptr, s->lapnext);
-            CHECK_LAUNCH();
+            TRY(laplacian(s, u, s->lapnext));
             st.Cbeta = s->Cbeta;
             st.lapother = s->lapother;
             st.lapu = s->lapnext;
@@ -1704,10 +1718,7 @@ static int dump_level(sa_solver *s, int w, int t, const float *u);
 // the Laplacian of u, which becomes the next step's.
 static int lap_other(sa_solver *s, const float *other) {
     if (!s->lapother) return fail(SA_E_STATE, "power-2 attenuation buffers missing");
-    if (s->lap_of != other) {
-        k_laplace<<<domain_grid(s->geo, kBlk), kBlk, 0, s->stream>>>(s->geo, other, nullptr, s->lapother);
-        CHECK_LAUNCH();
-    }
+    if (s->lap_of != other) TRY(laplacian(s, other, s->lapother));
     return SA_OK;
 }
 
