@@ -1,6 +1,7 @@
 """Shots on two solvers with their own streams and host threads (stride_amd.workers) give the
 same traces and gradients, bit for bit, as the same shots run one after the other on one
-solver: the 2D gradient scheme mix (CFS-PML, variable density, power-2 attenuation)."""
+solver, and match the oracle: the 2D gradient scheme mix (CFS-PML, variable density, power-2
+attenuation)."""
 
 import numpy as np
 import pytest
@@ -40,6 +41,15 @@ def test_two_workers_match_sequential_shots(gpu):
         assert np.array_equal(got[k][0], want[k][0]), k
         for key in ('grad_vp', 'prec_vp', 'grad_rho', 'prec_rho'):
             assert np.array_equal(got[k][1][key], want[k][1][key]), (k, key)
+    # and both against the oracle (traces 1e-5, raw gradient accumulators 1e-4), two of the shots
+    import oracle
+    from helpers import rel_l2
+    for k in (0, 3):
+        ref, saved, _ = oracle.forward(plans[k], save=True)
+        assert rel_l2(got[k][0].T, ref) < 1e-5, k
+        gref = oracle.adjoint_compat(plans[k], res[k], saved)
+        for key in ('grad_vp', 'grad_rho'):
+            assert rel_l2(got[k][1][key], gref[key]) < 1e-4, (k, key)
 
 
 def test_dev_shots_with_medium_changes_match_sequential(gpu):
