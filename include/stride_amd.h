@@ -196,9 +196,10 @@ int sa_set_tile_density(sa_solver *s, int rows, int chunk, int waves);
 int sa_stream_create(int device, void **stream);
 int sa_stream_destroy(void *stream);
 
-/* Kernel timing: when enabled, every launch of the wavefield-update kernel is bracketed by
- * HIP events on the solver stream. sa_timing_read returns the summed kernel milliseconds
- * and launch count since the last reset. */
+/* Kernel timing: when enabled, launches of the wavefield-update kernel are bracketed by HIP
+ * events on the solver stream: every launch for enable = 1, every enable-th launch for
+ * enable > 1 (a sample that leaves the other launches without events). sa_timing_read returns
+ * the summed kernel milliseconds and the count of timed launches since the last reset. */
 int sa_timing_enable(sa_solver *s, int enable);
 int sa_timing_read(sa_solver *s, double *kernel_ms, int64_t *launches);
 

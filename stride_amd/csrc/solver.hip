@@ -737,6 +737,9 @@ struct sa_solver {
     size_t out_n = 0;
     // timing
     bool timing = false;
+    int timing_stride = 1;     // every timing_stride-th update launch is timed
+    long long timing_seq = 0;  // update launches since timing was enabled
+    bool timing_this = false;  // the launch between timed_begin and timed_end is timed
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
     double kernel_ms = 0;
@@ -1257,6 +1260,8 @@ static int prepare(sa_solver *s) {
 
 static int timed_begin(sa_solver *s, hipEvent_t *e0) {
     if (!s->timing) return SA_OK;
+    s->timing_this = s->timing_seq++ % s->timing_stride == 0;
+    if (!s->timing_this) return SA_OK;
     if (s->ev_used + 2 > s->ev.size()) {
         for (int q = 0; q < 256; ++q) {
             hipEvent_t e;
@@ -1270,7 +1275,7 @@ static int timed_begin(sa_solver *s, hipEvent_t *e0) {
 }
 
 static int timed_end(sa_solver *s) {
-    if (!s->timing) return SA_OK;
+    if (!s->timing || !s->timing_this) return SA_OK;
     hipEvent_t e1 = s->ev[s->ev_used + 1];
     HIP_TRY(hipEventRecord(e1, s->stream));
     s->ev_used += 2;
@@ -2475,6 +2480,9 @@ extern "C" int sa_timing_enable(sa_solver *s, int enable) {
     if (!s) return fail(SA_E_ARG, "null solver");
     TRY(drain_timing(s));
     s->timing = enable != 0;
+    s->timing_stride = enable > 1 ? enable : 1;
+    s->timing_seq = 0;
+    s->timing_this = false;
     s->kernel_ms = 0;
     s->launches = 0;
     return SA_OK;
