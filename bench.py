@@ -625,7 +625,8 @@ def bench_autograd(nt=2500, checkpoint=0):
                        'vp/rho/alpha gradients',
                 value=round(2 * (nt - 1) * n / wall / 1e9, 3), unit='Gpts/s', seconds=round(wall, 3),
                 first_call_seconds=round(first, 3), checkpoint_every=K,
-                wavefield_store_gb=round(2 * (nt // K + 1) * box_gb + (K + 1) * pad_gb, 1), loss=loss,
+                # checkpoint pairs + the segment's levels and their K u (padded)
+                wavefield_store_gb=round(2 * (nt // K + 1) * box_gb + 2 * (K + 1) * pad_gb, 1), loss=loss,
                 gradients_finite_nonzero=bool(ok), dtype='f32', data='synthetic',
                 config={'workload': '3D 320x320x320 inner (360x360x360 extended), dx 0.4 mm, SO10 OT2, sponge 20, '
                                     'variable vp / rho / alpha (power 0), one shot, nt %d, 1024 receivers, '

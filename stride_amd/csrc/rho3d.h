@@ -23,6 +23,7 @@ struct Rho3DStep {
     Box box;
     float idt2;
     int konly = 0;  // out = K u only (A, B, G, other and save unused)
+    float *kout = nullptr;  // update without attenuation power 2: K u of every node also stored here
     // power-2 attenuation: out += Cbeta (lap u - lap other) with the centred SO10 Laplacians of
     // the other level and of u read from lapother and lapu
     const float *Cbeta = nullptr, *lapother = nullptr, *lapu = nullptr;

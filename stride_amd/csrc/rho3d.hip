@@ -42,6 +42,7 @@ struct RhoArgs {
     float idt2;
     Box box;
     int konly;  // out = K u (no update, no save)
+    float *kout;  // update form: K u of every node also stored here (null: not)
     // power-2 attenuation (Cbeta non-null): Cbeta and the centred Laplacians of the other level
     // and of u
     const float *Cbeta, *lapo, *lapu;
@@ -85,7 +86,7 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 struct Res {
-    rsrc_t u, o, b, A, B, G, out, Cb, Lo, Lu;
+    rsrc_t u, o, b, A, B, G, out, Cb, Lo, Lu, Ko;
 };
 
 template <int S, int RPT, int NW, bool VARB>
@@ -215,6 +216,7 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
             if (FORM == 2) acc += st.Cb[S][r] * (st.Lu[S][r] - st.Lo[S][r]);  // power 2: Cbeta (lap u - lap other)
             const float out = FORM == 1 ? st.nd[r][(J + 1) % 10] : st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][r] + acc;
             store_nt(out, rs.out, emit ? st.own[r] + po : kOOB);
+            if (FORM == 0 && a.kout) store_nt(st.nd[r][(J + 1) % 10], rs.Ko, emit ? st.own[r] + po : kOOB);
             if (SAVE && emit) {
                 const int y = y0 + w * RPT + r, z = z0 + lane;
                 if (sa_inbox(a.box, p, y, z))
@@ -253,6 +255,7 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     rs.B = make(a.B, a.bytes);
     rs.G = make(a.G, a.bytes);
     rs.out = make(a.out, a.bytes);
+    rs.Ko = make(a.kout ? a.kout : a.out, a.kout ? a.bytes : 0);
     if (FORM == 2) {
         rs.Cb = make(a.Cbeta, a.bytes);
         rs.Lo = make(a.lapo, a.bytes);
@@ -762,6 +765,8 @@ int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_
     a.idt2 = st.idt2;
     a.box = st.box;
     a.konly = st.konly;
+    a.kout = st.kout;
+    if (st.kout && (st.konly || st.Cbeta)) return -1;
     a.Cbeta = st.Cbeta;
     a.lapo = st.lapother;
     a.lapu = st.lapu;

@@ -714,6 +714,9 @@ struct sa_solver {
     float *src_keep = nullptr;
     size_t src_keep_n = 0;
     float *medium_stage = nullptr;  // unpadded medium array on its way to the padded layout
+    float *kseg = nullptr;  // save mode 4 with the fused density update: K u of the segment's levels
+    size_t kseg_bytes = 0;
+    bool kseg_valid = false;  // the current segment's K u slots were written by its recomputation
     size_t medium_stage_n = 0;
     // checkpointed raw saves (save_mode 5): `saved` holds the states (p[T_j], p[T_j - 1]),
     // T_j = (jK - 1) f, of segments j >= 1; `segment` is a ring of K + 2 snapshots (snapshot i in
@@ -936,6 +939,7 @@ extern "C" int sa_destroy(sa_solver *s) {
     dfree(s->fb[1]);
     dfree(s->src_keep);
     dfree(s->medium_stage);
+    dfree(s->kseg);
     dfree(s->xfer);
     dfree(s->dump[0]);
     dfree(s->dump[1]);
@@ -1392,10 +1396,14 @@ static int tile_points(sa_solver *s, Sparse &sp) {
 
 // One time step. With `inj` given and the fused 2D update in use, the injection is applied by
 // the update itself and *injected is set; otherwise the caller injects.
+// kout (the fused density update only): K u of every node is also stored there; *kout_done tells
+// whether it was.
 static int step_general(sa_solver *s, const float *u, const float *other, float *out, int backward, float *save,
                         int save_mode, const InjectSpec *inj = nullptr, bool *injected = nullptr,
-                        const GatherSpec *gat = nullptr, const GradSpec *grd = nullptr) {
+                        const GatherSpec *gat = nullptr, const GradSpec *grd = nullptr, float *kout = nullptr,
+                        bool *kout_done = nullptr) {
     if (injected) *injected = false;
+    if (kout_done) *kout_done = false;
     const Geo &g = s->geo;
     const sa_scheme &c = s->sch;
     hipEvent_t e0;
@@ -1424,8 +1432,10 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
             st.lapother = s->lapother;
             st.lapu = s->lapnext;
         }
+        if (kout && c.att_power != 2) st.kout = kout;
         if (rho3d_step(s->rho3d, g, st, s->stream) != 0)
             return fail(SA_E_HIP, "fused density update launch: %s", hipGetErrorString(hipGetLastError()));
+        if (kout_done) *kout_done = st.kout != nullptr;
         if (c.att_power == 2) lap_advance(s, u);
         TRY(timed_end(s));
         return SA_OK;
@@ -1706,6 +1716,20 @@ static int alloc_saved(sa_solver *s) {
         }
         // the recomputed levels of save mode 4 are read with their halo, which no step writes
         if (c.save_mode == 4) HIP_TRY(hipMemsetAsync(s->segment, 0, sb, s->stream));
+        // save mode 4 with the fused density update: a K u slot per recomputed level, so the
+        // exact adjoint reads K u_t instead of forming it again
+        s->kseg_valid = false;
+        const char *noku = getenv("STRIDE_AMD_NO_KU_REUSE");
+        if (c.save_mode == 4 && use_rho3d(s) && c.att_power != 2 && !(noku && noku[0] == '1')) {
+            if (!s->kseg || s->kseg_bytes < sb) {
+                dfree(s->kseg);
+                s->kseg = nullptr;
+                s->kseg_bytes = 0;
+                if (hipMalloc((void **)&s->kseg, sb) == hipSuccess) s->kseg_bytes = sb;
+                else s->kseg = nullptr;  // not enough memory: the adjoint forms K u itself
+                (void)hipGetLastError();
+            }
+        }
         TRY(dalloc(&s->fb[0], s->total));
         TRY(dalloc(&s->fb[1], s->total));
     }
@@ -1837,6 +1861,12 @@ static float *segment_level(sa_solver *s, int j, int t) {
     return s->segment + (size_t)(t - j * s->saved_f) * s->total;
 }
 
+// Slot of K u_t in segment j (save mode 4 with the fused density update: the recomputation's
+// update stores the K u it forms, which the exact adjoint's accumulation at step t reads).
+static float *segment_ku(sa_solver *s, int j, int t) {
+    return s->kseg + (size_t)(t - j * s->saved_f) * s->total;
+}
+
 // Recompute forward levels u_{jK} .. u_{min((j+1)K, nt-1)} into the segment buffer from checkpoint j.
 // Every step writes its new level into its own slot (the levels stay in the padded layout, so the
 // adjoint reads them without unpacking).
@@ -1853,11 +1883,17 @@ static int recompute_segment(sa_solver *s, int j) {
     if (tend >= t0 + 1)
         k_unpack<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->box, pair, segment_level(s, j, t0 + 1));
     CHECK_LAUNCH();
+    // K u of levels t0 + 1 .. tend - 1 (formed by their steps) when every step stored it
+    bool ku = s->kseg != nullptr;
     for (int t = t0 + 1; t < tend; ++t) {
         float *out = segment_level(s, j, t + 1);
-        TRY(step_general(s, segment_level(s, j, t), segment_level(s, j, t - 1), out, 0, nullptr, 0));
+        bool done = false;
+        TRY(step_general(s, segment_level(s, j, t), segment_level(s, j, t - 1), out, 0, nullptr, 0, nullptr, nullptr,
+                         nullptr, nullptr, s->kseg ? segment_ku(s, j, t) : nullptr, &done));
+        ku = ku && done;
         TRY(inject(s, s->src, s->src_keep + (size_t)t * s->src.npoint, 1.f, out, nullptr, 0.f, s->box));
     }
+    s->kseg_valid = ku;
     return SA_OK;
 }
 
@@ -2160,7 +2196,14 @@ static int adjoint_exact_core(sa_solver *s, const float *res_dev, float *lam_dev
             u1 = s->work1;
         }
         if ((rc = (hipGetLastError() == hipSuccess ? SA_OK : fail(SA_E_HIP, "launch failed"))) != SA_OK) break;
-        if ((rc = apply_K(s, u0, nullptr, KuT)) != SA_OK) break;
+        // K u_t: stored by the segment's recomputation for its levels t0 + 1 .. tend - 1, else formed
+        const float *Ku = KuT;
+        if (ckpt && s->kseg && s->kseg_valid && c.kernel == 2) {
+            const int K = s->saved_f, j = (t - 1) / K;
+            const int tend = std::min((j + 1) * K, nt - 1);
+            if (t >= j * K + 1 && t <= tend - 1) Ku = segment_ku(s, j, t);
+        }
+        if (Ku == KuT && (rc = apply_K(s, u0, nullptr, KuT)) != SA_OK) break;
         if (c.kernel == 4) {
             // gm += dt^2/12 (K mu)(K u); Ku += dt^2/12 K(m Ku)
             if ((rc = apply_K(s, m1, nullptr, Kmu)) != SA_OK) break;
@@ -2172,10 +2215,10 @@ static int adjoint_exact_core(sa_solver *s, const float *res_dev, float *lam_dev
         // mu_{t+1} (m1) formed in the same pass (3D: a march with LDS tiles)
         float *gbp = c.rho_mode == 2 ? acc[4] : nullptr;
         if (s->grid.dim == 3 && rho3d_supported(g)) {
-            if (rho3d_exact_acc(g, u0, m1, s->G, u1, KuT, acc[0], acc[1], acc[2], gbp, s->stream) != 0)
+            if (rho3d_exact_acc(g, u0, m1, s->G, u1, Ku, acc[0], acc[1], acc[2], gbp, s->stream) != 0)
                 rc = fail(SA_E_HIP, "exact accumulation launch: %s", hipGetErrorString(hipGetLastError()));
         } else {
-            k_exact_acc_fused<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, m1, s->G, u0, u1, KuT, acc[0],
+            k_exact_acc_fused<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, m1, s->G, u0, u1, Ku, acc[0],
                                                                               acc[1], acc[2], gbp);
         }
         if (hipGetLastError() != hipSuccess) rc = fail(SA_E_HIP, "launch failed");
@@ -2293,6 +2336,10 @@ extern "C" int sa_free_saved(sa_solver *s) {
     dfree(s->segment);
     s->segment = nullptr;
     s->segment_bytes = 0;
+    dfree(s->kseg);
+    s->kseg = nullptr;
+    s->kseg_bytes = 0;
+    s->kseg_valid = false;
     return SA_OK;
 }
 
