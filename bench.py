@@ -517,6 +517,75 @@ def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0, att_power=0):
                                     % (att_power, plan.nt)})
 
 
+def bench_density_pair(nt=2500, tile=None):
+    """Two 3D shots on the density workload's medium advanced together (interleaved wavefields,
+    one read of b and the coefficients per pass for both): forward with the raw wavefields saved
+    every 4th step, then the as-written adjoint with the summed vp and rho gradients. The second
+    shot is sourced at the receiver farthest from the first source. The rate counts each shot's own
+    updates; `roofline` covers the two-shot update (algorithmic bytes per cell and pass: read u and
+    the other level of both shots, b, A, B, G, write both new levels = 40 B, i.e. 20 B per point
+    update; + 8 B per cell on save steps). The gradient pass runs once per shot on guarded steps."""
+    from stride_amd.configs import config4
+    from stride_amd.physics.plan import build_plan
+    from stride_amd._native import Solver
+    cfg = config4(scale=1.0, nt=nt, nrec=1024)
+    space, time_ = cfg['space'], cfg['time']
+    src0 = cfg['sources']
+    far = cfg['receivers'][np.argmax(np.linalg.norm(cfg['receivers'] - src0[0], axis=1))][None]
+    plans = [build_plan(space, time_, cfg['wavelet'][None], cfg['vp'], rho=cfg['rho'], alpha=cfg['alpha'],
+                        attenuation_power=0, source_coordinates=sc, receiver_coordinates=cfg['receivers'],
+                        kernel='OT2', save_undersampling=4, grad_vp=True, grad_rho=True) for sc in (src0, far)]
+    plan = plans[0]
+    s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, plan.nt)
+    s.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
+                 plan.undersampling, 2, plan.tb0, plan.tb1, 1, 1, 0)
+    s.set_medium(plan.vp, plan.rho, plan.alpha)
+    s.set_sponge(plan.sponge_tables)
+    s.batch_create()
+    for k, p in enumerate(plans):
+        s.batch_set_taps(k, p.src_idx, p.src_w, p.rec_idx, p.rec_w)
+    if tile:
+        s.batch_set_tile_density(*tile)
+    rng = np.random.default_rng(0)
+    d_src = [s._to_device(p.src_data) for p in plans]
+    d_rec = [s.device_alloc(p.nt * p.nrec * 4) for p in plans]
+    d_res = [s._to_device(rng.standard_normal((p.nt, p.nrec)).astype(np.float32) * 1e-3) for p in plans]
+
+    def pair():
+        s.batch_forward_dev(d_src, d_rec)
+        s.batch_adjoint_dev(d_res)
+        s.synchronize()
+
+    pair()  # allocations (saved wavefields, gradient accumulators)
+    s.timing(True)
+    t0 = time.perf_counter()
+    pair()
+    wall = time.perf_counter() - t0
+    ms, launches = s.timing()
+    s.timing(False)
+    n = int(np.prod(plan.shape))
+    f = plan.undersampling
+    nsave = len(range(0, plan.nt - 1, f))
+    alg = launches * 40.0 * n + nsave * 8.0 * n
+    achieved = alg / (ms * 1e-3) / 1e9
+    own = 2 * ((plan.tb1 - 2) + (plan.tb1 - 1 - f - plan.tb0))
+    per_gb = plan.shape[0] * plan.shape[1] * (-(-plan.shape[2] // 32) * 32) * 4 / 1e9
+    for d in d_src + d_rec + d_res:
+        s.device_free(d)
+    s.close()
+    return dict(metric='Gpts/s fwd+adjoint 3D 320^3 SO10, variable vp/rho/alpha, vp+rho gradients, two shots per pass',
+                value=round(own * n / wall / 1e9, 3), unit='Gpts/s', seconds=round(wall, 3), shots=2,
+                snapshot_store_gb=round(2 * (plan.tb1 // f + 1) * per_gb, 1),
+                update_launches=int(launches), update_kernel_us=round(ms * 1e3 / launches, 1),
+                update_kernel_share=round(ms * 1e-3 / wall, 3),
+                roofline={'bound': 'hbm', 'kernel': 'k_rho3d (two shots)', 'achieved': round(achieved, 1),
+                          'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4)},
+                config={'workload': '3D 320x320x320 inner (360x360x360 extended), dx 0.4 mm, SO10 OT2, sponge 20, '
+                                    'variable vp / rho and alpha 0.5 (power 0), two shots per pass (interleaved '
+                                    'wavefields), nt %d, raw saves every 4th step, 1024 receivers per shot'
+                                    % plan.nt})
+
+
 def bench_power2(nt=500, general=False):
     """One 3D shot with power-2 attenuation (alpha 0.5 dB/cm/MHz, the 2D gradient workload's
     value) on the 360^3 grid of the headline workload: forward with p.dt2 saved every 4th step,
@@ -900,6 +969,9 @@ def main():
         return
     if args.density:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
+        if args.shots == 2 and not args.checkpoint and args.att_power == 0:
+            print(json.dumps(bench_density_pair(args.nt, tile)))
+            return
         print(json.dumps(bench_density(args.nt, tile, with_cpu=not args.no_cpu, checkpoint=args.checkpoint,
                                        att_power=args.att_power)))
         return

@@ -214,10 +214,14 @@ int sa_synchronize(sa_solver *s);
  * shots dispatched by stride/__init__.py:110-141, 325-334) and sums per-shot gradients in the
  * optimisation loop (stride/core.py:565-633). Two shots on the same grid, scheme, medium and
  * boundary advance together: their wavefields are interleaved per cell so one pass of the
- * specialised 3D update (3D, OT2, no density, sponge or no boundary, no attenuation) serves
- * both with one read of vp, and the adjoint accumulates the SUM of the two shots' vp
- * gradients and preconditioners (read with sa_read_gradient 0 / 1). Per-shot taps, samples,
- * traces and residuals. Other schemes return SA_E_UNSUPPORTED. */
+ * specialised 3D update (3D, OT2, no density, sponge or no boundary, no attenuation; save mode
+ * 0 or 1) serves both with one read of vp, and the adjoint accumulates the SUM of the two shots'
+ * vp gradients and preconditioners (read with sa_read_gradient 0 / 1). With density (3D, OT2,
+ * sponge or no boundary, attenuation power 0 / None / off; save mode 0 or 2) one pass of the
+ * fused density update serves both with one read of b and the coefficients, and the adjoint
+ * (variable density) sums the two shots' vp and rho gradients and preconditioners
+ * (sa_read_gradient 0 .. 3; stride/physics/iso_acoustic/devito.py:1115-1150). Per-shot taps,
+ * samples, traces and residuals. Other schemes return SA_E_UNSUPPORTED. */
 int sa_batch_create(sa_solver *s, int nshot /* 2 */);
 int sa_batch_set_taps(sa_solver *s, int shot, int nsrc, int src_ntap, const int64_t *src_index,
                       const float *src_weight, int nrec, int rec_ntap, const int64_t *rec_index,
@@ -230,6 +234,9 @@ int sa_batch_get_wavefield(sa_solver *s, int shot, int which, float *out);
 /* tile of the two-shot update: waves 8 with rows 1 or 2, or waves 4 with rows 2 (default rows 2,
  * chunk 24, waves 4) */
 int sa_batch_set_tile(sa_solver *s, int rows, int chunk, int waves);
+/* tile of the two-shot density update: rows 1 with waves 4 or 8 (default rows 1, chunk 60,
+ * waves 4) */
+int sa_batch_set_tile_density(sa_solver *s, int rows, int chunk, int waves);
 
 /* Many shots of uneven duration and receiver count, forward only. No counterpart in the
  * reference, which runs one shot per worker call (IsoAcousticDevito.forward,

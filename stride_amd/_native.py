@@ -46,7 +46,7 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_timing_enable', 'sa_timing_read', 'sa_stream_create', 'sa_stream_destroy',
            'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize', 'sa_batch_create',
            'sa_batch_set_taps', 'sa_batch_forward_dev', 'sa_batch_adjoint_dev', 'sa_batch_get_wavefield',
-           'sa_batch_set_tile', 'sa_traces_check', 'sa_traces_mute', 'sa_traces_fir', 'sa_traces_sosfilt',
+           'sa_batch_set_tile', 'sa_batch_set_tile_density', 'sa_traces_check', 'sa_traces_mute', 'sa_traces_fir', 'sa_traces_sosfilt',
            'sa_traces_norm', 'sa_traces_l2', 'sa_traces_l2_noalloc', 'sa_multi_create', 'sa_multi_set_shot',
            'sa_multi_forward_dev', 'sa_multi_get_wavefield', 'sa_multi_graph_ms', 'sa_nn_conv3x3',
            'sa_nn_flip_weights', 'sa_nn_wgrad3x3', 'sa_nn_bias_grad', 'sa_nn_avgpool', 'sa_nn_avgpool_back',
@@ -290,6 +290,10 @@ class Solver:
     def batch_set_tile(self, rows, chunk, waves):
         self._check(self._lib.sa_batch_set_tile(self._h, ctypes.c_int(rows), ctypes.c_int(chunk), ctypes.c_int(waves)))
 
+    def batch_set_tile_density(self, rows, chunk, waves):
+        self._check(self._lib.sa_batch_set_tile_density(self._h, ctypes.c_int(rows), ctypes.c_int(chunk),
+                                                        ctypes.c_int(waves)))
+
     def batch_forward_dev(self, src_ptrs, rec_ptrs):
         sp = (ctypes.c_void_p * 2)(*src_ptrs)
         rp = (ctypes.c_void_p * 2)(*rec_ptrs)
@@ -320,9 +324,10 @@ class Solver:
                 self.device_free(d)
         return recs
 
-    def batch_adjoint(self, residuals):
+    def batch_adjoint(self, residuals, density=False):
         """Host convenience: adjoint sources (nt, nrec_k) of the two shots -> summed vp gradient
-        and preconditioner (extended grid, as read_gradient(0) / read_gradient(1))."""
+        and preconditioner (extended grid, as read_gradient(0) / read_gradient(1)); with
+        ``density`` (a density scheme) also the summed rho gradient and preconditioner."""
         dres = [self._to_device(residuals[k]) for k in range(2)]
         try:
             self.batch_adjoint_dev(dres)
@@ -330,7 +335,10 @@ class Solver:
         finally:
             for d in dres:
                 self.device_free(d)
-        return dict(grad_vp=self.read_gradient(0), prec_vp=self.read_gradient(1))
+        out = dict(grad_vp=self.read_gradient(0), prec_vp=self.read_gradient(1))
+        if density:
+            out.update(grad_rho=self.read_gradient(2), prec_rho=self.read_gradient(3))
+        return out
 
     def batch_wavefield(self, shot, which=0):
         out = np.zeros(self.shape, np.float32)

@@ -38,4 +38,18 @@ __device__ __forceinline__ void store_nt(float v, rsrc_t r, unsigned off) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, kNT);
 }
 
+// two consecutive floats (8-byte aligned offset): the two shots of one cell of an interleaved field
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float2 load2_nt(rsrc_t r, unsigned off) {
+    const f2v v = __builtin_bit_cast(f2v, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kNT));
+    return make_float2(v.x, v.y);
+}
+
+__device__ __forceinline__ void store2_nt(float x, float y, rsrc_t r, unsigned off) {
+    const f2v v = {x, y};
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2v, v), r, off, 0, kNT);
+}
+
 }  // namespace bufio

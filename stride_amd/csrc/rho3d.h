@@ -27,10 +27,15 @@ struct Rho3DStep {
     // power-2 attenuation: out += Cbeta (lap u - lap other) with the centred SO10 Laplacians of
     // the other level and of u read from lapother and lapu
     const float *Cbeta = nullptr, *lapother = nullptr, *lapu = nullptr;
+    // two shots per pass (update form only: no konly, kout or power-2 term): u, other and out
+    // hold the two shots interleaved per cell (8 bytes per padded cell, shot-minor); b and the
+    // coefficients are shared; `save` takes shot 0's save and save1 shot 1's (same box layout)
+    int nshot = 1;
+    float *save1 = nullptr;
 };
 
-// true when the layout fits the kernel's 32-bit buffer offsets
-bool rho3d_supported(const Geo &g);
+// true when the layout of `nshot` interleaved shots fits the kernel's 32-bit buffer offsets
+bool rho3d_supported(const Geo &g, int nshot = 1);
 int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_t stream);
 // rows 1/2 with waves 4/8, or rows 1 with waves 16; chunk >= 1
 bool rho3d_valid_tune(int rows, int chunk, int waves);
@@ -39,8 +44,10 @@ bool rho3d_valid_tune(int rows, int chunk, int waves);
 // (whole-domain box): grad_rho / prec_rho and, when gv / pv are given, grad_vp / prec_vp from the
 // second time difference of the snapshots t - f, t, t + f (psm / psp null: zero).
 bool rho3d_grad_supported(const Geo &g, const Box &box);
+// pa may hold pa_nshot (1 or 2) interleaved shots, of which shot pa_shot is read.
 int rho3d_grad(const Geo &g, const Box &box, const float *ps, const float *psm, const float *psp, float fdt2,
-               const float *b, const float *pa, float *gv, float *pv, float *gr, float *pr, hipStream_t stream);
+               const float *b, const float *pa, float *gv, float *pv, float *gr, float *pr, hipStream_t stream,
+               int pa_nshot = 1, int pa_shot = 0);
 
 // Exact-adjoint accumulations of one step over the whole domain (3D): lam = mu / G, gA += lam u,
 // gB += lam um, gG += lam Ku and, with gb, gb -= 1/2 sum_a (D+ mu D+ u)(p) + (D+ mu D+ u)(p - s_a)

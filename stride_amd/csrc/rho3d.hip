@@ -35,8 +35,10 @@ constexpr int kTZ = 64;
 struct RhoArgs {
     const float *u, *o, *b, *A, *B, *G;
     float *out, *save;
-    long long bytes;
-    int plane_bytes;
+    float *save1;  // two shots: shot 1's save (box layout); `save` holds shot 0's
+    long long bytes;   // one-shot field
+    long long sbytes;  // field of the shots of a pass (u, o, out: NS floats per cell)
+    int plane_bytes, splane_bytes;
     int chunk, ntz, nty, ntiles, per_xcd;
     int save_mode;
     float idt2;
@@ -48,33 +50,36 @@ struct RhoArgs {
     const float *Cbeta, *lapo, *lapu;
 };
 
-template <int RPT, int NW, bool VARB>
+// NS shots per pass: u, o and out hold NS floats per cell (shot-minor); b and the coefficients
+// are read once for all shots. The LDS u tile and the flux tiles hold NS floats per cell.
+template <int RPT, int NW, bool VARB, int NS>
 struct Tile {
     static constexpr int NT = NW * 64, TY = NW * RPT;
-    static constexpr int UY = TY + 18, UZ = 88, UC = UZ / 4;  // u: rows y0-9.., columns z0-12..
-    static constexpr int BY = TY + 10, BZ = 80, BC = BZ / 4;  // b: rows y0-5.., columns z0-8..
-    static constexpr int FY = TY + 9;                         // axis-1 fluxes, half rows y0-5 .. y0+TY+3
-    static constexpr int FZN = 73, FZP = 76;                  // axis-2 fluxes, half columns z0-5 .. z0+67
+    static constexpr int UY = TY + 18, UZ = 88 * NS, UC = UZ / 4;  // u: rows y0-9.., columns z0-12.. (floats)
+    static constexpr int BY = TY + 10, BZ = 80, BC = BZ / 4;       // b: rows y0-5.., columns z0-8..
+    static constexpr int FY = TY + 9;                              // axis-1 fluxes, half rows y0-5 .. y0+TY+3
+    static constexpr int FZN = 73, FZP = 76;                       // axis-2 fluxes, half columns z0-5 .. z0+67
     static constexpr int NLU = (UY * UC + NT - 1) / NT;
     static constexpr int NLB = VARB ? (BY * BC + NT - 1) / NT : 1;
 };
 
-template <int RPT, int NW, bool VARB>
+template <int RPT, int NW, bool VARB, int NS>
 struct State {
-    using T = Tile<RPT, NW, VARB>;
-    float nd[RPT][10];  // node accumulators (K u); slot of plane p: (p - mbeg) % 10
-    float uq[RPT][10];  // own u; slot of plane p
-    float hb[RPT][5];   // axis-0 bbar of half plane q; slot (q - mbeg) % 5
-    float bprev[RPT];   // own b of the previous plane
-    float4 su[2][T::NLU];  // u tiles of the next two planes, staged in registers (slot: plane % 2)
+    using T = Tile<RPT, NW, VARB, NS>;
+    float nd[NS][RPT][10];  // node accumulators (K u); slot of plane p: (p - mbeg) % 10
+    float uq[NS][RPT][10];  // own u; slot of plane p
+    float hb[RPT][5];       // axis-0 bbar of half plane q; slot (q - mbeg) % 5
+    float bprev[RPT];       // own b of the previous plane
+    float4 su[2][T::NLU];   // u tiles of the next two planes, staged in registers (slot: plane % 2)
     float4 sb[2][T::NLB];
-    float ov[2][RPT], Av[2][RPT], Bv[2][RPT], Gv[2][RPT];  // own operands of the next node plane
+    float ov[2][NS][RPT], Av[2][RPT], Bv[2][RPT], Gv[2][RPT];  // own operands of the next node plane
     // power-2 attenuation: own Cbeta and the centred Laplacians of the other level and of u of the
     // next node plane
     float Cb[2][RPT], Lo[2][RPT], Lu[2][RPT];
     unsigned uoff[T::NLU], boff[T::NLB];  // in-plane byte offsets of the tile chunks (kOOB: none)
     int ulds[T::NLU], blds[T::NLB];       // their LDS float offsets (the dump slot: none)
     unsigned own[RPT];                    // own cells' in-plane byte offsets (kOOB outside the domain)
+    unsigned owns[RPT];                   // the same in the NS-shot fields (NS > 1)
 };
 
 // Workgroup barrier that orders LDS accesses only: a __syncthreads() would also wait for every
@@ -89,23 +94,51 @@ struct Res {
     rsrc_t u, o, b, A, B, G, out, Cb, Lo, Lu, Ko;
 };
 
-template <int S, int RPT, int NW, bool VARB>
-__device__ __forceinline__ void load_tile(State<RPT, NW, VARB> &st, const Res &rs, unsigned poff) {
-    using T = Tile<RPT, NW, VARB>;
+// plane pitch in bytes of the NS-shot fields (the one-shot pitch, known equal, for NS = 1)
+#define SPB(a) (NS == 1 ? (a).plane_bytes : (a).splane_bytes)
+
+// NS consecutive floats of LDS (8-byte aligned for NS = 2)
+template <int NS>
+__device__ __forceinline__ void lds_get(const float *p, float (&v)[NS]) {
+    if (NS == 1) {
+        v[0] = p[0];
+    } else {
+        const float2 q = *reinterpret_cast<const float2 *>(p);
+        v[0] = q.x;
+        v[NS - 1] = q.y;
+    }
+}
+
+template <int NS>
+__device__ __forceinline__ void lds_put(float *p, const float (&v)[NS]) {
+    if (NS == 1) *p = v[0];
+    else *reinterpret_cast<float2 *>(p) = make_float2(v[0], v[NS - 1]);
+}
+
+template <int S, int RPT, int NW, bool VARB, int NS>
+__device__ __forceinline__ void load_tile(State<RPT, NW, VARB, NS> &st, const Res &rs, unsigned poff, unsigned spoff) {
+    using T = Tile<RPT, NW, VARB, NS>;
 #pragma unroll
-    for (int i = 0; i < T::NLU; ++i) st.su[S][i] = load4(rs.u, st.uoff[i] + poff);
+    for (int i = 0; i < T::NLU; ++i) st.su[S][i] = load4(rs.u, st.uoff[i] + spoff);
     if (VARB) {
 #pragma unroll
         for (int i = 0; i < T::NLB; ++i) st.sb[S][i] = load4(rs.b, st.boff[i] + poff);
     }
 }
 
-template <int S, int RPT, int NW, bool VARB, int FORM>
-__device__ __forceinline__ void load_own(State<RPT, NW, VARB> &st, const Res &rs, unsigned poff) {
+template <int S, int RPT, int NW, bool VARB, int FORM, int NS>
+__device__ __forceinline__ void load_own(State<RPT, NW, VARB, NS> &st, const Res &rs, unsigned poff,
+                                         unsigned spoff) {
     if (FORM == 1) return;  // out = K u: no own operands
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        st.ov[S][r] = load_nt(rs.o, st.own[r] + poff);
+        if (NS == 1) {
+            st.ov[S][0][r] = load_nt(rs.o, st.own[r] + spoff);
+        } else {
+            const float2 o2 = load2_nt(rs.o, st.owns[r] + spoff);
+            st.ov[S][0][r] = o2.x;
+            st.ov[S][NS - 1][r] = o2.y;
+        }
         st.Av[S][r] = load_nt(rs.A, st.own[r] + poff);
         st.Bv[S][r] = load_nt(rs.B, st.own[r] + poff);
         st.Gv[S][r] = load_nt(rs.G, st.own[r] + poff);
@@ -117,11 +150,11 @@ __device__ __forceinline__ void load_own(State<RPT, NW, VARB> &st, const Res &rs
     }
 }
 
-template <int RPT, int NW, bool VARB, bool SAVE, int FORM, int J>
-__device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res &rs, State<RPT, NW, VARB> &st,
+template <int RPT, int NW, bool VARB, bool SAVE, int FORM, int NS, int J>
+__device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res &rs, State<RPT, NW, VARB, NS> &st,
                                       float *sU, float *sB, float *sFy, float *sFz, const int m, const int x0,
                                       const int x1, const int y0, const int z0, const int lane, const int w) {
-    using T = Tile<RPT, NW, VARB>;
+    using T = Tile<RPT, NW, VARB, NS>;
     // slots: plane m -> J, m - 9 -> (J + 1) % 10; staging slot of plane m: J % 2 (m - mbeg = J mod 10)
     constexpr int S = J % 2;
     lds_barrier();  // every wave is done with the previous plane's LDS tiles
@@ -134,10 +167,11 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
     // requests two planes ahead for the tiles, one node plane ahead for the own operands; all
     // unconditional (planes outside the chunk are read but never stored; past the array they
     // read zero), so the wait counts before each use are exact
-    load_tile<S>(st, rs, (unsigned)((m + 2) * a.plane_bytes));
+    load_tile<S>(st, rs, (unsigned)((m + 2) * a.plane_bytes), (unsigned)((m + 2) * SPB(a)));
     const int p = m - 9;
     const bool emit = p >= x0 && p < x1;
-    load_own<1 - S, RPT, NW, VARB, FORM>(st, rs, (unsigned)((p + 1) * a.plane_bytes));
+    load_own<1 - S, RPT, NW, VARB, FORM, NS>(st, rs, (unsigned)((p + 1) * a.plane_bytes),
+                                             (unsigned)((p + 1) * SPB(a)));
     lds_barrier();  // plane m's tiles are in LDS
     const bool inplane = m >= x0 && m < x1;
     if (inplane) {
@@ -148,22 +182,42 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
             if (e < T::FY * kTZ) {
                 // axis-1 flux at half row q = y0 - 5 + fr, column z0 + lane
                 const int fr = e >> 6;
-                const float *c = sU + (fr + 4) * T::UZ + lane + 12;  // u at row q
-                float d = 0.f;
+                const float *c = sU + (fr + 4) * T::UZ + (lane + 12) * NS;  // u at row q
+                float d[NS] = {};
 #pragma unroll
-                for (int k = 0; k < 5; ++k) d += g.ws[1][k] * (c[(k + 1) * T::UZ] - c[-k * T::UZ]);
-                if (VARB) d *= 0.5f * (sB[fr * T::BZ + lane + 8] + sB[(fr + 1) * T::BZ + lane + 8]);
-                sFy[fr * kTZ + lane] = d;
+                for (int k = 0; k < 5; ++k) {
+                    float hi[NS], lo[NS];
+                    lds_get<NS>(c + (k + 1) * T::UZ, hi);
+                    lds_get<NS>(c - k * T::UZ, lo);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) d[s] += g.ws[1][k] * (hi[s] - lo[s]);
+                }
+                if (VARB) {
+                    const float bb = 0.5f * (sB[fr * T::BZ + lane + 8] + sB[(fr + 1) * T::BZ + lane + 8]);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) d[s] *= bb;
+                }
+                lds_put<NS>(sFy + (fr * kTZ + lane) * NS, d);
             } else {
                 // axis-2 flux at half column q = z0 - 5 + fz, row y0 + row
                 const int ez = e - T::FY * kTZ;
                 const int row = ez / T::FZN, fz = ez - row * T::FZN;
-                const float *c = sU + (row + 9) * T::UZ + fz + 7;  // u at column q
-                float d = 0.f;
+                const float *c = sU + (row + 9) * T::UZ + (fz + 7) * NS;  // u at column q
+                float d[NS] = {};
 #pragma unroll
-                for (int k = 0; k < 5; ++k) d += g.ws[2][k] * (c[k + 1] - c[-k]);
-                if (VARB) d *= 0.5f * (sB[(row + 5) * T::BZ + fz + 3] + sB[(row + 5) * T::BZ + fz + 4]);
-                sFz[row * T::FZP + fz] = d;
+                for (int k = 0; k < 5; ++k) {
+                    float hi[NS], lo[NS];
+                    lds_get<NS>(c + (k + 1) * NS, hi);
+                    lds_get<NS>(c - k * NS, lo);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) d[s] += g.ws[2][k] * (hi[s] - lo[s]);
+                }
+                if (VARB) {
+                    const float bb = 0.5f * (sB[(row + 5) * T::BZ + fz + 3] + sB[(row + 5) * T::BZ + fz + 4]);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) d[s] *= bb;
+                }
+                lds_put<NS>(sFz + (row * T::FZP + fz) * NS, d);
             }
         }
     }
@@ -171,7 +225,10 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         const int ry = w * RPT + r;
-        st.uq[r][J] = sU[(ry + 9) * T::UZ + lane + 12];
+        float uc[NS];
+        lds_get<NS>(sU + (ry + 9) * T::UZ + (lane + 12) * NS, uc);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) st.uq[s][r][J] = uc[s];
         if (VARB) {
             const float bc = sB[(ry + 5) * T::BZ + lane + 8];
             st.hb[r][(J + 4) % 5] = 0.5f * (st.bprev[r] + bc);  // half plane m - 1
@@ -183,59 +240,92 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
             const int ry = w * RPT + r;
-            const float *fy = sFy + (ry + 5) * kTZ + lane;  // flux at half row y
-            const float *fz = sFz + ry * T::FZP + lane + 5;  // flux at half column z
-            float d = 0.f;
+            const float *fy = sFy + ((ry + 5) * kTZ + lane) * NS;  // flux at half row y
+            const float *fz = sFz + (ry * T::FZP + lane + 5) * NS;  // flux at half column z
+            float d[NS] = {};
 #pragma unroll
-            for (int k = 0; k < 5; ++k) d += g.ws[1][k] * (fy[k * kTZ] - fy[-(k + 1) * kTZ]);
+            for (int k = 0; k < 5; ++k) {
+                float hi[NS], lo[NS];
+                lds_get<NS>(fy + k * kTZ * NS, hi);
+                lds_get<NS>(fy - (k + 1) * kTZ * NS, lo);
 #pragma unroll
-            for (int k = 0; k < 5; ++k) d += g.ws[2][k] * (fz[k] - fz[-(k + 1)]);
-            st.nd[r][J] += d;
+                for (int s = 0; s < NS; ++s) d[s] += g.ws[1][k] * (hi[s] - lo[s]);
+            }
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                float hi[NS], lo[NS];
+                lds_get<NS>(fz + k * NS, hi);
+                lds_get<NS>(fz - (k + 1) * NS, lo);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) d[s] += g.ws[2][k] * (hi[s] - lo[s]);
+            }
+#pragma unroll
+            for (int s = 0; s < NS; ++s) st.nd[s][r][J] += d[s];
         }
     }
     // axis-0 flux at half plane m - 5 (u of planes m-9 .. m) goes to the nodes m-9 .. m
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) {
-        float f = 0.f;
+    for (int s = 0; s < NS; ++s) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) f += g.ws[0][k] * (st.uq[r][(J + 6 + k) % 10] - st.uq[r][(J + 15 - k) % 10]);
-        if (VARB) f *= st.hb[r][J % 5];
+        for (int r = 0; r < RPT; ++r) {
+            float f = 0.f;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            st.nd[r][(J + 5 + 10 - k) % 10] += g.ws[0][k] * f;  // node m - 5 - k
-            st.nd[r][(J + 6 + k) % 10] -= g.ws[0][k] * f;       // node m - 4 + k
+            for (int k = 0; k < 5; ++k)
+                f += g.ws[0][k] * (st.uq[s][r][(J + 6 + k) % 10] - st.uq[s][r][(J + 15 - k) % 10]);
+            if (VARB) f *= st.hb[r][J % 5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                st.nd[s][r][(J + 5 + 10 - k) % 10] += g.ws[0][k] * f;  // node m - 5 - k
+                st.nd[s][r][(J + 6 + k) % 10] -= g.ws[0][k] * f;       // node m - 4 + k
+            }
         }
     }
     // node plane m - 9 is complete (stored only inside the chunk)
     {
-        const unsigned po = (unsigned)(p * a.plane_bytes);
+        const unsigned po = (unsigned)(p * a.plane_bytes), spo = (unsigned)(p * SPB(a));
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
-            const float u = st.uq[r][(J + 1) % 10];
-            float acc = st.Gv[S][r] * st.nd[r][(J + 1) % 10];
-            if (FORM == 2) acc += st.Cb[S][r] * (st.Lu[S][r] - st.Lo[S][r]);  // power 2: Cbeta (lap u - lap other)
-            const float out = FORM == 1 ? st.nd[r][(J + 1) % 10] : st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][r] + acc;
-            store_nt(out, rs.out, emit ? st.own[r] + po : kOOB);
-            if (FORM == 0 && a.kout) store_nt(st.nd[r][(J + 1) % 10], rs.Ko, emit ? st.own[r] + po : kOOB);
+            float out[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const float u = st.uq[s][r][(J + 1) % 10];
+                float acc = st.Gv[S][r] * st.nd[s][r][(J + 1) % 10];
+                if (FORM == 2) acc += st.Cb[S][r] * (st.Lu[S][r] - st.Lo[S][r]);  // power 2: Cbeta (lap u - lap other)
+                out[s] = FORM == 1 ? st.nd[s][r][(J + 1) % 10] : st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][s][r] + acc;
+            }
+            if (NS == 1) store_nt(out[0], rs.out, emit ? st.own[r] + spo : kOOB);
+            else store2_nt(out[0], out[NS - 1], rs.out, emit ? st.owns[r] + spo : kOOB);
+            if (FORM == 0 && NS == 1 && a.kout) store_nt(st.nd[0][r][(J + 1) % 10], rs.Ko, emit ? st.own[r] + po : kOOB);
             if (SAVE && emit) {
                 const int y = y0 + w * RPT + r, z = z0 + lane;
-                if (sa_inbox(a.box, p, y, z))
-                    a.save[sa_boxidx(a.box, p, y, z)] = a.save_mode == 1 ? (out - 2.f * u + st.ov[S][r]) * a.idt2 : u;
+                if (sa_inbox(a.box, p, y, z)) {
+                    const long long bi = sa_boxidx(a.box, p, y, z);
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) {
+                        const float u = st.uq[s][r][(J + 1) % 10];
+                        (s == 0 ? a.save : a.save1)[bi] =
+                            a.save_mode == 1 ? (out[s] - 2.f * u + st.ov[S][s][r]) * a.idt2 : u;
+                    }
+                }
             }
         }
     }
 #pragma unroll
-    for (int r = 0; r < RPT; ++r) st.nd[r][(J + 1) % 10] = 0.f;
+    for (int s = 0; s < NS; ++s) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) st.nd[s][r][(J + 1) % 10] = 0.f;
+    }
 }
 
-template <int RPT, int NW, bool VARB, bool SAVE, int FORM>
+template <int RPT, int NW, bool VARB, bool SAVE, int FORM, int NS>
 __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
-    using T = Tile<RPT, NW, VARB>;
+    using T = Tile<RPT, NW, VARB, NS>;
     __shared__ float4 sU4[T::UY * T::UZ / 4 + 1];  // + a dump slot for threads without a chunk
     __shared__ float4 sB4[VARB ? T::BY * T::BZ / 4 + 1 : 1];
-    __shared__ float sFy[T::FY * kTZ];
-    __shared__ float sFz[T::TY * T::FZP];
+    __shared__ float2 sFy2[T::FY * kTZ * NS / 2 + 1];
+    __shared__ float2 sFz2[T::TY * T::FZP * NS / 2 + 1];
     float *sU = reinterpret_cast<float *>(sU4), *sB = reinterpret_cast<float *>(sB4);
+    float *sFy = reinterpret_cast<float *>(sFy2), *sFz = reinterpret_cast<float *>(sFz2);
 
     // XCD-aware tile order: hardware block b runs on XCD b % 8; each XCD takes a contiguous
     // run of tiles (axis 2 fastest, then axis 1, then chunks) so halo re-reads hit its L2
@@ -248,13 +338,13 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
 
     Res rs;
-    rs.u = make(a.u, a.bytes);
-    rs.o = make(a.o, a.bytes);
-    rs.b = make(VARB ? a.b : a.u, a.bytes);
+    rs.u = make(a.u, NS == 1 ? a.bytes : a.sbytes);
+    rs.o = make(a.o, NS == 1 ? a.bytes : a.sbytes);
+    rs.b = make(VARB ? a.b : a.A, a.bytes);
     rs.A = make(a.A, a.bytes);
     rs.B = make(a.B, a.bytes);
     rs.G = make(a.G, a.bytes);
-    rs.out = make(a.out, a.bytes);
+    rs.out = make(a.out, NS == 1 ? a.bytes : a.sbytes);
     rs.Ko = make(a.kout ? a.kout : a.out, a.kout ? a.bytes : 0);
     if (FORM == 2) {
         rs.Cb = make(a.Cbeta, a.bytes);
@@ -262,26 +352,33 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
         rs.Lu = make(a.lapu, a.bytes);
     }
 
-    State<RPT, NW, VARB> st;
+    State<RPT, NW, VARB, NS> st;
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
 #pragma unroll
-        for (int k = 0; k < 10; ++k) st.nd[r][k] = st.uq[r][k] = 0.f;
+        for (int k = 0; k < 10; ++k) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) st.nd[s][r][k] = st.uq[s][r][k] = 0.f;
+        }
 #pragma unroll
         for (int k = 0; k < 5; ++k) st.hb[r][k] = 0.f;
         st.bprev[r] = 0.f;
         const int y = y0 + w * RPT + r, z = z0 + lane;
-        st.own[r] = (y < g.n[1] && z < g.n[2]) ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
+        const bool in = y < g.n[1] && z < g.n[2];
+        const long long cell = g.base + (long long)y * g.s1 + z;
+        st.own[r] = in ? (unsigned)(cell * 4) : kOOB;
+        if (NS > 1) st.owns[r] = in ? (unsigned)(cell * 4 * NS) : kOOB;
     }
-    // rows past the padded plane (y > n1 + 9) would read the next plane: they read zero
+    // rows past the padded plane (y > n1 + 9) would read the next plane: they read zero. A u
+    // chunk is 4 floats: 4 / NS cells of NS shots.
 #pragma unroll
     for (int i = 0; i < T::NLU; ++i) {
         const int e = t + T::NT * i;
         const int row = e / T::UC, c = e - row * T::UC;
-        const int y = y0 - 9 + row, z = z0 - 12 + 4 * c;
+        const int y = y0 - 9 + row, z = z0 - 12 + 4 * c / NS;
         const bool ok = e < T::UY * T::UC;
         st.ulds[i] = ok ? row * T::UZ + 4 * c : T::UY * T::UZ;
-        st.uoff[i] = ok && y <= g.n[1] + 9 ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
+        st.uoff[i] = ok && y <= g.n[1] + 9 ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4 * NS) : kOOB;
     }
 #pragma unroll
     for (int i = 0; i < T::NLB; ++i) {
@@ -294,14 +391,14 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     }
 
     const int mbeg = x0 - 9;
-    load_tile<0>(st, rs, (unsigned)(mbeg * a.plane_bytes));
-    load_tile<1>(st, rs, (unsigned)((mbeg + 1) * a.plane_bytes));
+    load_tile<0>(st, rs, (unsigned)(mbeg * a.plane_bytes), (unsigned)(mbeg * SPB(a)));
+    load_tile<1>(st, rs, (unsigned)((mbeg + 1) * a.plane_bytes), (unsigned)((mbeg + 1) * SPB(a)));
     // whole periods of 10 planes (compile-time ring slots); planes past x1 + 8 store nothing
     const int nper = (x1 - x0 + 18 + 9) / 10;
     for (int q = 0; q < nper; ++q) {
         const int m = mbeg + 10 * q;
 #define SA_RHO_PLANE(J) \
-    plane<RPT, NW, VARB, SAVE, FORM, J>(g, a, rs, st, sU, sB, sFy, sFz, m + J, x0, x1, y0, z0, lane, w)
+    plane<RPT, NW, VARB, SAVE, FORM, NS, J>(g, a, rs, st, sU, sB, sFy, sFz, m + J, x0, x1, y0, z0, lane, w)
         SA_RHO_PLANE(0);
         SA_RHO_PLANE(1);
         SA_RHO_PLANE(2);
@@ -317,22 +414,29 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
 }
 
 template <int RPT, int NW>
-int launch(const Geo &g, RhoArgs a, bool varb, bool save, hipStream_t stream) {
+int launch(const Geo &g, RhoArgs a, bool varb, bool save, int nshot, hipStream_t stream) {
     constexpr int TY = RPT * NW;
     a.nty = (g.n[1] + TY - 1) / TY;
     a.ntiles = a.ntz * a.nty * ((g.n[0] + a.chunk - 1) / a.chunk);
     a.per_xcd = (a.ntiles + 7) / 8;
     const dim3 grid(8 * a.per_xcd), block(NW * 64);
-    if (a.konly && varb) k_rho3d<RPT, NW, true, false, 1><<<grid, block, 0, stream>>>(g, a);
-    else if (a.konly) k_rho3d<RPT, NW, false, false, 1><<<grid, block, 0, stream>>>(g, a);
-    else if (a.Cbeta && varb && save) k_rho3d<RPT, NW, true, true, 2><<<grid, block, 0, stream>>>(g, a);
-    else if (a.Cbeta && varb) k_rho3d<RPT, NW, true, false, 2><<<grid, block, 0, stream>>>(g, a);
-    else if (a.Cbeta && save) k_rho3d<RPT, NW, false, true, 2><<<grid, block, 0, stream>>>(g, a);
-    else if (a.Cbeta) k_rho3d<RPT, NW, false, false, 2><<<grid, block, 0, stream>>>(g, a);
-    else if (varb && save) k_rho3d<RPT, NW, true, true, 0><<<grid, block, 0, stream>>>(g, a);
-    else if (varb) k_rho3d<RPT, NW, true, false, 0><<<grid, block, 0, stream>>>(g, a);
-    else if (save) k_rho3d<RPT, NW, false, true, 0><<<grid, block, 0, stream>>>(g, a);
-    else k_rho3d<RPT, NW, false, false, 0><<<grid, block, 0, stream>>>(g, a);
+    if (nshot == 2) {
+        // two shots: the update form only (no K u output, no power-2 term)
+        if (a.konly || a.kout || a.Cbeta) return -1;
+        if (varb && save) k_rho3d<RPT, NW, true, true, 0, 2><<<grid, block, 0, stream>>>(g, a);
+        else if (varb) k_rho3d<RPT, NW, true, false, 0, 2><<<grid, block, 0, stream>>>(g, a);
+        else if (save) k_rho3d<RPT, NW, false, true, 0, 2><<<grid, block, 0, stream>>>(g, a);
+        else k_rho3d<RPT, NW, false, false, 0, 2><<<grid, block, 0, stream>>>(g, a);
+    } else if (a.konly && varb) k_rho3d<RPT, NW, true, false, 1, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.konly) k_rho3d<RPT, NW, false, false, 1, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cbeta && varb && save) k_rho3d<RPT, NW, true, true, 2, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cbeta && varb) k_rho3d<RPT, NW, true, false, 2, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cbeta && save) k_rho3d<RPT, NW, false, true, 2, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cbeta) k_rho3d<RPT, NW, false, false, 2, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (varb && save) k_rho3d<RPT, NW, true, true, 0, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (varb) k_rho3d<RPT, NW, true, false, 0, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (save) k_rho3d<RPT, NW, false, true, 0, 1><<<grid, block, 0, stream>>>(g, a);
+    else k_rho3d<RPT, NW, false, false, 0, 1><<<grid, block, 0, stream>>>(g, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -353,6 +457,7 @@ struct GradArgs {
     float *gv, *pv, *gr, *pr;     // accumulators (padded layout); gv / pv null: no vp gradient
     long long bytes, sbytes;
     int plane_bytes, splane_bytes, srow;  // srow: snapshot row pitch in floats
+    int pa_nshot, pa_shot;                // pa holds pa_nshot interleaved shots; this is shot pa_shot
     float fdt2;
     int lo[3], hi[3];  // interior [lo, hi)
     int chunk, ntz, nty, ntiles, per_xcd;
@@ -373,6 +478,7 @@ struct GState {
     unsigned poff[T::NL], boff[T::NL];
     int lds[T::NL];
     unsigned own, sown;  // own cell: padded-layout and snapshot byte offsets within a plane
+    unsigned aown;       // own cell's byte offset in pa (its shot of the interleaved shots)
     float pa[2], pm[2], pp[2], gv[2], pv[2], gr[2], prr[2];
 };
 
@@ -394,7 +500,7 @@ __device__ __forceinline__ void gload_tile(GState<NW> &st, const GRes &rs, const
 template <int S, int NW>
 __device__ __forceinline__ void gload_own(GState<NW> &st, const GRes &rs, const GradArgs &a, int p) {
     const unsigned po = st.own + (unsigned)(p * a.plane_bytes), so = st.sown + (unsigned)(p * a.splane_bytes);
-    st.pa[S] = load_nt(rs.pa, po);
+    st.pa[S] = load_nt(rs.pa, st.aown + (unsigned)(p * a.plane_bytes * a.pa_nshot));
     st.pm[S] = load_nt(rs.psm, so);
     st.pp[S] = load_nt(rs.psp, so);
     st.gr[S] = load_nt(rs.gr, po);
@@ -496,7 +602,7 @@ __global__ __launch_bounds__(NW * 64) void k_grad_density(Geo g, GradArgs a) {
     rs.psm = make(a.psm ? a.psm : a.ps, a.psm ? a.sbytes : 0);  // null: every read is zero
     rs.psp = make(a.psp ? a.psp : a.ps, a.psp ? a.sbytes : 0);
     rs.b = make(a.b, a.bytes);
-    rs.pa = make(a.pa, a.bytes);
+    rs.pa = make(a.pa, a.bytes * a.pa_nshot);
     rs.gr = make(a.gr, a.bytes);
     rs.pr = make(a.pr, a.bytes);
     rs.gv = make(a.gv ? a.gv : a.gr, a.gv ? a.bytes : 0);
@@ -507,7 +613,9 @@ __global__ __launch_bounds__(NW * 64) void k_grad_density(Geo g, GradArgs a) {
     {
         const int y = y0 + w, z = z0 + lane;
         const bool ok = y < a.hi[1] && z < a.hi[2];
-        st.own = ok ? (unsigned)((g.base + (long long)y * g.s1 + z) * 4) : kOOB;
+        const long long cell = g.base + (long long)y * g.s1 + z;
+        st.own = ok ? (unsigned)(cell * 4) : kOOB;
+        st.aown = ok ? (unsigned)((cell * a.pa_nshot + a.pa_shot) * 4) : kOOB;
         st.sown = ok ? (unsigned)(((long long)y * a.srow + z) * 4) : kOOB;
     }
 #pragma unroll
@@ -736,10 +844,10 @@ __global__ __launch_bounds__(NW * 64) void k_exact_acc_density(Geo g, XAccArgs a
 
 }  // namespace
 
-bool rho3d_supported(const Geo &g) {
+bool rho3d_supported(const Geo &g, int nshot) {
     if (!(g.act[0] && g.act[1] && g.act[2])) return false;
     // every byte offset (plus the plane offsets of the lead-in/out planes) stays below kOOB
-    return g.total * 4 + 32 * g.s0 * 4 < (long long)bufio::kOOB;
+    return (g.total * 4 + 32 * g.s0 * 4) * nshot < (long long)bufio::kOOB;
 }
 
 bool rho3d_valid_tune(int rows, int chunk, int waves) {
@@ -757,8 +865,13 @@ int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_
     a.G = st.G;
     a.out = st.out;
     a.save = st.save;
+    a.save1 = st.save1;
+    if (st.nshot != 1 && st.nshot != 2) return -1;
+    if (st.nshot == 2 && st.save && !st.save1) return -1;
     a.bytes = g.total * 4;
+    a.sbytes = a.bytes * st.nshot;
     a.plane_bytes = (int)(g.s0 * 4);
+    a.splane_bytes = a.plane_bytes * st.nshot;
     a.chunk = t.chunk;
     a.ntz = (g.n[2] + kTZ - 1) / kTZ;
     a.save_mode = st.save_mode;
@@ -773,11 +886,12 @@ int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_
     if (st.konly && (st.save || st.Cbeta)) return -1;
     if (st.Cbeta && (!st.lapother || !st.lapu)) return -1;
     const bool varb = st.b != nullptr, save = st.save != nullptr;
-    if (t.waves == 4 && t.rows == 1) return launch<1, 4>(g, a, varb, save, stream);
-    if (t.waves == 4 && t.rows == 2) return launch<2, 4>(g, a, varb, save, stream);
-    if (t.waves == 8 && t.rows == 1) return launch<1, 8>(g, a, varb, save, stream);
-    if (t.waves == 8 && t.rows == 2) return launch<2, 8>(g, a, varb, save, stream);
-    if (t.waves == 16 && t.rows == 1) return launch<1, 16>(g, a, varb, save, stream);
+    const int ns = st.nshot;
+    if (t.waves == 4 && t.rows == 1) return launch<1, 4>(g, a, varb, save, ns, stream);
+    if (t.waves == 4 && t.rows == 2) return launch<2, 4>(g, a, varb, save, ns, stream);
+    if (t.waves == 8 && t.rows == 1) return launch<1, 8>(g, a, varb, save, ns, stream);
+    if (t.waves == 8 && t.rows == 2) return launch<2, 8>(g, a, varb, save, ns, stream);
+    if (t.waves == 16 && t.rows == 1) return launch<1, 16>(g, a, varb, save, ns, stream);
     return -1;
 }
 
@@ -790,9 +904,14 @@ bool rho3d_grad_supported(const Geo &g, const Box &box) {
 }
 
 int rho3d_grad(const Geo &g, const Box &box, const float *ps, const float *psm, const float *psp, float fdt2,
-               const float *b, const float *pa, float *gv, float *pv, float *gr, float *pr, hipStream_t stream) {
+               const float *b, const float *pa, float *gv, float *pv, float *gr, float *pr, hipStream_t stream,
+               int pa_nshot, int pa_shot) {
     constexpr int NW = 8;
+    if (pa_nshot < 1 || pa_nshot > 2 || pa_shot < 0 || pa_shot >= pa_nshot || !rho3d_supported(g, pa_nshot))
+        return -1;
     GradArgs a;
+    a.pa_nshot = pa_nshot;
+    a.pa_shot = pa_shot;
     a.ps = ps;
     a.psm = psm;
     a.psp = psp;

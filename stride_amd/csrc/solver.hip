@@ -752,6 +752,7 @@ struct sa_solver {
     // fast 3D path
     Fast3D fast;
     Rho3DTune rho3d;  // fused div(b grad) update (3D, OT2, no PML, no power-2 attenuation)
+    Rho3DTune rho3d_b2 = {1, 60, 4, true};  // its two-shot form (sa_batch_*): rows 1 only
     // two shots per pass (sa_batch_*): interleaved levels, per-shot taps, interleaved snapshots
     int nshot = 0;
     float *bbuf[2] = {};
@@ -1212,6 +1213,11 @@ static bool use_fast(const sa_solver *s) {
 
 // the two-shot and multi-shot forms of the specialised update: no attenuation
 static bool use_fast_pairs(const sa_solver *s) { return use_fast(s) && s->sch.att_power == -2; }
+
+// the two-shot form of the fused density update: no power-2 attenuation
+static bool use_rho3d_pairs(const sa_solver *s) {
+    return use_rho3d(s) && s->sch.att_power != 2 && rho3d_supported(s->geo, 2);
+}
 
 // build per-point coefficients and boundary state for the current scheme + medium
 static int prepare(sa_solver *s) {
@@ -2523,6 +2529,16 @@ extern "C" int sa_set_tile_density(sa_solver *s, int rows, int chunk, int waves)
     return SA_OK;
 }
 
+extern "C" int sa_batch_set_tile_density(sa_solver *s, int rows, int chunk, int waves) {
+    if (!s) return fail(SA_E_ARG, "null solver");
+    if (rows != 1 || !rho3d_valid_tune(rows, chunk, waves) || waves == 16)
+        return fail(SA_E_ARG, "two-shot density tile: rows 1, 4 or 8 waves, chunk >= 1");
+    s->rho3d_b2.rows = rows;
+    s->rho3d_b2.chunk = chunk;
+    s->rho3d_b2.waves = waves;
+    return SA_OK;
+}
+
 extern "C" int sa_timing_enable(sa_solver *s, int enable) {
     if (!s) return fail(SA_E_ARG, "null solver");
     TRY(drain_timing(s));
@@ -2594,9 +2610,15 @@ extern "C" int sa_synchronize(sa_solver *s) {
 // read of vp, and the adjoint accumulates the sum of the two shots' vp gradients and
 // preconditioners (the quantity an FWI iteration uses) with one read-modify-write of each.
 
+// Two-shot runs: the specialised update (no density; save mode 0 or 1, the interior p.dt2) or
+// the fused density update (save mode 0 or 2, the raw wavefield of the whole domain).
+static bool batch_density(const sa_solver *s) { return s->sch.rho_mode != 0; }
+
 static bool batch_ok(const sa_solver *s) {
     const sa_scheme &c = s->sch;
-    return use_fast_pairs(s) && (c.save_mode == 0 || c.save_mode == 1) && s->nshot == 2;
+    if (s->nshot != 2) return false;
+    if (batch_density(s)) return use_rho3d_pairs(s) && (c.save_mode == 0 || c.save_mode == 2);
+    return use_fast_pairs(s) && (c.save_mode == 0 || c.save_mode == 1);
 }
 
 extern "C" int sa_batch_create(sa_solver *s, int nshot) {
@@ -2626,14 +2648,18 @@ extern "C" int sa_batch_set_tile(sa_solver *s, int rows, int chunk, int waves) {
     return SA_OK;
 }
 
+// Snapshot t / f of a batch starts at bsaved + (t / f) * 2 * box_points(bbox). Save mode 1: the
+// interior p.dt2 of the two shots interleaved per cell; save mode 2: the raw wavefield of the
+// whole domain, shot 0's snapshot followed by shot 1's.
 static int batch_alloc_saved(sa_solver *s) {
     const sa_scheme &c = s->sch;
     const Geo &g = s->geo;
     const int f = c.undersampling;
     Box b;
     for (int a = 0; a < 3; ++a) {
-        b.lo[a] = g.nabs[a];
-        b.n[a] = g.n[a] - 2 * g.nabs[a];
+        const bool interior = c.save_mode == 1;
+        b.lo[a] = interior ? g.nabs[a] : 0;
+        b.n[a] = interior ? g.n[a] - 2 * g.nabs[a] : g.n[a];
     }
     b.off2 = b.lo[2] % 32;
     b.p2 = (b.off2 + b.n[2] + 31) / 32 * 32;
@@ -2673,16 +2699,18 @@ extern "C" int sa_batch_forward_dev(sa_solver *s, const float *const *src, float
     HIP_TRY(hipSetDevice(s->device));
     TRY(prepare(s));
     if (!batch_ok(s))
-        return fail(SA_E_UNSUPPORTED, "two-shot runs need sa_batch_create and the specialised 3D scheme (OT2, no "
-                                      "density, sponge or no boundary, no attenuation, save mode 0 or 1)");
+        return fail(SA_E_UNSUPPORTED, "two-shot runs need sa_batch_create and a 3D OT2 scheme with sponge or no "
+                                      "boundary: without density, no attenuation and save mode 0 or 1; with "
+                                      "density, no power-2 attenuation and save mode 0 or 2");
     const sa_scheme &c = s->sch;
+    const bool dens = batch_density(s);
     for (int k = 0; k < 2; ++k) {
         TRY(coef_sparse(s, s->bsrc[k], 0));
         TRY(zero_field2(s, s->bbuf[k]));
         if (s->brec[k].npoint > 0)
             HIP_TRY(hipMemsetAsync(rec[k], 0, sizeof(float) * (size_t)s->grid.nt * s->brec[k].npoint, s->stream));
     }
-    if (c.save_mode == 1) TRY(batch_alloc_saved(s));
+    if (c.save_mode != 0) TRY(batch_alloc_saved(s));
     const int f = c.undersampling;
     const size_t per = 2 * box_points(s->bbox);
     const float idt2 = 1.f / (s->dt * s->dt);
@@ -2690,15 +2718,35 @@ extern "C" int sa_batch_forward_dev(sa_solver *s, const float *const *src, float
     for (int t = 1; t <= tlast; ++t) {
         float *u = s->bbuf[t % 2], *o = s->bbuf[(t + 1) % 2];
         float *save = nullptr;
-        if (c.save_mode == 1 && t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->bnsave)
+        if (c.save_mode != 0 && t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->bnsave)
             save = s->bsaved + (size_t)(t / f) * per;
         hipEvent_t e0;
         TRY(timed_begin(s, &e0));
-        TRY(fast3d_step_b2(s->fast, s->geo, u, o, o, save, s->bbox, idt2, s->stream));
+        if (dens) {
+            Rho3DStep st;
+            st.u = u;
+            st.other = o;
+            st.out = o;
+            st.b = c.rho_mode == 2 ? s->b : nullptr;
+            st.A = s->A;
+            st.B = s->B;
+            st.G = s->G;
+            st.save = save;
+            st.save1 = save ? save + box_points(s->bbox) : nullptr;
+            st.save_mode = 2;
+            st.box = s->bbox;
+            st.idt2 = idt2;
+            st.nshot = 2;
+            if (rho3d_step(s->rho3d_b2, s->geo, st, s->stream) != 0)
+                return fail(SA_E_HIP, "two-shot density update launch: %s", hipGetErrorString(hipGetLastError()));
+        } else {
+            TRY(fast3d_step_b2(s->fast, s->geo, u, o, o, save, s->bbox, idt2, s->stream));
+        }
         TRY(timed_end(s));
         const float *const sv[2] = {src[0] + (size_t)t * s->bsrc[0].npoint, src[1] + (size_t)t * s->bsrc[1].npoint};
         float *const rv[2] = {rec[0] + (size_t)t * s->brec[0].npoint, rec[1] + (size_t)t * s->brec[1].npoint};
-        TRY(inject2(s, s->bsrc, sv, 1.f, o, save, idt2, s->bbox));
+        // the p.dt2 save takes the injected term too; the raw save holds u, before this injection
+        TRY(inject2(s, s->bsrc, sv, 1.f, o, dens ? nullptr : save, idt2, s->bbox));
         TRY(gather2(s, s->brec, o, 1.f, rv));
     }
     s->cur = (tlast + 1) % 2;
@@ -2711,17 +2759,24 @@ extern "C" int sa_batch_adjoint_dev(sa_solver *s, const float *const *res) {
     if (!s || !res) return fail(SA_E_ARG, "null argument");
     HIP_TRY(hipSetDevice(s->device));
     TRY(prepare(s));
-    if (!batch_ok(s)) return fail(SA_E_UNSUPPORTED, "two-shot runs need sa_batch_create and the specialised 3D scheme");
-    if (!s->bsaved || s->sch.save_mode != 1)
-        return fail(SA_E_STATE, "the two-shot adjoint needs a two-shot forward run with save mode 1");
+    if (!batch_ok(s)) return fail(SA_E_UNSUPPORTED, "two-shot runs need sa_batch_create and a two-shot scheme");
     const sa_scheme &c = s->sch;
-    TRY(grad_outputs(s, 2, s->gacc));
+    const bool dens = batch_density(s);
+    if (!s->bsaved || (c.save_mode != 1 && c.save_mode != 2))
+        return fail(SA_E_STATE, "the two-shot adjoint needs a two-shot forward run with save mode 1 (no density) "
+                                "or 2 (density)");
+    if (dens && !(c.rho_mode == 2 && s->b && rho3d_grad_supported(s->geo, s->bbox)))
+        return fail(SA_E_UNSUPPORTED, "the two-shot density adjoint needs variable density and absorbing layers "
+                                      "of at least 8 cells");
+    // vp gradient and preconditioner (both schemes), rho gradient and preconditioner (density)
+    TRY(grad_outputs(s, dens ? 4 : 2, s->gacc));
     for (int k = 0; k < 2; ++k) {
         TRY(coef_sparse(s, s->brec[k], 1));
         TRY(zero_field2(s, s->bbuf[k]));
     }
     const int f = s->bf;
-    const size_t per = 2 * box_points(s->bbox);
+    const size_t per = 2 * box_points(s->bbox), one = box_points(s->bbox);
+    const float fdt2 = (float)(1.0 / ((double)f * s->grid.dt * (double)f * s->grid.dt));
     const int tstart = c.tb1 - 1 - f, tend = c.tb0 + 1;
     for (int t = tstart; t >= tend; --t) {
         float *u = s->bbuf[t % 2], *o = s->bbuf[(t + 1) % 2];
@@ -2729,10 +2784,41 @@ extern "C" int sa_batch_adjoint_dev(sa_solver *s, const float *const *res) {
         const float *ps = guard ? s->bsaved + (size_t)(t / f) * per : nullptr;
         hipEvent_t e0;
         TRY(timed_begin(s, &e0));
-        TRY(fast3d_step_adjoint_b2(s->fast, s->geo, u, o, o, ps, s->bbox, s->gacc[0], s->gacc[1], s->stream));
+        if (dens) {
+            Rho3DStep st;
+            st.u = u;
+            st.other = o;
+            st.out = o;
+            st.b = s->b;
+            st.A = s->A;
+            st.B = s->B;
+            st.G = s->G;
+            st.save = nullptr;
+            st.save_mode = 0;
+            st.box = s->bbox;
+            st.idt2 = 0.f;
+            st.nshot = 2;
+            if (rho3d_step(s->rho3d_b2, s->geo, st, s->stream) != 0)
+                return fail(SA_E_HIP, "two-shot density update launch: %s", hipGetErrorString(hipGetLastError()));
+        } else {
+            TRY(fast3d_step_adjoint_b2(s->fast, s->geo, u, o, o, ps, s->bbox, s->gacc[0], s->gacc[1], s->stream));
+        }
         TRY(timed_end(s));
         const float *const rv[2] = {res[0] + (size_t)t * s->brec[0].npoint, res[1] + (size_t)t * s->brec[1].npoint};
         TRY(inject2(s, s->brec, rv, -1.f, o, nullptr, 0.f, s->bbox));
+        if (dens && guard) {
+            // each shot's vp and rho terms from its raw snapshots t - f, t, t + f and its own
+            // adjoint level (shot k of u), summed into the shared accumulators
+            const int i = t / f;
+            for (int k = 0; k < 2; ++k) {
+                const float *psk = ps + k * one;
+                const float *psm = i >= 1 ? s->bsaved + (size_t)(i - 1) * per + k * one : nullptr;
+                const float *psp = i + 1 < s->bnsave ? s->bsaved + (size_t)(i + 1) * per + k * one : nullptr;
+                if (rho3d_grad(s->geo, s->bbox, psk, psm, psp, fdt2, s->b, u, s->gacc[0], s->gacc[1], s->gacc[2],
+                               s->gacc[3], s->stream, 2, k) != 0)
+                    return fail(SA_E_HIP, "density gradient launch: %s", hipGetErrorString(hipGetLastError()));
+            }
+        }
     }
     s->cur = (tend - 1 + 2) % 2;
     return SA_OK;
