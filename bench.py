@@ -731,10 +731,11 @@ def bench_gradient_2d(shots=8, workers=2, with_cpu=True):
     """The 2D gradient workload: 512^2 inner (572^2 extended), dx 0.2 mm, CFS-PML 30, variable vp /
     rho, alpha 0.5 with power 2, 128 ring receivers, nt 2500. Each shot: forward with the raw
     wavefield saved every f-th step, then the as-written adjoint with the vp and rho gradients
-    (sources at ring elements 0, 16, 32, ...). The update runs as the fused 2D kernel; the
-    Laplacian of the replaced level (power 2), injection and interpolation are launches of their
-    own. The shots run on ``workers`` solvers, each on its own stream and host thread (as that
-    many Mosaic workers sharing the GPU would); the line also gives the rate of one worker."""
+    (sources at ring elements 0, 16, 32, ...). The update runs as the fused 2D kernel (injection,
+    interpolation, the power-2 Laplacian and the gradients folded in). `value`: the ``shots`` shots
+    advanced in lockstep, one solver per shot and one launch per time step for all of them; the
+    line also gives the rate of the same shots on ``workers`` solvers, each on its own stream and
+    host thread (as that many Mosaic workers sharing the GPU would), and on one worker."""
     from stride_amd.configs import config2
     from stride_amd.workers import ShotWorkers
     from stride_amd.physics.plan import build_plan
@@ -805,11 +806,42 @@ def bench_gradient_2d(shots=8, workers=2, with_cpu=True):
     for sv in pool.solvers:
         release(sv)
     pool.close()
+    # the same shots in lockstep: one solver per shot, one launch per time step for all of them
+    from stride_amd._native import shots2d_forward_dev, shots2d_adjoint_dev
+    group = []
+    for k in range(shots):
+        sv = make_solver()
+        sv.set_sources(plans[k].src_idx, plans[k].src_w)
+        group.append(sv)
+    gbuf = [(sv._to_device(plans[k].src_data), sv.device_alloc(p.nt * p.nrec * 4), sv._to_device(res))
+            for k, sv in enumerate(group)]
+
+    def run_group():
+        shots2d_forward_dev(group, [b[0] for b in gbuf], [b[1] for b in gbuf])
+        shots2d_adjoint_dev(group, [b[2] for b in gbuf], 3)
+        group[0].synchronize()
+
+    run_group()  # allocations
+    t3 = time.perf_counter()
+    run_group()
+    wall_ls = time.perf_counter() - t3
+    group[0].timing(True)
+    run_group()
+    ms_ls, launches_ls = group[0].timing()
+    group[0].timing(False)
+    for sv, b in zip(group, gbuf):
+        for d in b:
+            sv.device_free(d)
+        sv.close()
     n = int(np.prod(p.shape))
     cpu = cpu_baseline_2d(cfg) if with_cpu else None
     return dict(metric='Gpts/s fwd+adjoint 2D 512^2 SO10, CFS-PML, variable vp/rho, power-2 attenuation, '
-                       'vp+rho gradients', value=round(launches * n / wall / 1e9, 3), unit='Gpts/s',
-                seconds_per_shot=round(wall / shots, 4), workers=workers,
+                       'vp+rho gradients', value=round(launches * n / wall_ls / 1e9, 3), unit='Gpts/s',
+                seconds_per_shot=round(wall_ls / shots, 4),
+                lockstep={'shots_per_launch': shots, 'launches': int(launches_ls),
+                          'launch_us': round(ms_ls * 1e3 / max(launches_ls, 1), 2)},
+                value_workers=round(launches * n / wall / 1e9, 3), seconds_per_shot_workers=round(wall / shots, 4),
+                workers=workers,
                 value_one_worker=round(launches * n / wall1 / 1e9, 3),
                 seconds_per_shot_one_worker=round(wall1 / shots, 4), shots=shots, update_launches=int(launches),
                 update_kernel_us=round(ms * 1e3 / launches, 2),
@@ -817,8 +849,9 @@ def bench_gradient_2d(shots=8, workers=2, with_cpu=True):
                 cpu_baseline=cpu, dtype='f32', data='synthetic',
                 config={'workload': '2D 512x512 inner (572x572 extended), dx 0.2 mm, SO10 OT2, CFS-PML 30, '
                                     'vp 1500 + 150 N1 in a 45 mm disc, rho 1000 + 50 N2, alpha 0.5 (power 2), '
-                                    '128-element 48 mm ring, nt %d, %d shots, %d workers (solvers with their own '
-                                    'streams and host threads)' % (p.nt, shots, workers)})
+                                    '128-element 48 mm ring, nt %d, %d shots advanced in lockstep (one launch '
+                                    'per time step for all shots); also on %d workers (solvers with their own '
+                                    'streams and host threads) and on one' % (p.nt, shots, workers)})
 
 
 def bench_forward_2d(reps=20):

@@ -238,6 +238,20 @@ int sa_batch_set_tile(sa_solver *s, int rows, int chunk, int waves);
  * waves 4) */
 int sa_batch_set_tile_density(sa_solver *s, int rows, int chunk, int waves);
 
+/* Several 2D shots per launch (lockstep). No counterpart in the reference, which runs one shot
+ * per worker call (IsoAcousticDevito.forward / adjoint, devito.py:515-581, 875-920; the shots of
+ * an iteration are spread over workers by stride/__init__.py:117-146). Shots on the same 2D grid
+ * and scheme (fused 2D update: OT2, any density form, sponge or CFS-PML, any attenuation; save
+ * mode 0, 1 or 2; no wavefield dumps), each set up on its own solver v[k] (medium, taps,
+ * nt), advance together: one launch per time step covers every shot's tiles. Each shot's traces,
+ * saved wavefield and gradients are those of its one-shot run (sa_forward_dev / sa_adjoint_dev
+ * on v[k]), and stay with v[k]. The work runs on v[0]'s stream after what is queued on each
+ * solver's stream; each solver's stream waits for it. */
+/* src[k], rec[k]: device (nt, nsrc_k) samples and (nt, nrec_k) traces of shot k */
+int sa_shots2d_forward_dev(sa_solver *const *v, int n, const float *const *src, float *const *rec);
+/* res[k]: device (nt, nrec_k) adjoint source of shot k; want: SA_WANT_VP | SA_WANT_RHO */
+int sa_shots2d_adjoint_dev(sa_solver *const *v, int n, const float *const *res, int want);
+
 /* Many shots of uneven duration and receiver count, forward only. No counterpart in the
  * reference, which runs one shot per worker call (IsoAcousticDevito.forward,
  * stride/physics/iso_acoustic/devito.py:515-581) over the shots of an iteration

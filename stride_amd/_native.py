@@ -46,7 +46,8 @@ EXPORTS = ['sa_last_error', 'sa_version', 'sa_create', 'sa_destroy', 'sa_set_str
            'sa_timing_enable', 'sa_timing_read', 'sa_stream_create', 'sa_stream_destroy',
            'sa_device_alloc', 'sa_device_free', 'sa_memcpy', 'sa_synchronize', 'sa_batch_create',
            'sa_batch_set_taps', 'sa_batch_forward_dev', 'sa_batch_adjoint_dev', 'sa_batch_get_wavefield',
-           'sa_batch_set_tile', 'sa_batch_set_tile_density', 'sa_traces_check', 'sa_traces_mute', 'sa_traces_fir', 'sa_traces_sosfilt',
+           'sa_batch_set_tile', 'sa_batch_set_tile_density', 'sa_shots2d_forward_dev', 'sa_shots2d_adjoint_dev',
+           'sa_traces_check', 'sa_traces_mute', 'sa_traces_fir', 'sa_traces_sosfilt',
            'sa_traces_norm', 'sa_traces_l2', 'sa_traces_l2_noalloc', 'sa_multi_create', 'sa_multi_set_shot',
            'sa_multi_forward_dev', 'sa_multi_get_wavefield', 'sa_multi_graph_ms', 'sa_nn_conv3x3',
            'sa_nn_flip_weights', 'sa_nn_wgrad3x3', 'sa_nn_bias_grad', 'sa_nn_avgpool', 'sa_nn_avgpool_back',
@@ -546,3 +547,66 @@ class Solver:
 
     def synchronize(self):
         self._check(self._lib.sa_synchronize(self._h))
+
+
+# ---- several 2D shots per launch (lockstep; one solver per shot) ----
+def _handles(solvers):
+    return (ctypes.c_void_p * len(solvers))(*[s._h.value for s in solvers])
+
+
+def shots2d_forward_dev(solvers, src_ptrs, rec_ptrs):
+    """Forward runs of the 2D shots held by ``solvers`` (same grid and scheme), one launch per
+    time step for all of them: device source samples / traces per shot."""
+    s0 = solvers[0]
+    n = len(solvers)
+    s0._check(s0._lib.sa_shots2d_forward_dev(_handles(solvers), ctypes.c_int(n), (ctypes.c_void_p * n)(*src_ptrs),
+                                             (ctypes.c_void_p * n)(*rec_ptrs)))
+
+
+def shots2d_adjoint_dev(solvers, res_ptrs, want=1):
+    """As-written adjoint runs of the 2D shots (after shots2d_forward_dev or one-shot forward runs
+    with saves), one launch per time step; each solver keeps its own shot's gradients."""
+    s0 = solvers[0]
+    n = len(solvers)
+    s0._check(s0._lib.sa_shots2d_adjoint_dev(_handles(solvers), ctypes.c_int(n), (ctypes.c_void_p * n)(*res_ptrs),
+                                             ctypes.c_int(want)))
+
+
+def shots2d_forward(solvers, srcs):
+    """Host convenience: source samples (nt, nsrc_k) per shot -> traces (nt, nrec_k) per shot."""
+    s0 = solvers[0]
+    ds = [s._to_device(x) for s, x in zip(solvers, srcs)]
+    recs = [np.zeros((s.nt, s.nrec), np.float32) for s in solvers]
+    dr = [s0.device_alloc(max(r.nbytes, 4)) for r in recs]
+    try:
+        shots2d_forward_dev(solvers, ds, dr)
+        s0.synchronize()
+        for r, d in zip(recs, dr):
+            s0.memcpy(r.ctypes.data, d, r.nbytes, 2)
+    finally:
+        for s, d in zip(solvers, ds):
+            s.device_free(d)
+        for d in dr:
+            s0.device_free(d)
+    return recs
+
+
+def shots2d_adjoint(solvers, residuals, want_vp=True, want_rho=False):
+    """Host convenience: adjoint sources (nt, nrec_k) per shot -> per-shot gradient dicts."""
+    s0 = solvers[0]
+    dres = [s._to_device(r) for s, r in zip(solvers, residuals)]
+    try:
+        shots2d_adjoint_dev(solvers, dres, (1 if want_vp else 0) | (2 if want_rho else 0))
+        s0.synchronize()
+    finally:
+        for s, d in zip(solvers, dres):
+            s.device_free(d)
+    out = []
+    for s in solvers:
+        g = {}
+        if want_vp:
+            g.update(grad_vp=s.read_gradient(0), prec_vp=s.read_gradient(1))
+        if want_rho:
+            g.update(grad_rho=s.read_gradient(2), prec_rho=s.read_gradient(3))
+        out.append(g)
+    return out

@@ -53,3 +53,8 @@ struct Flat2DStep {
 // true for the 2D padded layout within the kernel's 32-bit buffer offsets
 bool flat2d_supported(const Geo &g);
 int flat2d_step(const Geo &g, int rho_mode, const Flat2DStep &x, hipStream_t stream);
+// Several shots on one grid per launch (workgroups of shot k along grid axis z): shot k's
+// arguments are tab[k] (device memory, read by the kernel); `first` is the host copy of tab[0],
+// whose options (PML, power 2, save) select the kernel and must be those of every shot.
+int flat2d_step_shots(const Geo &g, int rho_mode, const Flat2DStep &first, const Flat2DStep *tab, int nshot,
+                      hipStream_t stream);

@@ -759,6 +759,11 @@ struct sa_solver {
     Sparse bsrc[2], brec[2];
     float *bsaved = nullptr;
     size_t bsaved_bytes = 0;
+    // lockstep 2D shots (sa_shots2d_*, as the first solver of a group): the per-step, per-shot
+    // argument table (pinned host copy and device copy) and the event after its upload
+    Flat2DStep *tab_h = nullptr, *tab_d = nullptr;
+    size_t tab_cap = 0;
+    hipEvent_t tab_ev = nullptr;
     int bnsave = 0, bf = 0;
     Box bbox;
     // many shots of uneven duration (sa_multi_*): shots paired longest first, pair j holding
@@ -956,6 +961,9 @@ extern "C" int sa_destroy(sa_solver *s) {
         dfree(s->bbuf[k]);
     }
     dfree(s->bsaved);
+    if (s->tab_h) (void)hipHostFree(s->tab_h);
+    if (s->tab_d) (void)hipFree(s->tab_d);
+    if (s->tab_ev) (void)hipEventDestroy(s->tab_ev);
     for (auto &m : s->mshot) {
         free_sparse(m.src);
         free_sparse(m.rec);
@@ -1400,6 +1408,85 @@ static int tile_points(sa_solver *s, Sparse &sp) {
     return SA_OK;
 }
 
+// The fused 2D update's arguments for one step (the one-shot loops and the lockstep shots build
+// them alike). With power-2 attenuation the launch reads lap(other) from lapother (the caller
+// forms it at the first step of a run) and writes lap(u) into lapnext; flat2d_lap_advance then
+// makes it the next step's lapother.
+static int flat2d_args(sa_solver *s, const float *u, const float *other, float *out, int backward, float *save,
+                       int save_mode, const InjectSpec *inj, const GatherSpec *gat, const GradSpec *grd,
+                       Flat2DStep &x) {
+    const sa_scheme &c = s->sch;
+    memset(&x, 0, sizeof(x));
+    x.u = u;
+    x.other = other;
+    x.out = out;
+    x.b = c.rho_mode == 2 ? s->b : nullptr;
+    x.A = s->A;
+    x.B = s->B;
+    x.G = s->G;
+    if (c.att_power == 2) {
+        x.Cbeta = s->Cbeta;
+        x.lapother = s->lapother;
+        x.lapout = s->lapnext;
+    }
+    if (c.boundary == 1) {
+        x.Cpml = s->Cpml;
+        for (int a = 0; a < 3; ++a) {
+            x.u1[a] = s->u1[a];
+            x.u2[a] = s->u2[a];
+            x.u3[a] = s->u3[a];
+            for (int r = 0; r < 4; ++r) x.pml[a][r] = s->pml[a][r];
+        }
+    }
+    x.dt = s->dt;
+    x.backward = backward;
+    x.save = save;
+    x.save_mode = save_mode;
+    x.box = s->box;
+    x.idt2 = 1.f / (s->dt * s->dt);
+    if (inj && inj->sp->ndest > 0) {
+        TRY(dest_map(s, *inj->sp));
+        x.imap = inj->sp->d_map;
+        x.ioff = inj->sp->d_off;
+        x.ipoint = inj->sp->d_point;
+        x.icoef = inj->sp->d_coef;
+        x.ivalues = inj->values;
+        x.isign = inj->sign;
+        x.isave = inj->into_save;
+    }
+    if (gat && gat->sp->npoint > 0) {
+        TRY(tile_points(s, *gat->sp));
+        x.goff = gat->sp->d_toff;
+        x.gpt = gat->sp->d_tpt;
+        x.gidx = gat->sp->d_idx;
+        x.gw = gat->sp->d_w;
+        x.gntap = gat->sp->ntap;
+        x.gsign = gat->sign;
+        x.gout = gat->out;
+    }
+    if (grd) {
+        x.gps = grd->ps;
+        x.gpsm = grd->psm;
+        x.gpsp = grd->psp;
+        x.gfdt2 = grd->fdt2;
+        x.graw = grd->raw;
+        x.ggv = grd->gv;
+        x.gpv = grd->pv;
+        x.ggr = grd->gr;
+        x.gpr = grd->pr;
+    }
+    return SA_OK;
+}
+
+static void flat2d_lap_advance(sa_solver *s, const Flat2DStep &x, const float *u) {
+    if (x.lapout) {
+        std::swap(s->lapother, s->lapnext);
+        s->lap_of = u;
+    } else {
+        s->lap_of = nullptr;
+    }
+}
+
 // One time step. With `inj` given and the fused 2D update in use, the injection is applied by
 // the update itself and *injected is set; otherwise the caller injects.
 // kout (the fused density update only): K u of every node is also stored there; *kout_done tells
@@ -1447,81 +1534,18 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
         return SA_OK;
     }
     if (use_flat2d(s)) {
-        Flat2DStep x;
-        memset(&x, 0, sizeof(x));
-        x.u = u;
-        x.other = other;
-        x.out = out;
-        x.b = c.rho_mode == 2 ? s->b : nullptr;
-        x.A = s->A;
-        x.B = s->B;
-        x.G = s->G;
-        if (c.att_power == 2) {
+        if (c.att_power == 2 && (s->lap_of != other || !s->lapnext)) {
             // `other` is overwritten in place, so its Laplacian must come from before: the
             // previous step's launch wrote it (as the Laplacian of that step's u), unless this is
             // the first step of a run
-            if (s->lap_of != other || !s->lapnext) {
-                k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, other, nullptr, s->lapother);
-                CHECK_LAUNCH();
-            }
-            x.Cbeta = s->Cbeta;
-            x.lapother = s->lapother;
-            x.lapout = s->lapnext;
+            k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, other, nullptr, s->lapother);
+            CHECK_LAUNCH();
         }
-        if (c.boundary == 1) {
-            x.Cpml = s->Cpml;
-            for (int a = 0; a < 3; ++a) {
-                x.u1[a] = s->u1[a];
-                x.u2[a] = s->u2[a];
-                x.u3[a] = s->u3[a];
-                for (int r = 0; r < 4; ++r) x.pml[a][r] = s->pml[a][r];
-            }
-        }
-        x.dt = s->dt;
-        x.backward = backward;
-        x.save = save;
-        x.save_mode = save_mode;
-        x.box = s->box;
-        x.idt2 = 1.f / (s->dt * s->dt);
-        if (inj && inj->sp->ndest > 0) {
-            TRY(dest_map(s, *inj->sp));
-            x.imap = inj->sp->d_map;
-            x.ioff = inj->sp->d_off;
-            x.ipoint = inj->sp->d_point;
-            x.icoef = inj->sp->d_coef;
-            x.ivalues = inj->values;
-            x.isign = inj->sign;
-            x.isave = inj->into_save;
-        }
-        if (gat && gat->sp->npoint > 0) {
-            TRY(tile_points(s, *gat->sp));
-            x.goff = gat->sp->d_toff;
-            x.gpt = gat->sp->d_tpt;
-            x.gidx = gat->sp->d_idx;
-            x.gw = gat->sp->d_w;
-            x.gntap = gat->sp->ntap;
-            x.gsign = gat->sign;
-            x.gout = gat->out;
-        }
-        if (grd) {
-            x.gps = grd->ps;
-            x.gpsm = grd->psm;
-            x.gpsp = grd->psp;
-            x.gfdt2 = grd->fdt2;
-            x.graw = grd->raw;
-            x.ggv = grd->gv;
-            x.gpv = grd->pv;
-            x.ggr = grd->gr;
-            x.gpr = grd->pr;
-        }
+        Flat2DStep x;
+        TRY(flat2d_args(s, u, other, out, backward, save, save_mode, inj, gat, grd, x));
         if (flat2d_step(g, c.rho_mode, x, s->stream) != 0)
             return fail(SA_E_HIP, "fused 2D update launch: %s", hipGetErrorString(hipGetLastError()));
-        if (x.lapout) {
-            std::swap(s->lapother, s->lapnext);
-            s->lap_of = u;
-        } else {
-            s->lap_of = nullptr;
-        }
+        flat2d_lap_advance(s, x, u);
         if (injected && inj) *injected = true;
         TRY(timed_end(s));
         return SA_OK;
@@ -2839,6 +2863,241 @@ extern "C" int sa_batch_get_wavefield(sa_solver *s, int shot, int which, float *
                 out[((long long)i * g.n[1] + j) * g.n[2] + k] =
                     tmp[2 * (g.base + (long long)i * g.s0 + (long long)j * g.s1 + k) + shot];
     return SA_OK;
+}
+
+// ---- several 2D shots per launch (lockstep) -------------------------------------------------
+// Shots on one 2D grid and scheme, each held by its own solver (medium, taps, levels, saved
+// wavefield, accumulators), advance together: every time step is ONE launch of the fused 2D
+// update over the tiles of all shots (grid axis z = shot). Each shot's arguments for each step
+// are those its one-shot run passes (flat2d_args), laid out in a table that goes to the device
+// once per run, so every shot's traces and gradients equal its one-shot run's. All work is issued
+// on the first solver's stream, after the work already queued on each solver's stream; each
+// solver's stream then waits for the group's work.
+
+namespace {
+// issues the group's work on one stream and restores every solver's own stream on exit
+struct GroupStream {
+    sa_solver *const *v;
+    int n;
+    std::vector<hipStream_t> own;
+    std::vector<hipEvent_t> evs;
+    hipStream_t st;
+    GroupStream(sa_solver *const *v_, int n_) : v(v_), n(n_), own(n_), st(v_[0]->stream) {
+        for (int k = 0; k < n; ++k) own[k] = v[k]->stream;
+    }
+    int begin() {
+        for (int k = 0; k < n; ++k) {
+            if (own[k] != st) {
+                hipEvent_t e;
+                HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                evs.push_back(e);
+                HIP_TRY(hipEventRecord(e, own[k]));
+                HIP_TRY(hipStreamWaitEvent(st, e, 0));
+            }
+            v[k]->stream = st;
+        }
+        return SA_OK;
+    }
+    ~GroupStream() {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess && hipEventRecord(e, st) == hipSuccess)
+            for (int k = 0; k < n; ++k)
+                if (own[k] != st) (void)hipStreamWaitEvent(own[k], e, 0);
+        if (e) (void)hipEventDestroy(e);
+        for (int k = 0; k < n; ++k) v[k]->stream = own[k];
+        for (auto x : evs) (void)hipEventDestroy(x);
+    }
+};
+}  // namespace
+
+static int shots2d_check(sa_solver *const *v, int n) {
+    if (!v || n < 1 || n > 1024) return fail(SA_E_ARG, "1 .. 1024 solvers");
+    for (int k = 0; k < n; ++k)
+        if (!v[k]) return fail(SA_E_ARG, "null solver");
+    HIP_TRY(hipSetDevice(v[0]->device));
+    const sa_scheme &c0 = v[0]->sch;
+    for (int k = 0; k < n; ++k) {
+        sa_solver *s = v[k];
+        if (s->device != v[0]->device) return fail(SA_E_ARG, "the solvers of a group share one device");
+        TRY(prepare(s));
+        if (!use_flat2d(s)) return fail(SA_E_UNSUPPORTED, "lockstep shots run the fused 2D update (2D, OT2)");
+        const sa_scheme &c = s->sch;
+        if (c.kernel != c0.kernel || c.rho_mode != c0.rho_mode || c.att_power != c0.att_power ||
+            c.boundary != c0.boundary || c.diff_source != c0.diff_source || c.undersampling != c0.undersampling ||
+            c.save_mode != c0.save_mode || c.tb0 != c0.tb0 || c.tb1 != c0.tb1 || s->grid.nt != v[0]->grid.nt)
+            return fail(SA_E_ARG, "the solvers of a group need the same scheme and number of steps");
+        for (int a = 0; a < 3; ++a)
+            if (s->geo.n[a] != v[0]->geo.n[a] || s->geo.nabs[a] != v[0]->geo.nabs[a] || s->P[a] != v[0]->P[a])
+                return fail(SA_E_ARG, "the solvers of a group need the same grid");
+        if (c.save_mode > 2) return fail(SA_E_UNSUPPORTED, "lockstep shots: save mode 0, 1 or 2");
+        if (s->dump_f[0] > 0 || s->dump_f[1] > 0)
+            return fail(SA_E_UNSUPPORTED, "wavefield dumps: run the shots one by one");
+    }
+    return SA_OK;
+}
+
+static int shots2d_table(sa_solver *s0, size_t n) {
+    if (s0->tab_ev) HIP_TRY(hipEventSynchronize(s0->tab_ev));  // the previous upload has read the table
+    if (n <= s0->tab_cap) return SA_OK;
+    if (s0->tab_h) HIP_TRY(hipHostFree(s0->tab_h));
+    if (s0->tab_d) HIP_TRY(hipFree(s0->tab_d));
+    s0->tab_h = nullptr;
+    s0->tab_d = nullptr;
+    s0->tab_cap = 0;
+    HIP_TRY(hipHostMalloc((void **)&s0->tab_h, n * sizeof(Flat2DStep), hipHostMallocDefault));
+    if (hipMalloc((void **)&s0->tab_d, n * sizeof(Flat2DStep)) != hipSuccess) {
+        s0->tab_d = nullptr;
+        return fail(SA_E_NOMEM, "cannot allocate the lockstep argument table");
+    }
+    s0->tab_cap = n;
+    return SA_OK;
+}
+
+// launches the steps of the table (nstep rows of n shots) on the group's stream
+static int shots2d_run(sa_solver *const *v, int n, int nstep) {
+    sa_solver *s0 = v[0];
+    const size_t bytes = (size_t)nstep * n * sizeof(Flat2DStep);
+    if (nstep <= 0) return SA_OK;
+    HIP_TRY(hipMemcpyAsync(s0->tab_d, s0->tab_h, bytes, hipMemcpyHostToDevice, s0->stream));
+    if (!s0->tab_ev) HIP_TRY(hipEventCreateWithFlags(&s0->tab_ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(s0->tab_ev, s0->stream));
+    for (int i = 0; i < nstep; ++i) {
+        hipEvent_t e0;
+        TRY(timed_begin(s0, &e0));
+        if (flat2d_step_shots(s0->geo, s0->sch.rho_mode, s0->tab_h[(size_t)i * n], s0->tab_d + (size_t)i * n, n,
+                              s0->stream) != 0)
+            return fail(SA_E_HIP, "lockstep 2D update launch: %s", hipGetErrorString(hipGetLastError()));
+        TRY(timed_end(s0));
+    }
+    return SA_OK;
+}
+
+// src[k], rec[k]: device (nt, nsrc_k) samples and (nt, nrec_k) traces of shot k (solver v[k])
+extern "C" int sa_shots2d_forward_dev(sa_solver *const *v, int n, const float *const *src, float *const *rec) {
+    if (!src || !rec) return fail(SA_E_ARG, "null argument");
+    TRY(shots2d_check(v, n));
+    GroupStream gs(v, n);
+    TRY(gs.begin());
+    const sa_scheme &c = v[0]->sch;
+    const int f = c.undersampling, tlast = c.tb1 - 2, nstep = std::max(0, tlast);
+    TRY(shots2d_table(v[0], (size_t)std::max(1, nstep) * n));
+    std::vector<float *> pend(n, nullptr);
+    for (int k = 0; k < n; ++k) {
+        sa_solver *s = v[k];
+        const Geo &g = s->geo;
+        s->lap_of = nullptr;
+        TRY(coef_sparse(s, s->src, 0));
+        TRY(zero_field(s, s->buf[0]));
+        TRY(zero_field(s, s->buf[1]));
+        if (c.boundary == 1)
+            for (int a = 0; a < 3; ++a)
+                if (g.act[a]) {
+                    TRY(zero_field(s, s->u1[a]));
+                    TRY(zero_field(s, s->u2[a]));
+                    TRY(zero_field(s, s->u3[a]));
+                }
+        if (s->lapo) TRY(zero_field(s, s->lapo));
+        if (s->rec.npoint > 0)
+            HIP_TRY(hipMemsetAsync(rec[k], 0, sizeof(float) * (size_t)s->grid.nt * s->rec.npoint, s->stream));
+        if (c.save_mode != 0) TRY(alloc_saved(s));
+        if (c.att_power == 2 && nstep > 0) {
+            // the first step's other level (buf[0]): its Laplacian
+            if (!s->lapother || !s->lapnext) return fail(SA_E_STATE, "power-2 attenuation buffers missing");
+            k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->buf[0], nullptr, s->lapother);
+            CHECK_LAUNCH();
+        }
+        const size_t per = box_points(s->box);
+        float *pending = nullptr;  // the previous step's receiver row, interpolated by this step's launch
+        for (int t = 1; t <= tlast; ++t) {
+            float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
+            float *save = nullptr;
+            int mode = 0;
+            if (c.save_mode != 0 && t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave) {
+                save = s->saved + (size_t)(t / f) * per;
+                mode = c.save_mode == 1 ? 1 : 2;
+            }
+            const InjectSpec spec{&s->src, src[k] + (size_t)t * s->src.npoint, 1.f, mode == 1};
+            const GatherSpec gspec{&s->rec, 1.f, pending};
+            Flat2DStep &x = v[0]->tab_h[(size_t)(t - 1) * n + k];
+            TRY(flat2d_args(s, u, o, o, 0, save, mode, &spec, pending ? &gspec : nullptr, nullptr, x));
+            flat2d_lap_advance(s, x, u);
+            pending = s->rec.npoint > 0 ? rec[k] + (size_t)t * s->rec.npoint : nullptr;
+        }
+        pend[k] = pending;
+        s->cur = (tlast + 1) % 2;
+    }
+    TRY(shots2d_run(v, n, nstep));
+    for (int k = 0; k < n; ++k)
+        if (pend[k]) TRY(gather(v[k], v[k]->rec, v[k]->buf[(tlast + 1) % 2], 1.f, pend[k]));
+    return SA_OK;
+}
+
+// res[k]: device (nt, nrec_k) adjoint source of shot k; want: SA_WANT_VP / SA_WANT_RHO. Each
+// solver's gradients are its own shot's (sa_read_gradient / sa_read_gradient_dev on it).
+extern "C" int sa_shots2d_adjoint_dev(sa_solver *const *v, int n, const float *const *res, int want) {
+    if (!res) return fail(SA_E_ARG, "null argument");
+    TRY(shots2d_check(v, n));
+    const sa_scheme &c = v[0]->sch;
+    const bool want_v = want & SA_WANT_VP, want_r = want & SA_WANT_RHO;
+    for (int k = 0; k < n; ++k) {
+        sa_solver *s = v[k];
+        if (s->saved_on_host) return fail(SA_E_STATE, "a saved wavefield is in host memory: sa_saved_from_host first");
+        if (!s->saved || (s->saved_mode != 1 && s->saved_mode != 2))
+            return fail(SA_E_STATE, "the lockstep adjoint needs forward runs that saved the wavefield (save mode 1 or 2)");
+        const bool raw = s->saved_mode == 2;
+        if (want_r && !(raw && c.rho_mode == 2 && s->geo.nabs[1] >= 5 && s->geo.nabs[2] >= 5))
+            return fail(SA_E_UNSUPPORTED, "the lockstep rho gradient needs variable density, the raw wavefield and "
+                                          "absorbing layers of at least 5 cells");
+    }
+    GroupStream gs(v, n);
+    TRY(gs.begin());
+    const int tstart = c.tb1 - 1 - v[0]->saved_f, tend = c.tb0 + 1;
+    const int nstep = std::max(0, tstart - tend + 1);
+    TRY(shots2d_table(v[0], (size_t)std::max(1, nstep) * n));
+    for (int k = 0; k < n; ++k) {
+        sa_solver *s = v[k];
+        const Geo &g = s->geo;
+        if (want_v) TRY(grad_outputs(s, 2, s->gacc));
+        if (want_r) TRY(grad_outputs(s, 2, s->gacc + 2));
+        float *gv = want_v ? s->gacc[0] : nullptr, *pv = want_v ? s->gacc[1] : nullptr;
+        float *gr = want_r ? s->gacc[2] : nullptr, *pr = want_r ? s->gacc[3] : nullptr;
+        TRY(coef_sparse(s, s->rec, 1));
+        TRY(zero_aux(s));
+        if (c.att_power == 2 && nstep > 0) {
+            if (!s->lapother || !s->lapnext) return fail(SA_E_STATE, "power-2 attenuation buffers missing");
+            k_laplace<<<domain_grid(g, kBlk), kBlk, 0, s->stream>>>(g, s->buf[(tstart + 1) % 2], nullptr,
+                                                                     s->lapother);
+            CHECK_LAUNCH();
+        }
+        const int f = s->saved_f;
+        const bool raw = s->saved_mode == 2;
+        const size_t per = box_points(s->box);
+        const float fdt2 = (float)(1.0 / ((double)f * s->grid.dt * (double)f * s->grid.dt));
+        for (int t = tstart; t >= tend; --t) {
+            float *u = s->buf[t % 2], *o = s->buf[(t + 1) % 2];
+            const bool guard = t % f == 0 && t >= c.tb0 && t <= c.tb1 && t / f < s->nsave && (gv || gr);
+            GradSpec gspec{};
+            if (guard) {
+                gspec.ps = s->saved + (size_t)(t / f) * per;
+                gspec.raw = raw;
+                if (raw) {
+                    gspec.psm = t / f >= 1 ? s->saved + (size_t)(t / f - 1) * per : nullptr;
+                    gspec.psp = t / f + 1 < s->nsave ? s->saved + (size_t)(t / f + 1) * per : nullptr;
+                }
+                gspec.fdt2 = fdt2;
+                gspec.gv = gv;
+                gspec.pv = pv;
+                gspec.gr = gr;
+                gspec.pr = pr;
+            }
+            const InjectSpec spec{&s->rec, res[k] + (size_t)t * s->rec.npoint, -1.f, false};
+            Flat2DStep &x = v[0]->tab_h[(size_t)(tstart - t) * n + k];
+            TRY(flat2d_args(s, u, o, o, 1, nullptr, 0, &spec, nullptr, guard ? &gspec : nullptr, x));
+            flat2d_lap_advance(s, x, u);
+        }
+        s->cur = (tend - 1 + 2) % 2;
+    }
+    return shots2d_run(v, n, nstep);
 }
 
 // ---- many shots of uneven duration and receiver count (forward) -----------------------------
