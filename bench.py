@@ -939,6 +939,9 @@ def main():
     ap.add_argument('--power2', action='store_true',
                     help='one 3D shot with power-2 attenuation on the fused update and on the general passes '
                          '(use with --nt) and exit')
+    ap.add_argument('--density-pair', action='store_true',
+                    help='two 3D density shots per pass (interleaved), fwd + adjoint with summed vp / rho '
+                         'gradients (--tile rows,chunk,waves)')
     ap.add_argument('--density', action='store_true',
                     help='only run one 3D shot on the variable vp / rho / alpha medium (fused density update, '
                          'vp and rho gradients) and exit')
@@ -1000,11 +1003,12 @@ def main():
     if args.power2:
         print(json.dumps({'fused': bench_power2(args.nt), 'general': bench_power2(args.nt, general=True)}))
         return
+    if args.density_pair:
+        tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
+        print(json.dumps(bench_density_pair(args.nt, tile)))
+        return
     if args.density:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
-        if args.shots == 2 and not args.checkpoint and args.att_power == 0:
-            print(json.dumps(bench_density_pair(args.nt, tile)))
-            return
         print(json.dumps(bench_density(args.nt, tile, with_cpu=not args.no_cpu, checkpoint=args.checkpoint,
                                        att_power=args.att_power)))
         return

@@ -20,6 +20,6 @@ def test_modes_are_listed():
                        timeout=120)
     assert r.returncode == 0
     for flag in ('--shots', '--schedule', '--pipelines', '--density', '--gradient-2d', '--forward-2d', '--workers',
-                 '--copy-check', '--no-cpu', '--power2', '--autograd', '--learned', '--checkpoint'):
+                 '--copy-check', '--no-cpu', '--power2', '--autograd', '--learned', '--checkpoint', '--density-pair'):
         assert flag in r.stdout, flag
 
