@@ -586,6 +586,56 @@ def bench_density_pair(nt=2500, tile=None):
                                     % plan.nt})
 
 
+def bench_pml3d(nt=500, general=False):
+    """One 3D shot with the CFS-PML boundary (20 cells, the reference's complex_frequency_shift_PML_2)
+    on the 360^3 grid and medium of the headline workload: forward with p.dt2 saved every 4th step,
+    then the as-written adjoint with the vp gradient. The update runs as the specialised one-shot
+    kernel without a boundary term, the auxiliary fields of every axis are advanced by one launch
+    over the absorbing layers before it and their term is applied by one after it; general=True
+    runs the separate general passes (the Laplacian, k_pml per axis, k_update) instead."""
+    from stride_amd.configs import config3
+    from stride_amd.physics.plan import build_plan
+    from stride_amd._native import Solver
+    if general:
+        os.environ['STRIDE_AMD_GENERAL_ONLY'] = '1'
+    try:
+        cfg = config3(scale=1.0, nt=nt, nrec=1024)
+        plan = build_plan(cfg['space'], cfg['time'], cfg['wavelet'][None], cfg['vp'],
+                          boundary_type='complex_frequency_shift_PML_2', source_coordinates=cfg['sources'],
+                          receiver_coordinates=cfg['receivers'], kernel='OT2', save_undersampling=4, grad_vp=True)
+        s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, plan.nt)
+        s.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
+                     plan.undersampling, 1, plan.tb0, plan.tb1, 1, 0, 0)
+        s.set_medium(plan.vp)
+        s.set_pml(plan.pml)
+        s.set_sources(plan.src_idx, plan.src_w)
+        s.set_receivers(plan.rec_idx, plan.rec_w)
+        d_src = s._to_device(plan.src_data)
+        d_rec = s.device_alloc(plan.nt * plan.nrec * 4)
+        d_res = s._to_device(np.random.default_rng(0).standard_normal((plan.nt, plan.nrec)).astype(np.float32) * 1e-3)
+
+        def shot():
+            s.forward_dev(d_src, d_rec)
+            s.adjoint_dev(d_res, 1)
+            s.synchronize()
+
+        shot()
+        t0 = time.perf_counter()
+        shot()
+        wall = time.perf_counter() - t0
+        for d in (d_src, d_rec, d_res):
+            s.device_free(d)
+        s.close()
+    finally:
+        if general:
+            os.environ.pop('STRIDE_AMD_GENERAL_ONLY', None)
+    n = int(np.prod(plan.shape))
+    fwd, adj = plan.tb1 - 2, plan.tb1 - 1 - plan.undersampling - plan.tb0
+    return dict(metric='Gpts/s fwd+adjoint 3D 320^3 SO10, CFS-PML 20, vp gradient',
+                value=round((fwd + adj) * n / wall / 1e9, 3), unit='Gpts/s', seconds=round(wall, 3), nt=plan.nt,
+                path='general passes' if general else 'specialised update + PML layer passes')
+
+
 def bench_power2(nt=500, general=False):
     """One 3D shot with power-2 attenuation (alpha 0.5 dB/cm/MHz, the 2D gradient workload's
     value) on the 360^3 grid of the headline workload: forward with p.dt2 saved every 4th step,
@@ -939,6 +989,9 @@ def main():
     ap.add_argument('--power2', action='store_true',
                     help='one 3D shot with power-2 attenuation on the fused update and on the general passes '
                          '(use with --nt) and exit')
+    ap.add_argument('--pml', action='store_true',
+                    help='one 3D shot with the CFS-PML boundary on the fused update and on the general passes '
+                         '(use --nt 500)')
     ap.add_argument('--density-pair', action='store_true',
                     help='two 3D density shots per pass (interleaved), fwd + adjoint with summed vp / rho '
                          'gradients (--tile rows,chunk,waves)')
@@ -999,6 +1052,9 @@ def main():
         return
     if args.autograd:
         print(json.dumps(bench_autograd(args.nt, args.checkpoint)))
+        return
+    if args.pml:
+        print(json.dumps({'fused': bench_pml3d(args.nt), 'general': bench_pml3d(args.nt, general=True)}))
         return
     if args.power2:
         print(json.dumps({'fused': bench_power2(args.nt), 'general': bench_power2(args.nt, general=True)}))

@@ -172,8 +172,9 @@ int sa_set_dump(sa_solver *s, int forward_factor, int adjoint_factor);
 int sa_dump_count(sa_solver *s, int which, long long *n);
 int sa_dump_to_host(sa_solver *s, int which, float *host, long long n);
 
-/* Tile of the specialised 3D update (3D, OT2, no density, sponge or no boundary, no
- * attenuation): 64 x waves*rows cells per workgroup, marching `chunk` planes of the slowest
+/* Tile of the specialised 3D update (3D, OT2, no density, sponge, no boundary or CFS-PML (the
+ * layers' auxiliary fields in passes of their own), any attenuation form; save mode 0 or 1):
+ * 64 x waves*rows cells per workgroup, marching `chunk` planes of the slowest
  * axis. version 7 (LDS ring of planes filled by direct-to-LDS loads two planes ahead, u read
  * once: axis-0 terms pushed into partial sums; 4 waves, rows 2) or 8 (as 7 with 8 waves; rows
  * 1, 2). Default: version 7, rows 2, chunk 24. Used by the autotuner (stride_amd.autotune).
@@ -181,7 +182,7 @@ int sa_dump_to_host(sa_solver *s, int which, float *host, long long n);
 int sa_set_tile(sa_solver *s, int rows, int chunk, int version);
 
 /* Tile of the fused 3D update for the div(b grad) forms (constant or variable density; OT2,
- * sponge or no boundary, attenuation power 0 / None / off), which replaces the staggered
+ * sponge, no boundary or CFS-PML, any attenuation form), which replaces the staggered
  * derivative, divergence and update passes of the operator built by
  * stride/physics/iso_acoustic/devito.py:1466-1608, 1646-1666 for those schemes. The tile is
  * 64 (axis 2) x waves*rows (axis 1) cells marched over `chunk` planes of axis 0; rows 1 or 2

@@ -206,13 +206,12 @@ __global__ void k_update(Geo g, UpdateArgs x) {
     }
 }
 
-// CFS-PML auxiliary update on the two slabs of axis a (boundaries/devito.py:133-171)
-__global__ void k_pml(Geo g, int a, const float *__restrict__ F, float *u1, float *u2, float *u3,
-                      const float *__restrict__ sig, const float *__restrict__ alp, const float *__restrict__ dsig,
-                      const float *__restrict__ dalp, float dt, int backward) {
-    DOMAIN_INDEX
-    int x = a == 0 ? i : (a == 1 ? j : k);
-    if (!(x < g.nabs[a] || x >= g.n[a] - g.nabs[a])) return;
+// CFS-PML auxiliary update of axis a at cell p, x its coordinate along a, inside a slab of a
+// (boundaries/devito.py:133-171)
+__device__ __forceinline__ void pml_cell(const Geo &g, int a, const float *__restrict__ F, long long p, int x,
+                                         float *u1, float *u2, float *u3, const float *__restrict__ sig,
+                                         const float *__restrict__ alp, const float *__restrict__ dsig,
+                                         const float *__restrict__ dalp, float dt, int backward) {
     float s = sig[x], al = alp[x], ds = dsig[x], da = dalp[x];
     float dF = sa_d1(g, F, p, a), d2F = sa_d2(g, F, p, a);
     float v3, v2, v1;
@@ -228,6 +227,80 @@ __global__ void k_pml(Geo g, int a, const float *__restrict__ F, float *u1, floa
     u3[p] = v3;
     u2[p] = v2;
     u1[p] = v1;
+}
+
+// CFS-PML auxiliary update on the two slabs of axis a
+__global__ void k_pml(Geo g, int a, const float *__restrict__ F, float *u1, float *u2, float *u3,
+                      const float *__restrict__ sig, const float *__restrict__ alp, const float *__restrict__ dsig,
+                      const float *__restrict__ dalp, float dt, int backward) {
+    DOMAIN_INDEX
+    int x = a == 0 ? i : (a == 1 ? j : k);
+    if (!(x < g.nabs[a] || x >= g.n[a] - g.nabs[a])) return;
+    pml_cell(g, a, F, p, x, u1, u2, u3, sig, alp, dsig, dalp, dt, backward);
+}
+
+// The CFS-PML fields of every active axis (3D fused updates)
+struct PmlSet {
+    float *u1[3], *u2[3], *u3[3];
+    const float *prof[3][4];  // sigma, alpha, dsigma, dalpha per axis
+    const float *Cpml;
+};
+
+__device__ __forceinline__ bool sa_in_slab(const Geo &g, int a, int x) {
+    return x < g.nabs[a] || x >= g.n[a] - g.nabs[a];
+}
+
+// Cells of the absorbing layers (the union of the slabs), one workgroup per row (i, j) of the
+// domain: every cell of a row inside an axis-0 or axis-1 slab, else the cells of the two axis-2
+// slabs (cell e of the row's list is k = slab_k(r, e)).
+struct SlabRow {
+    int i, j, cnt, nz, na;
+    bool whole;
+};
+
+__device__ __forceinline__ bool slab_row(const Geo &g, SlabRow &r) {
+    const int row = blockIdx.x;
+    r.i = row / g.n[1];
+    r.j = row - r.i * g.n[1];
+    if (r.i >= g.n[0]) return false;
+    r.nz = g.n[2];
+    r.na = g.nabs[2];
+    r.whole = sa_in_slab(g, 0, r.i) || sa_in_slab(g, 1, r.j) || 2 * r.na >= r.nz;
+    r.cnt = r.whole ? r.nz : 2 * r.na;
+    return true;
+}
+
+__device__ __forceinline__ int slab_k(const SlabRow &r, int e) {
+    return r.whole ? e : (e < r.na ? e : r.nz - 2 * r.na + e);
+}
+
+// every active axis's CFS-PML update on its slabs (the separate per-axis k_pml launches, in one)
+__global__ void k_pml_slabs(Geo g, const float *__restrict__ F, PmlSet ps, float dt, int backward) {
+    SlabRow r;
+    if (!slab_row(g, r)) return;
+    for (int e = threadIdx.x; e < r.cnt; e += blockDim.x) {
+        const int k = slab_k(r, e);
+        const long long p = sa_pidx(g, r.i, r.j, k);
+        const int xs[3] = {r.i, r.j, k};
+        for (int a = 0; a < 3; ++a)
+            if (g.act[a] && sa_in_slab(g, a, xs[a]))
+                pml_cell(g, a, F, p, xs[a], ps.u1[a], ps.u2[a], ps.u3[a], ps.prof[a][0], ps.prof[a][1],
+                         ps.prof[a][2], ps.prof[a][3], dt, backward);
+    }
+}
+
+// the PML term of (1) after a fused update that left it out: out -= Cpml sum_d u1_d on the slabs
+// (zero elsewhere: u1_d is only written on the slabs of d)
+__global__ void k_pml_correct(Geo g, PmlSet ps, float *out) {
+    SlabRow r;
+    if (!slab_row(g, r)) return;
+    for (int e = threadIdx.x; e < r.cnt; e += blockDim.x) {
+        const long long p = sa_pidx(g, r.i, r.j, slab_k(r, e));
+        float s1 = 0.f;
+        for (int a = 0; a < 3; ++a)
+            if (ps.u1[a]) s1 += ps.u1[a][p];
+        out[p] -= ps.Cpml[p] * s1;
+    }
 }
 
 // padded field -> unpadded C-order array of the domain
@@ -1201,30 +1274,63 @@ static int laplacian(sa_solver *s, const float *u, float *out) {
     return SA_OK;
 }
 
+// the fused 3D density update (with CFS-PML: the auxiliary fields by k_pml_slabs before it and
+// their term by k_pml_correct after it)
 static bool use_rho3d(const sa_solver *s) {
     const sa_scheme &c = s->sch;
-    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode != 0 && c.boundary != 1 && s->rho3d.enabled &&
-           rho3d_supported(s->geo);
+    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode != 0 && s->rho3d.enabled && rho3d_supported(s->geo);
 }
 
 static bool use_flat2d(const sa_solver *s) {
     return s->grid.dim == 2 && s->sch.kernel == 2 && s->rho3d.enabled && flat2d_supported(s->geo);
 }
 
-// the specialised 3D update: no density term; attenuation of any form with one shot per pass
+// the specialised 3D update: no density term; attenuation of any form with one shot per pass;
+// sponge, no boundary or CFS-PML (as for the density update)
 static bool use_fast(const sa_solver *s) {
     const sa_scheme &c = s->sch;
-    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 &&
-           c.boundary != 1 && (c.save_mode == 0 || c.save_mode == 1) && fast3d_enabled() &&
-           fast3d_supported(s->geo);
+    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 && (c.save_mode == 0 || c.save_mode == 1) &&
+           fast3d_enabled() && fast3d_supported(s->geo);
 }
 
-// the two-shot and multi-shot forms of the specialised update: no attenuation
-static bool use_fast_pairs(const sa_solver *s) { return use_fast(s) && s->sch.att_power == -2; }
+// the two-shot and multi-shot forms of the specialised update: no attenuation, no PML
+static bool use_fast_pairs(const sa_solver *s) {
+    return use_fast(s) && s->sch.att_power == -2 && s->sch.boundary != 1;
+}
 
-// the two-shot form of the fused density update: no power-2 attenuation
+// the two-shot form of the fused density update: no power-2 attenuation, no PML
 static bool use_rho3d_pairs(const sa_solver *s) {
-    return use_rho3d(s) && s->sch.att_power != 2 && rho3d_supported(s->geo, 2);
+    return use_rho3d(s) && s->sch.att_power != 2 && s->sch.boundary != 1 && rho3d_supported(s->geo, 2);
+}
+
+static PmlSet pml_set(const sa_solver *s) {
+    PmlSet ps;
+    memset(&ps, 0, sizeof(ps));
+    for (int a = 0; a < 3; ++a) {
+        if (!s->geo.act[a]) continue;
+        ps.u1[a] = s->u1[a];
+        ps.u2[a] = s->u2[a];
+        ps.u3[a] = s->u3[a];
+        for (int r = 0; r < 4; ++r) ps.prof[a][r] = s->pml[a][r];
+    }
+    ps.Cpml = s->Cpml;
+    return ps;
+}
+
+// CFS-PML around a fused 3D update (OT2, F = u): the auxiliary fields of every axis before it ...
+static int pml_before(sa_solver *s, const float *u, int backward) {
+    const Geo &g = s->geo;
+    k_pml_slabs<<<dim3((unsigned)(g.n[0] * g.n[1])), 64, 0, s->stream>>>(g, u, pml_set(s), s->dt, backward);
+    CHECK_LAUNCH();
+    return SA_OK;
+}
+
+// ... and their term of (1) after it
+static int pml_after(sa_solver *s, float *out) {
+    const Geo &g = s->geo;
+    k_pml_correct<<<dim3((unsigned)(g.n[0] * g.n[1])), 64, 0, s->stream>>>(g, pml_set(s), out);
+    CHECK_LAUNCH();
+    return SA_OK;
 }
 
 // build per-point coefficients and boundary state for the current scheme + medium
@@ -1271,7 +1377,10 @@ static int prepare(sa_solver *s) {
             }
         if (c.kernel == 4) TRY(dalloc(&s->lapo, s->total));
     }
-    if (use_fast(s)) TRY(fast3d_prepare(s->fast, g, s->vp, s->tables, s->tlen, c.boundary, s->dt, s->stream));
+    // with CFS-PML the specialised update runs without a boundary term (pml_before / pml_after)
+    if (use_fast(s))
+        TRY(fast3d_prepare(s->fast, g, s->vp, s->tables, s->tlen, c.boundary == 1 ? -1 : c.boundary, s->dt,
+                           s->stream));
     s->coef_dirty = false;
     return SA_OK;
 }
@@ -1526,8 +1635,10 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
             st.lapu = s->lapnext;
         }
         if (kout && c.att_power != 2) st.kout = kout;
+        if (c.boundary == 1) TRY(pml_before(s, u, backward));
         if (rho3d_step(s->rho3d, g, st, s->stream) != 0)
             return fail(SA_E_HIP, "fused density update launch: %s", hipGetErrorString(hipGetLastError()));
+        if (c.boundary == 1) TRY(pml_after(s, out));
         if (kout_done) *kout_done = st.kout != nullptr;
         if (c.att_power == 2) lap_advance(s, u);
         TRY(timed_end(s));
@@ -1800,11 +1911,13 @@ static int forward_step(sa_solver *s, int t, float *u, float *o, const float *sr
             at = {s->sch.att_power, s->alpha, s->lapother, s->lapnext};
             pat = &at;
         }
+        if (s->sch.boundary == 1) TRY(pml_before(s, u, 0));
         hipEvent_t e0;
         TRY(timed_begin(s, &e0));
         if (fast3d_step(s->fast, s->geo, u, o, o, save, s->box, idt2, s->stream, pat) != 0)
             return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
         TRY(timed_end(s));
+        if (s->sch.boundary == 1) TRY(pml_after(s, o));
         if (s->sch.att_power == 2) lap_advance(s, u);
     } else {
         const InjectSpec spec{&s->src, src + (size_t)t * s->src.npoint, 1.f, mode == 1};
@@ -2042,12 +2155,14 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
                 at = {c.att_power, s->alpha, s->lapother, s->lapnext};
                 pat = &at;
             }
+            if (c.boundary == 1) TRY(pml_before(s, u, 1));
             hipEvent_t e0;
             TRY(timed_begin(s, &e0));
             if (fast3d_step_adjoint(s->fast, g, u, o, o, s->saved_mode == 1 ? ps : nullptr, s->box, gv, pv, s->stream,
                                     pat) != 0)
                 return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
             TRY(timed_end(s));
+            if (c.boundary == 1) TRY(pml_after(s, o));
             if (c.att_power == 2) lap_advance(s, u);
         } else {
             const InjectSpec spec{&s->rec, res + (size_t)t * s->rec.npoint, -1.f, false};
