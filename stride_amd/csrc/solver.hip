@@ -250,16 +250,17 @@ __device__ __forceinline__ bool sa_in_slab(const Geo &g, int a, int x) {
     return x < g.nabs[a] || x >= g.n[a] - g.nabs[a];
 }
 
-// Cells of the absorbing layers (the union of the slabs), one workgroup per row (i, j) of the
-// domain: every cell of a row inside an axis-0 or axis-1 slab, else the cells of the two axis-2
-// slabs (cell e of the row's list is k = slab_k(r, e)).
+// Cells of the absorbing layers (the union of the slabs), one wave per row (i, j) of the domain
+// (kSlabRows rows per workgroup): every cell of a row inside an axis-0 or axis-1 slab, else the
+// cells of the two axis-2 slabs (cell e of the row's list is k = slab_k(r, e)).
+constexpr int kSlabRows = 4;
 struct SlabRow {
     int i, j, cnt, nz, na;
     bool whole;
 };
 
 __device__ __forceinline__ bool slab_row(const Geo &g, SlabRow &r) {
-    const int row = blockIdx.x;
+    const int row = blockIdx.x * kSlabRows + threadIdx.y;
     r.i = row / g.n[1];
     r.j = row - r.i * g.n[1];
     if (r.i >= g.n[0]) return false;
@@ -1320,7 +1321,8 @@ static PmlSet pml_set(const sa_solver *s) {
 // CFS-PML around a fused 3D update (OT2, F = u): the auxiliary fields of every axis before it ...
 static int pml_before(sa_solver *s, const float *u, int backward) {
     const Geo &g = s->geo;
-    k_pml_slabs<<<dim3((unsigned)(g.n[0] * g.n[1])), 64, 0, s->stream>>>(g, u, pml_set(s), s->dt, backward);
+    const unsigned rows = (unsigned)((g.n[0] * g.n[1] + kSlabRows - 1) / kSlabRows);
+    k_pml_slabs<<<rows, dim3(64, kSlabRows), 0, s->stream>>>(g, u, pml_set(s), s->dt, backward);
     CHECK_LAUNCH();
     return SA_OK;
 }
@@ -1328,7 +1330,8 @@ static int pml_before(sa_solver *s, const float *u, int backward) {
 // ... and their term of (1) after it
 static int pml_after(sa_solver *s, float *out) {
     const Geo &g = s->geo;
-    k_pml_correct<<<dim3((unsigned)(g.n[0] * g.n[1])), 64, 0, s->stream>>>(g, pml_set(s), out);
+    const unsigned rows = (unsigned)((g.n[0] * g.n[1] + kSlabRows - 1) / kSlabRows);
+    k_pml_correct<<<rows, dim3(64, kSlabRows), 0, s->stream>>>(g, pml_set(s), out);
     CHECK_LAUNCH();
     return SA_OK;
 }
