@@ -18,8 +18,9 @@ from stride_amd.physics.plan import build_plan, adjoint_source_samples
 pytestmark = pytest.mark.gpu
 
 
-def _two_shots(n, nabs, nt=100, rho='var', alpha='p0'):
+def _two_shots(n, nabs, nt=100, rho='var', alpha='p0', **extra):
     opts = dict(attenuation_power=2) if alpha == 'p2' else {}
+    opts.update(extra)
     plan0, ctx = small_case(dim=3, n=n, nabs=nabs, nt=nt, rho=rho, alpha=alpha, kernel='OT2', nrec=20,
                             grad_vp=True, grad_rho=rho == 'var', **opts)
     kw = dict(ctx['kw'])
@@ -80,6 +81,31 @@ def test_two_density_shots_match_one_shot_runs(gpu, n, nabs, tile, alpha):
         print('  %s vs summed one-shot runs %.3e' % (key, err))
         assert np.abs(want).max() > 0
         assert err < 1e-5, key
+
+
+def test_two_density_shots_time_bounds(gpu):
+    """time_bounds inside the run (devito.py:577-581, 916-920): raw saves on tb0 <= t <= tb1, the
+    adjoint from tb1 - 1 - f down to tb0 + 1; against two one-shot runs."""
+    plan0, plan1 = _two_shots(28, 8, nt=140, time_bounds=(20, 120))
+    assert (plan0.tb0, plan0.tb1) == (20, 120)
+    res = _residuals((plan0, plan1), seed=3)
+    single = []
+    for p, r in zip((plan0, plan1), res):
+        s = solver_for(p, save_mode=2)
+        rec = s.forward(p.src_data)
+        g = s.adjoint(r, want_vp=True, want_rho=True)
+        s.close()
+        single.append((rec, g))
+    s = _batched_solver(plan0, plan1, save_mode=2)
+    recs = s.batch_forward([plan0.src_data, plan1.src_data])
+    g = s.batch_adjoint(res, density=True)
+    s.close()
+    for k in range(2):
+        assert rel_l2(recs[k], single[k][0]) < 1e-6, k
+    for key in ('grad_vp', 'prec_vp', 'grad_rho', 'prec_rho'):
+        want = single[0][1][key].astype(np.float64) + single[1][1][key]
+        assert np.abs(want).max() > 0
+        assert rel_l2(g[key], want) < 1e-5, key
 
 
 def test_two_density_shots_match_oracle(gpu):

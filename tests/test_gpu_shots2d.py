@@ -42,10 +42,15 @@ CASES = [
     (2, 61, 7, 140, 1, dict(grad_vp=True)),
     # constant density, sponge, attenuation power 0, raw saves, vp gradient
     (4, 37, 9, 120, 2, dict(rho='const', alpha='p0', grad_vp=True)),
+    # time_bounds inside the run (saves on tb0 <= t <= tb1, the adjoint from tb1 - 1 - f down to
+    # tb0 + 1; devito.py:577-581, 916-920), variable density with CFS-PML
+    (2, 48, 10, 200, 2, dict(rho='var', grad_vp=True, grad_rho=True, time_bounds=(25, 180),
+                             boundary_type='complex_frequency_shift_PML_2')),
 ]
 
 
-@pytest.mark.parametrize('nshot,n,nabs,nt,save_mode,opts', CASES, ids=['var-pml-p2', 'plain-sponge', 'const-p0'])
+@pytest.mark.parametrize('nshot,n,nabs,nt,save_mode,opts', CASES,
+                         ids=['var-pml-p2', 'plain-sponge', 'const-p0', 'time-bounds'])
 def test_lockstep_shots_match_one_shot_runs(gpu, nshot, n, nabs, nt, save_mode, opts):
     from stride_amd._native import shots2d_forward, shots2d_adjoint
     plans = _shots(nshot, n, nabs, nt, **dict(opts))
