@@ -49,6 +49,10 @@ struct FastAtt {
     int power;
     const float *alpha, *lapo;
     float *lapn;
+    // CFS-PML (non-null cpml; the update prepared without a boundary): the term - Cpml sum_d u1_d
+    // of the general update, with the u1 fields already advanced for this step
+    const float *cpml = nullptr;
+    const float *u1[3] = {nullptr, nullptr, nullptr};
 };
 // forward step: out = step(u, o) (out may alias o); optional fused save of p.dt2 into `save` (box layout)
 int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,

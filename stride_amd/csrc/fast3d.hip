@@ -53,6 +53,10 @@ struct FastArgs {
     int att;
     const float *alpha, *lapo;
     float *lapn;
+    // CFS-PML (att bit 4): Cpml and the u1 auxiliary field of each axis (read on the cells of
+    // the absorbing layers only), the layer widths and the length of axis 0
+    const float *cpml, *pu1[3];
+    int nabs[3], n0;
 };
 
 // MODE 0: plain step, 1: fused save of p.dt2, 2: fused gradient accumulation
@@ -82,7 +86,7 @@ __device__ __forceinline__ void bstore_nt(float v, rsrc_t r, unsigned voff, int 
 }
 
 struct Rsrc {
-    rsrc_t u, o, vp, out, save, ps, grad, prec, alpha, lapo, lapn;
+    rsrc_t u, o, vp, out, save, ps, grad, prec, alpha, lapo, lapn, cp, q1, q2, q3;
     int plane_bytes, box_plane_bytes;
 };
 
@@ -120,27 +124,42 @@ __device__ __forceinline__ void dma16(rsrc_t r, float *lds, unsigned voff, int s
     __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void *)lds, 16, voff, soff, 0, 0);
 }
 
-// ATT: attenuation form, 0 none, 1 power 0, 2 power 2, 3 power None. alpha is an own-cell
-// operand with attenuation, and with power 2 also the other level's Laplacian.
+// ATT: attenuation form in bits 0-1 (0 none, 1 power 0, 2 power 2, 3 power None), bit 2 the
+// CFS-PML term. alpha is an own-cell operand with attenuation, and with power 2 also the other
+// level's Laplacian; with the PML, Cpml and the three u1 fields (zero off the layers).
 template <int RPT, int MODE, int ATT>
 struct Stage5 {
+    static constexpr int AF = ATT & 3;
+    static constexpr bool PM = (ATT & 4) != 0;
     float o[RPT], v[RPT];
     float ps[MODE == 2 ? RPT : 1], gr[MODE == 2 ? RPT : 1], pr[MODE == 2 ? RPT : 1];
-    float al[ATT ? RPT : 1], lo[ATT == 2 ? RPT : 1];
+    float al[AF ? RPT : 1], lo[AF == 2 ? RPT : 1];
+    float cp[PM ? RPT : 1], q1[PM ? RPT : 1], q2[PM ? RPT : 1], q3[PM ? RPT : 1];
 };
 
 template <int RPT, int MODE, int ATT>
 __device__ __forceinline__ void load_stage5(Stage5<RPT, MODE, ATT> &st, const Rsrc &rs, const unsigned (&row)[RPT],
                                             const unsigned (&grow)[RPT], const unsigned (&brow)[RPT], int x,
-                                            const FastArgs &a) {
+                                            const FastArgs &a, const int (&d1)[RPT], const int d2) {
+    using S5 = Stage5<RPT, MODE, ATT>;
     const int sp = (x + SA_HALO) * rs.plane_bytes;
     if (MODE == 3) return;  // the Laplacian alone: no own operands
+    // PML operands: read on the layer cells (plane, row or column within a layer of its axis;
+    // planes past the domain too, whose results are never stored), zero elsewhere
+    const bool xlay = S5::PM && min(x, a.n0 - 1 - x) < a.nabs[0];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
         st.o[r] = bload_nt(rs.o, row[r], sp);
         st.v[r] = bload_nt(rs.vp, row[r], sp);
-        if (ATT) st.al[r] = bload_nt(rs.alpha, row[r], sp);
-        if (ATT == 2) st.lo[r] = bload_nt(rs.lapo, row[r], sp);
+        if (S5::AF) st.al[r] = bload_nt(rs.alpha, row[r], sp);
+        if (S5::AF == 2) st.lo[r] = bload_nt(rs.lapo, row[r], sp);
+        if (S5::PM) {
+            const unsigned po = (xlay | (d1[r] < a.nabs[1]) | (d2 < a.nabs[2])) ? row[r] : kOOB;
+            st.cp[r] = bload_nt(rs.cp, po, sp);
+            st.q1[r] = bload_nt(rs.q1, po, sp);
+            st.q2[r] = bload_nt(rs.q2, po, sp);
+            st.q3[r] = bload_nt(rs.q3, po, sp);
+        }
     }
     if (MODE == 2) {
         // box planes outside [lo, lo + n) read as zero (offset past the end)
@@ -209,11 +228,12 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
         const float v = sx.v[r];
         const float v2 = v * v;
         float A = 2.f, B = -1.f, G = a.dt2 * v2, cq = a.dt2;
-        if (SPONGE || ATT == 1 || ATT == 3) {
+        constexpr int AF = ATT & 3;
+        if (SPONGE || AF == 1 || AF == 3) {
             // the coefficients of the general update (k_coefficients): power 0 adds dt alpha vp
             // to the damping, power None scales the step by W = alpha
             float aa = 0.f, e = 0.f;
-            if (ATT == 1) aa += a.dt * sx.al[r] * v;
+            if (AF == 1) aa += a.dt * sx.al[r] * v;
             if (SPONGE) {
                 // branch-free layer rule (axis d contributes its table value at distance d_d from
                 // the edge when no other axis is closer to its own edge); with sm = 0 every
@@ -225,7 +245,7 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
                 aa += a.dt * v2 * D;
                 e = a.dt * a.dt * v2 * D * D;
             }
-            const float W = ATT == 3 ? sx.al[r] : 1.f;
+            const float W = AF == 3 ? sx.al[r] : 1.f;
             const float ia = __builtin_amdgcn_rcpf(1.f + aa);  // v_rcp_f32 (1 ulp), no divide sequence
             A = W * (2.f - e) * ia;
             B = -W * (1.f - aa) * ia;
@@ -233,13 +253,15 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
             G = cq * v2;
         }
         float acc = G * lap[r];
-        if (ATT == 2) {
+        if (AF == 2) {
             // power-2 attenuation (the general update's Cbeta term): Cbeta (lap u - lap other),
             // Cbeta = Cq 2 alpha vp^3 / dt; the Laplacian of u is the next step's lap other
             const float cb = cq * 2.f * sx.al[r] * v2 * v / a.dt;
             acc += cb * (lap[r] - sx.lo[r]);
             bstore_nt(lap[r], rs.lapn, xin ? orow[r] : kOOB, sxo);
         }
+        // CFS-PML: - Cpml sum_d u1_d (the general update's order: the axes summed from zero)
+        if (ATT & 4) acc -= sx.cp[r] * ((sx.q1[r] + sx.q2[r]) + sx.q3[r]);
         const float val = A * c[r] + B * sx.o[r] + acc;
         bstore_nt(val, rs.out, xin ? orow[r] : kOOB, sxo);
         if (MODE == 1) bstore_nt((val - 2.f * c[r] + sx.o[r]) * a.idt2, rs.save, xin_box ? brow[r] : kOOB, sbx);
@@ -314,7 +336,7 @@ __device__ __forceinline__ void march7_step(const Geo &g, const FastArgs &a, con
     // 4. update and store plane p - 5; operands of plane p - 2 into its stage
     Stage5<RPT, MODE, ATT> &sx = st[(J + 1) % 3];
     finish7<RPT, MODE, SPONGE, ATT>(g, a, rs, p - 5, x0, x1, lap, c5, sx, orow, grow, brow, d1, t1v, d2, t2v);
-    load_stage5<RPT, MODE, ATT>(st[(J + 1) % 3], rs, row, grow, brow, min(max(p - 2, 0), xlast), a);
+    load_stage5<RPT, MODE, ATT>(st[(J + 1) % 3], rs, row, grow, brow, min(max(p - 2, 0), xlast), a, d1, d2);
 }
 
 // Epilogue step: plane p past the march (own column from memory), finishing plane p - 5.
@@ -338,7 +360,7 @@ __device__ __forceinline__ void tail7_step(const Geo &g, const FastArgs &a, cons
     }
     finish7<RPT, MODE, SPONGE, ATT>(g, a, rs, p - 5, x0, x1, lap, c5, st[(J + 1) % 3], orow, grow, brow, d1, t1v,
                                      d2, t2v);
-    if (J < 2) load_stage5<RPT, MODE, ATT>(st[(J + 1) % 3], rs, row, grow, brow, min(p - 2, xlast), a);
+    if (J < 2) load_stage5<RPT, MODE, ATT>(st[(J + 1) % 3], rs, row, grow, brow, min(p - 2, xlast), a, d1, d2);
 }
 
 // The march of one chunk; SPONGE: the layer coefficients are evaluated on every cell.
@@ -442,10 +464,16 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(RPT == 
         rs.grad = make_rsrc(a.grad, bytes);
         rs.prec = make_rsrc(a.prec, bytes);
     }
-    if (ATT) rs.alpha = make_rsrc(a.alpha, bytes);
-    if (ATT == 2) {
+    if (ATT & 3) rs.alpha = make_rsrc(a.alpha, bytes);
+    if ((ATT & 3) == 2) {
         rs.lapo = make_rsrc(a.lapo, bytes);
         rs.lapn = make_rsrc(a.lapn, bytes);
+    }
+    if (ATT & 4) {
+        rs.cp = make_rsrc(a.cpml, bytes);
+        rs.q1 = make_rsrc(a.pu1[0], bytes);
+        rs.q2 = make_rsrc(a.pu1[1], bytes);
+        rs.q3 = make_rsrc(a.pu1[2], bytes);
     }
     const unsigned zc = (unsigned)(min(z, nz + SA_RADIUS - 1) + SA_LEAD);
     unsigned row[RPT], orow[RPT], grow[RPT], brow[RPT];
@@ -883,6 +911,11 @@ void launch_v7(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastA
         case 1: launch_v7_att<RPT, NW, MODE, 1, NS>(sponge, grid, st, g, a); break;
         case 2: launch_v7_att<RPT, NW, MODE, 2, NS>(sponge, grid, st, g, a); break;
         case 3: launch_v7_att<RPT, NW, MODE, 3, NS>(sponge, grid, st, g, a); break;
+        // CFS-PML (no sponge): the layers' term fused
+        case 4: k_fast3d_v7<RPT, NW, MODE, false, 4, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
+        case 5: k_fast3d_v7<RPT, NW, MODE, false, 5, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
+        case 6: k_fast3d_v7<RPT, NW, MODE, false, 6, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
+        case 7: k_fast3d_v7<RPT, NW, MODE, false, 7, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
         default: launch_v7_att<RPT, NW, MODE, 0, NS>(sponge, grid, st, g, a); break;
     }
 }
@@ -1047,6 +1080,16 @@ int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *
         a.alpha = att->alpha;
         a.lapo = att->lapo;
         a.lapn = att->lapn;
+        if (att->cpml) {
+            if (!att->u1[0] || !att->u1[1] || !att->u1[2] || f.boundary != -1) return SA_E_ARG;
+            a.att |= 4;
+            a.cpml = att->cpml;
+            for (int d = 0; d < 3; ++d) {
+                a.pu1[d] = att->u1[d];
+                a.nabs[d] = g.nabs[d];
+            }
+            a.n0 = g.n[0];
+        }
     }
     if (save) {
         a.save = save;
@@ -1067,6 +1110,16 @@ int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o,
         a.alpha = att->alpha;
         a.lapo = att->lapo;
         a.lapn = att->lapn;
+        if (att->cpml) {
+            if (!att->u1[0] || !att->u1[1] || !att->u1[2] || f.boundary != -1) return SA_E_ARG;
+            a.att |= 4;
+            a.cpml = att->cpml;
+            for (int d = 0; d < 3; ++d) {
+                a.pu1[d] = att->u1[d];
+                a.nabs[d] = g.nabs[d];
+            }
+            a.n0 = g.n[0];
+        }
     }
     if (ps && grad && prec) {
         a.ps = ps;

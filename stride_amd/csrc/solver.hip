@@ -1327,7 +1327,18 @@ static int pml_before(sa_solver *s, const float *u, int backward) {
     return SA_OK;
 }
 
-// ... and their term of (1) after it
+// ... and their term of (1): applied by the specialised update itself (its operands added to
+// `at`, which becomes the attenuation argument when there was none) ...
+static void pml_term(const sa_solver *s, FastAtt &at, FastAtt *&pat) {
+    if (!pat) {
+        at.power = -2;
+        pat = &at;
+    }
+    at.cpml = s->Cpml;
+    for (int a = 0; a < 3; ++a) at.u1[a] = s->u1[a];
+}
+
+// ... or, after the fused density update, by a launch of its own
 static int pml_after(sa_solver *s, float *out) {
     const Geo &g = s->geo;
     const unsigned rows = (unsigned)((g.n[0] * g.n[1] + kSlabRows - 1) / kSlabRows);
@@ -1914,13 +1925,16 @@ static int forward_step(sa_solver *s, int t, float *u, float *o, const float *sr
             at = {s->sch.att_power, s->alpha, s->lapother, s->lapnext};
             pat = &at;
         }
-        if (s->sch.boundary == 1) TRY(pml_before(s, u, 0));
+        if (s->sch.boundary == 1) {
+            // the auxiliary fields first; the update applies their term
+            TRY(pml_before(s, u, 0));
+            pml_term(s, at, pat);
+        }
         hipEvent_t e0;
         TRY(timed_begin(s, &e0));
         if (fast3d_step(s->fast, s->geo, u, o, o, save, s->box, idt2, s->stream, pat) != 0)
             return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
         TRY(timed_end(s));
-        if (s->sch.boundary == 1) TRY(pml_after(s, o));
         if (s->sch.att_power == 2) lap_advance(s, u);
     } else {
         const InjectSpec spec{&s->src, src + (size_t)t * s->src.npoint, 1.f, mode == 1};
@@ -2158,14 +2172,16 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
                 at = {c.att_power, s->alpha, s->lapother, s->lapnext};
                 pat = &at;
             }
-            if (c.boundary == 1) TRY(pml_before(s, u, 1));
+            if (c.boundary == 1) {
+                TRY(pml_before(s, u, 1));
+                pml_term(s, at, pat);
+            }
             hipEvent_t e0;
             TRY(timed_begin(s, &e0));
             if (fast3d_step_adjoint(s->fast, g, u, o, o, s->saved_mode == 1 ? ps : nullptr, s->box, gv, pv, s->stream,
                                     pat) != 0)
                 return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
             TRY(timed_end(s));
-            if (c.boundary == 1) TRY(pml_after(s, o));
             if (c.att_power == 2) lap_advance(s, u);
         } else {
             const InjectSpec spec{&s->rec, res + (size_t)t * s->rec.npoint, -1.f, false};
