@@ -1022,8 +1022,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
-        print('bench.py measures one GPU: the solver shards by shot, and multi-GPU runs are out of scope '
-              'for this build (see DESIGN.md, sharding).', file=sys.stderr)
+        print('bench.py measures one GPU: shots shard across GPUs as independent solvers, one process per '
+              'GPU, and that multi-GPU run is not implemented here; run with --gpus 1.', file=sys.stderr)
         sys.exit(2)
 
     if args.copy_check:
