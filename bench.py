@@ -636,6 +636,48 @@ def bench_pml3d(nt=500, general=False):
                 path='general passes' if general else 'specialised update + PML layer passes')
 
 
+def bench_ot4_3d(nt=500, general=False):
+    """One 3D shot with fourth-order time stepping (kernel='OT4') on the 360^3 grid, medium and
+    sponge of the headline workload: forward with p.dt2 saved every 4th step, then the as-written
+    adjoint with the vp gradient. The fused path runs two marches per step (K u and w = vp^2 K u;
+    then the update staging w, so its Laplacian is K(m K u)); general=True runs the separate general
+    passes (the Laplacian, K of the scaled K u, the product, k_update) instead."""
+    from stride_amd.configs import config3
+    from stride_amd.physics.plan import build_plan
+    if general:
+        os.environ['STRIDE_AMD_GENERAL_ONLY'] = '1'
+    try:
+        cfg = config3(scale=1.0, nt=nt, nrec=1024)
+        plan = build_plan(cfg['space'], cfg['time'], cfg['wavelet'][None], cfg['vp'],
+                          source_coordinates=cfg['sources'], receiver_coordinates=cfg['receivers'], kernel='OT4',
+                          save_undersampling=4, grad_vp=True)
+        s = make_solver(plan)
+        d_src = s._to_device(plan.src_data)
+        d_rec = s.device_alloc(plan.nt * plan.nrec * 4)
+        d_res = s._to_device(np.random.default_rng(0).standard_normal((plan.nt, plan.nrec)).astype(np.float32) * 1e-3)
+
+        def shot():
+            s.forward_dev(d_src, d_rec)
+            s.adjoint_dev(d_res, 1)
+            s.synchronize()
+
+        shot()
+        t0 = time.perf_counter()
+        shot()
+        wall = time.perf_counter() - t0
+        for d in (d_src, d_rec, d_res):
+            s.device_free(d)
+        s.close()
+    finally:
+        if general:
+            os.environ.pop('STRIDE_AMD_GENERAL_ONLY', None)
+    n = int(np.prod(plan.shape))
+    fwd, adj = plan.tb1 - 2, plan.tb1 - 1 - plan.undersampling - plan.tb0
+    return dict(metric='Gpts/s fwd+adjoint 3D 320^3 SO10 OT4, vp gradient',
+                value=round((fwd + adj) * n / wall / 1e9, 3), unit='Gpts/s', seconds=round(wall, 3), nt=plan.nt,
+                kernel_order=plan.kernel_order, path='general passes' if general else 'two marches per step')
+
+
 def bench_power2(nt=500, general=False):
     """One 3D shot with power-2 attenuation (alpha 0.5 dB/cm/MHz, the 2D gradient workload's
     value) on the 360^3 grid of the headline workload: forward with p.dt2 saved every 4th step,
@@ -989,6 +1031,8 @@ def main():
     ap.add_argument('--power2', action='store_true',
                     help='one 3D shot with power-2 attenuation on the fused update and on the general passes '
                          '(use with --nt) and exit')
+    ap.add_argument('--ot4', action='store_true',
+                    help='one 3D OT4 shot on the fused marches and on the general passes (use --nt 500)')
     ap.add_argument('--pml', action='store_true',
                     help='one 3D shot with the CFS-PML boundary on the fused update and on the general passes '
                          '(use --nt 500)')
@@ -1052,6 +1096,9 @@ def main():
         return
     if args.autograd:
         print(json.dumps(bench_autograd(args.nt, args.checkpoint)))
+        return
+    if args.ot4:
+        print(json.dumps({'fused': bench_ot4_3d(args.nt), 'general': bench_ot4_3d(args.nt, general=True)}))
         return
     if args.pml:
         print(json.dumps({'fused': bench_pml3d(args.nt), 'general': bench_pml3d(args.nt, general=True)}))

@@ -172,8 +172,9 @@ int sa_set_dump(sa_solver *s, int forward_factor, int adjoint_factor);
 int sa_dump_count(sa_solver *s, int which, long long *n);
 int sa_dump_to_host(sa_solver *s, int which, float *host, long long n);
 
-/* Tile of the specialised 3D update (3D, OT2, no density, sponge, no boundary or CFS-PML (the
- * layers' auxiliary fields in passes of their own), any attenuation form; save mode 0 or 1):
+/* Tile of the specialised 3D update (3D, no density, save mode 0 or 1; OT2 with sponge, no
+ * boundary or CFS-PML (the layers' auxiliary fields in a pass of their own) and any attenuation
+ * form; OT4 in two marches with sponge or no boundary and attenuation off / power 0 / None):
  * 64 x waves*rows cells per workgroup, marching `chunk` planes of the slowest
  * axis. version 7 (LDS ring of planes filled by direct-to-LDS loads two planes ahead, u read
  * once: axis-0 terms pushed into partial sums; 4 waves, rows 2) or 8 (as 7 with 8 waves; rows

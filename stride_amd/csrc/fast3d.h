@@ -53,7 +53,14 @@ struct FastAtt {
     // of the general update, with the u1 fields already advanced for this step
     const float *cpml = nullptr;
     const float *u1[3] = {nullptr, nullptr, nullptr};
+    // OT4 second pass (non-null ot4_u; attenuation off / power 0 / None, no PML): the update's u
+    // argument is w = m K u from fast3d_ot4_first, ot4_u the level itself and ot4_ku its K u;
+    // out = A u + B other + G (K u + c4 K w), c4 = dt^2 / 12
+    const float *ot4_u = nullptr, *ot4_ku = nullptr;
+    float c4 = 0.f;
 };
+// OT4 first pass: ku = K u and w = vp^2 K u over the domain (one march)
+int fast3d_ot4_first(Fast3D &f, const Geo &g, const float *u, float *ku, float *w, hipStream_t stream);
 // forward step: out = step(u, o) (out may alias o); optional fused save of p.dt2 into `save` (box layout)
 int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,
                 float idt2, hipStream_t stream, const FastAtt *att = nullptr);

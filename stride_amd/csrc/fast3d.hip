@@ -57,6 +57,11 @@ struct FastArgs {
     // the absorbing layers only), the layer widths and the length of axis 0
     const float *cpml, *pu1[3];
     int nabs[3], n0;
+    // OT4 (att bit 8: the second pass, whose staged field is w = m K u): the own u and K u, and
+    // c4 = dt^2 / 12; MODE 4 (the first pass) writes K u into out and w into wout
+    const float *uown, *kuin;
+    float *wout;
+    float c4;
 };
 
 // MODE 0: plain step, 1: fused save of p.dt2, 2: fused gradient accumulation
@@ -86,7 +91,7 @@ __device__ __forceinline__ void bstore_nt(float v, rsrc_t r, unsigned voff, int 
 }
 
 struct Rsrc {
-    rsrc_t u, o, vp, out, save, ps, grad, prec, alpha, lapo, lapn, cp, q1, q2, q3;
+    rsrc_t u, o, vp, out, save, ps, grad, prec, alpha, lapo, lapn, cp, q1, q2, q3, uo, ku, wo;
     int plane_bytes, box_plane_bytes;
 };
 
@@ -125,12 +130,15 @@ __device__ __forceinline__ void dma16(rsrc_t r, float *lds, unsigned voff, int s
 }
 
 // ATT: attenuation form in bits 0-1 (0 none, 1 power 0, 2 power 2, 3 power None), bit 2 the
-// CFS-PML term. alpha is an own-cell operand with attenuation, and with power 2 also the other
-// level's Laplacian; with the PML, Cpml and the three u1 fields (zero off the layers).
+// CFS-PML term, bit 3 the OT4 second pass. alpha is an own-cell operand with attenuation, and
+// with power 2 also the other level's Laplacian; with the PML, Cpml and the three u1 fields (zero
+// off the layers); with OT4, u and K u (the staged field is then w = m K u).
 template <int RPT, int MODE, int ATT>
 struct Stage5 {
     static constexpr int AF = ATT & 3;
     static constexpr bool PM = (ATT & 4) != 0;
+    static constexpr bool O4 = (ATT & 8) != 0;
+    float uo[O4 ? RPT : 1], ku[O4 ? RPT : 1];
     float o[RPT], v[RPT];
     float ps[MODE == 2 ? RPT : 1], gr[MODE == 2 ? RPT : 1], pr[MODE == 2 ? RPT : 1];
     float al[AF ? RPT : 1], lo[AF == 2 ? RPT : 1];
@@ -144,6 +152,11 @@ __device__ __forceinline__ void load_stage5(Stage5<RPT, MODE, ATT> &st, const Rs
     using S5 = Stage5<RPT, MODE, ATT>;
     const int sp = (x + SA_HALO) * rs.plane_bytes;
     if (MODE == 3) return;  // the Laplacian alone: no own operands
+    if (MODE == 4) {        // OT4 first pass: vp only (m = vp^2)
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) st.v[r] = bload_nt(rs.vp, row[r], sp);
+        return;
+    }
     // PML operands: read on the layer cells (plane, row or column within a layer of its axis;
     // planes past the domain too, whose results are never stored), zero elsewhere
     const bool xlay = S5::PM && min(x, a.n0 - 1 - x) < a.nabs[0];
@@ -153,6 +166,10 @@ __device__ __forceinline__ void load_stage5(Stage5<RPT, MODE, ATT> &st, const Rs
         st.v[r] = bload_nt(rs.vp, row[r], sp);
         if (S5::AF) st.al[r] = bload_nt(rs.alpha, row[r], sp);
         if (S5::AF == 2) st.lo[r] = bload_nt(rs.lapo, row[r], sp);
+        if (S5::O4) {
+            st.uo[r] = bload_nt(rs.uo, row[r], sp);
+            st.ku[r] = bload_nt(rs.ku, row[r], sp);
+        }
         if (S5::PM) {
             const unsigned po = (xlay | (d1[r] < a.nabs[1]) | (d2 < a.nabs[2])) ? row[r] : kOOB;
             st.cp[r] = bload_nt(rs.cp, po, sp);
@@ -214,6 +231,15 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
         for (int r = 0; r < RPT; ++r) bstore_nt(lap[r], rs.out, xin ? orow[r] : kOOB, sxo);
         return;
     }
+    if (MODE == 4) {  // OT4 first pass: K u, and w = m K u with m = vp^2 (the general K(m K u) operand)
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+            const float v = sx.v[r];
+            bstore_nt(lap[r], rs.out, xin ? orow[r] : kOOB, sxo);
+            bstore_nt((v * v) * lap[r], rs.wo, xin ? orow[r] : kOOB, sxo);
+        }
+        return;
+    }
     const bool xin_box = MODE != 0 && (xin & (x >= a.box.lo[0]) & (x < a.box.lo[0] + a.box.n[0]));
     const int sbx = MODE == 1 && xin_box ? (x - a.box.lo[0]) * rs.box_plane_bytes : 0;
     int d0 = 0;
@@ -252,7 +278,10 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
             cq = W * a.dt2 * ia;
             G = cq * v2;
         }
-        float acc = G * lap[r];
+        // OT4: the staged field is w, so lap = K w = K(m K u); own u and K u are operands
+        constexpr bool O4 = (ATT & 8) != 0;
+        const float cc = O4 ? sx.uo[r] : c[r];
+        float acc = O4 ? G * (sx.ku[r] + a.c4 * lap[r]) : G * lap[r];
         if (AF == 2) {
             // power-2 attenuation (the general update's Cbeta term): Cbeta (lap u - lap other),
             // Cbeta = Cq 2 alpha vp^3 / dt; the Laplacian of u is the next step's lap other
@@ -262,13 +291,13 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
         }
         // CFS-PML: - Cpml sum_d u1_d (the general update's order: the axes summed from zero)
         if (ATT & 4) acc -= sx.cp[r] * ((sx.q1[r] + sx.q2[r]) + sx.q3[r]);
-        const float val = A * c[r] + B * sx.o[r] + acc;
+        const float val = A * cc + B * sx.o[r] + acc;
         bstore_nt(val, rs.out, xin ? orow[r] : kOOB, sxo);
-        if (MODE == 1) bstore_nt((val - 2.f * c[r] + sx.o[r]) * a.idt2, rs.save, xin_box ? brow[r] : kOOB, sbx);
+        if (MODE == 1) bstore_nt((val - 2.f * cc + sx.o[r]) * a.idt2, rs.save, xin_box ? brow[r] : kOOB, sbx);
         if (MODE == 2) {
             const float psv = sx.ps[r];
             const unsigned go = xin_box ? grow[r] : kOOB;
-            bstore_nt(sx.gr[r] + psv * c[r], rs.grad, go, sxo);
+            bstore_nt(sx.gr[r] + psv * cc, rs.grad, go, sxo);
             bstore_nt(sx.pr[r] + psv * psv, rs.prec, go, sxo);
         }
     }
@@ -468,6 +497,11 @@ __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(RPT == 
     if ((ATT & 3) == 2) {
         rs.lapo = make_rsrc(a.lapo, bytes);
         rs.lapn = make_rsrc(a.lapn, bytes);
+    }
+    if (MODE == 4) rs.wo = make_rsrc(a.wout, bytes);
+    if (ATT & 8) {
+        rs.uo = make_rsrc(a.uown, bytes);
+        rs.ku = make_rsrc(a.kuin, bytes);
     }
     if (ATT & 4) {
         rs.cp = make_rsrc(a.cpml, bytes);
@@ -907,16 +941,25 @@ void launch_v7_att(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const F
 
 template <int RPT, int NW, int MODE, int NS>
 void launch_v7(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
-    switch (a.att) {
-        case 1: launch_v7_att<RPT, NW, MODE, 1, NS>(sponge, grid, st, g, a); break;
-        case 2: launch_v7_att<RPT, NW, MODE, 2, NS>(sponge, grid, st, g, a); break;
-        case 3: launch_v7_att<RPT, NW, MODE, 3, NS>(sponge, grid, st, g, a); break;
-        // CFS-PML (no sponge): the layers' term fused
-        case 4: k_fast3d_v7<RPT, NW, MODE, false, 4, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
-        case 5: k_fast3d_v7<RPT, NW, MODE, false, 5, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
-        case 6: k_fast3d_v7<RPT, NW, MODE, false, 6, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
-        case 7: k_fast3d_v7<RPT, NW, MODE, false, 7, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
-        default: launch_v7_att<RPT, NW, MODE, 0, NS>(sponge, grid, st, g, a); break;
+    if constexpr (MODE >= 3) {
+        // the Laplacian-only and OT4 first passes: no own-operand forms
+        launch_v7_att<RPT, NW, MODE, 0, NS>(sponge, grid, st, g, a);
+    } else {
+        switch (a.att) {
+            case 1: launch_v7_att<RPT, NW, MODE, 1, NS>(sponge, grid, st, g, a); break;
+            case 2: launch_v7_att<RPT, NW, MODE, 2, NS>(sponge, grid, st, g, a); break;
+            case 3: launch_v7_att<RPT, NW, MODE, 3, NS>(sponge, grid, st, g, a); break;
+            // CFS-PML (no sponge): the layers' term fused
+            case 4: k_fast3d_v7<RPT, NW, MODE, false, 4, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
+            case 5: k_fast3d_v7<RPT, NW, MODE, false, 5, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
+            case 6: k_fast3d_v7<RPT, NW, MODE, false, 6, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
+            case 7: k_fast3d_v7<RPT, NW, MODE, false, 7, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
+            // OT4 second pass (attenuation off / power 0 / power None)
+            case 8: launch_v7_att<RPT, NW, MODE, 8, NS>(sponge, grid, st, g, a); break;
+            case 9: launch_v7_att<RPT, NW, MODE, 9, NS>(sponge, grid, st, g, a); break;
+            case 11: launch_v7_att<RPT, NW, MODE, 11, NS>(sponge, grid, st, g, a); break;
+            default: launch_v7_att<RPT, NW, MODE, 0, NS>(sponge, grid, st, g, a); break;
+        }
     }
 }
 
@@ -1090,6 +1133,13 @@ int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *
             }
             a.n0 = g.n[0];
         }
+        if (att->ot4_u) {
+            if (!att->ot4_ku || att->power == 2 || att->cpml) return SA_E_ARG;
+            a.att |= 8;
+            a.uown = att->ot4_u;
+            a.kuin = att->ot4_ku;
+            a.c4 = att->c4;
+        }
     }
     if (save) {
         a.save = save;
@@ -1120,6 +1170,13 @@ int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o,
             }
             a.n0 = g.n[0];
         }
+        if (att->ot4_u) {
+            if (!att->ot4_ku || att->power == 2 || att->cpml) return SA_E_ARG;
+            a.att |= 8;
+            a.uown = att->ot4_u;
+            a.kuin = att->ot4_ku;
+            a.c4 = att->c4;
+        }
     }
     if (ps && grad && prec) {
         a.ps = ps;
@@ -1129,6 +1186,17 @@ int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o,
         return launch<2>(f, g, a, stream);
     }
     return launch<0>(f, g, a, stream);
+}
+
+int fast3d_ot4_first(Fast3D &f, const Geo &g, const float *u, float *ku, float *w, hipStream_t stream) {
+    if (!u || !ku || !w) return SA_E_ARG;
+    FastArgs a = base_args(f, u, nullptr, ku);
+    a.wout = w;
+    const int boundary = f.boundary;
+    f.boundary = -1;  // no layer coefficients in this pass
+    const int rc = launch<4>(f, g, a, stream);
+    f.boundary = boundary;
+    return rc;
 }
 
 int fast3d_laplacian(Fast3D &f, const Geo &g, const float *u, float *out, hipStream_t stream) {

@@ -1288,15 +1288,17 @@ static bool use_flat2d(const sa_solver *s) {
 
 // the specialised 3D update: no density term; attenuation of any form with one shot per pass;
 // sponge, no boundary or CFS-PML (as for the density update)
+// (OT4: two passes, attenuation off / power 0 / None, no PML)
 static bool use_fast(const sa_solver *s) {
     const sa_scheme &c = s->sch;
-    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode == 0 && (c.save_mode == 0 || c.save_mode == 1) &&
+    const bool order = c.kernel == 2 || (c.kernel == 4 && c.boundary != 1 && c.att_power != 2);
+    return s->grid.dim == 3 && order && c.rho_mode == 0 && (c.save_mode == 0 || c.save_mode == 1) &&
            fast3d_enabled() && fast3d_supported(s->geo);
 }
 
 // the two-shot and multi-shot forms of the specialised update: no attenuation, no PML
 static bool use_fast_pairs(const sa_solver *s) {
-    return use_fast(s) && s->sch.att_power == -2 && s->sch.boundary != 1;
+    return use_fast(s) && s->sch.kernel == 2 && s->sch.att_power == -2 && s->sch.boundary != 1;
 }
 
 // the two-shot form of the fused density update: no power-2 attenuation, no PML
@@ -1336,6 +1338,23 @@ static void pml_term(const sa_solver *s, FastAtt &at, FastAtt *&pat) {
     }
     at.cpml = s->Cpml;
     for (int a = 0; a < 3; ++a) at.u1[a] = s->u1[a];
+}
+
+// OT4 on the specialised update: a first pass forms K u and w = m K u (m = vp^2, the operand of
+// the general path's K(m K u)); the update then stages w, so its Laplacian is K(m K u), and reads
+// u and K u as own-cell operands. Returns the field the update stages.
+static int ot4_first(sa_solver *s, const float *u, FastAtt &at, FastAtt *&pat, const float *&staged) {
+    if (fast3d_ot4_first(s->fast, s->geo, u, s->Ku, s->L2, s->stream) != 0)
+        return fail(SA_E_HIP, "OT4 first pass launch: %s", hipGetErrorString(hipGetLastError()));
+    if (!pat) {
+        at.power = -2;
+        pat = &at;
+    }
+    at.ot4_u = u;
+    at.ot4_ku = s->Ku;
+    at.c4 = s->dt * s->dt / 12.f;
+    staged = s->L2;
+    return SA_OK;
 }
 
 // ... or, after the fused density update, by a launch of its own
@@ -1930,9 +1949,11 @@ static int forward_step(sa_solver *s, int t, float *u, float *o, const float *sr
             TRY(pml_before(s, u, 0));
             pml_term(s, at, pat);
         }
+        const float *staged = u;
+        if (s->sch.kernel == 4) TRY(ot4_first(s, u, at, pat, staged));
         hipEvent_t e0;
         TRY(timed_begin(s, &e0));
-        if (fast3d_step(s->fast, s->geo, u, o, o, save, s->box, idt2, s->stream, pat) != 0)
+        if (fast3d_step(s->fast, s->geo, staged, o, o, save, s->box, idt2, s->stream, pat) != 0)
             return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
         TRY(timed_end(s));
         if (s->sch.att_power == 2) lap_advance(s, u);
@@ -2176,10 +2197,12 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
                 TRY(pml_before(s, u, 1));
                 pml_term(s, at, pat);
             }
+            const float *staged = u;
+            if (c.kernel == 4) TRY(ot4_first(s, u, at, pat, staged));
             hipEvent_t e0;
             TRY(timed_begin(s, &e0));
-            if (fast3d_step_adjoint(s->fast, g, u, o, o, s->saved_mode == 1 ? ps : nullptr, s->box, gv, pv, s->stream,
-                                    pat) != 0)
+            if (fast3d_step_adjoint(s->fast, g, staged, o, o, s->saved_mode == 1 ? ps : nullptr, s->box, gv, pv,
+                                    s->stream, pat) != 0)
                 return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
             TRY(timed_end(s));
             if (c.att_power == 2) lap_advance(s, u);

@@ -19,7 +19,7 @@ def test_modes_are_listed():
     r = subprocess.run([sys.executable, os.path.join(ROOT, 'bench.py'), '--help'], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0
-    for flag in ('--shots', '--schedule', '--pipelines', '--density', '--gradient-2d', '--forward-2d', '--workers', '--pml',
+    for flag in ('--shots', '--schedule', '--pipelines', '--density', '--gradient-2d', '--forward-2d', '--workers', '--pml', '--ot4',
                  '--copy-check', '--no-cpu', '--power2', '--autograd', '--learned', '--checkpoint', '--density-pair'):
         assert flag in r.stdout, flag
 
