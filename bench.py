@@ -436,7 +436,7 @@ def cpu_baseline_density(cfg, nt=24):
                        'adjoint steps with vp and rho gradients, %.1f s' % (fwd, adj, t1 - t0))
 
 
-def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0, att_power=0):
+def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0, att_power=0, kernel='OT2'):
     """One 3D shot on the variable vp / rho / alpha medium (360^3 extended, attenuation power 0,
     sponge 20, OT2, 1024 receivers): forward with the raw wavefield saved every 4th step over the
     whole domain, then the as-written adjoint with the vp and rho gradients. The update runs as the
@@ -453,7 +453,7 @@ def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0, att_power=0):
     space, time_ = cfg['space'], cfg['time']
     plan = build_plan(space, time_, cfg['wavelet'][None], cfg['vp'], rho=cfg['rho'], alpha=cfg['alpha'],
                       attenuation_power=att_power, source_coordinates=cfg['sources'],
-                      receiver_coordinates=cfg['receivers'], kernel='OT2', save_undersampling=4, grad_vp=True,
+                      receiver_coordinates=cfg['receivers'], kernel=kernel, save_undersampling=4, grad_vp=True,
                       grad_rho=True)
     s = Solver(plan.dim, plan.shape, plan.spacing, plan.absorbing, plan.dt, plan.nt)
     s.set_scheme(plan.kernel_order, plan.rho_mode, plan.att_power, plan.boundary, plan.diff_source,
@@ -486,7 +486,9 @@ def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0, att_power=0):
     f = plan.undersampling
     nsave = len(range(0, plan.nt - 1, f))
     # power 2: + Cbeta and the two Laplacians read (the Laplacian of u is its own launch)
-    per_update = 40.0 if att_power == 2 else 28.0
+    # OT4: two passes per step (u, b, m in, K u and m K u out = 20 B; then m K u, b, the other
+    # level, A, B, G, u, K u in, the new level out = 36 B)
+    per_update = 40.0 if att_power == 2 else (56.0 if kernel == 'OT4' else 28.0)
     alg = launches * per_update * n + (0 if checkpoint else nsave * 4.0 * n)
     achieved = alg / (ms * 1e-3) / 1e9
     own = (plan.tb1 - 2) + (plan.tb1 - 1 - f - plan.tb0)  # the shot's forward and adjoint steps
@@ -513,8 +515,8 @@ def bench_density(nt=2500, tile=None, with_cpu=True, checkpoint=0, att_power=0):
                 cpu_baseline=cpu, dtype='f32', data='synthetic',
                 config={'workload': '3D 320x320x320 inner (360x360x360 extended), dx 0.4 mm, SO10 OT2, sponge 20, '
                                     'variable vp / rho (Gaussian-filtered noise in the hemisphere) and alpha 0.5 '
-                                    '(power %d), one shot, nt %d, raw saves every 4th step, 1024 receivers'
-                                    % (att_power, plan.nt)})
+                                    '(power %d), one shot, nt %d, raw saves every 4th step, 1024 receivers, %s'
+                                    % (att_power, plan.nt, kernel)})
 
 
 def bench_density_pair(nt=2500, tile=None):
@@ -1032,7 +1034,8 @@ def main():
                     help='one 3D shot with power-2 attenuation on the fused update and on the general passes '
                          '(use with --nt) and exit')
     ap.add_argument('--ot4', action='store_true',
-                    help='one 3D OT4 shot on the fused marches and on the general passes (use --nt 500)')
+                    help='one 3D OT4 shot on the fused marches and on the general passes (use --nt 500); with '
+                         '--density, the density shot with OT4')
     ap.add_argument('--pml', action='store_true',
                     help='one 3D shot with the CFS-PML boundary on the fused update and on the general passes '
                          '(use --nt 500)')
@@ -1097,7 +1100,7 @@ def main():
     if args.autograd:
         print(json.dumps(bench_autograd(args.nt, args.checkpoint)))
         return
-    if args.ot4:
+    if args.ot4 and not args.density:
         print(json.dumps({'fused': bench_ot4_3d(args.nt), 'general': bench_ot4_3d(args.nt, general=True)}))
         return
     if args.pml:
@@ -1113,7 +1116,7 @@ def main():
     if args.density:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None
         print(json.dumps(bench_density(args.nt, tile, with_cpu=not args.no_cpu, checkpoint=args.checkpoint,
-                                       att_power=args.att_power)))
+                                       att_power=args.att_power, kernel='OT4' if args.ot4 else 'OT2')))
         return
     if args.schedule:
         tile = tuple(int(v) for v in args.tile.split(',')) if args.tile else None

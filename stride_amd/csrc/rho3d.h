@@ -32,6 +32,13 @@ struct Rho3DStep {
     // coefficients are shared; `save` takes shot 0's save and save1 shot 1's (same box layout)
     int nshot = 1;
     float *save1 = nullptr;
+    // OT4 (one shot, no konly / kout / power 2): pass 1 (ot4 = 1) writes K u into out and
+    // w = M K u into wout; pass 2 (ot4 = 2) takes w as u and reads the level from uown and its K u
+    // from kuin: out = A uown + B other + G (kuin + c4 K w)
+    int ot4 = 0;
+    const float *M = nullptr, *uown = nullptr, *kuin = nullptr;
+    float *wout = nullptr;
+    float c4 = 0.f;
 };
 
 // true when the layout of `nshot` interleaved shots fits the kernel's 32-bit buffer offsets

@@ -48,6 +48,12 @@ struct RhoArgs {
     // power-2 attenuation (Cbeta non-null): Cbeta and the centred Laplacians of the other level
     // and of u
     const float *Cbeta, *lapo, *lapu;
+    // OT4: pass 1 writes K u (out) and w = m K u (wout, m from M); pass 2 stages w (the u field)
+    // and reads the level itself (uown) and its K u (kuin): out = A u + B other + G (K u + c4 K w)
+    int ot4;
+    const float *M, *uown, *kuin;
+    float *wout;
+    float c4;
 };
 
 // NS shots per pass: u, o and out hold NS floats per cell (shot-minor); b and the coefficients
@@ -76,6 +82,7 @@ struct State {
     // power-2 attenuation: own Cbeta and the centred Laplacians of the other level and of u of the
     // next node plane
     float Cb[2][RPT], Lo[2][RPT], Lu[2][RPT];
+    float Mv[2][RPT], Uo[2][RPT], Kv[2][RPT];  // OT4: m (pass 1); u and K u (pass 2)
     unsigned uoff[T::NLU], boff[T::NLB];  // in-plane byte offsets of the tile chunks (kOOB: none)
     int ulds[T::NLU], blds[T::NLB];       // their LDS float offsets (the dump slot: none)
     unsigned own[RPT];                    // own cells' in-plane byte offsets (kOOB outside the domain)
@@ -91,7 +98,7 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 struct Res {
-    rsrc_t u, o, b, A, B, G, out, Cb, Lo, Lu, Ko;
+    rsrc_t u, o, b, A, B, G, out, Cb, Lo, Lu, Ko, M, Wo, Uo, Ki;
 };
 
 // plane pitch in bytes of the NS-shot fields (the one-shot pitch, known equal, for NS = 1)
@@ -130,8 +137,17 @@ template <int S, int RPT, int NW, bool VARB, int FORM, int NS>
 __device__ __forceinline__ void load_own(State<RPT, NW, VARB, NS> &st, const Res &rs, unsigned poff,
                                          unsigned spoff) {
     if (FORM == 1) return;  // out = K u: no own operands
+    if (FORM == 3) {        // OT4 pass 1: m
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) st.Mv[S][r] = load_nt(rs.M, st.own[r] + poff);
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
+        if (FORM == 4) {  // OT4 pass 2: the level and its K u
+            st.Uo[S][r] = load_nt(rs.Uo, st.own[r] + poff);
+            st.Kv[S][r] = load_nt(rs.Ki, st.own[r] + poff);
+        }
         if (NS == 1) {
             st.ov[S][0][r] = load_nt(rs.o, st.own[r] + spoff);
         } else {
@@ -288,21 +304,24 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
             float out[NS];
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                const float u = st.uq[s][r][(J + 1) % 10];
-                float acc = st.Gv[S][r] * st.nd[s][r][(J + 1) % 10];
+                // OT4 pass 2: the staged field is w, so nd = K w; the level is an operand
+                const float u = FORM == 4 ? st.Uo[S][r] : st.uq[s][r][(J + 1) % 10];
+                const float kn = st.nd[s][r][(J + 1) % 10];
+                float acc = FORM == 4 ? st.Gv[S][r] * (st.Kv[S][r] + a.c4 * kn) : st.Gv[S][r] * kn;
                 if (FORM == 2) acc += st.Cb[S][r] * (st.Lu[S][r] - st.Lo[S][r]);  // power 2: Cbeta (lap u - lap other)
-                out[s] = FORM == 1 ? st.nd[s][r][(J + 1) % 10] : st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][s][r] + acc;
+                out[s] = (FORM == 1 || FORM == 3) ? kn : st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][s][r] + acc;
             }
             if (NS == 1) store_nt(out[0], rs.out, emit ? st.own[r] + spo : kOOB);
             else store2_nt(out[0], out[NS - 1], rs.out, emit ? st.owns[r] + spo : kOOB);
             if (FORM == 0 && NS == 1 && a.kout) store_nt(st.nd[0][r][(J + 1) % 10], rs.Ko, emit ? st.own[r] + po : kOOB);
+            if (FORM == 3) store_nt(st.Mv[S][r] * out[0], rs.Wo, emit ? st.own[r] + po : kOOB);  // w = m K u
             if (SAVE && emit) {
                 const int y = y0 + w * RPT + r, z = z0 + lane;
                 if (sa_inbox(a.box, p, y, z)) {
                     const long long bi = sa_boxidx(a.box, p, y, z);
 #pragma unroll
                     for (int s = 0; s < NS; ++s) {
-                        const float u = st.uq[s][r][(J + 1) % 10];
+                        const float u = FORM == 4 ? st.Uo[S][r] : st.uq[s][r][(J + 1) % 10];
                         (s == 0 ? a.save : a.save1)[bi] =
                             a.save_mode == 1 ? (out[s] - 2.f * u + st.ov[S][s][r]) * a.idt2 : u;
                     }
@@ -346,6 +365,14 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     rs.G = make(a.G, a.bytes);
     rs.out = make(a.out, NS == 1 ? a.bytes : a.sbytes);
     rs.Ko = make(a.kout ? a.kout : a.out, a.kout ? a.bytes : 0);
+    if (FORM == 3) {
+        rs.M = make(a.M, a.bytes);
+        rs.Wo = make(a.wout, a.bytes);
+    }
+    if (FORM == 4) {
+        rs.Uo = make(a.uown, a.bytes);
+        rs.Ki = make(a.kuin, a.bytes);
+    }
     if (FORM == 2) {
         rs.Cb = make(a.Cbeta, a.bytes);
         rs.Lo = make(a.lapo, a.bytes);
@@ -427,7 +454,13 @@ int launch(const Geo &g, RhoArgs a, bool varb, bool save, int nshot, hipStream_t
         else if (varb) k_rho3d<RPT, NW, true, false, 0, 2><<<grid, block, 0, stream>>>(g, a);
         else if (save) k_rho3d<RPT, NW, false, true, 0, 2><<<grid, block, 0, stream>>>(g, a);
         else k_rho3d<RPT, NW, false, false, 0, 2><<<grid, block, 0, stream>>>(g, a);
-    } else if (a.konly && varb) k_rho3d<RPT, NW, true, false, 1, 1><<<grid, block, 0, stream>>>(g, a);
+    } else if (a.ot4 == 1 && varb) k_rho3d<RPT, NW, true, false, 3, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.ot4 == 1) k_rho3d<RPT, NW, false, false, 3, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.ot4 == 2 && varb && save) k_rho3d<RPT, NW, true, true, 4, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.ot4 == 2 && varb) k_rho3d<RPT, NW, true, false, 4, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.ot4 == 2 && save) k_rho3d<RPT, NW, false, true, 4, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.ot4 == 2) k_rho3d<RPT, NW, false, false, 4, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.konly && varb) k_rho3d<RPT, NW, true, false, 1, 1><<<grid, block, 0, stream>>>(g, a);
     else if (a.konly) k_rho3d<RPT, NW, false, false, 1, 1><<<grid, block, 0, stream>>>(g, a);
     else if (a.Cbeta && varb && save) k_rho3d<RPT, NW, true, true, 2, 1><<<grid, block, 0, stream>>>(g, a);
     else if (a.Cbeta && varb) k_rho3d<RPT, NW, true, false, 2, 1><<<grid, block, 0, stream>>>(g, a);
@@ -885,6 +918,16 @@ int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_
     a.lapu = st.lapu;
     if (st.konly && (st.save || st.Cbeta)) return -1;
     if (st.Cbeta && (!st.lapother || !st.lapu)) return -1;
+    a.ot4 = st.ot4;
+    a.M = st.M;
+    a.uown = st.uown;
+    a.kuin = st.kuin;
+    a.wout = st.wout;
+    a.c4 = st.c4;
+    if (st.ot4 < 0 || st.ot4 > 2) return -1;
+    if (st.ot4 && (st.nshot != 1 || st.konly || st.kout || st.Cbeta)) return -1;
+    if (st.ot4 == 1 && (!st.M || !st.wout || st.save)) return -1;
+    if (st.ot4 == 2 && (!st.uown || !st.kuin)) return -1;
     const bool varb = st.b != nullptr, save = st.save != nullptr;
     const int ns = st.nshot;
     if (t.waves == 4 && t.rows == 1) return launch<1, 4>(g, a, varb, save, ns, stream);

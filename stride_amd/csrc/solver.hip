@@ -1277,9 +1277,11 @@ static int laplacian(sa_solver *s, const float *u, float *out) {
 
 // the fused 3D density update (with CFS-PML: the auxiliary fields by k_pml_slabs before it and
 // their term by k_pml_correct after it)
+// (OT4: two passes, no PML, no power-2 attenuation)
 static bool use_rho3d(const sa_solver *s) {
     const sa_scheme &c = s->sch;
-    return s->grid.dim == 3 && c.kernel == 2 && c.rho_mode != 0 && s->rho3d.enabled && rho3d_supported(s->geo);
+    const bool order = c.kernel == 2 || (c.kernel == 4 && c.boundary != 1 && c.att_power != 2);
+    return s->grid.dim == 3 && order && c.rho_mode != 0 && s->rho3d.enabled && rho3d_supported(s->geo);
 }
 
 static bool use_flat2d(const sa_solver *s) {
@@ -1303,7 +1305,8 @@ static bool use_fast_pairs(const sa_solver *s) {
 
 // the two-shot form of the fused density update: no power-2 attenuation, no PML
 static bool use_rho3d_pairs(const sa_solver *s) {
-    return use_rho3d(s) && s->sch.att_power != 2 && s->sch.boundary != 1 && rho3d_supported(s->geo, 2);
+    return use_rho3d(s) && s->sch.kernel == 2 && s->sch.att_power != 2 && s->sch.boundary != 1 &&
+           rho3d_supported(s->geo, 2);
 }
 
 static PmlSet pml_set(const sa_solver *s) {
@@ -1667,7 +1670,25 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
             st.lapother = s->lapother;
             st.lapu = s->lapnext;
         }
-        if (kout && c.att_power != 2) st.kout = kout;
+        if (kout && c.att_power != 2 && c.kernel == 2) st.kout = kout;
+        if (c.kernel == 4) {
+            // OT4: pass 1 forms K u and w = m K u; the update (pass 2) stages w, so its K is
+            // K(m K u), and reads u and K u as operands
+            Rho3DStep p1 = st;
+            p1.save = nullptr;
+            p1.out = s->Ku;
+            p1.ot4 = 1;
+            p1.M = s->M;
+            p1.wout = s->L2;
+            if (rho3d_step(s->rho3d, g, p1, s->stream) != 0)
+                return fail(SA_E_HIP, "fused density OT4 first pass launch: %s",
+                            hipGetErrorString(hipGetLastError()));
+            st.u = s->L2;
+            st.ot4 = 2;
+            st.uown = u;
+            st.kuin = s->Ku;
+            st.c4 = s->dt * s->dt / 12.f;
+        }
         if (c.boundary == 1) TRY(pml_before(s, u, backward));
         if (rho3d_step(s->rho3d, g, st, s->stream) != 0)
             return fail(SA_E_HIP, "fused density update launch: %s", hipGetErrorString(hipGetLastError()));

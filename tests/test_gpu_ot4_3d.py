@@ -1,7 +1,9 @@
-"""OT4 on the specialised 3D update (no density, sponge boundary, attenuation off / power 0 / None): a first pass forms K u and w = vp^2 K u, the update stages w, so its Laplacian is K(m K u)
+"""OT4 on the fused 3D updates (sponge boundary, attenuation off / power 0 / None): on the
+specialised update (no density): a first pass forms K u and w = vp^2 K u, the update stages w, so its Laplacian is K(m K u)
 of the fourth-order time stepping (stride/physics/iso_acoustic/devito.py:1466-1608), and reads u
-and K u as own-cell operands. Against the oracle and against the separate general passes, forward
-and as-written adjoint.
+and K u as own-cell operands; on the fused density update the same two passes with K = div(b grad)
+and m = vp^2 rho. Against the oracle and against the separate general passes, forward and
+as-written adjoint.
 
 Grids: 44^3 and 51^3 extended (one tile across axis 2) and 86^3 extended (two tiles, several
 chunks).
@@ -24,6 +26,9 @@ CASES = [
     dict(alpha='p0'),
     dict(alpha='none-power'),
     dict(n=37, nabs=7),  # partial tiles on every axis
+    # the fused density update in two passes (K u and m K u; then the update staging m K u)
+    dict(rho='var', alpha='p0'),
+    dict(rho='const'),
 ]
 
 
@@ -33,16 +38,19 @@ def _label(o):
 
 def _case(n=28, nabs=8, nt=120, **opts):
     alpha = opts.pop('alpha', None)
-    return small_case(dim=3, n=n, nabs=nabs, nt=nt, alpha=alpha, kernel='OT4', nrec=20, **opts)[0]
+    rho = opts.pop('rho', 'none')
+    return small_case(dim=3, n=n, nabs=nabs, nt=nt, rho=rho, alpha=alpha, kernel='OT4', nrec=20, **opts)[0]
 
 
-def _run(plan, general, save_mode=0, res=None):
+def _run(plan, general, save_mode=0, res=None, want_rho=False):
     try:
         if general:
             os.environ['STRIDE_AMD_GENERAL_ONLY'] = '1'
         s = solver_for(plan, save_mode=save_mode)
+        if general:
+            s.set_tile_density(0)  # the density schemes' fused update off as well
         rec = s.forward(plan.src_data)
-        g = s.adjoint(res, want_vp=True) if res is not None else None
+        g = s.adjoint(res, want_vp=True, want_rho=want_rho) if res is not None else None
         s.close()
     finally:
         os.environ.pop('STRIDE_AMD_GENERAL_ONLY', None)
@@ -52,7 +60,7 @@ def _run(plan, general, save_mode=0, res=None):
 @pytest.mark.parametrize('opts', CASES, ids=[_label(o) for o in CASES])
 def test_ot4_3d_forward_fused_matches_general_and_oracle(gpu, opts):
     plan = _case(**dict(opts))
-    assert plan.kernel_order == 4 and plan.rho_mode == 0
+    assert plan.kernel_order == 4
     print('ot4 3d forward', _label(opts), plan.shape, flush=True)
     ref, _, _ = oracle.forward(plan)
     fused, _ = _run(plan, False)
@@ -64,20 +72,24 @@ def test_ot4_3d_forward_fused_matches_general_and_oracle(gpu, opts):
     assert e_gen <= 1e-5
 
 
-def test_ot4_3d_large_grid_adjoint(gpu):
-    """86^3 extended, attenuation power 0: p.dt2 saves fused into the second pass, the as-written
-    adjoint with the fused vp gradient, against the general passes and the oracle."""
-    plan = _case(n=70, nabs=8, nt=100, alpha='p0', grad_vp=True)
-    print('ot4 3d adjoint', plan.shape, flush=True)
+@pytest.mark.parametrize('rho', ['none', 'var'], ids=['vp-only', 'variable-density'])
+def test_ot4_3d_large_grid_adjoint(gpu, rho):
+    """86^3 extended, attenuation power 0: the as-written adjoint against the general passes and
+    the oracle (no density: p.dt2 saves and the vp gradient fused into the second pass; variable
+    density: raw saves, vp and rho gradients from the gradient pass)."""
+    dens = rho != 'none'
+    plan = _case(n=70, nabs=8, nt=100, alpha='p0', grad_vp=True, grad_rho=dens, rho=rho)
+    print('ot4 3d adjoint', rho, plan.shape, flush=True)
     rng = np.random.default_rng(29)
     res = adjoint_source_samples(plan, rng.standard_normal((plan.nrec, plan.nt)).astype(np.float32))
-    rec_f, g_f = _run(plan, False, 1, res)
-    rec_g, g_g = _run(plan, True, 1, res)
+    save_mode = 2 if dens else 1
+    rec_f, g_f = _run(plan, False, save_mode, res, want_rho=dens)
+    rec_g, g_g = _run(plan, True, save_mode, res, want_rho=dens)
     ref, saved, _ = oracle.forward(plan, save=True)
     adj = oracle.adjoint_compat(plan, res, saved)
     assert rel_l2(rec_f.T, ref) <= 1e-5
     assert rel_l2(rec_f, rec_g) <= 1e-5
-    for key in ('grad_vp', 'prec_vp'):
+    for key in ('grad_vp', 'prec_vp') + (('grad_rho', 'prec_rho') if dens else ()):
         e_ref, e_gen = rel_l2(g_f[key], adj[key]), rel_l2(g_f[key], g_g[key])
         print('  %s vs oracle %.3e, vs general passes %.3e' % (key, e_ref, e_gen))
         assert np.abs(adj[key]).max() > 0
