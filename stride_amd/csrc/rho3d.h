@@ -39,6 +39,10 @@ struct Rho3DStep {
     const float *M = nullptr, *uown = nullptr, *kuin = nullptr;
     float *wout = nullptr;
     float c4 = 0.f;
+    // CFS-PML term (one shot, OT2, no power 2, no konly / kout): out -= Cpml sum_d u1_d, the u1
+    // fields already advanced for this step (read on the absorbing-layer cells only)
+    const float *Cpml = nullptr;
+    const float *pu1[3] = {nullptr, nullptr, nullptr};
 };
 
 // true when the layout of `nshot` interleaved shots fits the kernel's 32-bit buffer offsets

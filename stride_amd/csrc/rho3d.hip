@@ -54,6 +54,10 @@ struct RhoArgs {
     const float *M, *uown, *kuin;
     float *wout;
     float c4;
+    // CFS-PML term (form 5): Cpml and the u1 field of each axis, read on the cells of the
+    // absorbing layers only (layer widths nabs, axis-0 length n0)
+    const float *Cpml, *pu1[3];
+    int nabs[3], n0;
 };
 
 // NS shots per pass: u, o and out hold NS floats per cell (shot-minor); b and the coefficients
@@ -83,6 +87,8 @@ struct State {
     // next node plane
     float Cb[2][RPT], Lo[2][RPT], Lu[2][RPT];
     float Mv[2][RPT], Uo[2][RPT], Kv[2][RPT];  // OT4: m (pass 1); u and K u (pass 2)
+    float Cp[2][RPT], Q1[2][RPT], Q2[2][RPT], Q3[2][RPT];  // CFS-PML: Cpml, u1 of each axis
+    bool lay[RPT];  // own cell's row or column inside a layer of axis 1 or 2
     unsigned uoff[T::NLU], boff[T::NLB];  // in-plane byte offsets of the tile chunks (kOOB: none)
     int ulds[T::NLU], blds[T::NLB];       // their LDS float offsets (the dump slot: none)
     unsigned own[RPT];                    // own cells' in-plane byte offsets (kOOB outside the domain)
@@ -98,7 +104,7 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 struct Res {
-    rsrc_t u, o, b, A, B, G, out, Cb, Lo, Lu, Ko, M, Wo, Uo, Ki;
+    rsrc_t u, o, b, A, B, G, out, Cb, Lo, Lu, Ko, M, Wo, Uo, Ki, Cp, Q1, Q2, Q3;
 };
 
 // plane pitch in bytes of the NS-shot fields (the one-shot pitch, known equal, for NS = 1)
@@ -134,8 +140,8 @@ __device__ __forceinline__ void load_tile(State<RPT, NW, VARB, NS> &st, const Re
 }
 
 template <int S, int RPT, int NW, bool VARB, int FORM, int NS>
-__device__ __forceinline__ void load_own(State<RPT, NW, VARB, NS> &st, const Res &rs, unsigned poff,
-                                         unsigned spoff) {
+__device__ __forceinline__ void load_own(State<RPT, NW, VARB, NS> &st, const Res &rs, const RhoArgs &a, int x,
+                                         unsigned poff, unsigned spoff) {
     if (FORM == 1) return;  // out = K u: no own operands
     if (FORM == 3) {        // OT4 pass 1: m
 #pragma unroll
@@ -163,6 +169,15 @@ __device__ __forceinline__ void load_own(State<RPT, NW, VARB, NS> &st, const Res
             st.Lo[S][r] = load_nt(rs.Lo, st.own[r] + poff);
             st.Lu[S][r] = load_nt(rs.Lu, st.own[r] + poff);
         }
+        if (FORM == 5) {
+            // node plane x (planes past the domain read too; their results are never stored)
+            const bool xl = min(x, a.n0 - 1 - x) < a.nabs[0];
+            const unsigned po = ((xl | st.lay[r]) ? st.own[r] : kOOB) + poff;
+            st.Cp[S][r] = load_nt(rs.Cp, po);
+            st.Q1[S][r] = load_nt(rs.Q1, po);
+            st.Q2[S][r] = load_nt(rs.Q2, po);
+            st.Q3[S][r] = load_nt(rs.Q3, po);
+        }
     }
 }
 
@@ -186,7 +201,7 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
     load_tile<S>(st, rs, (unsigned)((m + 2) * a.plane_bytes), (unsigned)((m + 2) * SPB(a)));
     const int p = m - 9;
     const bool emit = p >= x0 && p < x1;
-    load_own<1 - S, RPT, NW, VARB, FORM, NS>(st, rs, (unsigned)((p + 1) * a.plane_bytes),
+    load_own<1 - S, RPT, NW, VARB, FORM, NS>(st, rs, a, p + 1, (unsigned)((p + 1) * a.plane_bytes),
                                              (unsigned)((p + 1) * SPB(a)));
     lds_barrier();  // plane m's tiles are in LDS
     const bool inplane = m >= x0 && m < x1;
@@ -309,6 +324,7 @@ __device__ __forceinline__ void plane(const Geo &g, const RhoArgs &a, const Res 
                 const float kn = st.nd[s][r][(J + 1) % 10];
                 float acc = FORM == 4 ? st.Gv[S][r] * (st.Kv[S][r] + a.c4 * kn) : st.Gv[S][r] * kn;
                 if (FORM == 2) acc += st.Cb[S][r] * (st.Lu[S][r] - st.Lo[S][r]);  // power 2: Cbeta (lap u - lap other)
+                if (FORM == 5) acc -= st.Cp[S][r] * ((st.Q1[S][r] + st.Q2[S][r]) + st.Q3[S][r]);  // CFS-PML
                 out[s] = (FORM == 1 || FORM == 3) ? kn : st.Av[S][r] * u + st.Bv[S][r] * st.ov[S][s][r] + acc;
             }
             if (NS == 1) store_nt(out[0], rs.out, emit ? st.own[r] + spo : kOOB);
@@ -365,6 +381,12 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
     rs.G = make(a.G, a.bytes);
     rs.out = make(a.out, NS == 1 ? a.bytes : a.sbytes);
     rs.Ko = make(a.kout ? a.kout : a.out, a.kout ? a.bytes : 0);
+    if (FORM == 5) {
+        rs.Cp = make(a.Cpml, a.bytes);
+        rs.Q1 = make(a.pu1[0], a.bytes);
+        rs.Q2 = make(a.pu1[1], a.bytes);
+        rs.Q3 = make(a.pu1[2], a.bytes);
+    }
     if (FORM == 3) {
         rs.M = make(a.M, a.bytes);
         rs.Wo = make(a.wout, a.bytes);
@@ -394,6 +416,7 @@ __global__ __launch_bounds__(NW * 64) void k_rho3d(Geo g, RhoArgs a) {
         const bool in = y < g.n[1] && z < g.n[2];
         const long long cell = g.base + (long long)y * g.s1 + z;
         st.own[r] = in ? (unsigned)(cell * 4) : kOOB;
+        st.lay[r] = min(y, g.n[1] - 1 - y) < g.nabs[1] || min(z, g.n[2] - 1 - z) < g.nabs[2];
         if (NS > 1) st.owns[r] = in ? (unsigned)(cell * 4 * NS) : kOOB;
     }
     // rows past the padded plane (y > n1 + 9) would read the next plane: they read zero. A u
@@ -454,7 +477,11 @@ int launch(const Geo &g, RhoArgs a, bool varb, bool save, int nshot, hipStream_t
         else if (varb) k_rho3d<RPT, NW, true, false, 0, 2><<<grid, block, 0, stream>>>(g, a);
         else if (save) k_rho3d<RPT, NW, false, true, 0, 2><<<grid, block, 0, stream>>>(g, a);
         else k_rho3d<RPT, NW, false, false, 0, 2><<<grid, block, 0, stream>>>(g, a);
-    } else if (a.ot4 == 1 && varb) k_rho3d<RPT, NW, true, false, 3, 1><<<grid, block, 0, stream>>>(g, a);
+    } else if (a.Cpml && varb && save) k_rho3d<RPT, NW, true, true, 5, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cpml && varb) k_rho3d<RPT, NW, true, false, 5, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cpml && save) k_rho3d<RPT, NW, false, true, 5, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.Cpml) k_rho3d<RPT, NW, false, false, 5, 1><<<grid, block, 0, stream>>>(g, a);
+    else if (a.ot4 == 1 && varb) k_rho3d<RPT, NW, true, false, 3, 1><<<grid, block, 0, stream>>>(g, a);
     else if (a.ot4 == 1) k_rho3d<RPT, NW, false, false, 3, 1><<<grid, block, 0, stream>>>(g, a);
     else if (a.ot4 == 2 && varb && save) k_rho3d<RPT, NW, true, true, 4, 1><<<grid, block, 0, stream>>>(g, a);
     else if (a.ot4 == 2 && varb) k_rho3d<RPT, NW, true, false, 4, 1><<<grid, block, 0, stream>>>(g, a);
@@ -928,6 +955,15 @@ int rho3d_step(const Rho3DTune &t, const Geo &g, const Rho3DStep &st, hipStream_
     if (st.ot4 && (st.nshot != 1 || st.konly || st.kout || st.Cbeta)) return -1;
     if (st.ot4 == 1 && (!st.M || !st.wout || st.save)) return -1;
     if (st.ot4 == 2 && (!st.uown || !st.kuin)) return -1;
+    a.Cpml = st.Cpml;
+    for (int d = 0; d < 3; ++d) {
+        a.pu1[d] = st.pu1[d];
+        a.nabs[d] = g.nabs[d];
+    }
+    a.n0 = g.n[0];
+    if (st.Cpml && (st.nshot != 1 || st.konly || st.kout || st.Cbeta || st.ot4 || !st.pu1[0] || !st.pu1[1] ||
+                    !st.pu1[2]))
+        return -1;
     const bool varb = st.b != nullptr, save = st.save != nullptr;
     const int ns = st.nshot;
     if (t.waves == 4 && t.rows == 1) return launch<1, 4>(g, a, varb, save, ns, stream);

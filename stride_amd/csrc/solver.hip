@@ -1689,10 +1689,17 @@ static int step_general(sa_solver *s, const float *u, const float *other, float 
             st.kuin = s->Ku;
             st.c4 = s->dt * s->dt / 12.f;
         }
+        // CFS-PML: the auxiliary fields first; their term inside the update (or, with power-2
+        // attenuation, by a launch of its own after it)
+        const bool pml_in = c.boundary == 1 && c.att_power != 2 && c.kernel == 2 && !st.kout;
         if (c.boundary == 1) TRY(pml_before(s, u, backward));
+        if (pml_in) {
+            st.Cpml = s->Cpml;
+            for (int a = 0; a < 3; ++a) st.pu1[a] = s->u1[a];
+        }
         if (rho3d_step(s->rho3d, g, st, s->stream) != 0)
             return fail(SA_E_HIP, "fused density update launch: %s", hipGetErrorString(hipGetLastError()));
-        if (c.boundary == 1) TRY(pml_after(s, out));
+        if (c.boundary == 1 && !pml_in) TRY(pml_after(s, out));
         if (kout_done) *kout_done = st.kout != nullptr;
         if (c.att_power == 2) lap_advance(s, u);
         TRY(timed_end(s));
