@@ -61,9 +61,10 @@ struct FastAtt {
 };
 // OT4 first pass: ku = K u and w = vp^2 K u over the domain (one march)
 int fast3d_ot4_first(Fast3D &f, const Geo &g, const float *u, float *ku, float *w, hipStream_t stream);
-// forward step: out = step(u, o) (out may alias o); optional fused save of p.dt2 into `save` (box layout)
+// forward step: out = step(u, o) (out may alias o); optional fused save into `save` (box layout):
+// p.dt2 (save_mode 1) or the level u itself (2: raw snapshots)
 int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,
-                float idt2, hipStream_t stream, const FastAtt *att = nullptr);
+                float idt2, hipStream_t stream, const FastAtt *att = nullptr, int save_mode = 1);
 // adjoint step with optional fused gradient: grad += ps * u, prec += ps^2 on the interior (box layout ps)
 int fast3d_step_adjoint(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, const float *ps,
                         const Box &box, float *grad, float *prec, hipStream_t stream,

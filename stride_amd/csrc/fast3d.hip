@@ -62,6 +62,7 @@ struct FastArgs {
     const float *uown, *kuin;
     float *wout;
     float c4;
+    int raw_save;  // MODE 1: save the level itself (raw snapshots) instead of p.dt2
 };
 
 // MODE 0: plain step, 1: fused save of p.dt2, 2: fused gradient accumulation
@@ -293,7 +294,8 @@ __device__ __forceinline__ void finish7(const Geo &g, const FastArgs &a, const R
         if (ATT & 4) acc -= sx.cp[r] * ((sx.q1[r] + sx.q2[r]) + sx.q3[r]);
         const float val = A * cc + B * sx.o[r] + acc;
         bstore_nt(val, rs.out, xin ? orow[r] : kOOB, sxo);
-        if (MODE == 1) bstore_nt((val - 2.f * cc + sx.o[r]) * a.idt2, rs.save, xin_box ? brow[r] : kOOB, sbx);
+        if (MODE == 1)
+            bstore_nt(a.raw_save ? cc : (val - 2.f * cc + sx.o[r]) * a.idt2, rs.save, xin_box ? brow[r] : kOOB, sbx);
         if (MODE == 2) {
             const float psv = sx.ps[r];
             const unsigned go = xin_box ? grow[r] : kOOB;
@@ -1114,7 +1116,7 @@ int fast3d_prepare(Fast3D &f, const Geo &g, const float *vp, const float *tables
 }
 
 int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *out, float *save, const Box &box,
-                float idt2, hipStream_t stream, const FastAtt *att) {
+                float idt2, hipStream_t stream, const FastAtt *att, int save_mode) {
     FastArgs a = base_args(f, u, o, out);
     if (att) {
         if (att->power == 2 && (!att->alpha || !att->lapo || !att->lapn)) return SA_E_ARG;
@@ -1142,9 +1144,11 @@ int fast3d_step(Fast3D &f, const Geo &g, const float *u, const float *o, float *
         }
     }
     if (save) {
+        if (save_mode != 1 && save_mode != 2) return SA_E_ARG;
         a.save = save;
         a.box = box;
         a.idt2 = idt2;
+        a.raw_save = save_mode == 2;
         return launch<1>(f, g, a, stream);
     }
     return launch<0>(f, g, a, stream);

@@ -1290,12 +1290,14 @@ static bool use_flat2d(const sa_solver *s) {
 
 // the specialised 3D update: no density term; attenuation of any form with one shot per pass;
 // sponge, no boundary or CFS-PML (as for the density update)
-// (OT4: two passes, attenuation off / power 0 / None, no PML)
+// (OT4: two passes, attenuation off / power 0 / None, no PML); save modes 0 - 2 and 5 (p.dt2 or
+// raw snapshots fused; mode 5's recomputation runs the same steps). The exact adjoint's save modes
+// 3 and 4 stay on the general passes, which its checkpoint recomputation also runs.
 static bool use_fast(const sa_solver *s) {
     const sa_scheme &c = s->sch;
     const bool order = c.kernel == 2 || (c.kernel == 4 && c.boundary != 1 && c.att_power != 2);
-    return s->grid.dim == 3 && order && c.rho_mode == 0 && (c.save_mode == 0 || c.save_mode == 1) &&
-           fast3d_enabled() && fast3d_supported(s->geo);
+    const bool saves = c.save_mode <= 2 || c.save_mode == 5;
+    return s->grid.dim == 3 && order && c.rho_mode == 0 && saves && fast3d_enabled() && fast3d_supported(s->geo);
 }
 
 // the two-shot and multi-shot forms of the specialised update: no attenuation, no PML
@@ -1981,7 +1983,7 @@ static int forward_step(sa_solver *s, int t, float *u, float *o, const float *sr
         if (s->sch.kernel == 4) TRY(ot4_first(s, u, at, pat, staged));
         hipEvent_t e0;
         TRY(timed_begin(s, &e0));
-        if (fast3d_step(s->fast, s->geo, staged, o, o, save, s->box, idt2, s->stream, pat) != 0)
+        if (fast3d_step(s->fast, s->geo, staged, o, o, save, s->box, idt2, s->stream, pat, mode == 2 ? 2 : 1) != 0)
             return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
         TRY(timed_end(s));
         if (s->sch.att_power == 2) lap_advance(s, u);
