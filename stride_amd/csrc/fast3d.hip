@@ -941,6 +941,7 @@ void launch_v7_att(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const F
         k_fast3d_v7<RPT, NW, MODE, false, ATT, NS><<<grid, NW * 64, 0, st>>>(g, a);
 }
 
+// attenuation forms 0 - 3 on every tile
 template <int RPT, int NW, int MODE, int NS>
 void launch_v7(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
     if constexpr (MODE >= 3) {
@@ -951,25 +952,38 @@ void launch_v7(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastA
             case 1: launch_v7_att<RPT, NW, MODE, 1, NS>(sponge, grid, st, g, a); break;
             case 2: launch_v7_att<RPT, NW, MODE, 2, NS>(sponge, grid, st, g, a); break;
             case 3: launch_v7_att<RPT, NW, MODE, 3, NS>(sponge, grid, st, g, a); break;
-            // CFS-PML (no sponge): the layers' term fused
-            case 4: k_fast3d_v7<RPT, NW, MODE, false, 4, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
-            case 5: k_fast3d_v7<RPT, NW, MODE, false, 5, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
-            case 6: k_fast3d_v7<RPT, NW, MODE, false, 6, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
-            case 7: k_fast3d_v7<RPT, NW, MODE, false, 7, NS><<<grid, NW * 64, 0, st>>>(g, a); break;
-            // OT4 second pass (attenuation off / power 0 / power None)
-            case 8: launch_v7_att<RPT, NW, MODE, 8, NS>(sponge, grid, st, g, a); break;
-            case 9: launch_v7_att<RPT, NW, MODE, 9, NS>(sponge, grid, st, g, a); break;
-            case 11: launch_v7_att<RPT, NW, MODE, 11, NS>(sponge, grid, st, g, a); break;
             default: launch_v7_att<RPT, NW, MODE, 0, NS>(sponge, grid, st, g, a); break;
         }
+    }
+}
+
+// the CFS-PML (4 - 7) and OT4 second-pass (8, 9, 11) forms, on the version-7 tile only
+template <int MODE>
+bool launch_v7_forms(bool sponge, dim3 grid, hipStream_t st, const Geo &g, const FastArgs &a) {
+    if constexpr (MODE >= 3) {
+        return false;
+    } else {
+        switch (a.att) {
+            case 4: k_fast3d_v7<2, 4, MODE, false, 4, 3><<<grid, 4 * 64, 0, st>>>(g, a); break;
+            case 5: k_fast3d_v7<2, 4, MODE, false, 5, 3><<<grid, 4 * 64, 0, st>>>(g, a); break;
+            case 6: k_fast3d_v7<2, 4, MODE, false, 6, 3><<<grid, 4 * 64, 0, st>>>(g, a); break;
+            case 7: k_fast3d_v7<2, 4, MODE, false, 7, 3><<<grid, 4 * 64, 0, st>>>(g, a); break;
+            case 8: launch_v7_att<2, 4, MODE, 8, 3>(sponge, grid, st, g, a); break;
+            case 9: launch_v7_att<2, 4, MODE, 9, 3>(sponge, grid, st, g, a); break;
+            case 11: launch_v7_att<2, 4, MODE, 11, 3>(sponge, grid, st, g, a); break;
+            default: return false;
+        }
+        return true;
     }
 }
 
 template <int MODE>
 int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
     if (!fast3d_supported(g)) return SA_E_UNSUPPORTED;  // buffer offsets are 32-bit
-    const int rows = f.tune.rows;
-    const int version = f.tune.version;
+    // the PML and OT4 forms run on the version-7 tile (4 waves x 2 rows) whatever the tune
+    const bool forms = MODE < 3 && a.att >= 4;
+    const int rows = forms ? 2 : f.tune.rows;
+    const int version = forms ? 7 : f.tune.version;
     const int waves = version == 8 ? 8 : kWaves;
     const int TY = waves * rows;
     a.ntile_y = (g.n[1] + TY - 1) / TY;
@@ -980,7 +994,9 @@ int launch(const Fast3D &f, const Geo &g, FastArgs &a, hipStream_t st) {
     a.per_xcd = (a.nblocks + 7) / 8;
     dim3 grid(a.per_xcd * 8);
     const bool sponge = f.boundary == 0;
-    if (version == 7) {
+    if (forms) {
+        if (!launch_v7_forms<MODE>(sponge, grid, st, g, a)) return SA_E_ARG;
+    } else if (version == 7) {
         switch (rows) {
             case 2: launch_v7<2, 4, MODE, 3>(sponge, grid, st, g, a); break;
             default: return SA_E_ARG;
