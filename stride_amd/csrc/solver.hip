@@ -1290,14 +1290,13 @@ static bool use_flat2d(const sa_solver *s) {
 
 // the specialised 3D update: no density term; attenuation of any form with one shot per pass;
 // sponge, no boundary or CFS-PML (as for the density update)
-// (OT4: two passes, attenuation off / power 0 / None, no PML); save modes 0 - 2 and 5 (p.dt2 or
-// raw snapshots fused; mode 5's recomputation runs the same steps). The exact adjoint's save modes
-// 3 and 4 stay on the general passes, which its checkpoint recomputation also runs.
+// (OT4: two passes, attenuation off / power 0 / None, no PML); every save mode: p.dt2 or raw
+// snapshots fused, and the checkpoint recomputations (modes 4, 5) and the exact adjoint's steps run
+// the same update
 static bool use_fast(const sa_solver *s) {
     const sa_scheme &c = s->sch;
     const bool order = c.kernel == 2 || (c.kernel == 4 && c.boundary != 1 && c.att_power != 2);
-    const bool saves = c.save_mode <= 2 || c.save_mode == 5;
-    return s->grid.dim == 3 && order && c.rho_mode == 0 && saves && fast3d_enabled() && fast3d_supported(s->geo);
+    return s->grid.dim == 3 && order && c.rho_mode == 0 && fast3d_enabled() && fast3d_supported(s->geo);
 }
 
 // the two-shot and multi-shot forms of the specialised update: no attenuation, no PML
@@ -1960,6 +1959,37 @@ static void lap_advance(sa_solver *s, const float *u) {
     s->lap_of = u;
 }
 
+// One step of the specialised 3D update (use_fast): out = step(u, o) (out may alias o) with the
+// scheme's attenuation, CFS-PML and OT4 forms; forward: an optional fused save into `save` (mode 1
+// p.dt2, 2 raw); backward (the adjoints' steps): an optional fused vp gradient from the p.dt2
+// snapshot ps.
+static int fast_step(sa_solver *s, const float *u, const float *o, float *out, int backward, float *save, int mode,
+                     const float *ps, float *gv, float *pv) {
+    const sa_scheme &c = s->sch;
+    FastAtt at{}, *pat = nullptr;
+    if (c.att_power != -2) {
+        if (c.att_power == 2) TRY(lap_other(s, o));
+        at = {c.att_power, s->alpha, s->lapother, s->lapnext};
+        pat = &at;
+    }
+    if (c.boundary == 1) {
+        // the auxiliary fields first; the update applies their term
+        TRY(pml_before(s, u, backward));
+        pml_term(s, at, pat);
+    }
+    const float *staged = u;
+    if (c.kernel == 4) TRY(ot4_first(s, u, at, pat, staged));
+    hipEvent_t e0;
+    TRY(timed_begin(s, &e0));
+    const int rc = backward ? fast3d_step_adjoint(s->fast, s->geo, staged, o, out, ps, s->box, gv, pv, s->stream, pat)
+                            : fast3d_step(s->fast, s->geo, staged, o, out, save, s->box, 1.f / (s->dt * s->dt),
+                                          s->stream, pat, mode == 2 ? 2 : 1);
+    if (rc != 0) return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
+    TRY(timed_end(s));
+    if (c.att_power == 2) lap_advance(s, u);
+    return SA_OK;
+}
+
 // One forward step t: p[t+1] into o from u = p[t] and o = p[t-1], then the source injection.
 // The forward run and the checkpointed recomputation issue the same launches through it, so the
 // recomputed levels equal the forward run's bit for bit.
@@ -1968,25 +1998,7 @@ static int forward_step(sa_solver *s, int t, float *u, float *o, const float *sr
     const float idt2 = 1.f / (s->dt * s->dt);
     bool injected = false;
     if (use_fast(s)) {
-        FastAtt at{}, *pat = nullptr;
-        if (s->sch.att_power != -2) {
-            if (s->sch.att_power == 2) TRY(lap_other(s, o));
-            at = {s->sch.att_power, s->alpha, s->lapother, s->lapnext};
-            pat = &at;
-        }
-        if (s->sch.boundary == 1) {
-            // the auxiliary fields first; the update applies their term
-            TRY(pml_before(s, u, 0));
-            pml_term(s, at, pat);
-        }
-        const float *staged = u;
-        if (s->sch.kernel == 4) TRY(ot4_first(s, u, at, pat, staged));
-        hipEvent_t e0;
-        TRY(timed_begin(s, &e0));
-        if (fast3d_step(s->fast, s->geo, staged, o, o, save, s->box, idt2, s->stream, pat, mode == 2 ? 2 : 1) != 0)
-            return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
-        TRY(timed_end(s));
-        if (s->sch.att_power == 2) lap_advance(s, u);
+        TRY(fast_step(s, u, o, o, 0, save, mode, nullptr, nullptr, nullptr));
     } else {
         const InjectSpec spec{&s->src, src + (size_t)t * s->src.npoint, 1.f, mode == 1};
         TRY(step_general(s, u, o, o, 0, save, mode, &spec, &injected, gat));
@@ -2099,8 +2111,12 @@ static int recompute_segment(sa_solver *s, int j) {
     for (int t = t0 + 1; t < tend; ++t) {
         float *out = segment_level(s, j, t + 1);
         bool done = false;
-        TRY(step_general(s, segment_level(s, j, t), segment_level(s, j, t - 1), out, 0, nullptr, 0, nullptr, nullptr,
-                         nullptr, nullptr, s->kseg ? segment_ku(s, j, t) : nullptr, &done));
+        if (use_fast(s))  // the forward run's own steps, so the recomputed levels equal its bits
+            TRY(fast_step(s, segment_level(s, j, t), segment_level(s, j, t - 1), out, 0, nullptr, 0, nullptr, nullptr,
+                          nullptr));
+        else
+            TRY(step_general(s, segment_level(s, j, t), segment_level(s, j, t - 1), out, 0, nullptr, 0, nullptr,
+                             nullptr, nullptr, nullptr, s->kseg ? segment_ku(s, j, t) : nullptr, &done));
         ku = ku && done;
         TRY(inject(s, s->src, s->src_keep + (size_t)t * s->src.npoint, 1.f, out, nullptr, 0.f, s->box));
     }
@@ -2217,25 +2233,7 @@ static int adjoint_loop(sa_solver *s, const float *res, float *gv, float *pv, fl
         if (guard && raw_ckpt && (t / f) / s->ck_K != s->ck_seg) TRY(recompute_raw_segment(s, (t / f) / s->ck_K));
         const float *ps = guard ? snap(t / f) : nullptr;
         if (fast) {
-            FastAtt at{}, *pat = nullptr;
-            if (c.att_power != -2) {
-                if (c.att_power == 2) TRY(lap_other(s, o));
-                at = {c.att_power, s->alpha, s->lapother, s->lapnext};
-                pat = &at;
-            }
-            if (c.boundary == 1) {
-                TRY(pml_before(s, u, 1));
-                pml_term(s, at, pat);
-            }
-            const float *staged = u;
-            if (c.kernel == 4) TRY(ot4_first(s, u, at, pat, staged));
-            hipEvent_t e0;
-            TRY(timed_begin(s, &e0));
-            if (fast3d_step_adjoint(s->fast, g, staged, o, o, s->saved_mode == 1 ? ps : nullptr, s->box, gv, pv,
-                                    s->stream, pat) != 0)
-                return fail(SA_E_HIP, "specialised 3D update launch: %s", hipGetErrorString(hipGetLastError()));
-            TRY(timed_end(s));
-            if (c.att_power == 2) lap_advance(s, u);
+            TRY(fast_step(s, u, o, o, 1, nullptr, 0, s->saved_mode == 1 ? ps : nullptr, gv, pv));
         } else {
             const InjectSpec spec{&s->rec, res + (size_t)t * s->rec.npoint, -1.f, false};
             // the fused 2D update also accumulates this step's gradients (it reads u and the
@@ -2377,7 +2375,9 @@ static int adjoint_exact_core(sa_solver *s, const float *res_dev, float *lam_dev
     for (int lv = nt - 1; lv >= 1 && rc == SA_OK; --lv) {
         float *m1 = s->buf[(lv + 1) % 2], *m2 = s->buf[lv % 2];  // mu_{lv+1}, mu_{lv+2}
         if (lv >= 2) {
-            rc = step_general(s, m1, m2, m2, 1, nullptr, 0);  // mu_lv over mu_{lv+2}
+            // mu_lv over mu_{lv+2}
+            rc = use_fast(s) ? fast_step(s, m1, m2, m2, 1, nullptr, 0, nullptr, nullptr, nullptr)
+                             : step_general(s, m1, m2, m2, 1, nullptr, 0);
             if (rc == SA_OK && lv - 1 >= 1 && lv - 1 <= nt - 2)
                 rc = inject(s, s->rec, res_dev + (size_t)(lv - 1) * s->rec.npoint, 1.f, m2, nullptr, 0.f, s->box);
         }

@@ -121,6 +121,10 @@ EXACT_CASES = [
     (2, dict(kernel='OT2', rho='var')),
     (2, dict(kernel='OT2', alpha='p0')),
     (3, dict(kernel='OT2')),
+    # 3D without density: the forward, its every-step saves and the adjoint's steps on the
+    # specialised update (OT4: its two marches)
+    (3, dict(kernel='OT4')),
+    (3, dict(kernel='OT2', alpha='p0')),
 ]
 
 
