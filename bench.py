@@ -1012,6 +1012,8 @@ def main():
                     help='leave out the secondary 32-shot batched-forward line')
     ap.add_argument('--no-density', action='store_true',
                     help='leave out the secondary 3D variable-density gradient shot')
+    ap.add_argument('--no-gradient-2d', action='store_true',
+                    help='leave out the secondary 2D gradient workload (8 shots in lockstep)')
     ap.add_argument('--tile', type=str, default=None,
                     help='rows,chunk,version of the 3D update tile (rows,chunk,waves with --shots 2)')
     ap.add_argument('--shots', type=int, default=2, choices=(1, 2),
@@ -1256,6 +1258,16 @@ def main():
         cp = bench_density(args.nt, None, with_cpu=False, checkpoint=k)
         out['density']['checkpointed'] = {key: cp[key] for key in ('value', 'seconds', 'snapshots', 'snapshot_store_gb',
                                                                    'update_launches', 'update_kernel_us')}
+    if not args.no_gradient_2d:
+        # secondary line: the 2D gradient workload (572^2, CFS-PML, variable density, power 2), eight
+        # shots in lockstep; reported, never required
+        print('[bench] 2D gradient workload, 8 shots in lockstep', file=sys.stderr, flush=True)
+        try:
+            g2 = bench_gradient_2d(with_cpu=False)
+            out['gradient_2d'] = {key: g2[key] for key in ('metric', 'value', 'unit', 'seconds_per_shot', 'lockstep',
+                                                           'value_workers', 'value_one_worker', 'update_kernel_us')}
+        except Exception as exc:
+            out['gradient_2d'] = {'error': repr(exc)}
     if not args.no_cpu:
         try:
             out['cpu_baseline'] = cpu_baseline(plan)
